@@ -1,0 +1,117 @@
+/*
+ * ithaca_mf.h -- C ABI of the MI355X (gfx950) matrix-free spectral/hp operators.
+ *
+ * Drop-in boundary for the reference's MatrixFree operator layer.  Each entry point
+ * names the reference interface it replaces (paths relative to the reference root).
+ * Plain C types only: no C++, HIP or torch types cross this boundary; streams are
+ * passed as `void *` (a hipStream_t, NULL = the default stream).
+ *
+ * Layouts (un-interleaved, element-major; Collections/CoalescedGeomData.cpp:53-112,
+ * :251-313):
+ *   coefficients  u[e*M + (r*nm + q)*nm + p]          M = nm^dim
+ *   quad points   f[e*Q + (k*nq + j)*nq + i]          Q = nq^dim
+ *   jac           deformed: jac[e*Q + q]             regular: jac[e]
+ *   df            deformed: df[d*(nElmt*Q) + e*Q + q] regular: df[d*nElmt + e]
+ *                 d = c*dim + a holds d(xi_a)/d(x_c)   (dim*dim components)
+ * Status codes: 0 = OK; non-zero = error, message via imf_last_error().
+ */
+#ifndef ITHACA_MF_H
+#define ITHACA_MF_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct imf_op_s *imf_op;
+
+enum imf_status {
+    IMF_OK = 0,
+    IMF_ERR_ARG = 1,         /* bad argument / size mismatch */
+    IMF_ERR_UNSUPPORTED = 2, /* unknown key or (modes, points) combination */
+    IMF_ERR_HIP = 3,         /* HIP runtime failure */
+    IMF_ERR_STATE = 4        /* missing geometry, wrong precision, ... */
+};
+
+enum imf_precision { IMF_FP64 = 0, IMF_FP32 = 1 };
+
+/* Replaces MatrixFree::GetOperatorFactory().CreateInstance(key, basis, nElmt)
+ * (MatrixFreeOps/Operator.hpp:303-310, keys from Operator.cpp:12-56 such as
+ * "Helmholtz_Hex_Deformed", "IProduct_Quad_Regular", "BwdTrans_Hex_Regular") together
+ * with the Helper<DIM,DEFORMED> constructor that copies the basis tables
+ * (Operator.hpp:205-275).  Per direction d < dim: bdata[d] (nm*nq, bdata[p*nq+i]),
+ * dbdata[d] (nm*nq), D[d] (nq*nq, D[a*nq+b] = l_a'(z_b)), w[d] (nq).  Like the
+ * reference kernels the operator requires homogeneous modes/points and supports
+ * nm in 2..8 with nq in nm..2*nm (Helmholtz.h:669-760).  No padding restriction on
+ * nElmt (the reference's vec_t::width padding is handled internally). */
+int imf_create(const char *key, int dim, const int *nm, const int *nq,
+               const double *const *bdata, const double *const *dbdata,
+               const double *const *D, const double *const *w, int nElmt,
+               int precision, imf_op *op);
+
+/* Releases device memory held by the operator. */
+int imf_destroy(imf_op op);
+
+/* Operator::Ndof (Operator.hpp:29-32; BwdTrans.h:190-193): nm^dim * nElmt.
+ * Defined for every operator type, including Helmholtz. */
+double imf_ndof(imf_op op);
+
+/* Operator::NeedsJac / NeedsDF (Operator.hpp:35-45, 88-91, 163-171). */
+int imf_needs_jac(imf_op op);
+int imf_needs_df(imf_op op);
+
+/* Number of input / output scalars per apply (nElmt * M or nElmt * Q). */
+size_t imf_in_size(imf_op op);
+size_t imf_out_size(imf_op op);
+
+/* Helper::SetJac / SetDF (Operator.hpp:277-291), in the un-interleaved GetJac /
+ * GetDerivFactors layouts above.  n is the number of doubles; the data are copied
+ * to device memory (host pointers) or device-to-device (_device variants). */
+int imf_set_jac(imf_op op, const double *jac, size_t n);
+int imf_set_df(imf_op op, const double *df, size_t n);
+int imf_set_jac_device(imf_op op, const double *d_jac, size_t n);
+int imf_set_df_device(imf_op op, const double *d_df, size_t n);
+
+/* Helmholtz::SetLambda (Operator.hpp:193-196). */
+int imf_set_lambda(imf_op op, double lambda);
+
+/* operator()(in, out) (Operator.hpp:70-72, 99-101, 176-178) on host arrays: copies in,
+ * applies, copies out; synchronous.  out is overwritten.  FP64 operators only. */
+int imf_apply(imf_op op, const double *in, double *out);
+
+/* operator()(in, out) on device arrays of the operator's precision (double for
+ * IMF_FP64, float for IMF_FP32), asynchronous on `stream`.  Allocates nothing. */
+int imf_apply_device(imf_op op, const void *d_in, void *d_out, void *stream);
+
+/* Transposed (adjoint) apply under the Euclidean inner product, device arrays, no
+ * reference counterpart (used by the adjoint/autograd layer):
+ *   BwdTrans^T:  f (nElmt*Q) -> B^T f            (IProduct with unit weights, J = 1)
+ *   IProduct^T:  g (nElmt*M) -> diag(w J) B g    (BwdTrans followed by w*J scaling)
+ *   Helmholtz^T: identical to Helmholtz (the operator is symmetric). */
+int imf_apply_transpose_device(imf_op op, const void *d_in, void *d_out, void *stream);
+
+/* Message of the last failing call on this thread ("" if none). */
+const char *imf_last_error(void);
+
+/* ---- set-up helpers (no reference FFI counterpart; host-side restatements) ---- */
+
+/* LibUtilities::Basis::GenBasis for eModified_A on nq Gauss-Lobatto-Legendre points
+ * (Foundations/Basis.cpp:392-421, Polylib.cpp zwglj/Dglj).  Outputs z[nq], w[nq],
+ * D[nq*nq], bdata[nm*nq], dbdata[nm*nq]. */
+int imf_basis_modified_a(int nm, int nq, double *z, double *w, double *D,
+                         double *bdata, double *dbdata);
+
+/* GeomFactors::ComputeDerivFactors (SpatialDomains/GeomFactors.cpp:399-478) for
+ * straight-sided (trilinear hex / bilinear quad) elements, evaluated on the GPU.
+ * d_verts: nElmt x 8 x 3 (hex) or nElmt x 4 x 2 (quad) in geometric vertex order;
+ * z: host GLL points (nq).  Writes d_jac / d_df in the layouts above. */
+int imf_geom_hex_device(const double *d_verts, int nElmt, int nq, const double *z,
+                        int deformed, double *d_jac, double *d_df, void *stream);
+int imf_geom_quad_device(const double *d_verts, int nElmt, int nq, const double *z,
+                         int deformed, double *d_jac, double *d_df, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

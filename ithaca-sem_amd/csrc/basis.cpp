@@ -1,0 +1,131 @@
+// Host-side 1D basis tables: Gauss-Lobatto-Legendre points/weights, the collocation
+// derivative matrix and the Modified_A (C0 hierarchical) basis.
+//
+// Restates LibUtilities/Polylib/Polylib.cpp (jacobfd, Jacobz, zwglj :195-232,
+// Dglj :690-730) and LibUtilities/Foundations/Basis.cpp:392-421 (eModified_A) for
+// alpha = beta = 0, which is what the operators need at set-up time.
+#include <cmath>
+#include <vector>
+
+#include "../../include/ithaca_mf.h"
+
+namespace {
+
+void jacobfd(int np, const double *z, double *poly, double *polyd, int n, double a,
+             double b)
+{
+    if (!np) return;
+    if (n == 0) {
+        for (int i = 0; i < np; ++i) {
+            if (poly) poly[i] = 1.0;
+            if (polyd) polyd[i] = 0.0;
+        }
+        return;
+    }
+    if (n == 1) {
+        for (int i = 0; i < np; ++i) {
+            if (poly) poly[i] = 0.5 * (a - b + (a + b + 2.0) * z[i]);
+            if (polyd) polyd[i] = 0.5 * (a + b + 2.0);
+        }
+        return;
+    }
+    const double apb = a + b;
+    std::vector<double> p1(np), p2(np), pv(np);
+    double *P = poly ? poly : pv.data();
+    for (int i = 0; i < np; ++i) {
+        p2[i] = 1.0;
+        p1[i] = 0.5 * (a - b + (a + b + 2.0) * z[i]);
+    }
+    for (int k = 2; k <= n; ++k) {
+        double a1 = 2.0 * k * (k + apb) * (2.0 * k + apb - 2.0);
+        double a2 = (2.0 * k + apb - 1.0) * (a * a - b * b) / a1;
+        double a3 = (2.0 * k + apb - 2.0) * (2.0 * k + apb - 1.0) * (2.0 * k + apb) / a1;
+        double a4 = 2.0 * (k + a - 1.0) * (k + b - 1.0) * (2.0 * k + apb) / a1;
+        for (int i = 0; i < np; ++i) {
+            P[i] = (a2 + a3 * z[i]) * p1[i] - a4 * p2[i];
+            p2[i] = p1[i];
+            p1[i] = P[i];
+        }
+    }
+    if (polyd) {
+        const double c4 = 2.0 * n + a + b;
+        const double c1 = n * (a - b) / c4, c2 = n * (2.0 * n + a + b) / c4,
+                     c3 = 2.0 * (n + a) * (n + b) / c4;
+        for (int i = 0; i < np; ++i)
+            polyd[i] = ((c1 - c2 * z[i]) * P[i] + c3 * p2[i]) / (1.0 - z[i] * z[i]);
+    }
+}
+
+// Newton iteration with deflation from Chebyshev guesses (Polylib Jacobz).
+void jacobz(int n, double *z, double a, double b)
+{
+    const double eps = 100 * 2.220446049250313e-16;
+    const double dth = M_PI / (2.0 * n);
+    double rlast = 0.0;
+    for (int k = 0; k < n; ++k) {
+        double r = -std::cos((2.0 * k + 1.0) * dth);
+        if (k) r = 0.5 * (r + rlast);
+        for (int it = 1; it < 30; ++it) {
+            double p, dp, sum = 0.0;
+            jacobfd(1, &r, &p, &dp, n, a, b);
+            for (int i = 0; i < k; ++i) sum += 1.0 / (r - z[i]);
+            const double delr = -p / (dp - sum * p);
+            r += delr;
+            if (std::fabs(delr) < eps) break;
+        }
+        z[k] = r;
+        rlast = r;
+    }
+}
+
+}  // namespace
+
+extern "C" int imf_basis_modified_a(int nm, int nq, double *z, double *w, double *D,
+                                    double *bdata, double *dbdata)
+{
+    if (nm < 2 || nq < 2 || nq > 64 || !z || !w || !D || !bdata || !dbdata) return IMF_ERR_ARG;
+    // GLL zeros / weights
+    z[0] = -1.0;
+    z[nq - 1] = 1.0;
+    if (nq > 2) jacobz(nq - 2, z + 1, 1.0, 1.0);
+    jacobfd(nq, z, w, nullptr, nq - 1, 0.0, 0.0);
+    const double fac = 2.0 / (double(nq) * double(nq - 1));
+    for (int i = 0; i < nq; ++i) w[i] = fac / (w[i] * w[i]);
+    // derivative matrix, flat D[a*nq + b] = l_a'(z_b)
+    std::vector<double> pd(nq);
+    pd[0] = 2.0 * std::pow(-1.0, nq) * double(nq - 1);
+    if (nq > 2) {
+        jacobfd(nq - 2, z + 1, pd.data() + 1, nullptr, nq - 3, 2.0, 2.0);
+        for (int i = 1; i < nq - 1; ++i) pd[i] *= 0.5 * double(nq + 1) * (1.0 - z[i] * z[i]);
+    }
+    pd[nq - 1] = -2.0 * double(nq - 1);
+    for (int i = 0; i < nq; ++i)
+        for (int j = 0; j < nq; ++j) {
+            if (i != j)
+                D[i * nq + j] = pd[j] / (pd[i] * (z[j] - z[i]));
+            else if (j == 0)
+                D[i * nq + j] = -double(nq - 1) * double(nq) / 4.0;
+            else if (j == nq - 1)
+                D[i * nq + j] = double(nq - 1) * double(nq) / 4.0;
+            else
+                D[i * nq + j] = 0.0;
+        }
+    // Modified_A modes: vertices (1 -+ z)/2, interior (1-z)(1+z)/4 P_{p-2}^{(1,1)}(z)
+    for (int i = 0; i < nq; ++i) {
+        bdata[i] = 0.5 * (1 - z[i]);
+        bdata[nq + i] = 0.5 * (1 + z[i]);
+    }
+    for (int p = 2; p < nm; ++p) {
+        double *mode = bdata + p * nq;
+        jacobfd(nq, z, mode, nullptr, p - 2, 1.0, 1.0);
+        for (int i = 0; i < nq; ++i) mode[i] *= bdata[i] * bdata[nq + i];
+    }
+    // dbdata = D * bdata with D(i,k) = flat[k*nq + i]
+    for (int p = 0; p < nm; ++p)
+        for (int i = 0; i < nq; ++i) {
+            double s = 0.0;
+            for (int k = 0; k < nq; ++k) s += D[k * nq + i] * bdata[p * nq + k];
+            dbdata[p * nq + i] = s;
+        }
+    return IMF_OK;
+}

@@ -1,0 +1,324 @@
+// C ABI of the matrix-free operators (declared in include/ithaca_mf.h).
+//
+// An imf_op owns device copies of the caller's geometry (the un-interleaved GetJac /
+// GetDerivFactors arrays, Collections/CoalescedGeomData.cpp:53-112, :251-313) and the
+// packed streamed form derived from them, plus the 1D basis tables as a kernel-argument
+// image.  Applies on device arrays allocate nothing.
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/ithaca_mf.h"
+#include "imf_common.hpp"
+#include "mf_geom.hpp"
+#include "mf_launch.hpp"
+
+struct imf_op_s {
+    int kind = 0, shape = 0, deformed = 0, prec = 0, dim = 0, nm = 0, nq = 0;
+    int nElmt = 0;
+    size_t M = 0, Q = 0;
+    std::vector<double> w;                 // direction-0 weights (host)
+    std::vector<unsigned char> tab;        // Tab<NM,NQ,T> image
+    double lambda = 1.0;                   // Helmholtz default m_lambda(1.0)
+    double *d_jac = nullptr, *d_df = nullptr;
+    size_t n_jac = 0, n_df = 0;
+    void *d_geom = nullptr;
+    bool geom_ready = false;
+    void *d_in = nullptr, *d_out = nullptr;  // staging for host applies
+    imf::LaunchFn fwd = nullptr, tr = nullptr;
+};
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg)
+{
+    g_err = msg;
+    return code;
+}
+
+int hip_fail(hipError_t e, const char *what)
+{
+    return fail(IMF_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+size_t elt_size(const imf_op op) { return op->prec == IMF_FP64 ? sizeof(double) : sizeof(float); }
+
+size_t jac_len(const imf_op op) { return (size_t)op->nElmt * (op->deformed ? op->Q : 1); }
+size_t df_len(const imf_op op) { return (size_t)op->dim * op->dim * jac_len(op); }
+
+size_t geom_len(const imf_op op)
+{
+    const int nc = imf::geo_comps(op->kind, op->shape);
+    return (size_t)nc * jac_len(op);
+}
+
+bool needs_jac(const imf_op op) { return op->kind != imf::kBwdTrans; }
+bool needs_df(const imf_op op) { return op->kind == imf::kHelmholtz; }
+
+// Re-pack the streamed geometry once everything it depends on has been set.
+int maybe_pack(imf_op op)
+{
+    op->geom_ready = false;
+    if (!needs_jac(op) || !op->d_jac) return IMF_OK;
+    if (needs_df(op) && !op->d_df) return IMF_OK;
+    if (!op->d_geom) {
+        hipError_t e = hipMalloc(&op->d_geom, geom_len(op) * elt_size(op));
+        if (e != hipSuccess) return hip_fail(e, "hipMalloc(geometry)");
+    }
+    hipError_t e = imf::pack_geometry(op->kind, op->shape, op->deformed, op->prec, op->nElmt,
+                                      op->nq, op->w.data(), op->d_jac, op->d_df, op->d_geom,
+                                      nullptr);
+    if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
+    if (e != hipSuccess) return hip_fail(e, "pack_geometry");
+    op->geom_ready = true;
+    return IMF_OK;
+}
+
+int set_array(imf_op op, double **dst, size_t *ndst, const double *src, size_t n, size_t want,
+              const char *name, hipMemcpyKind kind)
+{
+    if (!op || !src) return fail(IMF_ERR_ARG, std::string(name) + ": null argument");
+    if (n != want)
+        return fail(IMF_ERR_ARG, std::string(name) + ": expected " + std::to_string(want) +
+                                     " values, got " + std::to_string(n));
+    if (!*dst) {
+        hipError_t e = hipMalloc((void **)dst, want * sizeof(double));
+        if (e != hipSuccess) return hip_fail(e, "hipMalloc(geometry input)");
+        *ndst = want;
+    }
+    hipError_t e = hipMemcpy(*dst, src, want * sizeof(double), kind);
+    if (e != hipSuccess) return hip_fail(e, "hipMemcpy(geometry input)");
+    return maybe_pack(op);
+}
+
+template <typename T>
+void fill_tab(imf_op op, const double *B, const double *D, const double *w)
+{
+    const int nm = op->nm, nq = op->nq;
+    std::vector<T> img;
+    img.reserve(nm * nq + nq * nq + nq);
+    for (int i = 0; i < nm * nq; ++i) img.push_back(T(B[i]));
+    for (int i = 0; i < nq * nq; ++i) img.push_back(T(D[i]));
+    for (int i = 0; i < nq; ++i) img.push_back(T(w[i]));
+    op->tab.resize(img.size() * sizeof(T));
+    std::memcpy(op->tab.data(), img.data(), op->tab.size());
+}
+
+int check_op(const imf_op op)
+{
+    if (!op) return fail(IMF_ERR_ARG, "null operator handle");
+    return IMF_OK;
+}
+
+int run(imf_op op, imf::LaunchFn fn, const void *in, void *out, void *stream)
+{
+    if (needs_jac(op) && !op->geom_ready)
+        return fail(IMF_ERR_STATE, needs_df(op) ? "Jacobian and derivative factors not set"
+                                                : "Jacobian not set");
+    if (!in || !out) return fail(IMF_ERR_ARG, "null input/output array");
+    imf::LaunchArgs a{op->tab.data(), op->nElmt, op->lambda, op->d_geom, in, out,
+                      (hipStream_t)stream};
+    hipError_t e = fn(a);
+    if (e != hipSuccess) return hip_fail(e, "kernel launch");
+    return IMF_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *imf_last_error(void) { return g_err.c_str(); }
+
+int imf_create(const char *key, int dim, const int *nm, const int *nq,
+               const double *const *bdata, const double *const *dbdata,
+               const double *const *D, const double *const *w, int nElmt, int precision,
+               imf_op *out)
+{
+    if (!key || !nm || !nq || !bdata || !D || !w || !out)
+        return fail(IMF_ERR_ARG, "imf_create: null argument");
+    (void)dbdata;  // D bdata is applied in factored form by the kernels
+    *out = nullptr;
+    const std::string k(key);
+    int kind, shape, deformed;
+    auto starts = [&](const char *s) { return k.rfind(s, 0) == 0; };
+    if (starts("BwdTrans_")) kind = imf::kBwdTrans;
+    else if (starts("IProduct_")) kind = imf::kIProduct;
+    else if (starts("Helmholtz_")) kind = imf::kHelmholtz;
+    else return fail(IMF_ERR_UNSUPPORTED, "No such operator key: " + k);
+    const std::string rest = k.substr(k.find('_') + 1);
+    if (rest == "Quad_Regular") { shape = imf::kQuad; deformed = 0; }
+    else if (rest == "Quad_Deformed") { shape = imf::kQuad; deformed = 1; }
+    else if (rest == "Hex_Regular") { shape = imf::kHex; deformed = 0; }
+    else if (rest == "Hex_Deformed") { shape = imf::kHex; deformed = 1; }
+    else return fail(IMF_ERR_UNSUPPORTED, "No such operator key: " + k);
+    // The reference registers BwdTrans only as _Regular (MatrixFreeOps/BwdTrans.cpp:7-33).
+    if (kind == imf::kBwdTrans && deformed)
+        return fail(IMF_ERR_UNSUPPORTED, "No such operator key: " + k);
+    if (dim != shape)
+        return fail(IMF_ERR_ARG, "imf_create: dim does not match the shape of " + k);
+    if (precision != IMF_FP64 && precision != IMF_FP32)
+        return fail(IMF_ERR_ARG, "imf_create: unknown precision");
+    if (nElmt <= 0) return fail(IMF_ERR_ARG, "imf_create: nElmt must be positive");
+    for (int d = 1; d < dim; ++d) {
+        if (nm[d] != nm[0] || nq[d] != nq[0])
+            return fail(IMF_ERR_UNSUPPORTED, "MatrixFree requires homogenous modes/points");
+        const size_t nb = (size_t)nm[0] * nq[0];
+        if (std::memcmp(bdata[d], bdata[0], nb * sizeof(double)) ||
+            std::memcmp(D[d], D[0], (size_t)nq[0] * nq[0] * sizeof(double)) ||
+            std::memcmp(w[d], w[0], (size_t)nq[0] * sizeof(double)))
+            return fail(IMF_ERR_UNSUPPORTED, "MatrixFree requires identical bases in all directions");
+    }
+    const int m = nm[0], q = nq[0];
+    if (m < 2 || m > 8 || q < m || q > 2 * m)
+        return fail(IMF_ERR_UNSUPPORTED, std::string(kind == imf::kHelmholtz ? "Helmholtz" :
+                                                     kind == imf::kIProduct ? "IProduct" : "BwdTrans") +
+                                             (shape == imf::kHex ? "Hex" : "Quad") +
+                                             ": # of modes / points combo not initialised.");
+    imf_op op = new imf_op_s();
+    op->kind = kind; op->shape = shape; op->deformed = deformed; op->prec = precision;
+    op->dim = dim; op->nm = m; op->nq = q; op->nElmt = nElmt;
+    op->M = 1; op->Q = 1;
+    for (int d = 0; d < dim; ++d) { op->M *= m; op->Q *= q; }
+    op->w.assign(w[0], w[0] + q);
+    if (precision == IMF_FP64) fill_tab<double>(op, bdata[0], D[0], w[0]);
+    else fill_tab<float>(op, bdata[0], D[0], w[0]);
+    const int geo = kind == imf::kBwdTrans ? imf::kUnit : (deformed ? imf::kDeformed : imf::kRegular);
+    // BwdTrans^T uses no geometry; IProduct^T uses the IProduct geometry.
+    const int geo_t = kind == imf::kBwdTrans ? imf::kUnit : geo;
+    if (shape == imf::kHex) {
+        op->fwd = imf::lookup_hex(m, kind, imf::kForward, geo, q, precision);
+        op->tr = imf::lookup_hex(m, kind, imf::kTranspose, geo_t, q, precision);
+    } else {
+        op->fwd = imf::lookup_quad(m, kind, imf::kForward, geo, q, precision);
+        op->tr = imf::lookup_quad(m, kind, imf::kTranspose, geo_t, q, precision);
+    }
+    if (!op->fwd) {
+        delete op;
+        return fail(IMF_ERR_UNSUPPORTED, k + ": # of modes / points combo not initialised.");
+    }
+    *out = op;
+    return IMF_OK;
+}
+
+int imf_destroy(imf_op op)
+{
+    if (!op) return IMF_OK;
+    (void)hipFree(op->d_jac);
+    (void)hipFree(op->d_df);
+    (void)hipFree(op->d_geom);
+    (void)hipFree(op->d_in);
+    (void)hipFree(op->d_out);
+    delete op;
+    return IMF_OK;
+}
+
+double imf_ndof(imf_op op) { return op ? double(op->M) * op->nElmt : 0.0; }
+int imf_needs_jac(imf_op op) { return op && needs_jac(op) ? 1 : 0; }
+int imf_needs_df(imf_op op) { return op && needs_df(op) ? 1 : 0; }
+
+size_t imf_in_size(imf_op op)
+{
+    if (!op) return 0;
+    return (size_t)op->nElmt * (op->kind == imf::kIProduct ? op->Q : op->M);
+}
+
+size_t imf_out_size(imf_op op)
+{
+    if (!op) return 0;
+    return (size_t)op->nElmt * (op->kind == imf::kBwdTrans ? op->Q : op->M);
+}
+
+int imf_set_jac(imf_op op, const double *jac, size_t n)
+{
+    if (int s = check_op(op)) return s;
+    return set_array(op, &op->d_jac, &op->n_jac, jac, n, jac_len(op), "SetJac",
+                     hipMemcpyHostToDevice);
+}
+
+int imf_set_df(imf_op op, const double *df, size_t n)
+{
+    if (int s = check_op(op)) return s;
+    return set_array(op, &op->d_df, &op->n_df, df, n, df_len(op), "SetDF",
+                     hipMemcpyHostToDevice);
+}
+
+int imf_set_jac_device(imf_op op, const double *d_jac, size_t n)
+{
+    if (int s = check_op(op)) return s;
+    return set_array(op, &op->d_jac, &op->n_jac, d_jac, n, jac_len(op), "SetJac",
+                     hipMemcpyDeviceToDevice);
+}
+
+int imf_set_df_device(imf_op op, const double *d_df, size_t n)
+{
+    if (int s = check_op(op)) return s;
+    return set_array(op, &op->d_df, &op->n_df, d_df, n, df_len(op), "SetDF",
+                     hipMemcpyDeviceToDevice);
+}
+
+int imf_set_lambda(imf_op op, double lambda)
+{
+    if (int s = check_op(op)) return s;
+    op->lambda = lambda;
+    return IMF_OK;
+}
+
+int imf_apply_device(imf_op op, const void *d_in, void *d_out, void *stream)
+{
+    if (int s = check_op(op)) return s;
+    return run(op, op->fwd, d_in, d_out, stream);
+}
+
+int imf_apply_transpose_device(imf_op op, const void *d_in, void *d_out, void *stream)
+{
+    if (int s = check_op(op)) return s;
+    if (!op->tr) return fail(IMF_ERR_UNSUPPORTED, "transpose not available");
+    if (op->kind == imf::kHelmholtz) return run(op, op->fwd, d_in, d_out, stream);
+    return run(op, op->tr, d_in, d_out, stream);
+}
+
+int imf_apply(imf_op op, const double *in, double *out)
+{
+    if (int s = check_op(op)) return s;
+    if (op->prec != IMF_FP64) return fail(IMF_ERR_STATE, "imf_apply: host applies are FP64 only");
+    if (!in || !out) return fail(IMF_ERR_ARG, "null input/output array");
+    const size_t nin = imf_in_size(op), nout = imf_out_size(op);
+    hipError_t e;
+    if (!op->d_in) {
+        if ((e = hipMalloc(&op->d_in, nin * sizeof(double))) != hipSuccess)
+            return hip_fail(e, "hipMalloc(staging)");
+        if ((e = hipMalloc(&op->d_out, nout * sizeof(double))) != hipSuccess)
+            return hip_fail(e, "hipMalloc(staging)");
+    }
+    if ((e = hipMemcpy(op->d_in, in, nin * sizeof(double), hipMemcpyHostToDevice)) != hipSuccess)
+        return hip_fail(e, "hipMemcpy(in)");
+    if (int s = run(op, op->fwd, op->d_in, op->d_out, nullptr)) return s;
+    if ((e = hipMemcpy(out, op->d_out, nout * sizeof(double), hipMemcpyDeviceToHost)) != hipSuccess)
+        return hip_fail(e, "hipMemcpy(out)");
+    return IMF_OK;
+}
+
+int imf_geom_hex_device(const double *d_verts, int nElmt, int nq, const double *z, int deformed,
+                        double *d_jac, double *d_df, void *stream)
+{
+    if (!d_verts || !z || !d_jac || !d_df || nElmt <= 0 || nq < 2 || nq > 16)
+        return fail(IMF_ERR_ARG, "imf_geom_hex_device: bad argument");
+    hipError_t e = imf::geom_from_vertices(imf::kHex, d_verts, nElmt, nq, z, deformed, d_jac,
+                                           d_df, (hipStream_t)stream);
+    return e == hipSuccess ? IMF_OK : hip_fail(e, "geom_hex");
+}
+
+int imf_geom_quad_device(const double *d_verts, int nElmt, int nq, const double *z, int deformed,
+                         double *d_jac, double *d_df, void *stream)
+{
+    if (!d_verts || !z || !d_jac || !d_df || nElmt <= 0 || nq < 2 || nq > 16)
+        return fail(IMF_ERR_ARG, "imf_geom_quad_device: bad argument");
+    hipError_t e = imf::geom_from_vertices(imf::kQuad, d_verts, nElmt, nq, z, deformed, d_jac,
+                                           d_df, (hipStream_t)stream);
+    return e == hipSuccess ? IMF_OK : hip_fail(e, "geom_quad");
+}
+
+}  // extern "C"

@@ -1,0 +1,253 @@
+// Geometry kernels: packing of (jac, df) into the streamed per-point form, and the
+// geometric factors of straight-sided elements computed on the GPU.
+#include "mf_geom.hpp"
+
+#include <cmath>
+
+namespace imf {
+
+struct Pts {
+    double v[16];
+};
+
+// Metric G^{ab} = sum_c df[c*dim+a] df[c*dim+b], accumulated in the reference order
+// (Helmholtz.h:858-966 for hexes, :172-190 for quads).
+__device__ __forceinline__ void metric3(const double *f, double *m)
+{
+    m[0] = f[0] * f[0]; m[0] += f[3] * f[3]; m[0] += f[6] * f[6];
+    m[1] = f[0] * f[1]; m[1] += f[3] * f[4]; m[1] += f[6] * f[7];
+    m[2] = f[0] * f[2]; m[2] += f[3] * f[5]; m[2] += f[6] * f[8];
+    m[3] = f[1] * f[1]; m[3] += f[4] * f[4]; m[3] += f[7] * f[7];
+    m[4] = f[1] * f[2]; m[4] += f[4] * f[5]; m[4] += f[7] * f[8];
+    m[5] = f[2] * f[2]; m[5] += f[5] * f[5]; m[5] += f[8] * f[8];
+}
+
+__device__ __forceinline__ void metric2(const double *f, double *m)
+{
+    m[0] = f[0] * f[0]; m[0] += f[2] * f[2];
+    m[1] = f[0] * f[1]; m[1] += f[2] * f[3];
+    m[2] = f[1] * f[1]; m[2] += f[3] * f[3];
+}
+
+// Helmholtz, deformed: out[e][c][q] = Jw, Jw*G (6 or 3 unique entries), Jw = J w_i w_j (w_k).
+template <typename T, int DIM>
+__global__ void pack_helm_deformed(long nPts, int Q, int nq, Pts w, const double *__restrict__ jac,
+                                   const double *__restrict__ df, T *__restrict__ out)
+{
+    constexpr int NC = DIM == 3 ? 7 : 4, NDF = DIM * DIM;
+    const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= nPts) return;
+    const long e = g / Q;
+    const int q = (int)(g - e * Q);
+    const int i = q % nq, j = (q / nq) % nq, k = q / (nq * nq);
+    double wq = w.v[i] * w.v[j];
+    if (DIM == 3) wq *= w.v[k];
+    const double jw = jac[g] * wq;
+    double f[NDF], m[6];
+#pragma unroll
+    for (int d = 0; d < NDF; ++d) f[d] = df[(long)d * nPts + g];
+    if (DIM == 3) metric3(f, m); else metric2(f, m);
+    T *o = out + e * NC * Q + q;
+    o[0] = T(jw);
+#pragma unroll
+    for (int c = 1; c < NC; ++c) o[(long)c * Q] = T(jw * m[c - 1]);
+}
+
+// Helmholtz, regular: out[e][c] = J, J*G (weights applied by the operator kernel).
+template <typename T, int DIM>
+__global__ void pack_helm_regular(long nElmt, const double *__restrict__ jac,
+                                  const double *__restrict__ df, T *__restrict__ out)
+{
+    constexpr int NC = DIM == 3 ? 7 : 4, NDF = DIM * DIM;
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nElmt) return;
+    double f[NDF], m[6];
+#pragma unroll
+    for (int d = 0; d < NDF; ++d) f[d] = df[(long)d * nElmt + e];
+    if (DIM == 3) metric3(f, m); else metric2(f, m);
+    const double J = jac[e];
+    out[e * NC] = T(J);
+#pragma unroll
+    for (int c = 1; c < NC; ++c) out[e * NC + c] = T(J * m[c - 1]);
+}
+
+// IProduct: deformed out[e][q] = J w_i w_j (w_k); regular out[e] = J.
+template <typename T, int DIM>
+__global__ void pack_jw(long n, int Q, int nq, int deformed, Pts w,
+                        const double *__restrict__ jac, T *__restrict__ out)
+{
+    const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n) return;
+    if (!deformed) {
+        out[g] = T(jac[g]);
+        return;
+    }
+    const int q = (int)(g % Q);
+    const int i = q % nq, j = (q / nq) % nq, k = q / (nq * nq);
+    double wq = w.v[i] * w.v[j];
+    if (DIM == 3) wq *= w.v[k];
+    out[g] = T(jac[g] * wq);
+}
+
+hipError_t pack_geometry(int kind, int shape, int deformed, int prec, long nElmt, int nq,
+                         const double *w, const double *d_jac, const double *d_df,
+                         void *d_out, hipStream_t s)
+{
+    Pts pw{};
+    for (int i = 0; i < nq && i < 16; ++i) pw.v[i] = w[i];
+    const int Q = shape == kHex ? nq * nq * nq : nq * nq;
+    const int bs = 256;
+    if (kind == kHelmholtz) {
+        if (deformed) {
+            const long n = nElmt * Q;
+            const dim3 g((unsigned)((n + bs - 1) / bs));
+            if (shape == kHex) {
+                if (prec == 0) pack_helm_deformed<double, 3><<<g, bs, 0, s>>>(n, Q, nq, pw, d_jac, d_df, (double *)d_out);
+                else pack_helm_deformed<float, 3><<<g, bs, 0, s>>>(n, Q, nq, pw, d_jac, d_df, (float *)d_out);
+            } else {
+                if (prec == 0) pack_helm_deformed<double, 2><<<g, bs, 0, s>>>(n, Q, nq, pw, d_jac, d_df, (double *)d_out);
+                else pack_helm_deformed<float, 2><<<g, bs, 0, s>>>(n, Q, nq, pw, d_jac, d_df, (float *)d_out);
+            }
+        } else {
+            const dim3 g((unsigned)((nElmt + bs - 1) / bs));
+            if (shape == kHex) {
+                if (prec == 0) pack_helm_regular<double, 3><<<g, bs, 0, s>>>(nElmt, d_jac, d_df, (double *)d_out);
+                else pack_helm_regular<float, 3><<<g, bs, 0, s>>>(nElmt, d_jac, d_df, (float *)d_out);
+            } else {
+                if (prec == 0) pack_helm_regular<double, 2><<<g, bs, 0, s>>>(nElmt, d_jac, d_df, (double *)d_out);
+                else pack_helm_regular<float, 2><<<g, bs, 0, s>>>(nElmt, d_jac, d_df, (float *)d_out);
+            }
+        }
+    } else {
+        const long n = deformed ? nElmt * Q : nElmt;
+        const dim3 g((unsigned)((n + bs - 1) / bs));
+        if (shape == kHex) {
+            if (prec == 0) pack_jw<double, 3><<<g, bs, 0, s>>>(n, Q, nq, deformed, pw, d_jac, (double *)d_out);
+            else pack_jw<float, 3><<<g, bs, 0, s>>>(n, Q, nq, deformed, pw, d_jac, (float *)d_out);
+        } else {
+            if (prec == 0) pack_jw<double, 2><<<g, bs, 0, s>>>(n, Q, nq, deformed, pw, d_jac, (double *)d_out);
+            else pack_jw<float, 2><<<g, bs, 0, s>>>(n, Q, nq, deformed, pw, d_jac, (float *)d_out);
+        }
+    }
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Geometric factors of straight-sided elements (GeomFactors.cpp:399-478).
+// Map derivatives deriv[i][k] = dx_k/dxi_i of the trilinear / bilinear vertex map
+// (vertex order of StdHexExp.cpp:1042-1082 / StdQuadExp), then g = deriv deriv^T,
+// jac = sqrt(det g), df[k*dim+j] = sum_i deriv[i][k] (g^{-1})[i][j].
+// ---------------------------------------------------------------------------
+
+__device__ void factors_from_deriv(int dim, const double *deriv, double *jac, double *fac)
+{
+    double g[9], gi[9];
+    for (int i = 0, l = 0; i < dim; ++i)
+        for (int j = 0; j < dim; ++j, ++l) {
+            double s = 0.0;
+            for (int k = 0; k < dim; ++k) s += deriv[i * dim + k] * deriv[j * dim + k];
+            g[l] = s;
+        }
+    double det = 0.0;
+    if (dim == 3) {
+        gi[0] = g[4] * g[8] - g[5] * g[7];
+        gi[1] = g[2] * g[7] - g[1] * g[8];
+        gi[2] = g[1] * g[5] - g[2] * g[4];
+        gi[3] = g[5] * g[6] - g[3] * g[8];
+        gi[4] = g[0] * g[8] - g[2] * g[6];
+        gi[5] = g[2] * g[3] - g[0] * g[5];
+        gi[6] = g[3] * g[7] - g[4] * g[6];
+        gi[7] = g[1] * g[6] - g[0] * g[7];
+        gi[8] = g[0] * g[4] - g[1] * g[3];
+        for (int i = 0; i < 3; ++i) det += g[i] * gi[i * 3];
+    } else {
+        gi[0] = g[3]; gi[1] = -g[1]; gi[2] = -g[2]; gi[3] = g[0];
+        for (int i = 0; i < 2; ++i) det += g[i] * gi[i * 2];
+    }
+    for (int i = 0; i < dim * dim; ++i) gi[i] /= det;
+    *jac = sqrt(det);
+    for (int k = 0, l = 0; k < dim; ++k)
+        for (int j = 0; j < dim; ++j, ++l) {
+            double s = 0.0;
+            for (int i = 0; i < dim; ++i) s += deriv[i * dim + k] * gi[dim * i + j];
+            fac[l] = s;
+        }
+}
+
+__global__ void geom_hex_kernel(const double *__restrict__ verts, long nElmt, int nq, Pts z,
+                                int deformed, double *__restrict__ jac, double *__restrict__ df)
+{
+    const int Q = deformed ? nq * nq * nq : 1;
+    const long n = nElmt * Q;
+    const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n) return;
+    const long e = g / Q;
+    const int q = (int)(g - e * Q);
+    const int i = q % nq, j = (q / nq) % nq, k = q / (nq * nq);
+    const double xi[3] = {z.v[i], z.v[j], z.v[k]};
+    const double *v = verts + e * 24;
+    double deriv[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int nv = 0; nv < 8; ++nv) {
+        const int c[3] = {(nv % 4) % 3 > 0, (nv % 4) > 1, nv > 3};
+        double f[3], d[3];
+        for (int a = 0; a < 3; ++a) {
+            f[a] = c[a] ? 0.5 * (1 + xi[a]) : 0.5 * (1 - xi[a]);
+            d[a] = c[a] ? 0.5 : -0.5;
+        }
+        const double dN[3] = {d[0] * f[1] * f[2], f[0] * d[1] * f[2], f[0] * f[1] * d[2]};
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b) deriv[a * 3 + b] += dN[a] * v[nv * 3 + b];
+    }
+    double J, fac[9];
+    factors_from_deriv(3, deriv, &J, fac);
+    jac[g] = J;
+    for (int d = 0; d < 9; ++d) df[(long)d * n + g] = fac[d];
+}
+
+__global__ void geom_quad_kernel(const double *__restrict__ verts, long nElmt, int nq, Pts z,
+                                 int deformed, double *__restrict__ jac, double *__restrict__ df)
+{
+    const int Q = deformed ? nq * nq : 1;
+    const long n = nElmt * Q;
+    const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n) return;
+    const long e = g / Q;
+    const int q = (int)(g - e * Q);
+    const double xi[2] = {z.v[q % nq], z.v[q / nq]};
+    const double *v = verts + e * 8;
+    double deriv[4] = {0, 0, 0, 0};
+    for (int nv = 0; nv < 4; ++nv) {
+        const int c[2] = {nv % 3 > 0, nv > 1};
+        double f[2], d[2];
+        for (int a = 0; a < 2; ++a) {
+            f[a] = c[a] ? 0.5 * (1 + xi[a]) : 0.5 * (1 - xi[a]);
+            d[a] = c[a] ? 0.5 : -0.5;
+        }
+        const double dN[2] = {d[0] * f[1], f[0] * d[1]};
+        for (int a = 0; a < 2; ++a)
+            for (int b = 0; b < 2; ++b) deriv[a * 2 + b] += dN[a] * v[nv * 2 + b];
+    }
+    double J, fac[4];
+    factors_from_deriv(2, deriv, &J, fac);
+    jac[g] = J;
+    for (int d = 0; d < 4; ++d) df[(long)d * n + g] = fac[d];
+}
+
+hipError_t geom_from_vertices(int shape, const double *d_verts, long nElmt, int nq,
+                              const double *z, int deformed, double *d_jac, double *d_df,
+                              hipStream_t s)
+{
+    Pts pz{};
+    for (int i = 0; i < nq && i < 16; ++i) pz.v[i] = z[i];
+    const long Q = deformed ? (shape == kHex ? (long)nq * nq * nq : (long)nq * nq) : 1;
+    const long n = nElmt * Q;
+    const int bs = 256;
+    const dim3 g((unsigned)((n + bs - 1) / bs));
+    if (shape == kHex)
+        geom_hex_kernel<<<g, bs, 0, s>>>(d_verts, nElmt, nq, pz, deformed, d_jac, d_df);
+    else
+        geom_quad_kernel<<<g, bs, 0, s>>>(d_verts, nElmt, nq, pz, deformed, d_jac, d_df);
+    return hipGetLastError();
+}
+
+}  // namespace imf

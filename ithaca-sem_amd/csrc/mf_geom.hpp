@@ -1,0 +1,19 @@
+// Geometry preparation entry points (implemented in mf_geom.hip).
+#pragma once
+
+#include "imf_common.hpp"
+
+namespace imf {
+
+// Pack (jac, df) in the un-interleaved CoalescedGeomData layouts into the per-point
+// (deformed) or per-element (regular) streamed form the operator kernels read.
+hipError_t pack_geometry(int kind, int shape, int deformed, int prec, long nElmt, int nq,
+                         const double *w, const double *d_jac, const double *d_df,
+                         void *d_out, hipStream_t s);
+
+// Geometric factors of straight-sided hexes (shape = kHex) or quads (kQuad).
+hipError_t geom_from_vertices(int shape, const double *d_verts, long nElmt, int nq,
+                              const double *z, int deformed, double *d_jac, double *d_df,
+                              hipStream_t s);
+
+}  // namespace imf

@@ -1,0 +1,390 @@
+// Hexahedral sum-factorised operators for gfx950.
+//
+// Work decomposition ("pencil rotation").  One element is owned by NQ*NQ threads.
+// Every 1D contraction is done in registers on a pencil (a line of the element along
+// the contracted direction) held by one thread; the data are re-oriented between
+// contractions through LDS (one write + one read per value, odd strides, conflict
+// free), instead of each output reading all NQ inputs from LDS.  The last pencil
+// orientation of the BwdTrans is the k-column of thread (i,j), which is also the
+// orientation in which the geometry stream is read coalesced ([e][comp][k][j][i]).
+//
+// Reference semantics (all paths relative to the reference root):
+//   BwdTrans   BwdTransKernels.hpp:302-372      u = (B x B x B) u_hat
+//   IProduct   IProductKernels.hpp:236-311      out = (B x B x B)^T diag(w J) f
+//   PhysDeriv  PhysDerivKernels.hpp:219-294     d_a = D_a u
+//   Helmholtz  Helmholtz.h:764-996              out = lambda M u_hat + L u_hat
+// The Helmholtz kernel applies dbdata = D bdata (Basis.cpp:418) in factored form:
+// sum_i dB[p,i] g_i = sum_l B[p,l] (D^T g)_l, so the three IProducts of the reference
+// collapse into one D^T application per direction followed by one (B x B x B)^T.
+#pragma once
+
+#include "imf_common.hpp"
+
+namespace imf {
+
+template <typename T>
+__device__ __forceinline__ T ld_stream(const T *p)
+{
+    return __builtin_nontemporal_load(p);
+}
+
+template <typename T>
+__device__ __forceinline__ void st_stream(T *p, T v)
+{
+    __builtin_nontemporal_store(v, p);
+}
+
+// ---------------------------------------------------------------------------
+// Building blocks.  `t` is the thread's index inside its element (0 <= t < NQ*NQ).
+// buffers are NQ^3 scalars each.
+// ---------------------------------------------------------------------------
+
+// u_hat (global, p-pencils of thread (q,r)) -> u (registers, k-column of thread (i,j)).
+// Uses buf0 and buf1.  Ends with u in registers of every active thread.
+template <int NM, int NQ, typename T>
+__device__ __forceinline__ void hex_bwd_to_columns(const Tab<NM, NQ, T> &tab,
+                                                   const T *__restrict__ src, bool valid,
+                                                   bool active, int t, T *buf0, T *buf1,
+                                                   T (&u)[NQ])
+{
+    // p -> i on p-pencils (thread (q, r))
+    if (active && t < NM * NM) {
+        T uh[NM];
+        if (valid) {
+#pragma unroll
+            for (int p = 0; p < NM; ++p) uh[p] = ld_stream(src + t * NM + p);
+        } else {
+#pragma unroll
+            for (int p = 0; p < NM; ++p) uh[p] = T(0);
+        }
+#pragma unroll
+        for (int i = 0; i < NQ; ++i) {
+            T s = uh[0] * tab.B[i];
+#pragma unroll
+            for (int p = 1; p < NM; ++p) s = fma(uh[p], tab.B[p * NQ + i], s);
+            buf0[t * NQ + i] = s;  // s1[r][q][i], t = r*NM + q
+        }
+    }
+    __syncthreads();
+    // q -> j on q-pencils (thread (i, r))
+    if (active && t < NQ * NM) {
+        const int i = t % NQ, r = t / NQ;
+        T s1[NM];
+#pragma unroll
+        for (int q = 0; q < NM; ++q) s1[q] = buf0[(r * NM + q) * NQ + i];
+#pragma unroll
+        for (int j = 0; j < NQ; ++j) {
+            T s = s1[0] * tab.B[j];
+#pragma unroll
+            for (int q = 1; q < NM; ++q) s = fma(s1[q], tab.B[q * NQ + j], s);
+            buf1[(r * NQ + j) * NQ + i] = s;  // s2[r][j][i]
+        }
+    }
+    __syncthreads();
+    // r -> k on r-pencils (thread (i, j)): the element's k-columns
+    if (active) {
+        T s2[NM];
+#pragma unroll
+        for (int r = 0; r < NM; ++r) s2[r] = buf1[r * NQ * NQ + t];
+#pragma unroll
+        for (int k = 0; k < NQ; ++k) {
+            T s = s2[0] * tab.B[k];
+#pragma unroll
+            for (int r = 1; r < NM; ++r) s = fma(s2[r], tab.B[r * NQ + k], s);
+            u[k] = s;
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < NQ; ++k) u[k] = T(0);
+    }
+}
+
+// k-columns acc (registers of thread (i,j)) -> out = (B x B x B)^T acc (global).
+// Uses buf0 and buf1; the caller must have synchronised any earlier readers.
+template <int NM, int NQ, typename T>
+__device__ __forceinline__ void hex_columns_to_modes(const Tab<NM, NQ, T> &tab,
+                                                     const T (&acc)[NQ], T *__restrict__ dst,
+                                                     bool valid, bool active, int t, T *buf0,
+                                                     T *buf1)
+{
+    // k -> r in registers
+    if (active) {
+#pragma unroll
+        for (int r = 0; r < NM; ++r) {
+            T s = acc[0] * tab.B[r * NQ];
+#pragma unroll
+            for (int k = 1; k < NQ; ++k) s = fma(acc[k], tab.B[r * NQ + k], s);
+            buf0[r * NQ * NQ + t] = s;  // z2[r][j][i]
+        }
+    }
+    __syncthreads();
+    // j -> q on j-pencils (thread (i, r))
+    if (active && t < NQ * NM) {
+        const int i = t % NQ, r = t / NQ;
+        T v[NQ];
+#pragma unroll
+        for (int j = 0; j < NQ; ++j) v[j] = buf0[(r * NQ + j) * NQ + i];
+#pragma unroll
+        for (int q = 0; q < NM; ++q) {
+            T s = v[0] * tab.B[q * NQ];
+#pragma unroll
+            for (int j = 1; j < NQ; ++j) s = fma(v[j], tab.B[q * NQ + j], s);
+            buf1[(r * NM + q) * NQ + i] = s;  // z1[r][q][i]
+        }
+    }
+    __syncthreads();
+    // i -> p on i-pencils (thread (q, r))
+    if (active && t < NM * NM) {
+        T v[NQ];
+#pragma unroll
+        for (int i = 0; i < NQ; ++i) v[i] = buf1[t * NQ + i];
+        if (valid) {
+#pragma unroll
+            for (int p = 0; p < NM; ++p) {
+                T s = v[0] * tab.B[p * NQ];
+#pragma unroll
+                for (int i = 1; i < NQ; ++i) s = fma(v[i], tab.B[p * NQ + i], s);
+                st_stream(dst + t * NM + p, s);
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Kernels
+// ---------------------------------------------------------------------------
+
+// BwdTrans (DIR = kForward, GEO ignored) and the Euclidean transpose of IProduct
+// (DIR = kTranspose: out = diag(w J) B g, GEO = kDeformed or kRegular).
+template <int NM, int NQ, typename T, int DIR, int GEO>
+__global__ __launch_bounds__(HexCfg<NQ>::BLOCK) void hex_bwdtrans_kernel(
+    const Tab<NM, NQ, T> tab, int nElmt, const T *__restrict__ in,
+    const T *__restrict__ geom, T *__restrict__ out)
+{
+    using C = HexCfg<NQ>;
+    constexpr int M = NM * NM * NM, Q = NQ * NQ * NQ;
+    __shared__ T lds[C::EPB][2][Q];
+    const int eb = threadIdx.x / C::TPE;
+    const int t = threadIdx.x - eb * C::TPE;
+    const bool active = eb < C::EPB;
+    const int es = active ? eb : 0;
+    const long e = (long)blockIdx.x * C::EPB + eb;
+    const bool valid = active && e < nElmt;
+
+    T u[NQ];
+    hex_bwd_to_columns<NM, NQ, T>(tab, in + e * M, valid, active, t, lds[es][0], lds[es][1],
+                                  u);
+    if (!valid) return;
+    if (DIR == kTranspose) {
+        if (GEO == kDeformed) {
+#pragma unroll
+            for (int k = 0; k < NQ; ++k) u[k] *= ld_stream(geom + e * Q + k * C::TPE + t);
+        } else {
+            const T wij = tab.w[t % NQ] * tab.w[t / NQ] * geom[e];
+#pragma unroll
+            for (int k = 0; k < NQ; ++k) u[k] *= wij * tab.w[k];
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < NQ; ++k) st_stream(out + e * Q + k * C::TPE + t, u[k]);
+}
+
+// IProductWRTBase: out = (B x B x B)^T diag(w J) f.  GEO = kUnit gives the Euclidean
+// transpose of BwdTrans (w J == 1).
+template <int NM, int NQ, typename T, int GEO>
+__global__ __launch_bounds__(HexCfg<NQ>::BLOCK) void hex_iproduct_kernel(
+    const Tab<NM, NQ, T> tab, int nElmt, const T *__restrict__ in,
+    const T *__restrict__ geom, T *__restrict__ out)
+{
+    using C = HexCfg<NQ>;
+    constexpr int M = NM * NM * NM, Q = NQ * NQ * NQ;
+    __shared__ T lds[C::EPB][2][Q];
+    const int eb = threadIdx.x / C::TPE;
+    const int t = threadIdx.x - eb * C::TPE;
+    const bool active = eb < C::EPB;
+    const int es = active ? eb : 0;
+    const long e = (long)blockIdx.x * C::EPB + eb;
+    const bool valid = active && e < nElmt;
+
+    T acc[NQ];
+    if (valid) {
+        if (GEO == kDeformed) {
+#pragma unroll
+            for (int k = 0; k < NQ; ++k) {
+                const int q = k * C::TPE + t;
+                acc[k] = ld_stream(in + e * Q + q) * ld_stream(geom + e * Q + q);
+            }
+        } else if (GEO == kRegular) {
+            const T wij = tab.w[t % NQ] * tab.w[t / NQ] * geom[e];
+#pragma unroll
+            for (int k = 0; k < NQ; ++k)
+                acc[k] = ld_stream(in + e * Q + k * C::TPE + t) * (wij * tab.w[k]);
+        } else {
+#pragma unroll
+            for (int k = 0; k < NQ; ++k) acc[k] = ld_stream(in + e * Q + k * C::TPE + t);
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < NQ; ++k) acc[k] = T(0);
+    }
+    hex_columns_to_modes<NM, NQ, T>(tab, acc, out + e * M, valid, active, t, lds[es][0],
+                                    lds[es][1]);
+}
+
+// Helmholtz: out = lambda M u_hat + sum_ab (D_a B)^T diag(w J G^{ab}) (D_b B) u_hat.
+// Packed geometry: deformed [e][7][Q] (Jw, JwG00, JwG01, JwG02, JwG11, JwG12, JwG22);
+// regular [e][7] (J, J G00, ...), weights applied in the kernel.
+template <int NM, int NQ, typename T, int GEO>
+__global__ __launch_bounds__(HexCfg<NQ>::BLOCK) void hex_helmholtz_kernel(
+    const Tab<NM, NQ, T> tab, int nElmt, T lambda, const T *__restrict__ in,
+    const T *__restrict__ geom, T *__restrict__ out)
+{
+    using C = HexCfg<NQ>;
+    constexpr int M = NM * NM * NM, Q = NQ * NQ * NQ, TPE = C::TPE;
+    __shared__ T lds[C::EPB][4][Q];
+    const int eb = threadIdx.x / TPE;
+    const int t = threadIdx.x - eb * TPE;
+    const bool active = eb < C::EPB;
+    const int es = active ? eb : 0;
+    const long e = (long)blockIdx.x * C::EPB + eb;
+    const bool valid = active && e < nElmt;
+    T *buf0 = lds[es][0], *buf1 = lds[es][1], *buf2 = lds[es][2], *buf3 = lds[es][3];
+    const int ci = t % NQ, cj = t / NQ;  // this thread's column (i, j)
+
+    // 1. BwdTrans into k-columns; keep a copy of u in LDS for the cross derivatives.
+    T u[NQ];
+    hex_bwd_to_columns<NM, NQ, T>(tab, in + e * M, valid, active, t, buf0, buf1, u);
+    if (active) {
+#pragma unroll
+        for (int k = 0; k < NQ; ++k) buf2[k * TPE + t] = u[k];
+    }
+    __syncthreads();
+
+    // 2. Reference-space derivatives (PhysDerivTensor3DKernel).  d0 on i-pencils of
+    // thread (j,k) = (t%NQ, t/NQ), d1 on j-pencils of thread (i,k) = (t%NQ, t/NQ),
+    // d2 on this thread's k-column.
+    if (active) {
+        T v[NQ];
+        {
+            const int pj = t % NQ, pk = t / NQ;
+#pragma unroll
+            for (int l = 0; l < NQ; ++l) v[l] = buf2[(pk * NQ + pj) * NQ + l];
+#pragma unroll
+            for (int i = 0; i < NQ; ++i) {
+                T s = v[0] * tab.D[i];
+#pragma unroll
+                for (int l = 1; l < NQ; ++l) s = fma(v[l], tab.D[l * NQ + i], s);
+                buf0[(pk * NQ + pj) * NQ + i] = s;
+            }
+        }
+        {
+            const int pi = t % NQ, pk = t / NQ;
+#pragma unroll
+            for (int l = 0; l < NQ; ++l) v[l] = buf2[(pk * NQ + l) * NQ + pi];
+#pragma unroll
+            for (int j = 0; j < NQ; ++j) {
+                T s = v[0] * tab.D[j];
+#pragma unroll
+                for (int l = 1; l < NQ; ++l) s = fma(v[l], tab.D[l * NQ + j], s);
+                buf1[(pk * NQ + j) * NQ + pi] = s;
+            }
+        }
+    }
+    __syncthreads();
+
+    // 3. Pointwise metric contraction and mass term on this thread's column.
+    T g0[NQ], g1[NQ], g2[NQ], acc[NQ];
+    if (valid) {
+#pragma unroll
+        for (int k = 0; k < NQ; ++k) {
+            T d2 = u[0] * tab.D[k];
+#pragma unroll
+            for (int l = 1; l < NQ; ++l) d2 = fma(u[l], tab.D[l * NQ + k], d2);
+            const T d0 = buf0[k * TPE + t];
+            const T d1 = buf1[k * TPE + t];
+            T jw, m00, m01, m02, m11, m12, m22;
+            if (GEO == kDeformed) {
+                const T *g = geom + e * (7 * Q) + k * TPE + t;
+                jw = ld_stream(g);
+                m00 = ld_stream(g + Q);
+                m01 = ld_stream(g + 2 * Q);
+                m02 = ld_stream(g + 3 * Q);
+                m11 = ld_stream(g + 4 * Q);
+                m12 = ld_stream(g + 5 * Q);
+                m22 = ld_stream(g + 6 * Q);
+            } else {
+                const T wq = tab.w[ci] * tab.w[cj] * tab.w[k];
+                const T *g = geom + e * 7;
+                jw = g[0] * wq;
+                m00 = g[1] * wq;
+                m01 = g[2] * wq;
+                m02 = g[3] * wq;
+                m11 = g[4] * wq;
+                m12 = g[5] * wq;
+                m22 = g[6] * wq;
+            }
+            g0[k] = fma(m02, d2, fma(m01, d1, m00 * d0));
+            g1[k] = fma(m12, d2, fma(m11, d1, m01 * d0));
+            g2[k] = fma(m22, d2, fma(m12, d1, m02 * d0));
+            acc[k] = lambda * jw * u[k];
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < NQ; ++k) g0[k] = g1[k] = g2[k] = acc[k] = T(0);
+    }
+    if (active) {
+#pragma unroll
+        for (int k = 0; k < NQ; ++k) {
+            buf2[k * TPE + t] = g0[k];
+            buf3[k * TPE + t] = g1[k];
+        }
+    }
+    // D^T along k in registers.
+#pragma unroll
+    for (int k = 0; k < NQ; ++k) {
+        T s = acc[k];
+#pragma unroll
+        for (int l = 0; l < NQ; ++l) s = fma(g2[l], tab.D[k * NQ + l], s);
+        acc[k] = s;
+    }
+    __syncthreads();
+
+    // 4. D^T along i (pencils of thread (j,k)) and along j (pencils of thread (i,k)).
+    if (active) {
+        T v[NQ];
+        {
+            const int pj = t % NQ, pk = t / NQ;
+#pragma unroll
+            for (int l = 0; l < NQ; ++l) v[l] = buf2[(pk * NQ + pj) * NQ + l];
+#pragma unroll
+            for (int i = 0; i < NQ; ++i) {
+                T s = v[0] * tab.D[i * NQ];
+#pragma unroll
+                for (int l = 1; l < NQ; ++l) s = fma(v[l], tab.D[i * NQ + l], s);
+                buf0[(pk * NQ + pj) * NQ + i] = s;
+            }
+        }
+        {
+            const int pi = t % NQ, pk = t / NQ;
+#pragma unroll
+            for (int l = 0; l < NQ; ++l) v[l] = buf3[(pk * NQ + l) * NQ + pi];
+#pragma unroll
+            for (int j = 0; j < NQ; ++j) {
+                T s = v[0] * tab.D[j * NQ];
+#pragma unroll
+                for (int l = 1; l < NQ; ++l) s = fma(v[l], tab.D[j * NQ + l], s);
+                buf1[(pk * NQ + j) * NQ + pi] = s;
+            }
+        }
+    }
+    __syncthreads();
+    if (active) {
+#pragma unroll
+        for (int k = 0; k < NQ; ++k) acc[k] += buf0[k * TPE + t] + buf1[k * TPE + t];
+    }
+
+    // 5. (B x B x B)^T of the combined quadrature-space field.
+    hex_columns_to_modes<NM, NQ, T>(tab, acc, out + e * M, valid, active, t, buf2, buf3);
+}
+
+}  // namespace imf

@@ -1,0 +1,202 @@
+"""Python binding of the gfx950 matrix-free operators (C ABI in include/ithaca_mf.h).
+
+The operators mirror the reference MatrixFree layer (MatrixFreeOps/Operator.hpp):
+a string key such as ``"Helmholtz_Hex_Deformed"`` selects the operator, ``set_jac`` /
+``set_df`` / ``set_lambda`` configure it, and ``apply(in, out)`` is ``operator()``.
+Device arrays are torch tensors on ``cuda:0`` (torch is plumbing for device memory and
+streams only).  There is no CPU fallback: a missing library raises immediately.
+"""
+import ctypes
+import os
+
+import numpy as np
+import torch  # noqa: F401  -- loads the HIP runtime before the operator library
+
+_PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.abspath(os.path.join(_PKG, "..", ".."))
+LIB_PATH = os.path.join(ROOT, "lib", "libithaca_mf.so")
+
+FP64, FP32 = 0, 1
+_LIB = None
+_dp = ctypes.POINTER(ctypes.c_double)
+
+
+class MFError(RuntimeError):
+    """A non-zero status returned by the C ABI (the reference throws NekError)."""
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(
+                f"{LIB_PATH} is missing: build it with `make -C ithaca-sem_amd` "
+                "(there is no CPU fallback)")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, sz, i, d = ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int, ctypes.c_double
+        L.imf_create.argtypes = [ctypes.c_char_p, i, ctypes.POINTER(i), ctypes.POINTER(i),
+                                 ctypes.POINTER(_dp), ctypes.POINTER(_dp), ctypes.POINTER(_dp),
+                                 ctypes.POINTER(_dp), i, i, ctypes.POINTER(vp)]
+        L.imf_destroy.argtypes = [vp]
+        L.imf_ndof.argtypes = [vp]
+        L.imf_ndof.restype = d
+        L.imf_needs_jac.argtypes = [vp]
+        L.imf_needs_df.argtypes = [vp]
+        L.imf_in_size.argtypes = [vp]
+        L.imf_in_size.restype = sz
+        L.imf_out_size.argtypes = [vp]
+        L.imf_out_size.restype = sz
+        for n in ("imf_set_jac", "imf_set_df", "imf_set_jac_device", "imf_set_df_device"):
+            getattr(L, n).argtypes = [vp, vp, sz]
+        L.imf_set_lambda.argtypes = [vp, d]
+        L.imf_apply.argtypes = [vp, vp, vp]
+        L.imf_apply_device.argtypes = [vp, vp, vp, vp]
+        L.imf_apply_transpose_device.argtypes = [vp, vp, vp, vp]
+        L.imf_last_error.restype = ctypes.c_char_p
+        L.imf_basis_modified_a.argtypes = [i, i, vp, vp, vp, vp, vp]
+        L.imf_geom_hex_device.argtypes = [vp, i, i, vp, i, vp, vp, vp]
+        L.imf_geom_quad_device.argtypes = [vp, i, i, vp, i, vp, vp, vp]
+        _LIB = L
+    return _LIB
+
+
+def _check(status):
+    if status != 0:
+        raise MFError(lib().imf_last_error().decode())
+
+
+def _np_ptr(a):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _stream_ptr(stream):
+    if stream is None:
+        stream = torch.cuda.current_stream()
+    return ctypes.c_void_p(stream.cuda_stream)
+
+
+def basis_modified_a(nm, nq):
+    """Modified_A basis on nq GLL points (Basis.cpp:392-421), computed by the library."""
+    z, w = np.zeros(nq), np.zeros(nq)
+    D, b, db = np.zeros(nq * nq), np.zeros(nm * nq), np.zeros(nm * nq)
+    _check(lib().imf_basis_modified_a(nm, nq, _np_ptr(z), _np_ptr(w), _np_ptr(D), _np_ptr(b),
+                                      _np_ptr(db)))
+    return dict(z=z, w=w, D=D, bdata=b, dbdata=db, nm=nm, nq=nq)
+
+
+def geom_device(shape, verts, nq, z, deformed=True, stream=None):
+    """Geometric factors of straight-sided elements on the GPU (GeomFactors.cpp:399-478).
+
+    verts: cuda float64 tensor (nElmt, 8, 3) for hexes or (nElmt, 4, 2) for quads.
+    Returns (jac, df) cuda tensors in the GetJac / GetDerivFactors layouts."""
+    verts = verts.contiguous()
+    E = verts.shape[0]
+    dim = 3 if shape == "hex" else 2
+    npts = E * (nq ** dim if deformed else 1)
+    jac = torch.empty(npts, dtype=torch.float64, device=verts.device)
+    df = torch.empty(dim * dim * npts, dtype=torch.float64, device=verts.device)
+    zz = np.ascontiguousarray(z, dtype=np.float64)
+    fn = lib().imf_geom_hex_device if dim == 3 else lib().imf_geom_quad_device
+    _check(fn(ctypes.c_void_p(verts.data_ptr()), E, nq, _np_ptr(zz), int(deformed),
+              ctypes.c_void_p(jac.data_ptr()), ctypes.c_void_p(df.data_ptr()),
+              _stream_ptr(stream)))
+    return jac, df
+
+
+class Operator:
+    """One MatrixFree operator instance: ``GetOperatorFactory().CreateInstance(key, basis,
+    nElmt)`` of the reference, implemented on the GPU."""
+
+    def __init__(self, key, nm, nq, nElmt, precision=FP64, basis=None):
+        self.key = key
+        shape = key.split("_")[1]
+        self.dim = 3 if shape == "Hex" else 2
+        self.nm, self.nq, self.nElmt, self.precision = nm, nq, nElmt, precision
+        self.basis = basis if basis is not None else basis_modified_a(nm, nq)
+        b = self.basis
+        self._keep = [np.ascontiguousarray(b[k], dtype=np.float64)
+                      for k in ("bdata", "dbdata", "D", "w")]
+        arr = lambda a: (_dp * self.dim)(*([a.ctypes.data_as(_dp)] * self.dim))
+        nm_a = (ctypes.c_int * self.dim)(*([nm] * self.dim))
+        nq_a = (ctypes.c_int * self.dim)(*([nq] * self.dim))
+        h = ctypes.c_void_p()
+        _check(lib().imf_create(key.encode(), self.dim, nm_a, nq_a, arr(self._keep[0]),
+                                arr(self._keep[1]), arr(self._keep[2]), arr(self._keep[3]),
+                                nElmt, precision, ctypes.byref(h)))
+        self._h = h
+        self.dtype = torch.float64 if precision == FP64 else torch.float32
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value and _LIB is not None:
+            _LIB.imf_destroy(h)
+            self._h = None
+
+    # -- reference Operator interface -------------------------------------------------
+    def ndof(self):
+        return lib().imf_ndof(self._h)
+
+    def needs_jac(self):
+        return bool(lib().imf_needs_jac(self._h))
+
+    def needs_df(self):
+        return bool(lib().imf_needs_df(self._h))
+
+    @property
+    def in_size(self):
+        return lib().imf_in_size(self._h)
+
+    @property
+    def out_size(self):
+        return lib().imf_out_size(self._h)
+
+    def _set(self, name, a):
+        if isinstance(a, torch.Tensor) and a.is_cuda:
+            a = a.contiguous()
+            assert a.dtype == torch.float64
+            _check(getattr(lib(), f"imf_set_{name}_device")(self._h, ctypes.c_void_p(a.data_ptr()),
+                                                           a.numel()))
+        else:
+            a = np.ascontiguousarray(a, dtype=np.float64)
+            _check(getattr(lib(), f"imf_set_{name}")(self._h, _np_ptr(a), a.size))
+
+    def set_jac(self, jac):
+        self._set("jac", jac)
+
+    def set_df(self, df):
+        self._set("df", df)
+
+    def set_lambda(self, lam):
+        _check(lib().imf_set_lambda(self._h, float(lam)))
+
+    def apply(self, inp, out=None, stream=None):
+        """operator()(in, out) on cuda tensors; asynchronous on the stream."""
+        assert inp.is_cuda and inp.dtype == self.dtype and inp.is_contiguous()
+        assert inp.numel() == self.in_size
+        if out is None:
+            out = torch.empty(self.out_size, dtype=self.dtype, device=inp.device)
+        assert out.is_cuda and out.dtype == self.dtype and out.numel() == self.out_size
+        _check(lib().imf_apply_device(self._h, ctypes.c_void_p(inp.data_ptr()),
+                                      ctypes.c_void_p(out.data_ptr()), _stream_ptr(stream)))
+        return out
+
+    def apply_transpose(self, inp, out=None, stream=None):
+        """Euclidean transpose of the operator (adjoint), cuda tensors."""
+        assert inp.is_cuda and inp.dtype == self.dtype and inp.is_contiguous()
+        assert inp.numel() == self.out_size
+        if out is None:
+            out = torch.empty(self.in_size, dtype=self.dtype, device=inp.device)
+        assert out.numel() == self.in_size
+        _check(lib().imf_apply_transpose_device(self._h, ctypes.c_void_p(inp.data_ptr()),
+                                                ctypes.c_void_p(out.data_ptr()),
+                                                _stream_ptr(stream)))
+        return out
+
+    def apply_host(self, inp):
+        """operator()(in, out) on host numpy arrays (synchronous, FP64)."""
+        inp = np.ascontiguousarray(inp, dtype=np.float64)
+        assert inp.size == self.in_size
+        out = np.empty(self.out_size)
+        _check(lib().imf_apply(self._h, _np_ptr(inp), _np_ptr(out)))
+        return out

@@ -1,0 +1,73 @@
+/*
+ * CPU oracle for the matrix-free spectral/hp operator path (TEST INFRASTRUCTURE).
+ *
+ * This is a plain-C restatement of the reference CPU algorithms, kept only as the
+ * checker for the HIP product path.  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load it; the product never links or calls it.
+ *
+ * Layout conventions (boundary layout of the reference Collections layer):
+ *   coefficients  u[e*M + (r*nm + q)*nm + p]            (p fastest)
+ *   quad points   f[e*Q + (k*nq + j)*nq + i]            (i fastest)
+ *   jac           deformed: jac[e*Q + q]   regular: jac[e]
+ *   df            deformed: df[d*(E*Q) + e*Q + q]  regular: df[d*E + e]
+ *                 with d = c*dim + a holding d(xi_a)/d(x_c)
+ * These follow Collections/CoalescedGeomData.cpp:53-112 (GetJac) and :251-313
+ * (GetDerivFactors).
+ */
+#ifndef ITHACA_MF_ORACLE_H
+#define ITHACA_MF_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- Polylib (LibUtilities/Polylib/Polylib.cpp) ---- */
+void or_jacobfd(int np, const double *z, double *poly, double *polyd, int n,
+                double alpha, double beta);
+void or_zwglj(double *z, double *w, int np);
+void or_Dglj(double *D, const double *z, int np);
+
+/* ---- Basis (LibUtilities/Foundations/Basis.cpp:392-421, eModified_A on GLL) ---- */
+void or_basis_modified_a(int nm, int nq, double *z, double *w, double *D,
+                         double *bdata, double *dbdata);
+
+/* ---- Geometry (SpatialDomains/GeomFactors.cpp:399-478) ---- */
+/* verts: E x 8 x 3 (hex) / E x 4 x 2 (quad), vertex order of HexGeom/QuadGeom.
+ * deformed != 0 -> per-point factors, else the value at the first point. */
+void or_geom_hex(const double *verts, int E, int nq, const double *z, int deformed,
+                 double *jac, double *df);
+void or_geom_quad(const double *verts, int E, int nq, const double *z, int deformed,
+                  double *jac, double *df);
+/* physical coordinates of the quadrature points: xyz[c*(E*Q) + e*Q + q] */
+void or_coords_hex(const double *verts, int E, int nq, const double *z, double *xyz);
+void or_coords_quad(const double *verts, int E, int nq, const double *z, double *xy);
+
+/* ---- MatrixFree operators (width-1 element interleave, reference loop order) ---- */
+void or_set_threads(int nthreads);
+
+void or_bwdtrans_hex(int nm, int nq, const double *bdata, int E,
+                     const double *in, double *out);
+void or_iproduct_hex(int nm, int nq, const double *bdata, const double *w, int E,
+                     int deformed, const double *jac, const double *in, double *out);
+void or_physderiv_hex(int nq, const double *D, int E, int deformed, const double *df,
+                      const double *in, double *out0, double *out1, double *out2);
+void or_helmholtz_hex(int nm, int nq, const double *bdata, const double *dbdata,
+                      const double *D, const double *w, int E, int deformed,
+                      const double *jac, const double *df, double lambda,
+                      const double *in, double *out);
+
+void or_bwdtrans_quad(int nm, int nq, const double *bdata, int E,
+                      const double *in, double *out);
+void or_iproduct_quad(int nm, int nq, const double *bdata, const double *w, int E,
+                      int deformed, const double *jac, const double *in, double *out);
+void or_physderiv_quad(int nq, const double *D, int E, int deformed, const double *df,
+                       const double *in, double *out0, double *out1);
+void or_helmholtz_quad(int nm, int nq, const double *bdata, const double *dbdata,
+                       const double *D, const double *w, int E, int deformed,
+                       const double *jac, const double *df, double lambda,
+                       const double *in, double *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,96 @@
+"""Independent dense references used to pin the oracle (pure numpy/scipy).
+
+Nothing here shares code with the oracle or the product: GLL nodes come from numpy's
+Legendre module, the Modified_A basis and its derivative from closed forms with scipy's
+Jacobi polynomials, and the physical gradient from the inverse of the map Jacobian
+(not from the adjugate/derivative-factor formulae).  Element operators are assembled as
+explicit matrices (Kronecker products), so a dropped term, a transposed operand or a
+wrong index in the sum-factorised code shows up as a mismatch.
+"""
+import numpy as np
+from numpy.polynomial import legendre as L
+from scipy.special import eval_jacobi
+
+
+def gll(nq):
+    n = nq - 1
+    c = np.zeros(nq); c[-1] = 1.0           # P_n
+    inner = np.sort(L.legroots(L.legder(c)))  # roots of P_n'
+    z = np.concatenate([[-1.0], inner, [1.0]])
+    w = 2.0 / (n * (n + 1) * L.legval(z, c) ** 2)
+    return z, w
+
+
+def modified_a(nm, z):
+    """B[i, p] = phi_p(z_i) and dB[i, p] = phi_p'(z_i) in closed form."""
+    B = np.zeros((len(z), nm)); dB = np.zeros_like(B)
+    B[:, 0] = 0.5 * (1 - z); dB[:, 0] = -0.5
+    B[:, 1] = 0.5 * (1 + z); dB[:, 1] = 0.5
+    for p in range(2, nm):
+        n = p - 2
+        P = eval_jacobi(n, 1, 1, z)
+        dP = 0.5 * (n + 3) * eval_jacobi(n - 1, 2, 2, z) if n > 0 else np.zeros_like(z)
+        B[:, p] = 0.25 * (1 - z * z) * P
+        dB[:, p] = -0.5 * z * P + 0.25 * (1 - z * z) * dP
+    return B, dB
+
+
+def hex_map(verts, xi):
+    """x(xi) and A[c, a] = dx_c/dxi_a for a trilinear hex (verts 8x3)."""
+    corners = np.array([[0, 0, 0], [1, 0, 0], [1, 1, 0], [0, 1, 0],
+                        [0, 0, 1], [1, 0, 1], [1, 1, 1], [0, 1, 1]])
+    x = np.zeros(3); A = np.zeros((3, 3))
+    for v, c in enumerate(corners):
+        f = np.where(c, 0.5 * (1 + xi), 0.5 * (1 - xi))
+        d = np.where(c, 0.5, -0.5)
+        x += np.prod(f) * verts[v]
+        grad = np.array([d[0] * f[1] * f[2], f[0] * d[1] * f[2], f[0] * f[1] * d[2]])
+        A += np.outer(verts[v], grad)
+    return x, A
+
+
+def quad_map(verts, xi):
+    corners = np.array([[0, 0], [1, 0], [1, 1], [0, 1]])
+    x = np.zeros(2); A = np.zeros((2, 2))
+    for v, c in enumerate(corners):
+        f = np.where(c, 0.5 * (1 + xi), 0.5 * (1 - xi))
+        d = np.where(c, 0.5, -0.5)
+        x += np.prod(f) * verts[v]
+        A += np.outer(verts[v], np.array([d[0] * f[1], f[0] * d[1]]))
+    return x, A
+
+
+def element_mats(shape, nm, nq, verts):
+    """Dense per-element matrices: B (Q x M), grads G_a (Q x M) in reference space,
+    quadrature weights W (Q), |det A| (Q), A^{-1} (Q x dim x dim), coordinates."""
+    z, w = gll(nq)
+    B1, dB1 = modified_a(nm, z)
+    if shape == "hex":
+        B = np.kron(np.kron(B1, B1), B1)
+        G = [np.kron(np.kron(B1, B1), dB1), np.kron(np.kron(B1, dB1), B1),
+             np.kron(np.kron(dB1, B1), B1)]
+        W = np.kron(np.kron(w, w), w)
+        pts = [(z[i], z[j], z[k]) for k in range(nq) for j in range(nq) for i in range(nq)]
+        mp = hex_map
+    else:
+        B = np.kron(B1, B1)
+        G = [np.kron(B1, dB1), np.kron(dB1, B1)]
+        W = np.kron(w, w)
+        pts = [(z[i], z[j]) for j in range(nq) for i in range(nq)]
+        mp = quad_map
+    dets, Ainv, X = [], [], []
+    for p in pts:
+        x, A = mp(verts, np.array(p))
+        dets.append(abs(np.linalg.det(A))); Ainv.append(np.linalg.inv(A)); X.append(x)
+    return dict(B=B, G=G, W=W, J=np.array(dets), Ainv=np.array(Ainv), X=np.array(X), z=z, w=w)
+
+
+def helmholtz_matrix(m, lam):
+    """lambda B^T W J B + sum_c (d_c B)^T W J (d_c B), d_c = sum_a (A^{-1})[a, c] G_a."""
+    WJ = m["W"] * m["J"]
+    H = lam * m["B"].T @ (WJ[:, None] * m["B"])
+    dim = len(m["G"])
+    for c in range(dim):
+        dc = sum(m["Ainv"][:, a, c][:, None] * m["G"][a] for a in range(dim))
+        H += dc.T @ (WJ[:, None] * dc)
+    return H

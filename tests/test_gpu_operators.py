@@ -1,0 +1,216 @@
+"""GPU parity of the matrix-free operators against the CPU oracle, through the C ABI.
+
+Bar: FP64 outputs within 1e-12 relative L2 of the oracle (the reference CPU path);
+FP32 variants within 2e-6 relative L2 (single-precision storage and arithmetic).
+"""
+import numpy as np
+import pytest
+import torch
+
+import ithaca_hip as H
+import oracle as O
+from ithaca_hip import mesh
+
+pytestmark = pytest.mark.gpu
+
+TOL64 = 1e-12
+TOL32 = 2e-6
+
+HEX_COMBOS = [(2, 2), (2, 4), (3, 5), (4, 4), (4, 8), (5, 6), (6, 7), (6, 12), (7, 9),
+              (8, 8), (8, 16)]
+QUAD_COMBOS = [(2, 3), (3, 6), (5, 6), (6, 7), (7, 14), (8, 16)]
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def _case(shape, nm, nq, deformed, seed, n=3):
+    print(f"[case] {shape} nm={nm} nq={nq} deformed={deformed} seed={seed}", flush=True)
+    rng = np.random.default_rng(seed)
+    if shape == "hex":
+        verts = mesh.hex_box(n, 0.1 if deformed else 0.0, rng)
+        geom = O.geom_hex
+    else:
+        verts = mesh.quad_box(n + 2, 0.1 if deformed else 0.0, rng)
+        geom = O.geom_quad
+    b = O.basis(nm, nq)
+    jac, df = geom(verts, nq, b["z"], deformed)
+    return rng, verts, b, jac, df
+
+
+def _dev(a, dtype=torch.float64):
+    return torch.from_numpy(np.ascontiguousarray(a)).to("cuda", dtype=dtype)
+
+
+def _op(key, nm, nq, E, b, jac=None, df=None, prec=H.FP64):
+    op = H.Operator(key, nm, nq, E, precision=prec)
+    if jac is not None:
+        op.set_jac(jac)
+    if df is not None:
+        op.set_df(df)
+    return op
+
+
+@pytest.mark.parametrize("shape,nm,nq", [("hex", c[0], c[1]) for c in HEX_COMBOS] +
+                         [("quad", c[0], c[1]) for c in QUAD_COMBOS])
+def test_bwdtrans_parity(shape, nm, nq):
+    rng, verts, b, jac, df = _case(shape, nm, nq, False, 1)
+    E = verts.shape[0]
+    dim = 3 if shape == "hex" else 2
+    u = rng.uniform(-1, 1, E * nm ** dim)
+    ref = O.bwdtrans(shape, b, E, u)
+    op = _op(f"BwdTrans_{shape.capitalize()}_Regular", nm, nq, E, b)
+    got = op.apply(_dev(u)).cpu().numpy()
+    assert rel(got, ref) < TOL64
+    op32 = _op(f"BwdTrans_{shape.capitalize()}_Regular", nm, nq, E, b, prec=H.FP32)
+    got32 = op32.apply(_dev(u, torch.float32)).double().cpu().numpy()
+    assert rel(got32, ref) < TOL32
+
+
+@pytest.mark.parametrize("deformed", [True, False])
+@pytest.mark.parametrize("shape,nm,nq", [("hex", c[0], c[1]) for c in HEX_COMBOS] +
+                         [("quad", c[0], c[1]) for c in QUAD_COMBOS])
+def test_iproduct_parity(shape, nm, nq, deformed):
+    rng, verts, b, jac, df = _case(shape, nm, nq, deformed, 2)
+    E = verts.shape[0]
+    dim = 3 if shape == "hex" else 2
+    f = rng.uniform(-1, 1, E * nq ** dim)
+    ref = O.iproduct(shape, b, E, deformed, jac, f)
+    key = f"IProduct_{shape.capitalize()}_{'Deformed' if deformed else 'Regular'}"
+    got = _op(key, nm, nq, E, b, jac).apply(_dev(f)).cpu().numpy()
+    assert rel(got, ref) < TOL64
+    got32 = _op(key, nm, nq, E, b, jac, prec=H.FP32).apply(_dev(f, torch.float32))
+    assert rel(got32.double().cpu().numpy(), ref) < TOL32
+
+
+@pytest.mark.parametrize("deformed", [True, False])
+@pytest.mark.parametrize("shape,nm,nq", [("hex", c[0], c[1]) for c in HEX_COMBOS] +
+                         [("quad", c[0], c[1]) for c in QUAD_COMBOS])
+def test_helmholtz_parity(shape, nm, nq, deformed):
+    rng, verts, b, jac, df = _case(shape, nm, nq, deformed, 3)
+    E = verts.shape[0]
+    dim = 3 if shape == "hex" else 2
+    u = rng.uniform(-1, 1, E * nm ** dim)
+    lam = 2.5
+    ref = O.helmholtz(shape, b, E, deformed, jac, df, lam, u)
+    key = f"Helmholtz_{shape.capitalize()}_{'Deformed' if deformed else 'Regular'}"
+    op = _op(key, nm, nq, E, b, jac, df)
+    op.set_lambda(lam)
+    got = op.apply(_dev(u)).cpu().numpy()
+    assert rel(got, ref) < TOL64
+    op32 = _op(key, nm, nq, E, b, jac, df, prec=H.FP32)
+    op32.set_lambda(lam)
+    got32 = op32.apply(_dev(u, torch.float32)).double().cpu().numpy()
+    assert rel(got32, ref) < TOL32 * 10
+
+
+@pytest.mark.parametrize("shape", ["hex", "quad"])
+def test_geometry_kernel_parity(shape):
+    nq = 7
+    rng = np.random.default_rng(4)
+    verts = mesh.hex_box(4, 0.1, rng) if shape == "hex" else mesh.quad_box(6, 0.1, rng)
+    b = O.basis(6, nq)
+    for deformed in (True, False):
+        geom = O.geom_hex if shape == "hex" else O.geom_quad
+        jac, df = geom(verts, nq, b["z"], deformed)
+        gj, gd = H.geom_device(shape, _dev(verts), nq, b["z"], deformed)
+        torch.cuda.synchronize()
+        assert rel(gj.cpu().numpy(), jac) < 1e-14
+        assert rel(gd.cpu().numpy(), df) < 1e-14
+
+
+@pytest.mark.parametrize("key", ["BwdTrans_Hex_Regular", "IProduct_Hex_Deformed",
+                                 "Helmholtz_Hex_Deformed", "IProduct_Quad_Regular",
+                                 "BwdTrans_Quad_Regular", "Helmholtz_Quad_Deformed"])
+def test_transpose_is_adjoint(key):
+    shape = key.split("_")[1].lower()
+    deformed = key.endswith("Deformed")
+    nm, nq = (6, 7) if shape == "hex" else (5, 6)
+    rng, verts, b, jac, df = _case(shape, nm, nq, deformed, 9)
+    E = verts.shape[0]
+    op = _op(key, nm, nq, E, b, jac if not key.startswith("BwdTrans") else None,
+             df if key.startswith("Helmholtz") else None)
+    u = _dev(rng.uniform(-1, 1, op.in_size))
+    v = _dev(rng.uniform(-1, 1, op.out_size))
+    lhs = torch.dot(v, op.apply(u)).item()
+    rhs = torch.dot(op.apply_transpose(v), u).item()
+    assert abs(lhs - rhs) <= 1e-12 * max(abs(lhs), 1.0)
+
+
+def test_host_apply_matches_device_apply():
+    rng, verts, b, jac, df = _case("hex", 6, 7, True, 5)
+    E = verts.shape[0]
+    op = _op("Helmholtz_Hex_Deformed", 6, 7, E, b, jac, df)
+    op.set_lambda(10.0)
+    u = rng.uniform(-1, 1, E * 216)
+    dev = op.apply(_dev(u)).cpu().numpy()
+    host = op.apply_host(u)
+    assert np.array_equal(dev, host)
+    assert op.ndof() == E * 216
+    assert op.needs_jac() and op.needs_df()
+
+
+def test_error_paths():
+    b = O.basis(6, 7)
+    with pytest.raises(H.MFError, match="No such operator key"):
+        H.Operator("Helmholtz_Tet_Deformed", 6, 7, 4)
+    with pytest.raises(H.MFError, match="No such operator key"):
+        H.Operator("BwdTrans_Hex_Deformed", 6, 7, 4)
+    with pytest.raises(H.MFError, match="modes / points combo"):
+        H.Operator("Helmholtz_Hex_Deformed", 6, 13, 4)
+    op = H.Operator("Helmholtz_Hex_Deformed", 6, 7, 4)
+    with pytest.raises(H.MFError, match="not set"):
+        op.apply(torch.zeros(4 * 216, dtype=torch.float64, device="cuda"))
+    with pytest.raises(H.MFError, match="expected"):
+        op.set_jac(np.zeros(10))
+    del b
+
+
+def test_full_hex_workload_parity():
+    """48^3 jittered hexes, P=5, 7 GLL points, lambda=10: every element checked."""
+    w = mesh.workload_hex_helmholtz()
+    nm, nq, E = w["nm"], w["nq"], w["nElmt"]
+    print(f"[case] full hex workload E={E}", flush=True)
+    b = O.basis(nm, nq)
+    O.set_threads(16)
+    jac, df = O.geom_hex(w["verts"], nq, b["z"], True)
+    ref = O.helmholtz("hex", b, E, True, jac, df, w["lam"], w["u"])
+    gj, gd = H.geom_device("hex", _dev(w["verts"]), nq, b["z"], True)
+    op = H.Operator("Helmholtz_Hex_Deformed", nm, nq, E)
+    op.set_jac(gj)
+    op.set_df(gd)
+    op.set_lambda(w["lam"])
+    u = _dev(w["u"])
+    got = op.apply(u).cpu().numpy()
+    assert rel(got, ref) < TOL64
+    # linearity and symmetry at full size
+    v = torch.rand_like(u) * 2 - 1
+    Hu = op.apply(u)
+    Hv = op.apply(v)
+    H2 = op.apply(2.0 * u - 3.0 * v)
+    assert (torch.linalg.norm(H2 - (2.0 * Hu - 3.0 * Hv)) / torch.linalg.norm(H2)).item() < 1e-13
+    a, c = torch.dot(v, Hu).item(), torch.dot(u, Hv).item()
+    assert abs(a - c) < 1e-12 * abs(a)
+    # FP32 variant of the same workload
+    op32 = H.Operator("Helmholtz_Hex_Deformed", nm, nq, E, precision=H.FP32)
+    op32.set_jac(gj)
+    op32.set_df(gd)
+    op32.set_lambda(w["lam"])
+    got32 = op32.apply(u.float()).double().cpu().numpy()
+    assert rel(got32, ref) < 1e-5
+
+
+def test_quad_workload_parity():
+    """32x32 regular quads, P=4, 6 GLL points, lambda=1, seed 0."""
+    w = mesh.workload_quad_helmholtz()
+    nm, nq, E = w["nm"], w["nq"], w["nElmt"]
+    b = O.basis(nm, nq)
+    jac, df = O.geom_quad(w["verts"], nq, b["z"], False)
+    ref = O.helmholtz("quad", b, E, False, jac, df, w["lam"], w["u"])
+    op = H.Operator("Helmholtz_Quad_Regular", nm, nq, E)
+    op.set_jac(jac)
+    op.set_df(df)
+    op.set_lambda(w["lam"])
+    got = op.apply(_dev(w["u"])).cpu().numpy()
+    assert rel(got, ref) < TOL64

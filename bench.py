@@ -30,7 +30,9 @@ def parse():
     p.add_argument("--steps", type=int, default=50)
     p.add_argument("--warmup", type=int, default=5)
     p.add_argument("--precision", choices=["fp64", "fp32"], default="fp64")
-    p.add_argument("--op", choices=["helmholtz", "bwdtrans", "iproduct"], default="helmholtz")
+    p.add_argument("--op", choices=["helmholtz", "bwdtrans", "iproduct", "copy"],
+                   default="helmholtz",
+                   help="copy: plain 1 GiB device copy (HBM bandwidth / counter calibration)")
     p.add_argument("--n", type=int, default=48, help="elements per direction")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     p.add_argument("--cpu-seconds", type=float, default=12.0)
@@ -74,8 +76,34 @@ def cpu_baseline(w, seconds):
                       f"width 1), OpenMP over contiguous element chunks, {threads} threads"}
 
 
+def copy_calibration(args):
+    """Device-to-device copy of 1 GiB: reads 1 GiB, writes 1 GiB per launch."""
+    import torch
+
+    n = (1 << 30) // 8
+    x = torch.ones(n, dtype=torch.float64, device="cuda:0")
+    y = torch.empty_like(x)
+    for _ in range(args.warmup):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    ev0.record()
+    for _ in range(args.steps):
+        y.copy_(x)
+    ev1.record()
+    torch.cuda.synchronize()
+    ms = ev0.elapsed_time(ev1) / args.steps
+    gbs = 2 * n * 8 / (ms * 1e-3) / 1e9
+    print(json.dumps({"metric": "device copy bandwidth", "value": gbs, "unit": "GB/s",
+                      "bytes_read_per_launch": n * 8, "bytes_written_per_launch": n * 8,
+                      "ms_per_step": ms, "steps": args.steps, "frac_of_8TBs": gbs / HBM_PEAK_GBS}))
+
+
 def main():
     args = parse()
+    if args.op == "copy":
+        return copy_calibration(args)
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1 "
                                    "(multi-GPU sharding is described in DESIGN.md, not run)"}))

@@ -1,0 +1,359 @@
+// Implementation of the host mirror (Basis, MatrixFree operators, Collections adapters)
+// over the C ABI of include/ithaca_mf.h.
+#include <iostream>
+#include <sstream>
+
+#include "../../../include/ithaca_mf.h"
+#include "ithaca/Collections.hpp"
+
+namespace Nektar {
+
+namespace {
+
+void check(int status)
+{
+    if (status != IMF_OK) throw NekError(imf_last_error());
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// LibUtilities::Basis
+// ---------------------------------------------------------------------------
+namespace LibUtilities {
+
+Basis::Basis(const BasisKey &key) : m_key(key)
+{
+    ASSERTL0(key.GetBasisType() == eModified_A &&
+                 key.GetPointsKey().GetPointsType() == eGaussLobattoLegendre,
+             "Basis: only Modified_A on Gauss-Lobatto-Legendre points is provided");
+    const int nm = key.GetNumModes(), nq = key.GetPointsKey().GetNumPoints();
+    m_z.resize(nq);
+    m_w.resize(nq);
+    m_D.resize(nq * nq);
+    m_bdata.resize(nm * nq);
+    m_dbdata.resize(nm * nq);
+    check(imf_basis_modified_a(nm, nq, m_z.data(), m_w.data(), m_D.data(), m_bdata.data(),
+                               m_dbdata.data()));
+}
+
+}  // namespace LibUtilities
+
+// ---------------------------------------------------------------------------
+// MatrixFree
+// ---------------------------------------------------------------------------
+namespace MatrixFree {
+
+Operator::Operator(const std::string &key, std::vector<LibUtilities::BasisSharedPtr> basis,
+                   int nElmt)
+    : m_basis(basis), m_nElmt(nElmt)
+{
+    const int dim = (int)basis.size();
+    std::vector<int> nm(dim), nq(dim);
+    std::vector<const double *> b(dim), db(dim), D(dim), w(dim);
+    for (int d = 0; d < dim; ++d) {
+        nm[d] = basis[d]->GetNumModes();
+        nq[d] = basis[d]->GetNumPoints();
+        b[d] = basis[d]->GetBdata().data();
+        db[d] = basis[d]->GetDbdata().data();
+        D[d] = basis[d]->GetD().data();
+        w[d] = basis[d]->GetW().data();
+    }
+    check(imf_create(key.c_str(), dim, nm.data(), nq.data(), b.data(), db.data(), D.data(),
+                     w.data(), nElmt, IMF_FP64, &m_op));
+}
+
+Operator::~Operator() { imf_destroy(m_op); }
+
+NekDouble Operator::Ndof() { return imf_ndof(m_op); }
+
+void Operator::SetJac(const std::shared_ptr<std::vector<NekDouble>> &jac)
+{
+    check(imf_set_jac(m_op, jac->data(), jac->size()));
+}
+
+void Operator::SetDF(const std::shared_ptr<std::vector<NekDouble>> &df)
+{
+    check(imf_set_df(m_op, df->data(), df->size()));
+}
+
+void Operator::SetJacDevice(const NekDouble *d_jac, size_t n) { check(imf_set_jac_device(m_op, d_jac, n)); }
+void Operator::SetDFDevice(const NekDouble *d_df, size_t n) { check(imf_set_df_device(m_op, d_df, n)); }
+
+void Operator::ApplyDevice(const NekDouble *d_in, NekDouble *d_out, void *stream)
+{
+    check(imf_apply_device(m_op, d_in, d_out, stream));
+}
+
+void Operator::ApplyHost(const NekDouble *in, NekDouble *out) { check(imf_apply(m_op, in, out)); }
+
+void BwdTrans::operator()(const Array<OneD, const NekDouble> &in, Array<OneD, NekDouble> &out)
+{
+    ASSERTL0(in.size() >= imf_in_size(m_op) && out.size() >= imf_out_size(m_op),
+             "BwdTrans: array too small");
+    ApplyHost(in.data(), out.data());
+}
+
+void IProduct::operator()(const Array<OneD, const NekDouble> &in, Array<OneD, NekDouble> &out)
+{
+    ASSERTL0(in.size() >= imf_in_size(m_op) && out.size() >= imf_out_size(m_op),
+             "IProduct: array too small");
+    ApplyHost(in.data(), out.data());
+}
+
+void Helmholtz::operator()(const Array<OneD, const NekDouble> &in, Array<OneD, NekDouble> &out)
+{
+    ASSERTL0(in.size() >= imf_in_size(m_op) && out.size() >= imf_out_size(m_op),
+             "Helmholtz: array too small");
+    ApplyHost(in.data(), out.data());
+}
+
+void Helmholtz::SetLambda(NekDouble lambda)
+{
+    m_lambda = lambda;
+    check(imf_set_lambda(m_op, lambda));
+}
+
+OperatorFactory &GetOperatorFactory()
+{
+    static OperatorFactory tmp;
+    return tmp;
+}
+
+std::string GetOpstring(LibUtilities::ShapeType shape, bool deformed)
+{
+    std::string s = "_";
+    switch (shape) {
+    case LibUtilities::eSegment: s += "Seg"; break;
+    case LibUtilities::eTriangle: s += "Tri"; break;
+    case LibUtilities::eQuadrilateral: s += "Quad"; break;
+    case LibUtilities::eTetrahedron: s += "Tet"; break;
+    case LibUtilities::ePyramid: s += "Pyr"; break;
+    case LibUtilities::ePrism: s += "Prism"; break;
+    case LibUtilities::eHexahedron: s += "Hex"; break;
+    default: break;
+    }
+    s += deformed ? "_Deformed" : "_Regular";
+    return s;
+}
+
+namespace {
+
+template <class T>
+OperatorSharedPtr make(const std::string &key, std::vector<LibUtilities::BasisSharedPtr> b, int n)
+{
+    return std::make_shared<T>(key, b, n);
+}
+
+// Keys registered by the reference for quads and hexes: MatrixFreeOps/BwdTrans.cpp:7-33,
+// IProductQuad.cpp, IProductHex.cpp, Helmholtz.cpp:7-23.
+const bool registered = [] {
+    auto &f = GetOperatorFactory();
+    for (const char *shape : {"Quad", "Hex"}) {
+        const std::string s(shape);
+        const std::string kb = "BwdTrans_" + s + "_Regular";
+        f.RegisterCreatorFunction(kb, [kb](std::vector<LibUtilities::BasisSharedPtr> b, int n) {
+            return make<BwdTrans>(kb, b, n);
+        });
+        for (const char *d : {"_Regular", "_Deformed"}) {
+            const std::string ki = "IProduct_" + s + d, kh = "Helmholtz_" + s + d;
+            f.RegisterCreatorFunction(ki, [ki](std::vector<LibUtilities::BasisSharedPtr> b, int n) {
+                return make<IProduct>(ki, b, n);
+            });
+            f.RegisterCreatorFunction(kh, [kh](std::vector<LibUtilities::BasisSharedPtr> b, int n) {
+                return make<Helmholtz>(kh, b, n);
+            });
+        }
+    }
+    return true;
+}();
+
+}  // namespace
+}  // namespace MatrixFree
+
+// ---------------------------------------------------------------------------
+// Collections
+// ---------------------------------------------------------------------------
+namespace Collections {
+
+int CollectionExp::GetNcoeffs() const
+{
+    int n = 1;
+    for (auto &b : basis) n *= b->GetNumModes();
+    return n;
+}
+
+int CollectionExp::GetTotPoints() const
+{
+    int n = 1;
+    for (auto &b : basis) n *= b->GetNumPoints();
+    return n;
+}
+
+OperatorImpMap SetFixedImpType(ImplementationType defaultType)
+{
+    OperatorImpMap m;
+    for (int i = 0; i < SIZE_OperatorType; ++i) m[(OperatorType)i] = defaultType;
+    return m;
+}
+
+std::ostream &operator<<(std::ostream &os, const OperatorKey &p)
+{
+    os << LibUtilities::ShapeTypeMap[std::get<0>(p)] << ", "
+       << OperatorTypeMap[std::get<1>(p)] << ", " << ImplementationTypeMap[std::get<2>(p)];
+    if (std::get<3>(p)) os << ", Nodal";
+    return os;
+}
+
+OperatorFactory &GetOperatorFactory()
+{
+    static OperatorFactory tmp;
+    return tmp;
+}
+
+Operator::Operator(const CollectionExp &exp, CoalescedGeomDataSharedPtr geom)
+    : m_isDeformed(geom ? geom->IsDeformed() : false), m_numElmt(exp.nElmt)
+{
+}
+
+namespace {
+
+std::shared_ptr<MatrixFree::Operator> create_mf(const std::string &op, const CollectionExp &exp,
+                                                bool deformed)
+{
+    const std::string key = op + MatrixFree::GetOpstring(exp.shape, deformed);
+    return MatrixFree::GetOperatorFactory().CreateInstance(key, exp.basis, exp.nElmt);
+}
+
+// Collections/BwdTrans.cpp:140-240 (BwdTrans_MatrixFree), on the GPU.
+class BwdTrans_MatrixFreeHip final : public Operator {
+public:
+    BwdTrans_MatrixFreeHip(const CollectionExp &exp, CoalescedGeomDataSharedPtr geom)
+        : Operator(exp, geom)
+    {
+        m_oper = std::dynamic_pointer_cast<MatrixFree::BwdTrans>(create_mf("BwdTrans", exp, false));
+        ASSERTL0(m_oper, "Failed to cast pointer.");
+    }
+    void operator()(const Array<OneD, const NekDouble> &input, Array<OneD, NekDouble> &output0,
+                    Array<OneD, NekDouble> &, Array<OneD, NekDouble> &, Array<OneD, NekDouble> &,
+                    const StdRegions::ConstFactorMap &) final
+    {
+        (*m_oper)(input, output0);
+    }
+    void operator()(int, const Array<OneD, const NekDouble> &, Array<OneD, NekDouble> &,
+                    Array<OneD, NekDouble> &) final
+    {
+        NEKERROR(ErrorUtil::efatal, "BwdTrans_MatrixFree: Not valid for this operator.");
+    }
+    std::shared_ptr<MatrixFree::BwdTrans> m_oper;
+};
+
+// Collections/IProductWRTBase.cpp:168-277 (IProductWRTBase_MatrixFree), on the GPU.
+class IProductWRTBase_MatrixFreeHip final : public Operator {
+public:
+    IProductWRTBase_MatrixFreeHip(const CollectionExp &exp, CoalescedGeomDataSharedPtr geom)
+        : Operator(exp, geom)
+    {
+        auto oper = create_mf("IProduct", exp, m_isDeformed);
+        oper->SetJac(geom->GetJac());
+        m_oper = std::dynamic_pointer_cast<MatrixFree::IProduct>(oper);
+        ASSERTL0(m_oper, "Failed to cast pointer.");
+    }
+    void operator()(const Array<OneD, const NekDouble> &input, Array<OneD, NekDouble> &output,
+                    Array<OneD, NekDouble> &, Array<OneD, NekDouble> &, Array<OneD, NekDouble> &,
+                    const StdRegions::ConstFactorMap &) final
+    {
+        (*m_oper)(input, output);
+    }
+    void operator()(int, const Array<OneD, const NekDouble> &, Array<OneD, NekDouble> &,
+                    Array<OneD, NekDouble> &) final
+    {
+        NEKERROR(ErrorUtil::efatal, "Not valid for this operator.");
+    }
+    std::shared_ptr<MatrixFree::IProduct> m_oper;
+};
+
+// Collections/Helmholtz.cpp:391-503 (Helmholtz_MatrixFree), on the GPU.  lambda is read
+// from factors[eFactorLambda] on every call (Helmholtz.cpp:408-414).
+class Helmholtz_MatrixFreeHip final : public Operator {
+public:
+    Helmholtz_MatrixFreeHip(const CollectionExp &exp, CoalescedGeomDataSharedPtr geom)
+        : Operator(exp, geom)
+    {
+        auto oper = create_mf("Helmholtz", exp, m_isDeformed);
+        oper->SetJac(geom->GetJac());
+        oper->SetDF(geom->GetDerivFactors());
+        m_oper = std::dynamic_pointer_cast<MatrixFree::Helmholtz>(oper);
+        ASSERTL0(m_oper, "Failed to cast pointer.");
+    }
+    void operator()(const Array<OneD, const NekDouble> &input, Array<OneD, NekDouble> &output0,
+                    Array<OneD, NekDouble> &, Array<OneD, NekDouble> &, Array<OneD, NekDouble> &,
+                    const StdRegions::ConstFactorMap &factors) final
+    {
+        auto x = factors.find(StdRegions::eFactorLambda);
+        ASSERTL0(x != factors.end(), "Constant factor not defined: FactorLambda");
+        m_oper->SetLambda(x->second);
+        (*m_oper)(input, output0);
+    }
+    void operator()(int, const Array<OneD, const NekDouble> &, Array<OneD, NekDouble> &,
+                    Array<OneD, NekDouble> &) final
+    {
+        NEKERROR(ErrorUtil::efatal, "Not valid for this operator.");
+    }
+    std::shared_ptr<MatrixFree::Helmholtz> m_oper;
+};
+
+template <class T>
+OperatorSharedPtr create(const CollectionExp &e, CoalescedGeomDataSharedPtr g)
+{
+    return std::make_shared<T>(e, g);
+}
+
+const bool registered = [] {
+    auto &f = GetOperatorFactory();
+    for (auto shape : {LibUtilities::eQuadrilateral, LibUtilities::eHexahedron}) {
+        f.RegisterCreatorFunction(OperatorKey(shape, eBwdTrans, eMatrixFreeHip, false),
+                                  create<BwdTrans_MatrixFreeHip>, "BwdTrans_MatrixFreeHip");
+        f.RegisterCreatorFunction(OperatorKey(shape, eIProductWRTBase, eMatrixFreeHip, false),
+                                  create<IProductWRTBase_MatrixFreeHip>,
+                                  "IProductWRTBase_MatrixFreeHip");
+        f.RegisterCreatorFunction(OperatorKey(shape, eHelmholtz, eMatrixFreeHip, false),
+                                  create<Helmholtz_MatrixFreeHip>, "Helmholtz_MatrixFreeHip");
+    }
+    return true;
+}();
+
+}  // namespace
+
+Collection::Collection(const CollectionExp &exp, CoalescedGeomDataSharedPtr geom,
+                       const OperatorImpMap &impTypes)
+    : m_exp(exp), m_geomData(geom), m_impTypes(impTypes)
+{
+}
+
+void Collection::Initialise(const OperatorType opType)
+{
+    if (HasOperator(opType)) return;
+    auto it = m_impTypes.find(opType);
+    if (it == m_impTypes.end()) {
+        NEKERROR(ErrorUtil::ewarning, "Failed to determine implmentation to initialise "
+                                      "collection operator");
+        return;
+    }
+    OperatorKey key(m_exp.shape, opType, it->second, m_exp.nodal);
+    std::stringstream ss;
+    ss << key;
+    ASSERTL0(GetOperatorFactory().ModuleExists(key), "Requested unknown operator " + ss.str());
+    m_ops[opType] = GetOperatorFactory().CreateInstance(key, m_exp, m_geomData);
+}
+
+void Collection::ApplyOperator(const OperatorType &op, const Array<OneD, const NekDouble> &inarray,
+                               Array<OneD, NekDouble> &output,
+                               const StdRegions::ConstFactorMap &factors)
+{
+    Array<OneD, NekDouble> wsp(m_ops[op]->GetWspSize());
+    (*m_ops[op])(inarray, output, NullNekDouble1DArray, NullNekDouble1DArray, wsp, factors);
+}
+
+}  // namespace Collections
+}  // namespace Nektar

@@ -1,0 +1,81 @@
+/*
+ * ithaca_rom.h -- C ABI of the MI355X reduced-basis kernels.
+ *
+ * The reference ROM (solvers/IncNavierStokesSolver/EquationSystems/CoupledLinearNS_ROM.cpp)
+ * runs its offline and online phases on the CPU with Eigen 3.3.7:
+ *   offline  ROM_offline_phase  :3916-4001  (BDCSVD of the snapshot matrix, truncation)
+ *            gen_reference_matrices / gen_proj_adv_terms  :2730-2830, :3639-3911
+ *            (Galerkin projections RB^T A RB)
+ *   online   ROM_online_phase   :4003-4100  one parameter at a time:
+ *            gen_affine_mat_proj :4694-4704 (A(mu) = sum_q theta_q(mu) A_q),
+ *            colPivHouseholderQr().solve :4053.
+ * The entry points below replace those loops with batched device kernels.  All arrays are
+ * device pointers unless stated; streams are hipStream_t passed as void* (NULL = default).
+ * Status codes as in ithaca_mf.h (0 = OK), message via irom_last_error().
+ */
+#ifndef ITHACA_ROM_H
+#define ITHACA_ROM_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct irom_batch_s *irom_batch;
+
+enum irom_method {
+    IROM_LU_FP64 = 0,    /* assemble + LU (partial pivoting) + solve, all FP64 */
+    IROM_LU_FP32_IR = 1  /* FP32 LU + FP64 residuals / iterative refinement */
+};
+
+/* ---- batched online solves: A(mu_i) x_i = b_i, i < nInst ----------------------------
+ * Replaces the per-parameter loop of ROM_online_phase (CoupledLinearNS_ROM.cpp:4003-4100):
+ * gen_affine_mat_proj (:4694-4704) and colPivHouseholderQr().solve (:4053).
+ * Instance i has size N[i] (1..256) and Q[i] affine terms.  Packed device layouts (offsets
+ * in elements, instances in the given order):
+ *   A      : A_q of instance i, row-major N x N, at a_off[i] + q*N*N      (double)
+ *   theta  : theta_q(mu_i) at t_off[i] + q                                 (double)
+ *   b, x   : vectors at v_off[i]                                           (double)
+ * The handle owns its workspace and scheduling data; solve calls allocate nothing. */
+int irom_batch_create(int nInst, const int *N, const int *Q, irom_batch *out);
+int irom_batch_destroy(irom_batch b);
+/* offsets (host arrays of nInst entries) and total sizes of the packed arrays */
+int irom_batch_layout(irom_batch b, long *a_off, long *t_off, long *v_off, size_t *a_size,
+                      size_t *t_size, size_t *v_size);
+
+/* Solve every instance (transpose != 0 solves A(mu)^T x = b, the adjoint system).
+ * tol: relative backward-error target of the refinement (IROM_LU_FP32_IR); max_iter caps it.
+ * One persistent launch (plus a counter reset).  Per-instance refinement iterations and
+ * status (0 ok, 1 zero pivot, 2 refinement not converged) go to d_iters / d_status when
+ * non-NULL (int arrays of nInst). */
+int irom_batch_solve(irom_batch b, int method, int transpose, const double *d_A,
+                     const double *d_theta, const double *d_b, double *d_x, double tol,
+                     int max_iter, int *d_iters, int *d_status, void *stream);
+
+/* Tile variant used by irom_batch_solve for a method: set explicitly, or choose by timing
+ * every variant on the given data (the autotuner; returns the chosen index).  */
+int irom_batch_num_variants(void);
+const char *irom_batch_variant_name(int v);
+int irom_batch_set_variant(irom_batch b, int method, int v);
+int irom_batch_autotune(irom_batch b, int method, const double *d_A, const double *d_theta,
+                        const double *d_b, double *d_x, int reps, int *chosen, void *stream);
+
+/* hipGraph: capture one irom_batch_solve with these arguments into a graph held by the
+ * handle, then replay it.  No memset nodes are recorded (counters are reset by a kernel). */
+int irom_batch_capture(irom_batch b, int method, int transpose, const double *d_A,
+                       const double *d_theta, const double *d_b, double *d_x, double tol,
+                       int max_iter, int *d_iters, int *d_status);
+int irom_batch_replay(irom_batch b, void *stream);
+
+/* Parameter gradient of J = l^T x(mu) for theta_q = mu_q: with A(mu)^T lambda = l,
+ * dJ/dmu_q = -lambda^T A_q x.  Writes d_grad[t_off[i] + q].  No reference counterpart. */
+int irom_batch_param_grad(irom_batch b, const double *d_A, const double *d_lambda,
+                          const double *d_x, double *d_grad, void *stream);
+
+const char *irom_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -73,6 +73,35 @@ int irom_batch_replay(irom_batch b, void *stream);
 int irom_batch_param_grad(irom_batch b, const double *d_A, const double *d_lambda,
                           const double *d_x, double *d_grad, void *stream);
 
+/* ---- POD offline: Gram / Galerkin products on MFMA --------------------------------
+ * C (row-major m x n, leading dimension ldc) (+)= X^T Y, X: K x m and Y: K x n column-major
+ * (columns contiguous, leading dimensions ldx / ldy), K up to ~1e8.  Replaces the dense
+ * products of the reference's offline phase (snapshot correlation / BDCSVD input,
+ * CoupledLinearNS_ROM.cpp:3925; RB^T A RB, :2788-2802, :3661-3663).  Modes:
+ *   IROM_GRAM_FP64   FP64 MFMA, FP64 accumulation
+ *   IROM_GRAM_FP32X3 hi/lo FP32 split, 3 FP32 MFMA products, FP32 accumulation within a
+ *                    16-deep panel, FP64 across panels
+ *   IROM_GRAM_BF16X3 hi/lo BF16 split, 3 BF16 MFMA products, FP32 accumulation within a
+ *                    32-deep panel, FP64 across panels
+ * X == Y (same pointer, m == n, ldx == ldy) computes only the upper blocks and mirrors.
+ * d_work must hold irom_gram_workspace_size(m, n, K) bytes (split-K partial blocks). */
+enum irom_gram_mode { IROM_GRAM_FP64 = 0, IROM_GRAM_FP32X3 = 1, IROM_GRAM_BF16X3 = 2 };
+size_t irom_gram_workspace_size(int m, int n, long K);
+int irom_gram(int mode, int m, int n, long K, const double *d_X, long ldx, const double *d_Y,
+              long ldy, double *d_C, long ldc, int accumulate, void *d_work, size_t work_bytes,
+              void *stream);
+const char *irom_gram_last_error(void);
+
+/* ---- POD offline: FP64 symmetric eigen-decomposition --------------------------------
+ * Eigenpairs of the symmetric positive semi-definite n x n matrix d_G (row-major, device):
+ * h_evals (host, n) in descending order, d_evecs (device, column-major n x n, column j the
+ * eigenvector of h_evals[j]).  One-sided Jacobi, rotations while |w_p.w_q| > tol ||w_p|| ||w_q||.
+ * Replaces the reference's BDCSVD of the snapshot matrix (CoupledLinearNS_ROM.cpp:3925)
+ * through the method of snapshots.  Synchronous; allocates its own workspace. */
+int irom_eig_sym(int n, const double *d_G, double *h_evals, double *d_evecs, double tol,
+                 int max_sweeps, int *sweeps, void *stream);
+const char *irom_eig_last_error(void);
+
 const char *irom_last_error(void);
 
 #ifdef __cplusplus

@@ -65,6 +65,14 @@ constexpr int geo_comps(int kind, int shape)
     return kind == kHelmholtz ? (shape == kHex ? 7 : 4) : 1;
 }
 
+// Per-element stride (in scalars) of deformed packed geometry: the nc*Q values of an
+// element padded to a whole number of 128-byte lines, so that no cache line is shared by
+// two elements (which different workgroups read at different times).
+__host__ __device__ constexpr long geo_stride(long nc_q, int elt_bytes)
+{
+    return ((nc_q * elt_bytes + 127) / 128) * 128 / elt_bytes;
+}
+
 // Launcher signature: everything an apply needs, already on the device.
 struct LaunchArgs {
     const void *tab;   // host pointer to a Tab<NM,NQ,T> image

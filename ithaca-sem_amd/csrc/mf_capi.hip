@@ -52,6 +52,8 @@ size_t df_len(const imf_op op) { return (size_t)op->dim * op->dim * jac_len(op);
 size_t geom_len(const imf_op op)
 {
     const int nc = imf::geo_comps(op->kind, op->shape);
+    if (op->kind == imf::kHelmholtz && op->shape == imf::kHex && op->deformed)
+        return (size_t)op->nElmt * imf::geo_stride((long)nc * op->Q, (int)elt_size(op));
     return (size_t)nc * jac_len(op);
 }
 

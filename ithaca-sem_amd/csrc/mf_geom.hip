@@ -47,7 +47,9 @@ __global__ void pack_helm_deformed(long nPts, int Q, int nq, Pts w, const double
 #pragma unroll
     for (int d = 0; d < NDF; ++d) f[d] = df[(long)d * nPts + g];
     if (DIM == 3) metric3(f, m); else metric2(f, m);
-    T *o = out + e * NC * Q + q;
+    // hexes: per-element blocks padded to whole 128-byte lines (geo_stride)
+    const long stride = DIM == 3 ? geo_stride((long)NC * Q, sizeof(T)) : (long)NC * Q;
+    T *o = out + e * stride + q;
     o[0] = T(jw);
 #pragma unroll
     for (int c = 1; c < NC; ++c) o[(long)c * Q] = T(jw * m[c - 1]);

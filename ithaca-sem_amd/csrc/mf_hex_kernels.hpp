@@ -36,27 +36,78 @@ __device__ __forceinline__ void st_stream(T *p, T v)
 
 // ---------------------------------------------------------------------------
 // Building blocks.  `t` is the thread's index inside its element (0 <= t < NQ*NQ).
-// buffers are NQ^3 scalars each.
+// Coefficient pencils live in LDS with an odd row stride NM+1 (value (t, p) of the
+// element's mode set, t = r*NM + q, at t*(NM+1) + p) so that the p-pencil reads of
+// neighbouring threads hit different banks.
 // ---------------------------------------------------------------------------
 
-// u_hat (global, p-pencils of thread (q,r)) -> u (registers, k-column of thread (i,j)).
-// Uses buf0 and buf1.  Ends with u in registers of every active thread.
+template <int NM, int NQ>
+struct HexLds {
+    static constexpr int M = NM * NM * NM, Q = NQ * NQ * NQ;
+    static constexpr int MP = NM * NM * (NM + 1);  // padded pencil span of a mode set
+    static constexpr int QB = Q > MP ? Q : MP;     // one LDS buffer
+};
+
+// Block-cooperative, fully coalesced load of the block's contiguous span of nv elements'
+// coefficients (nv*M scalars from src) into the padded pencil buffers buf(slot).
+template <int NM, int NQ, typename T, typename BufFn>
+__device__ __forceinline__ void load_modes_block(const T *__restrict__ src, int nv, BufFn buf)
+{
+    constexpr int M = NM * NM * NM;
+    for (int f = threadIdx.x; f < nv * M; f += blockDim.x) {
+        const int sl = f / M, o = f - sl * M;
+        const int tp = o / NM, p = o - tp * NM;
+        buf(sl)[tp * (NM + 1) + p] = ld_stream(src + f);
+    }
+}
+
+// Block-cooperative, fully coalesced store of nv elements' coefficients from the padded
+// pencil buffers buf(slot) to dst.
+template <int NM, int NQ, typename T, typename BufFn>
+__device__ __forceinline__ void store_modes_block(T *__restrict__ dst, int nv, BufFn buf)
+{
+    constexpr int M = NM * NM * NM;
+    for (int f = threadIdx.x; f < nv * M; f += blockDim.x) {
+        const int sl = f / M, o = f - sl * M;
+        const int tp = o / NM, p = o - tp * NM;
+        st_stream(dst + f, buf(sl)[tp * (NM + 1) + p]);
+    }
+}
+
+// Coalesced copy of nv elements' quadrature values (nv*Q scalars) into buf(slot) (natural
+// order), and back.
+template <int NQ, typename T, typename BufFn>
+__device__ __forceinline__ void load_points_block(const T *__restrict__ src, int nv, BufFn buf)
+{
+    constexpr int Q = NQ * NQ * NQ;
+    for (int f = threadIdx.x; f < nv * Q; f += blockDim.x) {
+        const int sl = f / Q;
+        buf(sl)[f - sl * Q] = ld_stream(src + f);
+    }
+}
+
+template <int NQ, typename T, typename BufFn>
+__device__ __forceinline__ void store_points_block(T *__restrict__ dst, int nv, BufFn buf)
+{
+    constexpr int Q = NQ * NQ * NQ;
+    for (int f = threadIdx.x; f < nv * Q; f += blockDim.x) {
+        const int sl = f / Q;
+        st_stream(dst + f, buf(sl)[f - sl * Q]);
+    }
+}
+
+// Mode pencils (LDS, padded) -> u (registers, k-column of thread (i,j)).  Reads uh from
+// `modes`, uses buf0 and buf1.
 template <int NM, int NQ, typename T>
-__device__ __forceinline__ void hex_bwd_to_columns(const Tab<NM, NQ, T> &tab,
-                                                   const T *__restrict__ src, bool valid,
+__device__ __forceinline__ void hex_bwd_to_columns(const Tab<NM, NQ, T> &tab, const T *modes,
                                                    bool active, int t, T *buf0, T *buf1,
                                                    T (&u)[NQ])
 {
     // p -> i on p-pencils (thread (q, r))
     if (active && t < NM * NM) {
         T uh[NM];
-        if (valid) {
 #pragma unroll
-            for (int p = 0; p < NM; ++p) uh[p] = ld_stream(src + t * NM + p);
-        } else {
-#pragma unroll
-            for (int p = 0; p < NM; ++p) uh[p] = T(0);
-        }
+        for (int p = 0; p < NM; ++p) uh[p] = modes[t * (NM + 1) + p];
 #pragma unroll
         for (int i = 0; i < NQ; ++i) {
             T s = uh[0] * tab.B[i];
@@ -99,13 +150,13 @@ __device__ __forceinline__ void hex_bwd_to_columns(const Tab<NM, NQ, T> &tab,
     }
 }
 
-// k-columns acc (registers of thread (i,j)) -> out = (B x B x B)^T acc (global).
-// Uses buf0 and buf1; the caller must have synchronised any earlier readers.
+// k-columns acc (registers of thread (i,j)) -> (B x B x B)^T acc as padded mode pencils in
+// `modes`.  Uses buf0 and buf1; `modes` may alias buf0.  The caller synchronises before
+// reading `modes`.
 template <int NM, int NQ, typename T>
 __device__ __forceinline__ void hex_columns_to_modes(const Tab<NM, NQ, T> &tab,
-                                                     const T (&acc)[NQ], T *__restrict__ dst,
-                                                     bool valid, bool active, int t, T *buf0,
-                                                     T *buf1)
+                                                     const T (&acc)[NQ], bool active, int t,
+                                                     T *buf0, T *buf1, T *modes)
 {
     // k -> r in registers
     if (active) {
@@ -138,14 +189,12 @@ __device__ __forceinline__ void hex_columns_to_modes(const Tab<NM, NQ, T> &tab,
         T v[NQ];
 #pragma unroll
         for (int i = 0; i < NQ; ++i) v[i] = buf1[t * NQ + i];
-        if (valid) {
 #pragma unroll
-            for (int p = 0; p < NM; ++p) {
-                T s = v[0] * tab.B[p * NQ];
+        for (int p = 0; p < NM; ++p) {
+            T s = v[0] * tab.B[p * NQ];
 #pragma unroll
-                for (int i = 1; i < NQ; ++i) s = fma(v[i], tab.B[p * NQ + i], s);
-                st_stream(dst + t * NM + p, s);
-            }
+            for (int i = 1; i < NQ; ++i) s = fma(v[i], tab.B[p * NQ + i], s);
+            modes[t * (NM + 1) + p] = s;
         }
     }
 }
@@ -162,20 +211,22 @@ __global__ __launch_bounds__(HexCfg<NQ>::BLOCK) void hex_bwdtrans_kernel(
     const T *__restrict__ geom, T *__restrict__ out)
 {
     using C = HexCfg<NQ>;
-    constexpr int M = NM * NM * NM, Q = NQ * NQ * NQ;
-    __shared__ T lds[C::EPB][2][Q];
+    using Lb = HexLds<NM, NQ>;
+    constexpr int M = Lb::M, Q = Lb::Q;
+    __shared__ T lds[C::EPB][3][Lb::QB];
     const int eb = threadIdx.x / C::TPE;
     const int t = threadIdx.x - eb * C::TPE;
     const bool active = eb < C::EPB;
     const int es = active ? eb : 0;
-    const long e = (long)blockIdx.x * C::EPB + eb;
-    const bool valid = active && e < nElmt;
+    const long e0 = (long)blockIdx.x * C::EPB, e = e0 + eb;
+    const int nv = (int)min((long)C::EPB, (long)nElmt - e0);
+    const bool valid = active && eb < nv;
 
+    load_modes_block<NM, NQ, T>(in + e0 * M, nv, [&](int sl) { return lds[sl][2]; });
+    __syncthreads();
     T u[NQ];
-    hex_bwd_to_columns<NM, NQ, T>(tab, in + e * M, valid, active, t, lds[es][0], lds[es][1],
-                                  u);
-    if (!valid) return;
-    if (DIR == kTranspose) {
+    hex_bwd_to_columns<NM, NQ, T>(tab, lds[es][2], active, t, lds[es][0], lds[es][1], u);
+    if (valid && DIR == kTranspose) {
         if (GEO == kDeformed) {
 #pragma unroll
             for (int k = 0; k < NQ; ++k) u[k] *= ld_stream(geom + e * Q + k * C::TPE + t);
@@ -185,8 +236,13 @@ __global__ __launch_bounds__(HexCfg<NQ>::BLOCK) void hex_bwdtrans_kernel(
             for (int k = 0; k < NQ; ++k) u[k] *= wij * tab.w[k];
         }
     }
+    // lds[.][0] was last read before the final barrier of hex_bwd_to_columns
+    if (active) {
 #pragma unroll
-    for (int k = 0; k < NQ; ++k) st_stream(out + e * Q + k * C::TPE + t, u[k]);
+        for (int k = 0; k < NQ; ++k) lds[es][0][k * C::TPE + t] = u[k];
+    }
+    __syncthreads();
+    store_points_block<NQ, T>(out + e0 * Q, nv, [&](int sl) { return lds[sl][0]; });
 }
 
 // IProductWRTBase: out = (B x B x B)^T diag(w J) f.  GEO = kUnit gives the Euclidean
@@ -197,63 +253,75 @@ __global__ __launch_bounds__(HexCfg<NQ>::BLOCK) void hex_iproduct_kernel(
     const T *__restrict__ geom, T *__restrict__ out)
 {
     using C = HexCfg<NQ>;
-    constexpr int M = NM * NM * NM, Q = NQ * NQ * NQ;
-    __shared__ T lds[C::EPB][2][Q];
+    using Lb = HexLds<NM, NQ>;
+    constexpr int M = Lb::M, Q = Lb::Q;
+    __shared__ T lds[C::EPB][2][Lb::QB];
     const int eb = threadIdx.x / C::TPE;
     const int t = threadIdx.x - eb * C::TPE;
     const bool active = eb < C::EPB;
     const int es = active ? eb : 0;
-    const long e = (long)blockIdx.x * C::EPB + eb;
-    const bool valid = active && e < nElmt;
+    const long e0 = (long)blockIdx.x * C::EPB, e = e0 + eb;
+    const int nv = (int)min((long)C::EPB, (long)nElmt - e0);
+    const bool valid = active && eb < nv;
 
+    load_points_block<NQ, T>(in + e0 * Q, nv, [&](int sl) { return lds[sl][1]; });
+    if (GEO == kDeformed)
+        load_points_block<NQ, T>(geom + e0 * Q, nv, [&](int sl) { return lds[sl][0]; });
+    __syncthreads();
     T acc[NQ];
     if (valid) {
         if (GEO == kDeformed) {
 #pragma unroll
-            for (int k = 0; k < NQ; ++k) {
-                const int q = k * C::TPE + t;
-                acc[k] = ld_stream(in + e * Q + q) * ld_stream(geom + e * Q + q);
-            }
+            for (int k = 0; k < NQ; ++k)
+                acc[k] = lds[es][1][k * C::TPE + t] * lds[es][0][k * C::TPE + t];
         } else if (GEO == kRegular) {
             const T wij = tab.w[t % NQ] * tab.w[t / NQ] * geom[e];
 #pragma unroll
-            for (int k = 0; k < NQ; ++k)
-                acc[k] = ld_stream(in + e * Q + k * C::TPE + t) * (wij * tab.w[k]);
+            for (int k = 0; k < NQ; ++k) acc[k] = lds[es][1][k * C::TPE + t] * (wij * tab.w[k]);
         } else {
 #pragma unroll
-            for (int k = 0; k < NQ; ++k) acc[k] = ld_stream(in + e * Q + k * C::TPE + t);
+            for (int k = 0; k < NQ; ++k) acc[k] = lds[es][1][k * C::TPE + t];
         }
     } else {
 #pragma unroll
         for (int k = 0; k < NQ; ++k) acc[k] = T(0);
     }
-    hex_columns_to_modes<NM, NQ, T>(tab, acc, out + e * M, valid, active, t, lds[es][0],
-                                    lds[es][1]);
+    __syncthreads();
+    hex_columns_to_modes<NM, NQ, T>(tab, acc, active, t, lds[es][0], lds[es][1], lds[es][0]);
+    __syncthreads();
+    store_modes_block<NM, NQ, T>(out + e0 * M, nv, [&](int sl) { return lds[sl][0]; });
 }
 
 // Helmholtz: out = lambda M u_hat + sum_ab (D_a B)^T diag(w J G^{ab}) (D_b B) u_hat.
-// Packed geometry: deformed [e][7][Q] (Jw, JwG00, JwG01, JwG02, JwG11, JwG12, JwG22);
-// regular [e][7] (J, J G00, ...), weights applied in the kernel.
+// Packed geometry: deformed [e][GS] with GS = 7Q rounded up to 128-byte lines and the
+// components (Jw, JwG00, JwG01, JwG02, JwG11, JwG12, JwG22) at c*Q; regular [e][7]
+// (J, J G00, ...), weights applied in the kernel.
 template <int NM, int NQ, typename T, int GEO>
 __global__ __launch_bounds__(HexCfg<NQ>::BLOCK) void hex_helmholtz_kernel(
     const Tab<NM, NQ, T> tab, int nElmt, T lambda, const T *__restrict__ in,
     const T *__restrict__ geom, T *__restrict__ out)
 {
     using C = HexCfg<NQ>;
-    constexpr int M = NM * NM * NM, Q = NQ * NQ * NQ, TPE = C::TPE;
-    __shared__ T lds[C::EPB][4][Q];
+    using Lb = HexLds<NM, NQ>;
+    constexpr int M = Lb::M, Q = Lb::Q, TPE = C::TPE;
+    constexpr long GS = geo_stride(7 * Q, sizeof(T));
+    __shared__ T lds[C::EPB][4][Lb::QB];
     const int eb = threadIdx.x / TPE;
     const int t = threadIdx.x - eb * TPE;
     const bool active = eb < C::EPB;
     const int es = active ? eb : 0;
-    const long e = (long)blockIdx.x * C::EPB + eb;
-    const bool valid = active && e < nElmt;
+    const long e0 = (long)blockIdx.x * C::EPB, e = e0 + eb;
+    const int nv = (int)min((long)C::EPB, (long)nElmt - e0);
+    const bool valid = active && eb < nv;
     T *buf0 = lds[es][0], *buf1 = lds[es][1], *buf2 = lds[es][2], *buf3 = lds[es][3];
     const int ci = t % NQ, cj = t / NQ;  // this thread's column (i, j)
 
     // 1. BwdTrans into k-columns; keep a copy of u in LDS for the cross derivatives.
+    load_modes_block<NM, NQ, T>(in + e0 * M, nv, [&](int sl) { return lds[sl][2]; });
+    __syncthreads();
     T u[NQ];
-    hex_bwd_to_columns<NM, NQ, T>(tab, in + e * M, valid, active, t, buf0, buf1, u);
+    hex_bwd_to_columns<NM, NQ, T>(tab, buf2, active, t, buf0, buf1, u);
+    // buf2 (the mode pencils) was last read before the first barrier above
     if (active) {
 #pragma unroll
         for (int k = 0; k < NQ; ++k) buf2[k * TPE + t] = u[k];
@@ -304,7 +372,7 @@ __global__ __launch_bounds__(HexCfg<NQ>::BLOCK) void hex_helmholtz_kernel(
             const T d1 = buf1[k * TPE + t];
             T jw, m00, m01, m02, m11, m12, m22;
             if (GEO == kDeformed) {
-                const T *g = geom + e * (7 * Q) + k * TPE + t;
+                const T *g = geom + e * GS + k * TPE + t;
                 jw = ld_stream(g);
                 m00 = ld_stream(g + Q);
                 m01 = ld_stream(g + 2 * Q);
@@ -383,8 +451,10 @@ __global__ __launch_bounds__(HexCfg<NQ>::BLOCK) void hex_helmholtz_kernel(
         for (int k = 0; k < NQ; ++k) acc[k] += buf0[k * TPE + t] + buf1[k * TPE + t];
     }
 
-    // 5. (B x B x B)^T of the combined quadrature-space field.
-    hex_columns_to_modes<NM, NQ, T>(tab, acc, out + e * M, valid, active, t, buf2, buf3);
+    // 5. (B x B x B)^T of the combined quadrature-space field, then one coalesced store.
+    hex_columns_to_modes<NM, NQ, T>(tab, acc, active, t, buf2, buf3, buf2);
+    __syncthreads();
+    store_modes_block<NM, NQ, T>(out + e0 * M, nv, [&](int sl) { return lds[sl][2]; });
 }
 
 }  // namespace imf

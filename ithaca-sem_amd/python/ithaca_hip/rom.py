@@ -129,9 +129,13 @@ def affine_batch(n_inst=4096, seed=3, nmin=16, nmax=256, qmin=3, qmax=8):
 
     Per instance: Q ~ U{qmin..qmax} terms and size N ~ U{nmin..nmax};
     A_0 = U diag(10^{U[0,4]}) U^T with U Haar-orthogonal (QR of a Gaussian matrix, signs
-    fixed by diag(R)); A_q (q >= 1) = 0.1 (G + G^T)/2 with G Gaussian;
-    theta_q = mu_q ~ U[0.1, 1]; b ~ N(0, 1).  Returns the packed arrays of
-    include/ithaca_rom.h in instance order."""
+    fixed by diag(R)); A_q (q >= 1) = 0.1 sym(G) / sqrt(2N) with G Gaussian and
+    sym(G) = (G + G^T)/2, i.e. random symmetric matrices scaled to spectral norm ~0.1
+    (sqrt(2N) is the semicircle edge of sym(G)); theta_q = mu_q ~ U[0.1, 1]; b ~ N(0, 1).
+    The normalisation keeps every A(mu) symmetric positive definite with condition number
+    ~1e4 (unnormalised 0.1 sym(G) would make a third of the instances indefinite and
+    push condition numbers to ~1e7).  Returns the packed arrays of include/ithaca_rom.h
+    in instance order."""
     rng = np.random.default_rng(seed)
     Q = rng.integers(qmin, qmax + 1, size=n_inst)
     N = rng.integers(nmin, nmax + 1, size=n_inst)
@@ -150,12 +154,16 @@ def affine_batch(n_inst=4096, seed=3, nmin=16, nmax=256, qmin=3, qmax=8):
         U, R = np.linalg.qr(G)
         U *= np.sign(np.diagonal(R, axis1=1, axis2=2))[:, None, :]
         lam = 10.0 ** rng.uniform(0.0, 4.0, size=(len(idx), n))
-        A0 = np.einsum("kij,kj,klj->kil", U, lam, U)
+        A0 = (U * lam[:, None, :]) @ np.swapaxes(U, 1, 2)
+        npert = int(np.sum(Q[idx] - 1))
+        P = rng.standard_normal((npert, n, n))
+        P = (0.05 / np.sqrt(2.0 * n)) * (P + np.swapaxes(P, 1, 2))
+        c = 0
         for r, i in enumerate(idx):
             base = a_off[i]
             A[base:base + n * n] = A0[r].ravel()
-            for q in range(1, Q[i]):
-                Gq = rng.standard_normal((n, n))
-                A[base + q * n * n: base + (q + 1) * n * n] = (0.05 * (Gq + Gq.T)).ravel()
+            k = Q[i] - 1
+            A[base + n * n: base + Q[i] * n * n] = P[c:c + k].ravel()
+            c += k
     return dict(N=N.astype(np.int32), Q=Q.astype(np.int32), A=A, theta=theta, b=b,
                 a_off=a_off, t_off=t_off, v_off=v_off)

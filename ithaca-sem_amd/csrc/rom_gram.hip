@@ -84,7 +84,8 @@ __global__ __launch_bounds__(GT, 2) void gram_f64_kernel(int m, int n, long K, l
         }
         __syncthreads();
     }
-    // partial block -> part[split][i][j] (row-major m x n)
+    // partial block -> part[split][i][j] (row-major m x n).  The f64 16x16x4 accumulator
+    // holds D[lane/16 + 4r][lane%16] in component r.
     double *P = part + (long)split * m * n;
 #pragma unroll
     for (int a = 0; a < 4; ++a)
@@ -92,7 +93,7 @@ __global__ __launch_bounds__(GT, 2) void gram_f64_kernel(int m, int n, long K, l
         for (int b = 0; b < 4; ++b)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int i = i0 + wr + a * 16 + 4 * (lane >> 4) + r;
+                const int i = i0 + wr + a * 16 + (lane >> 4) + 4 * r;
                 const int j = j0 + wc + b * 16 + (lane & 15);
                 if (i < m && j < n) P[(long)i * n + j] = acc[a][b][r];
             }

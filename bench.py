@@ -36,6 +36,8 @@ def parse():
     p.add_argument("--n", type=int, default=48, help="elements per direction")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     p.add_argument("--cpu-seconds", type=float, default=12.0)
+    p.add_argument("--no-rom", action="store_true", help="skip the batched reduced solves")
+    p.add_argument("--rom-instances", type=int, default=4096)
     return p.parse_args()
 
 
@@ -74,6 +76,146 @@ def cpu_baseline(w, seconds):
             "sample": f"{E} of {w['nElmt']} elements, {len(times)} applies, median; "
                       "oracle restatement of HelmholtzHexImpl (scalar element loop, "
                       f"width 1), OpenMP over contiguous element chunks, {threads} threads"}
+
+
+def _event_time(fn, steps, warmup):
+    import torch
+
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(steps):
+        fn()
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / steps
+
+
+def rom_bench(args):
+    """Batched online reduced solves: 4096 affine systems A(mu) = sum_q theta_q A_q,
+    N ~ U{16..256}, Q ~ U{3..8}, A_0 SPD with eigenvalues log-uniform in [1, 1e4], A_q
+    random symmetric of spectral norm ~0.1, theta ~ U[0.1, 1], b ~ N(0,1) (seed 3).
+    One step = assemble + factor + solve every instance in one persistent launch."""
+    import numpy as np
+    import torch
+
+    from ithaca_hip import rom
+
+    bt = rom.affine_batch_device(args.rom_instances, seed=3)
+    B = rom.Batch(bt["N"], bt["Q"])
+    A = bt["A"]
+    th = torch.from_numpy(bt["theta"]).cuda()
+    b = torch.from_numpy(bt["b"]).cuda()
+    n = B.n
+    res = {"workload": f"{n} affine reduced systems, N~U{{16..256}}, Q~U{{3..8}}, seed 3",
+           "flops_lu_total": float(np.sum((2.0 / 3.0) * bt["N"].astype(float) ** 3)),
+           "bytes_A_q_total": float(bt["A"].numel() * 8)}
+    xs = {}
+    times = {}
+    for method, name in ((rom.LU_FP64, "fp64_lu"), (rom.LU_FP32_IR, "fp32_lu_fp64_ir")):
+        x = torch.empty_like(b)
+        v = B.autotune(A, th, b, x, method, reps=2)
+        ms = _event_time(lambda: B.solve(A, th, b, x, method=method), 10, 2)
+        iters = torch.zeros(n, dtype=torch.int32, device="cuda")
+        status = torch.full((n,), -1, dtype=torch.int32, device="cuda")
+        B.solve(A, th, b, x, method=method, iters=iters, status=status)
+        torch.cuda.synchronize()
+        xs[name] = x.cpu().numpy()
+        times[name] = ms
+        res[name] = {"solves_per_s": n / (ms * 1e-3), "ms_per_batch": ms,
+                     "variant": B.variants()[v],
+                     "refinement_iters_max": int(iters.max()),
+                     "refinement_iters_mean": float(iters.float().mean()),
+                     "status_nonzero": int((status != 0).sum())}
+    xg = torch.empty_like(b)
+    B.capture(A, th, b, xg, method=rom.LU_FP32_IR)
+    res["fp32_lu_fp64_ir"]["graph_replay_ms"] = _event_time(lambda: B.replay(), 10, 2)
+    res["speedup_fp32ir_over_fp64"] = times["fp64_lu"] / times["fp32_lu_fp64_ir"]
+    if not args.no_cpu:
+        res["cpu_baseline"] = rom_cpu_baseline(bt, xs)
+    return res
+
+
+def rom_cpu_baseline(bt, xs):
+    """The reference's online solve (Eigen 3.3.7 ColPivHouseholderQR after the affine
+    assembly, CoupledLinearNS_ROM.cpp:4053, :4694-4704) on every 16th instance, one core;
+    its solutions are the FP64 reference for the GPU errors."""
+    import numpy as np
+
+    import oracle.rom as R
+
+    idx = list(range(0, len(bt["N"]), 16))
+    host = dict(bt)
+    A_host = {}
+    for i in idx:
+        n, q = int(bt["N"][i]), int(bt["Q"][i])
+        a0 = int(bt["a_off"][i])
+        A_host[i] = bt["A"][a0:a0 + q * n * n].cpu().numpy().reshape(q, n, n)
+    use_eigen = R.eigen_lib() is not None
+    errs = {k: 0.0 for k in xs}
+    t_tot = 0.0
+    for i in idx:
+        n = int(bt["N"][i])
+        th = bt["theta"][int(bt["t_off"][i]):int(bt["t_off"][i]) + int(bt["Q"][i])]
+        v0 = int(bt["v_off"][i])
+        rhs = np.ascontiguousarray(bt["b"][v0:v0 + n])
+        t0 = time.perf_counter()
+        Am = np.ascontiguousarray(R.assemble(A_host[i], th))
+        if use_eigen:
+            xr = R.eigen_colpivqr(Am, rhs)
+        else:
+            xr = np.linalg.solve(Am, rhs)
+        t_tot += time.perf_counter() - t0
+        for k, x in xs.items():
+            errs[k] = max(errs[k], float(np.linalg.norm(x[v0:v0 + n] - xr) / np.linalg.norm(xr)))
+    del host
+    return {"value": len(idx) / t_tot, "unit": "solves/s", "cores": 1,
+            "kind": "reference" if use_eigen else "port",
+            "sample": f"every 16th instance ({len(idx)} of {len(bt['N'])}): affine assembly + "
+                      + ("Eigen 3.3.7 ColPivHouseholderQR (reference solver, oracle/_ref)"
+                         if use_eigen else "LAPACK LU (numpy)"),
+            "max_rel_err_vs_reference": errs}
+
+
+def quad_bench(args):
+    """32x32 regular quads, P=4 (5 modes), 6 GLL points, lambda=1, seed 0, 100 applies."""
+    import numpy as np
+    import torch
+
+    import ithaca_hip as H
+    from ithaca_hip import mesh
+
+    w = mesh.workload_quad_helmholtz()
+    nm, nq, E = w["nm"], w["nq"], w["nElmt"]
+    b = H.basis_modified_a(nm, nq)
+    verts = torch.from_numpy(w["verts"]).cuda()
+    gj, gd = H.geom_device("quad", verts, nq, b["z"], False)
+    op = H.Operator("Helmholtz_Quad_Regular", nm, nq, E, basis=b)
+    op.set_jac(gj)
+    op.set_df(gd)
+    op.set_lambda(w["lam"])
+    x = torch.from_numpy(w["u"]).cuda()
+    y = torch.empty(op.out_size, dtype=torch.float64, device="cuda")
+    ms = _event_time(lambda: op.apply(x, y), 100, 10)
+    res = {"workload": "Helmholtz_Quad_Regular 32x32 quads, P=4, 6 GLL pts, lambda=1",
+           "gpu_us_per_apply": ms * 1e3, "ndof": E * nm * nm}
+    if not args.no_cpu:
+        import oracle as O
+        O.set_threads(1)
+        bo = O.basis(nm, nq)
+        jac, df = O.geom_quad(w["verts"], nq, bo["z"], False)
+        O.helmholtz("quad", bo, E, False, jac, df, w["lam"], w["u"])
+        t0 = time.perf_counter()
+        for _ in range(100):
+            O.helmholtz("quad", bo, E, False, jac, df, w["lam"], w["u"])
+        res["cpu_baseline"] = {"us_per_apply": (time.perf_counter() - t0) / 100 * 1e6,
+                               "cores": 1, "kind": "port",
+                               "sample": "100 applies, oracle restatement of HelmholtzQuadImpl"}
+        O.set_threads(min(16, os.cpu_count() or 1))
+    return res
 
 
 def copy_calibration(args):
@@ -187,6 +329,12 @@ def main():
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(w, args.cpu_seconds)
         out["gpu_over_cpu"] = out["value"] / out["cpu_baseline"]["value"]
+    if args.op == "helmholtz" and args.precision == "fp64" and args.n == 48:
+        del op, x, y
+        torch.cuda.empty_cache()
+        if not args.no_rom:
+            out["batched_reduced_solves"] = rom_bench(args)
+        out["quad_helmholtz"] = quad_bench(args)
     print(json.dumps(out))
 
 

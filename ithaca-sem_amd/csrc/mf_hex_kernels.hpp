@@ -74,32 +74,10 @@ __device__ __forceinline__ void store_modes_block(T *__restrict__ dst, int nv, B
     }
 }
 
-// Coalesced copy of nv elements' quadrature values (nv*Q scalars) into buf(slot) (natural
-// order), and back.
-template <int NQ, typename T, typename BufFn>
-__device__ __forceinline__ void load_points_block(const T *__restrict__ src, int nv, BufFn buf)
-{
-    constexpr int Q = NQ * NQ * NQ;
-    for (int f = threadIdx.x; f < nv * Q; f += blockDim.x) {
-        const int sl = f / Q;
-        buf(sl)[f - sl * Q] = ld_stream(src + f);
-    }
-}
-
-template <int NQ, typename T, typename BufFn>
-__device__ __forceinline__ void store_points_block(T *__restrict__ dst, int nv, BufFn buf)
-{
-    constexpr int Q = NQ * NQ * NQ;
-    for (int f = threadIdx.x; f < nv * Q; f += blockDim.x) {
-        const int sl = f / Q;
-        st_stream(dst + f, buf(sl)[f - sl * Q]);
-    }
-}
-
-// Mode pencils (LDS, padded) -> u (registers, k-column of thread (i,j)).  Reads uh from
-// `modes`, uses buf0 and buf1.
-template <int NM, int NQ, typename T>
-__device__ __forceinline__ void hex_bwd_to_columns(const Tab<NM, NQ, T> &tab, const T *modes,
+// Mode pencils -> u (registers, k-column of thread (i,j)).  pencil(p) returns the p-th
+// coefficient of this thread's p-pencil (t = r*NM + q); uses buf0 and buf1.
+template <int NM, int NQ, typename T, typename PencilFn>
+__device__ __forceinline__ void hex_bwd_to_columns(const Tab<NM, NQ, T> &tab, PencilFn pencil,
                                                    bool active, int t, T *buf0, T *buf1,
                                                    T (&u)[NQ])
 {
@@ -107,7 +85,7 @@ __device__ __forceinline__ void hex_bwd_to_columns(const Tab<NM, NQ, T> &tab, co
     if (active && t < NM * NM) {
         T uh[NM];
 #pragma unroll
-        for (int p = 0; p < NM; ++p) uh[p] = modes[t * (NM + 1) + p];
+        for (int p = 0; p < NM; ++p) uh[p] = pencil(p);
 #pragma unroll
         for (int i = 0; i < NQ; ++i) {
             T s = uh[0] * tab.B[i];
@@ -150,13 +128,13 @@ __device__ __forceinline__ void hex_bwd_to_columns(const Tab<NM, NQ, T> &tab, co
     }
 }
 
-// k-columns acc (registers of thread (i,j)) -> (B x B x B)^T acc as padded mode pencils in
-// `modes`.  Uses buf0 and buf1; `modes` may alias buf0.  The caller synchronises before
-// reading `modes`.
-template <int NM, int NQ, typename T>
+// k-columns acc (registers of thread (i,j)) -> (B x B x B)^T acc; emit(p, v) receives the
+// p-th coefficient of this thread's p-pencil (t = r*NM + q).  Uses buf0 and buf1; emit may
+// write to buf0.
+template <int NM, int NQ, typename T, typename EmitFn>
 __device__ __forceinline__ void hex_columns_to_modes(const Tab<NM, NQ, T> &tab,
                                                      const T (&acc)[NQ], bool active, int t,
-                                                     T *buf0, T *buf1, T *modes)
+                                                     T *buf0, T *buf1, EmitFn emit)
 {
     // k -> r in registers
     if (active) {
@@ -194,7 +172,7 @@ __device__ __forceinline__ void hex_columns_to_modes(const Tab<NM, NQ, T> &tab,
             T s = v[0] * tab.B[p * NQ];
 #pragma unroll
             for (int i = 1; i < NQ; ++i) s = fma(v[i], tab.B[p * NQ + i], s);
-            modes[t * (NM + 1) + p] = s;
+            emit(p, s);
         }
     }
 }
@@ -204,29 +182,39 @@ __device__ __forceinline__ void hex_columns_to_modes(const Tab<NM, NQ, T> &tab,
 // ---------------------------------------------------------------------------
 
 // BwdTrans (DIR = kForward, GEO ignored) and the Euclidean transpose of IProduct
-// (DIR = kTranspose: out = diag(w J) B g, GEO = kDeformed or kRegular).
+// (DIR = kTranspose: out = diag(w J) B g, GEO = kDeformed or kRegular).  These light
+// kernels read and write global memory directly from the pencil owners (staging their
+// spans through LDS costs occupancy and is slower here).
 template <int NM, int NQ, typename T, int DIR, int GEO>
 __global__ __launch_bounds__(HexCfg<NQ>::BLOCK) void hex_bwdtrans_kernel(
     const Tab<NM, NQ, T> tab, int nElmt, const T *__restrict__ in,
     const T *__restrict__ geom, T *__restrict__ out)
 {
     using C = HexCfg<NQ>;
-    using Lb = HexLds<NM, NQ>;
-    constexpr int M = Lb::M, Q = Lb::Q;
-    __shared__ T lds[C::EPB][3][Lb::QB];
+    constexpr int M = NM * NM * NM, Q = NQ * NQ * NQ;
+    __shared__ T lds[C::EPB][2][Q];
     const int eb = threadIdx.x / C::TPE;
     const int t = threadIdx.x - eb * C::TPE;
     const bool active = eb < C::EPB;
     const int es = active ? eb : 0;
-    const long e0 = (long)blockIdx.x * C::EPB, e = e0 + eb;
-    const int nv = (int)min((long)C::EPB, (long)nElmt - e0);
-    const bool valid = active && eb < nv;
+    const long e = (long)blockIdx.x * C::EPB + eb;
+    const bool valid = active && e < nElmt;
 
-    load_modes_block<NM, NQ, T>(in + e0 * M, nv, [&](int sl) { return lds[sl][2]; });
-    __syncthreads();
+    // this thread's p-pencil, loaded before the contraction (as one unconditional block)
+    T uh[NM];
+    if (valid && t < NM * NM) {
+        const T *src = in + e * M + t * NM;
+#pragma unroll
+        for (int p = 0; p < NM; ++p) uh[p] = ld_stream(src + p);
+    } else {
+#pragma unroll
+        for (int p = 0; p < NM; ++p) uh[p] = T(0);
+    }
     T u[NQ];
-    hex_bwd_to_columns<NM, NQ, T>(tab, lds[es][2], active, t, lds[es][0], lds[es][1], u);
-    if (valid && DIR == kTranspose) {
+    hex_bwd_to_columns<NM, NQ, T>(tab, [&](int p) { return uh[p]; }, active, t, lds[es][0],
+                                  lds[es][1], u);
+    if (!valid) return;
+    if (DIR == kTranspose) {
         if (GEO == kDeformed) {
 #pragma unroll
             for (int k = 0; k < NQ; ++k) u[k] *= ld_stream(geom + e * Q + k * C::TPE + t);
@@ -236,13 +224,8 @@ __global__ __launch_bounds__(HexCfg<NQ>::BLOCK) void hex_bwdtrans_kernel(
             for (int k = 0; k < NQ; ++k) u[k] *= wij * tab.w[k];
         }
     }
-    // lds[.][0] was last read before the final barrier of hex_bwd_to_columns
-    if (active) {
 #pragma unroll
-        for (int k = 0; k < NQ; ++k) lds[es][0][k * C::TPE + t] = u[k];
-    }
-    __syncthreads();
-    store_points_block<NQ, T>(out + e0 * Q, nv, [&](int sl) { return lds[sl][0]; });
+    for (int k = 0; k < NQ; ++k) st_stream(out + e * Q + k * C::TPE + t, u[k]);
 }
 
 // IProductWRTBase: out = (B x B x B)^T diag(w J) f.  GEO = kUnit gives the Euclidean
@@ -253,43 +236,41 @@ __global__ __launch_bounds__(HexCfg<NQ>::BLOCK) void hex_iproduct_kernel(
     const T *__restrict__ geom, T *__restrict__ out)
 {
     using C = HexCfg<NQ>;
-    using Lb = HexLds<NM, NQ>;
-    constexpr int M = Lb::M, Q = Lb::Q;
-    __shared__ T lds[C::EPB][2][Lb::QB];
+    constexpr int M = NM * NM * NM, Q = NQ * NQ * NQ;
+    __shared__ T lds[C::EPB][2][Q];
     const int eb = threadIdx.x / C::TPE;
     const int t = threadIdx.x - eb * C::TPE;
     const bool active = eb < C::EPB;
     const int es = active ? eb : 0;
-    const long e0 = (long)blockIdx.x * C::EPB, e = e0 + eb;
-    const int nv = (int)min((long)C::EPB, (long)nElmt - e0);
-    const bool valid = active && eb < nv;
+    const long e = (long)blockIdx.x * C::EPB + eb;
+    const bool valid = active && e < nElmt;
 
-    load_points_block<NQ, T>(in + e0 * Q, nv, [&](int sl) { return lds[sl][1]; });
-    if (GEO == kDeformed)
-        load_points_block<NQ, T>(geom + e0 * Q, nv, [&](int sl) { return lds[sl][0]; });
-    __syncthreads();
     T acc[NQ];
     if (valid) {
         if (GEO == kDeformed) {
 #pragma unroll
-            for (int k = 0; k < NQ; ++k)
-                acc[k] = lds[es][1][k * C::TPE + t] * lds[es][0][k * C::TPE + t];
+            for (int k = 0; k < NQ; ++k) {
+                const int q = k * C::TPE + t;
+                acc[k] = ld_stream(in + e * Q + q) * ld_stream(geom + e * Q + q);
+            }
         } else if (GEO == kRegular) {
             const T wij = tab.w[t % NQ] * tab.w[t / NQ] * geom[e];
 #pragma unroll
-            for (int k = 0; k < NQ; ++k) acc[k] = lds[es][1][k * C::TPE + t] * (wij * tab.w[k]);
+            for (int k = 0; k < NQ; ++k)
+                acc[k] = ld_stream(in + e * Q + k * C::TPE + t) * (wij * tab.w[k]);
         } else {
 #pragma unroll
-            for (int k = 0; k < NQ; ++k) acc[k] = lds[es][1][k * C::TPE + t];
+            for (int k = 0; k < NQ; ++k) acc[k] = ld_stream(in + e * Q + k * C::TPE + t);
         }
     } else {
 #pragma unroll
         for (int k = 0; k < NQ; ++k) acc[k] = T(0);
     }
-    __syncthreads();
-    hex_columns_to_modes<NM, NQ, T>(tab, acc, active, t, lds[es][0], lds[es][1], lds[es][0]);
-    __syncthreads();
-    store_modes_block<NM, NQ, T>(out + e0 * M, nv, [&](int sl) { return lds[sl][0]; });
+    T *dst = out + e * M + t * NM;
+    hex_columns_to_modes<NM, NQ, T>(
+        tab, acc, active, t, lds[es][0], lds[es][1], [&](int p, T v) {
+            if (valid) st_stream(dst + p, v);
+        });
 }
 
 // Helmholtz: out = lambda M u_hat + sum_ab (D_a B)^T diag(w J G^{ab}) (D_b B) u_hat.
@@ -320,7 +301,8 @@ __global__ __launch_bounds__(HexCfg<NQ>::BLOCK) void hex_helmholtz_kernel(
     load_modes_block<NM, NQ, T>(in + e0 * M, nv, [&](int sl) { return lds[sl][2]; });
     __syncthreads();
     T u[NQ];
-    hex_bwd_to_columns<NM, NQ, T>(tab, buf2, active, t, buf0, buf1, u);
+    hex_bwd_to_columns<NM, NQ, T>(
+        tab, [&](int p) { return buf2[t * (NM + 1) + p]; }, active, t, buf0, buf1, u);
     // buf2 (the mode pencils) was last read before the first barrier above
     if (active) {
 #pragma unroll
@@ -452,7 +434,8 @@ __global__ __launch_bounds__(HexCfg<NQ>::BLOCK) void hex_helmholtz_kernel(
     }
 
     // 5. (B x B x B)^T of the combined quadrature-space field, then one coalesced store.
-    hex_columns_to_modes<NM, NQ, T>(tab, acc, active, t, buf2, buf3, buf2);
+    hex_columns_to_modes<NM, NQ, T>(tab, acc, active, t, buf2, buf3,
+                                    [&](int p, T v) { buf2[t * (NM + 1) + p] = v; });
     __syncthreads();
     store_modes_block<NM, NQ, T>(out + e0 * M, nv, [&](int sl) { return lds[sl][2]; });
 }

@@ -167,3 +167,48 @@ def affine_batch(n_inst=4096, seed=3, nmin=16, nmax=256, qmin=3, qmax=8):
             c += k
     return dict(N=N.astype(np.int32), Q=Q.astype(np.int32), A=A, theta=theta, b=b,
                 a_off=a_off, t_off=t_off, v_off=v_off)
+
+
+def affine_batch_device(n_inst=4096, seed=3, nmin=16, nmax=256, qmin=3, qmax=8,
+                        device="cuda"):
+    """The distribution of ``affine_batch`` generated on the GPU (for large batches).
+
+    N, Q, theta and b are drawn exactly as in ``affine_batch`` (numpy PCG64, same seed);
+    the matrices come from torch's Philox generator seeded with ``seed`` on the device, so
+    their values differ from the host generator's.  Returns the same dict with ``A`` a cuda
+    float64 tensor and ``theta``, ``b`` as host arrays."""
+    rng = np.random.default_rng(seed)
+    Q = rng.integers(qmin, qmax + 1, size=n_inst)
+    N = rng.integers(nmin, nmax + 1, size=n_inst)
+    a_off = np.zeros(n_inst, dtype=np.int64)
+    t_off = np.zeros(n_inst, dtype=np.int64)
+    v_off = np.zeros(n_inst, dtype=np.int64)
+    a_off[1:] = np.cumsum(Q * N * N)[:-1]
+    t_off[1:] = np.cumsum(Q)[:-1]
+    v_off[1:] = np.cumsum(N)[:-1]
+    theta = rng.uniform(0.1, 1.0, size=int(Q.sum()))
+    b = rng.standard_normal(int(N.sum()))
+    gen = torch.Generator(device=device).manual_seed(seed)
+    A = torch.empty(int(np.sum(Q * N * N)), dtype=torch.float64, device=device)
+    for n in np.unique(N):
+        n = int(n)
+        idx = np.nonzero(N == n)[0]
+        G = torch.randn((len(idx), n, n), generator=gen, dtype=torch.float64, device=device)
+        U, R = torch.linalg.qr(G)
+        U = U * torch.sign(torch.diagonal(R, dim1=1, dim2=2))[:, None, :]
+        lam = 10.0 ** (4.0 * torch.rand((len(idx), n), generator=gen, dtype=torch.float64,
+                                        device=device))
+        A0 = (U * lam[:, None, :]) @ U.transpose(1, 2).contiguous()
+        npert = int(np.sum(Q[idx] - 1))
+        P = torch.randn((npert, n, n), generator=gen, dtype=torch.float64, device=device)
+        P = (0.05 / np.sqrt(2.0 * n)) * (P + P.transpose(1, 2))
+        c = 0
+        for r, i in enumerate(idx):
+            base = int(a_off[i])
+            A[base:base + n * n] = A0[r].reshape(-1)
+            k = int(Q[i]) - 1
+            A[base + n * n: base + (k + 1) * n * n] = P[c:c + k].reshape(-1)
+            c += k
+    return dict(N=N.astype(np.int32), Q=Q.astype(np.int32), A=A, theta=theta, b=b,
+                a_off=a_off, t_off=t_off, v_off=v_off)
+

@@ -69,6 +69,17 @@ def solve_eigen(batch, idx):
     return x
 
 
+def eigen_colpivqr(A, b):
+    """x = A.colPivHouseholderQr().solve(b) with the reference's Eigen (row-major A)."""
+    L = eigen_lib()
+    dp = ctypes.POINTER(ctypes.c_double)
+    A = np.ascontiguousarray(A)
+    b = np.ascontiguousarray(b)
+    x = np.zeros(len(b))
+    L.ref_colpivqr_solve(len(b), A.ctypes.data_as(dp), b.ctypes.data_as(dp), x.ctypes.data_as(dp))
+    return x
+
+
 def param_grad(batch, lam, x):
     g = np.zeros(int(np.sum(batch["Q"])))
     for i in range(len(batch["N"])):

@@ -38,6 +38,8 @@ def parse():
     p.add_argument("--cpu-seconds", type=float, default=12.0)
     p.add_argument("--no-rom", action="store_true", help="skip the batched reduced solves")
     p.add_argument("--rom-instances", type=int, default=4096)
+    p.add_argument("--no-pod", action="store_true", help="skip the full-size POD offline stage")
+    p.add_argument("--pod-snapshots", type=int, default=512)
     return p.parse_args()
 
 
@@ -71,11 +73,38 @@ def cpu_baseline(w, seconds):
         O.helmholtz("hex", b, E, True, jac, df, w["lam"], u)
         times.append(time.perf_counter() - t0)
     t = float(np.median(times))
+    # one core on a smaller contiguous sample (the reference runs one MPI rank per core)
+    O.set_threads(1)
+    E1 = 1024
+    u1 = np.ascontiguousarray(u[:E1 * nm ** 3])
+    jac1 = np.ascontiguousarray(jac[:E1 * nq ** 3])
+    df1 = np.ascontiguousarray(df.reshape(9, -1)[:, :E1 * nq ** 3])
+    O.helmholtz("hex", b, E1, True, jac1, df1, w["lam"], u1)
+    t1s = []
+    for _ in range(5):
+        t0 = time.perf_counter()
+        O.helmholtz("hex", b, E1, True, jac1, df1, w["lam"], u1)
+        t1s.append(time.perf_counter() - t0)
+    O.set_threads(threads)
     return {"value": E * nm ** 3 / t / 1e9, "unit": "GDOF/s", "cores": threads,
             "kind": "port",
             "sample": f"{E} of {w['nElmt']} elements, {len(times)} applies, median; "
                       "oracle restatement of HelmholtzHexImpl (scalar element loop, "
-                      f"width 1), OpenMP over contiguous element chunks, {threads} threads"}
+                      f"width 1), OpenMP over contiguous element chunks, {threads} threads",
+            "single_core_value": E1 * nm ** 3 / float(np.median(t1s)) / 1e9,
+            "single_core_sample": f"{E1} elements, 5 applies, median",
+            "cpu_model": _cpu_model(), "host_cpus": os.cpu_count()}
+
+
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def _event_time(fn, steps, warmup):
@@ -179,6 +208,196 @@ def rom_cpu_baseline(bt, xs):
                          if use_eigen else "LAPACK LU (numpy)"),
             "max_rel_err_vs_reference": errs}
 
+
+def pod_bench(args):
+    """POD offline stage on the 48^3 deformed hex space (N = 23 887 872 coefficients):
+    512 snapshots, each sum_k a_k prod_i sin(m_{k,i} pi x_i) over 20 fields (integer
+    wavenumbers U{1..6}, a_k ~ N(0,1) 2^-k, seed 2), the fields projected to coefficients
+    by the elemental L2 projection (preconditioned CG, converged to 1e-13), plus N(0,1)*1e-6
+    noise added in coefficient space (torch Philox, seed 2).  Then M S (matrix-free mass),
+    G = S^T M S in FP64 / FP32x3 / BF16x3 on MFMA, the 512^2 FP64 eigendecomposition,
+    truncation at retained energy 1 - 1e-8, the basis V, and the Galerkin projection
+    V^T H V of the Helmholtz operator (lambda = 10)."""
+    import numpy as np
+    import torch
+
+    import ithaca_hip as H
+    from ithaca_hip import mesh, pod
+
+    dev = torch.device("cuda:0")
+    torch.cuda.empty_cache()
+    w = mesh.workload_hex_helmholtz(args.n)
+    nm, nq, E = w["nm"], w["nq"], w["nElmt"]
+    M, Q = nm ** 3, nq ** 3
+    N = E * M
+    k_snap = args.pod_snapshots
+    b = H.basis_modified_a(nm, nq)
+    verts = torch.from_numpy(w["verts"]).to(dev)
+    jac, df = H.geom_device("hex", verts, nq, b["z"], True)
+    proj = pod.ElementalProjector(E, nm, nq, b, jac)
+    res = {"workload": f"{E} deformed hexes (P=5, 7 GLL pts), N={N}, {k_snap} snapshots of "
+                       "20 sine-product fields (seed 2) + 1e-6 noise",
+           "projection": "elemental L2 (HexExp::v_FwdTrans) by CG preconditioned with the "
+                         "constant-Jacobian inverse mass, tol 1e-13; noise added in "
+                         "coefficient space"}
+
+    # 1. the 20 fields, projected
+    m, a = mesh.pod_fields(20, k_snap, seed=2)
+    X = mesh.hex_quad_points(verts, b["z"])
+    Phi = torch.empty((20, N), dtype=torch.float64, device=dev)
+    its, ress = [], []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(20):
+        f = (torch.sin(float(m[k, 0]) * np.pi * X[..., 0])
+             * torch.sin(float(m[k, 1]) * np.pi * X[..., 1])
+             * torch.sin(float(m[k, 2]) * np.pi * X[..., 2])).reshape(-1)
+        Phi[k], it, rr = proj.project(f)
+        its.append(it)
+        ress.append(rr)
+    torch.cuda.synchronize()
+    res["projection_s"] = time.perf_counter() - t0
+    res["projection_cg_iters_max"] = int(max(its))
+    res["projection_rel_residual_max"] = float(max(ress))
+    del X, f
+
+    # 2. snapshots S (k_snap, N) = a Phi + noise
+    S = torch.empty((k_snap, N), dtype=torch.float64, device=dev)
+    torch.matmul(torch.from_numpy(a).to(dev), Phi, out=S)
+    gen = torch.Generator(device=dev).manual_seed(2)
+    for c0 in range(0, k_snap, 16):
+        c1 = min(k_snap, c0 + 16)
+        S[c0:c1].add_(torch.randn((c1 - c0, N), generator=gen, dtype=torch.float64,
+                                  device=dev), alpha=1e-6)
+    del Phi
+
+    # 3. M S: composed (BwdTrans then IProduct per snapshot, the reference's
+    # MassMatrixOp_MatFree) and fused (Mass_Hex_Deformed, 64 snapshots per launch,
+    # geometry read once per launch); the fused result feeds the Gram
+    MS = torch.empty_like(S)
+    ms_comp = _event_time(lambda: pod.mass_apply(S, proj.bwd, proj.ipr, out=MS,
+                                                 phys=proj.phys), 1, 0)
+    MS_comp_row = MS[k_snap // 2].clone()
+    ms_fused = _event_time(lambda: pod.mass_apply_fused(S, proj.mass_op, out=MS), 2, 1)
+    comp_bytes = k_snap * E * 8 * (2 * M + 3 * Q)
+    fused_bytes = E * 8 * (2 * M * k_snap + Q * ((k_snap + 63) // 64))
+    res["mass_apply"] = {
+        "composed": {"ms_total": ms_comp, "algorithmic_GBs": comp_bytes / (ms_comp * 1e-3) / 1e9,
+                     "bytes_per_snapshot_per_element": 8 * (2 * M + 3 * Q)},
+        "fused": {"ms_total": ms_fused, "algorithmic_GBs": fused_bytes / (ms_fused * 1e-3) / 1e9,
+                  "bytes_per_snapshot_per_element": 8 * 2 * M,
+                  "geometry_bytes_per_launch_per_element": 8 * Q, "snapshots_per_launch": 64},
+        "fused_vs_composed_rel_diff": (torch.linalg.vector_norm(MS[k_snap // 2] - MS_comp_row)
+                                       / torch.linalg.vector_norm(MS_comp_row)).item()}
+    del MS_comp_row
+
+    # 4. Gram in three precisions
+    work = torch.empty((pod._lib().irom_gram_workspace_size(k_snap, k_snap, N) + 7) // 8,
+                       dtype=torch.float64, device=dev)
+    G = {}
+    flops = 2.0 * k_snap * k_snap * N
+    for mode, name in ((pod.GRAM_FP64, "fp64"), (pod.GRAM_FP32X3, "fp32x3"),
+                       (pod.GRAM_BF16X3, "bf16x3")):
+        out = torch.empty((k_snap, k_snap), dtype=torch.float64, device=dev)
+        ms = _event_time(lambda: pod.gram(S, MS, mode=mode, out=out, work=work), 2, 1)
+        G[name] = out
+        res[f"gram_{name}"] = {"ms": ms, "TFLOPs": flops / (ms * 1e-3) / 1e12}
+    gn = torch.linalg.matrix_norm(G["fp64"]).item()
+    for name in ("fp32x3", "bf16x3"):
+        res[f"gram_{name}"]["rel_err_vs_fp64"] = (
+            torch.linalg.matrix_norm(G[name] - G["fp64"]).item() / gn)
+    res["gram_fp64"]["mfma_frac_of_78.6TF"] = res["gram_fp64"]["TFLOPs"] / 78.6
+
+    # 5. eigendecomposition (FP64 Jacobi on the GPU), truncation, basis
+    t0 = time.perf_counter()
+    Gs = 0.5 * (G["fp64"] + G["fp64"].T)
+    ev, Psi, sweeps = pod.eig_sym(Gs)
+    torch.cuda.synchronize()
+    res["eig_ms"] = (time.perf_counter() - t0) * 1e3
+    res["eig_sweeps"] = sweeps
+    r = pod.energy_truncation(ev, 1.0 - 1e-8)
+    res["rank_energy_1e-8"] = r
+    C = Psi[:r] / torch.sqrt(torch.as_tensor(ev[:r], device=dev))[:, None]
+    for name in ("fp32x3", "bf16x3"):
+        ev_m, Psi_m, _ = pod.eig_sym(0.5 * (G[name] + G[name].T))
+        C_m = Psi_m[:r] / torch.sqrt(torch.as_tensor(ev_m[:r], device=dev))[:, None]
+        sgn = torch.sign((C_m * C).sum(1, keepdim=True))
+        Dm = C_m * sgn - C
+        # ||V_m,i - V_i||_M = sqrt(d_i G d_i^T), ||V_i||_M = 1
+        verr = torch.sqrt(torch.clamp(((Dm @ Gs) * Dm).sum(1), min=0.0)).max().item()
+        res[f"gram_{name}"]["eig_rel_err_top_r"] = float(
+            np.max(np.abs(ev_m[:r] - ev[:r]) / ev[:r]))
+        res[f"gram_{name}"]["basis_max_M_norm_err"] = verr
+    V = C @ S
+    cpu_slice = None
+    if not args.no_cpu:
+        Ks = 1024 * M
+        cpu_slice = (S[:, :Ks].cpu().numpy(), MS[:, :Ks].cpu().numpy(),
+                     pod.gram(S[:, :Ks], MS[:, :Ks], work=work).cpu().numpy())
+    del MS
+    MV = torch.empty_like(V)
+    pod.mass_apply_fused(V, proj.mass_op, out=MV)
+    O = pod.gram(V, MV)
+    res["basis_M_orthonormality_err"] = (O - torch.eye(r, dtype=torch.float64,
+                                                       device=dev)).abs().max().item()
+    # 6. Galerkin projection of the Helmholtz operator: A_r = V^T H V
+    Hop = H.Operator("Helmholtz_Hex_Deformed", nm, nq, E, basis=b)
+    Hop.set_jac(jac)
+    Hop.set_df(df)
+    Hop.set_lambda(w["lam"])
+    HV = MV
+
+    def project():
+        for i in range(r):
+            Hop.apply(V[i], HV[i])
+        return pod.gram(V, HV)
+    ms_proj = _event_time(project, 2, 1)
+    Ar = project()
+    res["galerkin_projection"] = {
+        "ms": ms_proj, "r": r,
+        "symmetry_rel_err": (torch.linalg.matrix_norm(Ar - Ar.T)
+                             / torch.linalg.matrix_norm(Ar)).item()}
+    for name, mode in (("fp32x3", pod.GRAM_FP32X3), ("bf16x3", pod.GRAM_BF16X3)):
+        Am = pod.gram(V, HV, mode=mode)
+        res["galerkin_projection"][f"{name}_rel_err_vs_fp64"] = (
+            torch.linalg.matrix_norm(Am - Ar) / torch.linalg.matrix_norm(Ar)).item()
+
+    if not args.no_cpu:
+        res["cpu_baseline"] = pod_cpu_baseline(cpu_slice, G["fp64"], ev, r)
+    del S, V, MV, HV, Hop, proj, work, G
+    torch.cuda.empty_cache()
+    return res
+
+
+def pod_cpu_baseline(cpu_slice, G64, ev, r):
+    """FP64 CPU oracle of the reduced quantities within a bounded CPU budget: the numpy
+    (OpenBLAS) Gram of a K-slice of S and M S (the first 1024 elements' coefficients; M is
+    block diagonal, so the slice of M S is exact), compared with the GPU FP64 Gram kernel on
+    the same slice, and numpy eigh of the full GPU Gram compared with the GPU eigenvalues."""
+    import numpy as np
+    from threadpoolctl import threadpool_limits
+
+    S_h, MS_h, G_gpu = cpu_slice
+    threads = min(16, os.cpu_count() or 1)
+    with threadpool_limits(limits=threads):
+        S_h @ MS_h[:8].T  # warm-up
+        t0 = time.perf_counter()
+        G_cpu = S_h @ MS_h.T
+        t_gram = time.perf_counter() - t0
+        Gh = G64.cpu().numpy()
+        t0 = time.perf_counter()
+        ev_cpu = np.linalg.eigh(0.5 * (Gh + Gh.T))[0][::-1]
+        t_eig = time.perf_counter() - t0
+    flops = 2.0 * S_h.shape[0] ** 2 * S_h.shape[1]
+    return {"value": flops / t_gram / 1e9, "unit": "GFLOP/s (FP64 Gram)", "cores": threads,
+            "kind": "port",
+            "sample": f"Gram of a {S_h.shape[0]} x {S_h.shape[1]} K-slice (numpy/OpenBLAS), "
+                      "eigh of the full 512^2 Gram (LAPACK)",
+            "gram_slice_s": t_gram, "eigh_ms": t_eig * 1e3,
+            "gram_slice_rel_err_gpu_vs_cpu": float(np.linalg.norm(G_gpu - G_cpu)
+                                                   / np.linalg.norm(G_cpu)),
+            "eig_rel_err_top_r_gpu_vs_cpu": float(np.max(np.abs(ev[:r] - ev_cpu[:r])
+                                                         / ev_cpu[:r]))}
 
 def quad_bench(args):
     """32x32 regular quads, P=4 (5 modes), 6 GLL points, lambda=1, seed 0, 100 applies."""
@@ -335,6 +554,11 @@ def main():
         if not args.no_rom:
             out["batched_reduced_solves"] = rom_bench(args)
         out["quad_helmholtz"] = quad_bench(args)
+        if not args.no_pod:
+            try:
+                out["pod_offline"] = pod_bench(args)
+            except Exception as exc:  # noqa: BLE001 -- reported, the main line stands
+                out["pod_offline"] = {"error": f"{type(exc).__name__}: {exc}"}
     print(json.dumps(out))
 
 

@@ -45,6 +45,11 @@ enum imf_precision { IMF_FP64 = 0, IMF_FP32 = 1 };
  * reference kernels the operator requires homogeneous modes/points and supports
  * nm in 2..8 with nq in nm..2*nm (Helmholtz.h:669-760).  No padding restriction on
  * nElmt (the reference's vec_t::width padding is handled internally). */
+/* Also accepted, with no MatrixFreeOps counterpart: "Mass_Hex_Deformed" and
+ * "Mass_Hex_Regular", the elemental mass action out = B^T diag(w J) B in of
+ * StdRegions/StdExpansion.cpp:647-661 (MassMatrixOp_MatFree: BwdTrans, then
+ * IProductWRTBase) fused into one kernel; needs SetJac only; in and out are
+ * coefficient arrays (nElmt*M). */
 int imf_create(const char *key, int dim, const int *nm, const int *nq,
                const double *const *bdata, const double *const *dbdata,
                const double *const *D, const double *const *w, int nElmt,
@@ -88,8 +93,16 @@ int imf_apply_device(imf_op op, const void *d_in, void *d_out, void *stream);
  * reference counterpart (used by the adjoint/autograd layer):
  *   BwdTrans^T:  f (nElmt*Q) -> B^T f            (IProduct with unit weights, J = 1)
  *   IProduct^T:  g (nElmt*M) -> diag(w J) B g    (BwdTrans followed by w*J scaling)
- *   Helmholtz^T: identical to Helmholtz (the operator is symmetric). */
+ *   Helmholtz^T, Mass^T: identical to the operator (both are symmetric). */
 int imf_apply_transpose_device(imf_op op, const void *d_in, void *d_out, void *stream);
+
+/* operator()(in_v, out_v) for v < nvec vectors at d_in + v*ld_in and d_out + v*ld_out
+ * (ld in scalars of the operator's precision, ld_in >= imf_in_size, ld_out >=
+ * imf_out_size), device arrays, asynchronous, allocates nothing.  No reference
+ * counterpart (the reference applies one field at a time); the Mass operator does all
+ * vectors in one launch, reading the geometry once; other operators loop over vectors. */
+int imf_apply_many_device(imf_op op, int nvec, const void *d_in, long ld_in, void *d_out,
+                          long ld_out, void *stream);
 
 /* Message of the last failing call on this thread ("" if none). */
 const char *imf_last_error(void);

@@ -14,7 +14,7 @@
 namespace imf {
 
 // Operator kinds and element shapes handled by the library.
-enum Kind : int { kBwdTrans = 0, kIProduct = 1, kHelmholtz = 2 };
+enum Kind : int { kBwdTrans = 0, kIProduct = 1, kHelmholtz = 2, kMass = 3 };
 enum Shape : int { kQuad = 2, kHex = 3 };
 
 // Geometry modes.  kUnit: no geometry (the Euclidean transposes).  kDeformed:
@@ -57,7 +57,7 @@ struct QuadCfg {
 // Number of packed geometry components per point (deformed) or per element (regular).
 //   Helmholtz hex : Jw, Jw*G00, Jw*G01, Jw*G02, Jw*G11, Jw*G12, Jw*G22
 //   Helmholtz quad: Jw, Jw*G00, Jw*G01, Jw*G11
-//   IProduct      : Jw
+//   IProduct, Mass: Jw
 // with G^{ab} = sum_c df[c*dim+a] df[c*dim+b] (Helmholtz.h:858-966, :172-190) and
 // Jw = J w_i w_j (w_k).  Regular elements store J instead of Jw (the kernel applies w).
 constexpr int geo_comps(int kind, int shape)
@@ -82,6 +82,8 @@ struct LaunchArgs {
     const void *in;    // device
     void *out;         // device
     hipStream_t stream;
+    int nvec = 1;      // vectors per launch (Mass), vector v at in + v*ldin, out + v*ldout
+    long ldin = 0, ldout = 0;
 };
 using LaunchFn = hipError_t (*)(const LaunchArgs &);
 

@@ -115,14 +115,15 @@ int check_op(const imf_op op)
     return IMF_OK;
 }
 
-int run(imf_op op, imf::LaunchFn fn, const void *in, void *out, void *stream)
+int run(imf_op op, imf::LaunchFn fn, const void *in, void *out, void *stream, int nvec = 1,
+        long ldin = 0, long ldout = 0)
 {
     if (needs_jac(op) && !op->geom_ready)
         return fail(IMF_ERR_STATE, needs_df(op) ? "Jacobian and derivative factors not set"
                                                 : "Jacobian not set");
     if (!in || !out) return fail(IMF_ERR_ARG, "null input/output array");
     imf::LaunchArgs a{op->tab.data(), op->nElmt, op->lambda, op->d_geom, in, out,
-                      (hipStream_t)stream};
+                      (hipStream_t)stream, nvec, ldin, ldout};
     hipError_t e = fn(a);
     if (e != hipSuccess) return hip_fail(e, "kernel launch");
     return IMF_OK;
@@ -149,6 +150,7 @@ int imf_create(const char *key, int dim, const int *nm, const int *nq,
     if (starts("BwdTrans_")) kind = imf::kBwdTrans;
     else if (starts("IProduct_")) kind = imf::kIProduct;
     else if (starts("Helmholtz_")) kind = imf::kHelmholtz;
+    else if (starts("Mass_")) kind = imf::kMass;
     else return fail(IMF_ERR_UNSUPPORTED, "No such operator key: " + k);
     const std::string rest = k.substr(k.find('_') + 1);
     if (rest == "Quad_Regular") { shape = imf::kQuad; deformed = 0; }
@@ -158,6 +160,8 @@ int imf_create(const char *key, int dim, const int *nm, const int *nq,
     else return fail(IMF_ERR_UNSUPPORTED, "No such operator key: " + k);
     // The reference registers BwdTrans only as _Regular (MatrixFreeOps/BwdTrans.cpp:7-33).
     if (kind == imf::kBwdTrans && deformed)
+        return fail(IMF_ERR_UNSUPPORTED, "No such operator key: " + k);
+    if (kind == imf::kMass && shape != imf::kHex)
         return fail(IMF_ERR_UNSUPPORTED, "No such operator key: " + k);
     if (dim != shape)
         return fail(IMF_ERR_ARG, "imf_create: dim does not match the shape of " + k);
@@ -176,7 +180,8 @@ int imf_create(const char *key, int dim, const int *nm, const int *nq,
     const int m = nm[0], q = nq[0];
     if (m < 2 || m > 8 || q < m || q > 2 * m)
         return fail(IMF_ERR_UNSUPPORTED, std::string(kind == imf::kHelmholtz ? "Helmholtz" :
-                                                     kind == imf::kIProduct ? "IProduct" : "BwdTrans") +
+                                                     kind == imf::kIProduct ? "IProduct" :
+                                                     kind == imf::kMass ? "Mass" : "BwdTrans") +
                                              (shape == imf::kHex ? "Hex" : "Quad") +
                                              ": # of modes / points combo not initialised.");
     imf_op op = new imf_op_s();
@@ -197,6 +202,7 @@ int imf_create(const char *key, int dim, const int *nm, const int *nq,
         op->fwd = imf::lookup_quad(m, kind, imf::kForward, geo, q, precision);
         op->tr = imf::lookup_quad(m, kind, imf::kTranspose, geo_t, q, precision);
     }
+    if (kind == imf::kMass) op->tr = op->fwd;  // symmetric
     if (!op->fwd) {
         delete op;
         return fail(IMF_ERR_UNSUPPORTED, k + ": # of modes / points combo not initialised.");
@@ -278,8 +284,27 @@ int imf_apply_transpose_device(imf_op op, const void *d_in, void *d_out, void *s
 {
     if (int s = check_op(op)) return s;
     if (!op->tr) return fail(IMF_ERR_UNSUPPORTED, "transpose not available");
-    if (op->kind == imf::kHelmholtz) return run(op, op->fwd, d_in, d_out, stream);
+    if (op->kind == imf::kHelmholtz || op->kind == imf::kMass)
+        return run(op, op->fwd, d_in, d_out, stream);
     return run(op, op->tr, d_in, d_out, stream);
+}
+
+int imf_apply_many_device(imf_op op, int nvec, const void *d_in, long ld_in, void *d_out,
+                          long ld_out, void *stream)
+{
+    if (int s = check_op(op)) return s;
+    if (nvec < 0) return fail(IMF_ERR_ARG, "imf_apply_many_device: nvec < 0");
+    if (nvec == 0) return IMF_OK;
+    if (ld_in < (long)imf_in_size(op) || ld_out < (long)imf_out_size(op))
+        return fail(IMF_ERR_ARG, "imf_apply_many_device: leading dimension smaller than the "
+                                 "vector size");
+    if (op->kind == imf::kMass) return run(op, op->fwd, d_in, d_out, stream, nvec, ld_in, ld_out);
+    const size_t es = elt_size(op);
+    for (int v = 0; v < nvec; ++v)
+        if (int s = run(op, op->fwd, (const char *)d_in + (size_t)v * ld_in * es,
+                        (char *)d_out + (size_t)v * ld_out * es, stream))
+            return s;
+    return IMF_OK;
 }
 
 int imf_apply(imf_op op, const double *in, double *out)

@@ -273,6 +273,65 @@ __global__ __launch_bounds__(HexCfg<NQ>::BLOCK) void hex_iproduct_kernel(
         });
 }
 
+// Mass: out_v = (B x B x B)^T diag(w J) (B x B x B) in_v for nvec vectors, the fused form
+// of the reference's MassMatrixOp_MatFree (StdRegions/StdExpansion.cpp:647-661:
+// BwdTrans, then IProductWRTBase).  The thread's geometry column (NQ values of w J) is
+// read once and reused for every vector, so each further vector streams only its
+// coefficients in and out.
+template <int NM, int NQ, typename T, int GEO>
+__global__ __launch_bounds__(HexCfg<NQ>::BLOCK) void hex_mass_kernel(
+    const Tab<NM, NQ, T> tab, int nElmt, int nvec, const T *__restrict__ in, long ldin,
+    const T *__restrict__ geom, T *__restrict__ out, long ldout)
+{
+    using C = HexCfg<NQ>;
+    constexpr int M = NM * NM * NM, Q = NQ * NQ * NQ;
+    __shared__ T lds[C::EPB][2][Q];
+    const int eb = threadIdx.x / C::TPE;
+    const int t = threadIdx.x - eb * C::TPE;
+    const bool active = eb < C::EPB;
+    const int es = active ? eb : 0;
+    const long e = (long)blockIdx.x * C::EPB + eb;
+    const bool valid = active && e < nElmt;
+
+    T jw[NQ];
+    if (valid) {
+        if (GEO == kDeformed) {
+#pragma unroll
+            for (int k = 0; k < NQ; ++k) jw[k] = geom[e * Q + k * C::TPE + t];
+        } else {
+            const T wij = tab.w[t % NQ] * tab.w[t / NQ] * geom[e];
+#pragma unroll
+            for (int k = 0; k < NQ; ++k) jw[k] = wij * tab.w[k];
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < NQ; ++k) jw[k] = T(0);
+    }
+    // Buffer reuse across vectors: each phase writes a buffer only after a barrier that
+    // follows the previous vector's last read of it.
+    for (int v = 0; v < nvec; ++v) {
+        T uh[NM];
+        if (valid && t < NM * NM) {
+            const T *src = in + v * ldin + e * M + t * NM;
+#pragma unroll
+            for (int p = 0; p < NM; ++p) uh[p] = ld_stream(src + p);
+        } else {
+#pragma unroll
+            for (int p = 0; p < NM; ++p) uh[p] = T(0);
+        }
+        T u[NQ];
+        hex_bwd_to_columns<NM, NQ, T>(tab, [&](int p) { return uh[p]; }, active, t,
+                                      lds[es][0], lds[es][1], u);
+#pragma unroll
+        for (int k = 0; k < NQ; ++k) u[k] *= jw[k];
+        T *dst = out + v * ldout + e * M + t * NM;
+        hex_columns_to_modes<NM, NQ, T>(
+            tab, u, active, t, lds[es][0], lds[es][1], [&](int p, T x) {
+                if (valid) st_stream(dst + p, x);
+            });
+    }
+}
+
 // Helmholtz: out = lambda M u_hat + sum_ab (D_a B)^T diag(w J G^{ab}) (D_b B) u_hat.
 // Packed geometry: deformed [e][GS] with GS = 7Q rounded up to 128-byte lines and the
 // components (Jw, JwG00, JwG01, JwG02, JwG11, JwG12, JwG22) at c*Q; regular [e][7]

@@ -28,6 +28,9 @@ hipError_t launch_hex(const LaunchArgs &a)
     if constexpr (KIND == kHelmholtz) {
         hex_helmholtz_kernel<NM, NQ, T, GEO>
             <<<grid, block, 0, a.stream>>>(tab, a.nElmt, T(a.lambda), in, geom, out);
+    } else if constexpr (KIND == kMass) {
+        hex_mass_kernel<NM, NQ, T, GEO><<<grid, block, 0, a.stream>>>(
+            tab, a.nElmt, a.nvec, in, a.ldin, geom, out, a.ldout);
     } else if constexpr ((KIND == kBwdTrans && DIR == kForward) ||
                          (KIND == kIProduct && DIR == kTranspose)) {
         hex_bwdtrans_kernel<NM, NQ, T, DIR, GEO>
@@ -67,9 +70,17 @@ hipError_t launch_quad(const LaunchArgs &a)
 //   Helmholtz            -> helmholtz<geo>                (the operator is symmetric)
 //   BwdTrans  forward    -> bwdtrans<fwd>;  transpose -> iproduct<unit>
 //   IProduct  forward    -> iproduct<geo>;  transpose -> bwdtrans<transpose, geo>
+//   Mass (hex)           -> mass<geo>                     (the operator is symmetric)
 template <int SHAPE, int NM, int NQ, typename T>
 LaunchFn pick(int kind, int dir, int geo)
 {
+    if (kind == kMass) {
+        if constexpr (SHAPE == kHex) {
+            if (geo == kDeformed) return &launch_hex<NM, NQ, T, kMass, kForward, kDeformed>;
+            if (geo == kRegular) return &launch_hex<NM, NQ, T, kMass, kForward, kRegular>;
+        }
+        return nullptr;
+    }
 #define IMF_L(K, D, G)                                                        \
     (SHAPE == kHex ? (LaunchFn)&launch_hex<NM, NQ, T, K, D, G>                \
                    : (LaunchFn)&launch_quad<NM, NQ, T, K, D, G>)

@@ -53,36 +53,6 @@ struct Args {
     float *ws32;    // per workgroup NMAX*NMAX (FP32 factors, MIXED only)
 };
 
-// Block-wide argmax of v (ties -> smallest index).  Ends synchronised.
-__device__ __forceinline__ void block_argmax(double v, int ix, double *sv, int *si, double &bv,
-                                             int &bi)
-{
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        const double ov = __shfl_xor(v, off);
-        const int oi = __shfl_xor(ix, off);
-        if (ov > v || (ov == v && oi < ix)) {
-            v = ov;
-            ix = oi;
-        }
-    }
-    const int w = threadIdx.x >> 6;
-    if ((threadIdx.x & 63) == 0) {
-        sv[w] = v;
-        si[w] = ix;
-    }
-    __syncthreads();
-    bv = sv[0];
-    bi = si[0];
-#pragma unroll
-    for (int k = 1; k < NT / 64; ++k)
-        if (sv[k] > bv || (sv[k] == bv && si[k] < bi)) {
-            bv = sv[k];
-            bi = si[k];
-        }
-    __syncthreads();
-}
-
 __device__ __forceinline__ double block_max(double v, double *sv)
 {
 #pragma unroll
@@ -96,97 +66,230 @@ __device__ __forceinline__ double block_max(double v, double *sv)
     return r;
 }
 
-// In-place LU with partial pivoting of the n x n row-major matrix L (ld n).
-// ipiv[k] = row swapped with k at step k.  Returns through *zero a zero-pivot flag.
-template <typename T, int NB, int TM, int TN>
-__device__ void block_lu(T *__restrict__ L, int n, T *P, int *ipiv, double *sv, int *si,
-                         int *zero)
+// Orders LDS accesses of the lanes of one wave (no hardware barrier needed).
+__device__ __forceinline__ void wave_sync()
 {
-    constexpr int NBP = NB + 1;
-    const int tid = threadIdx.x;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Value of v in lane l of this wave (l uniform).
+__device__ __forceinline__ float read_lane(float v, int l)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, v), l));
+}
+__device__ __forceinline__ double read_lane(double v, int l)
+{
+    const long long b = __builtin_bit_cast(long long, v);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), l);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned int)lo);
+}
+
+// LDS column stride of the panel and of U12: reads of a 4-wide register tile may run up
+// to 3 entries past the last valid row / column (their products are discarded).
+constexpr int PLD = NMAX + 4;
+
+// Per-workgroup LDS bookkeeping of the row interchanges.
+struct PivLds {
+    int posP[NMAX];    // panel-start row whose content is at row r (rows of the panel step)
+    int posAll[NMAX];  // original row whose content is at row r (the final permutation)
+    int mark[NMAX];    // k0 of the panel step that put row r into list
+    int list[NMAX];    // rows below the panel's top block that receive a top row
+    int cnt;
+};
+
+// In-place LU with partial pivoting of the n x n row-major matrix L (ld n), right-looking
+// with panels of NB columns.  On return L holds the unit-lower and upper factors of
+// P A = L U with the rows permuted, pv.posAll[i] = row of A that became row i, and *zero
+// is set when a pivot is zero or NaN.  Per panel:
+//   1. the m x nb panel is copied to LDS (column-major, P) and factored by wave 0 alone:
+//      lane-per-row pivot search reduced by shuffles, interchanges and rank-1 updates
+//      ordered by wave barriers only; lane 0 records which rows the interchanges touch;
+//   2. every other column applies the panel's interchanges as one gather (all loads of
+//      a column issued before its stores); the columns right of the panel then solve
+//      U12 = L11^{-1} A12 in registers and write U12 to global and to LDS (U);
+//   3. A22 -= L21 U12 in TM x TN register tiles, both operands read from LDS, the next
+//      tile's global values loaded while the current one is computed.
+template <typename T, int NB, int TM, int TN>
+__device__ void block_lu(T *__restrict__ L, int n, T *P, T *U, PivLds &pv, int *zero)
+{
+    const int tid = threadIdx.x, lane = tid & 63;
+    for (int i = tid; i < n; i += NT) {
+        pv.posAll[i] = i;
+        pv.mark[i] = -1;
+    }
     for (int k0 = 0; k0 < n; k0 += NB) {
         const int nb = min(NB, n - k0), m = n - k0;
-        for (int idx = tid; idx < m * nb; idx += NT) {
-            const int r = idx / nb, c = idx - r * nb;
-            P[r * NBP + c] = L[(long)(k0 + r) * n + k0 + c];
+        // 1. panel to LDS (loads issued 8 at a time per thread)
+        for (int i = tid; i < m; i += NT) pv.posP[k0 + i] = k0 + i;
+        for (int base = tid; base < m * nb; base += 8 * NT) {
+            T v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int idx = base + u * NT;
+                const int r = idx / nb, c = idx - r * nb;
+                v[u] = idx < m * nb ? L[(long)(k0 + r) * n + k0 + c] : T(0);
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int idx = base + u * NT;
+                const int r = idx / nb, c = idx - r * nb;
+                if (idx < m * nb) P[c * PLD + r] = v[u];
+            }
         }
         __syncthreads();
-        for (int c = 0; c < nb; ++c) {
-            double v = -1.0;
-            int ix = 0x7fffffff;
-            if (tid >= c && tid < m) {
-                v = fabs((double)P[tid * NBP + c]);
-                ix = tid;
-            }
-            double bv;
-            int bi;
-            block_argmax(v, ix, sv, si, bv, bi);
-            if (tid == 0) {
-                ipiv[k0 + c] = k0 + bi;
-                if (!(bv > 0.0)) *zero = 1;
-            }
-            if (bi != c && tid < nb) {
-                const T t = P[c * NBP + tid];
-                P[c * NBP + tid] = P[bi * NBP + tid];
-                P[bi * NBP + tid] = t;
-            }
-            __syncthreads();
-            const T piv = P[c * NBP + c];
-            if (tid > c && tid < m) {
-                const T l = P[tid * NBP + c] / piv;
-                P[tid * NBP + c] = l;
-                for (int cc = c + 1; cc < nb; ++cc) P[tid * NBP + cc] -= l * P[c * NBP + cc];
-            }
-            __syncthreads();
-        }
-        for (int idx = tid; idx < m * nb; idx += NT) {
-            const int r = idx / nb, c = idx - r * nb;
-            L[(long)(k0 + r) * n + k0 + c] = P[r * NBP + c];
-        }
-        // row interchanges of this panel applied to the other columns
-        for (int j = tid; j < n; j += NT) {
-            if (j >= k0 && j < k0 + nb) continue;
+        if (tid < 64) {
+            int cnt = 0;
             for (int c = 0; c < nb; ++c) {
-                const int p = ipiv[k0 + c];
-                if (p != k0 + c) {
-                    const T t = L[(long)(k0 + c) * n + j];
-                    L[(long)(k0 + c) * n + j] = L[(long)p * n + j];
-                    L[(long)p * n + j] = t;
+                double v = -1.0;
+                int ix = 0x7fffffff;
+#pragma unroll
+                for (int s = 0; s < NMAX / 64; ++s) {
+                    const int r = lane + 64 * s;
+                    if (r >= c && r < m) {
+                        const double a = fabs((double)P[c * PLD + r]);
+                        if (a > v) {
+                            v = a;
+                            ix = r;
+                        }
+                    }
                 }
+#pragma unroll
+                for (int off = 32; off > 0; off >>= 1) {
+                    const double ov = __shfl_xor(v, off);
+                    const int oi = __shfl_xor(ix, off);
+                    if (ov > v || (ov == v && oi < ix)) {
+                        v = ov;
+                        ix = oi;
+                    }
+                }
+                if (ix >= m) ix = c;  // a column of NaNs: no pivot found
+                if (lane == 0 && !(v > 0.0)) *zero = 1;
+                if (ix != c) {
+                    if (lane < nb) {
+                        const T t = P[lane * PLD + c];
+                        P[lane * PLD + c] = P[lane * PLD + ix];
+                        P[lane * PLD + ix] = t;
+                    }
+                    if (lane == 0) {
+                        const int r1 = k0 + c, r2 = k0 + ix;
+                        if (ix >= nb && pv.mark[r2] != k0) {
+                            pv.mark[r2] = k0;
+                            pv.list[cnt++] = r2;
+                        }
+                        int t = pv.posP[r1];
+                        pv.posP[r1] = pv.posP[r2];
+                        pv.posP[r2] = t;
+                        t = pv.posAll[r1];
+                        pv.posAll[r1] = pv.posAll[r2];
+                        pv.posAll[r2] = t;
+                    }
+                }
+                wave_sync();
+                const T piv = P[c * PLD + c];
+#pragma unroll
+                for (int s = 0; s < NMAX / 64; ++s) {
+                    const int r = lane + 64 * s;
+                    if (r > c && r < m) {
+                        const T l = P[c * PLD + r] / piv;
+                        P[c * PLD + r] = l;
+                        for (int cc = c + 1; cc < nb; ++cc) P[cc * PLD + r] -= l * P[cc * PLD + c];
+                    }
+                }
+                wave_sync();
             }
+            if (lane == 0) pv.cnt = cnt;
         }
         __syncthreads();
-        // U12 = L11^{-1} A12 (unit lower L11 from the panel); each thread owns a column and
-        // re-reads the rows it has already produced (L1-resident, same thread).
-        for (int j = k0 + nb + tid; j < n; j += NT) {
-            for (int c = 1; c < nb; ++c) {
-                T s = L[(long)(k0 + c) * n + j];
-                for (int cp = 0; cp < c; ++cp) s -= P[c * NBP + cp] * L[(long)(k0 + cp) * n + j];
-                L[(long)(k0 + c) * n + j] = s;
+        // panel back to global
+        for (int idx = tid; idx < m * nb; idx += NT) {
+            const int r = idx / nb, c = idx - r * nb;
+            L[(long)(k0 + r) * n + k0 + c] = P[c * PLD + r];
+        }
+        // 2. the other columns: q < n2 -> column k0+nb+q (right), else column q-n2 (left)
+        const int n2 = n - k0 - nb, cnt = pv.cnt;
+        for (int q = tid; q < n - nb; q += NT) {
+            const bool right = q < n2;
+            const int j = right ? k0 + nb + q : q - n2;
+            // top rows' new values: from below the top block now, from LDS after the stage
+            T fin[NB];
+#pragma unroll
+            for (int c = 0; c < NB; ++c) {
+                const int src = c < nb ? pv.posP[k0 + c] : 0;
+                fin[c] = (c < nb && src >= k0 + nb) ? L[(long)src * n + j] : T(0);
             }
+            // top rows' old values, staged in this thread's LDS column of U
+            T old[NB];
+#pragma unroll
+            for (int c = 0; c < NB; ++c) old[c] = c < nb ? L[(long)(k0 + c) * n + j] : T(0);
+#pragma unroll
+            for (int c = 0; c < NB; ++c)
+                if (c < nb) U[c * PLD + q] = old[c];
+            // rows below the top block that receive a top row (their old values are read
+            // above into fin where needed, so they may be overwritten now)
+            for (int e = 0; e < cnt; ++e) {
+                const int r = pv.list[e];
+                L[(long)r * n + j] = U[(pv.posP[r] - k0) * PLD + q];
+            }
+#pragma unroll
+            for (int c = 0; c < NB; ++c) {
+                const int src = c < nb ? pv.posP[k0 + c] : 0;
+                if (c < nb && src < k0 + nb) fin[c] = U[(src - k0) * PLD + q];
+            }
+            if (right) {
+                // U12 = L11^{-1} A12 (unit lower, L11[c][cp] = P[cp*PLD + c]) in this
+                // thread's LDS column
+#pragma unroll
+                for (int c = 0; c < NB; ++c)
+                    if (c < nb) U[c * PLD + q] = fin[c];
+                for (int c = 1; c < nb; ++c) {
+                    T s = U[c * PLD + q];
+                    for (int cp = 0; cp < c; ++cp) s -= P[cp * PLD + c] * U[cp * PLD + q];
+                    U[c * PLD + q] = s;
+                }
+#pragma unroll
+                for (int c = 0; c < NB; ++c)
+                    if (c < nb) fin[c] = U[c * PLD + q];
+            }
+#pragma unroll
+            for (int c = 0; c < NB; ++c)
+                if (c < nb) L[(long)(k0 + c) * n + j] = fin[c];
         }
         __syncthreads();
-        // A22 -= L21 U12
-        const int m2 = n - k0 - nb;
-        if (m2 > 0) {
-            const int tr_n = (m2 + TM - 1) / TM, tc_n = (m2 + TN - 1) / TN;
-            for (int t = tid; t < tr_n * tc_n; t += NT) {
+        // 3. A22 -= L21 U12
+        if (n2 > 0) {
+            const int tr_n = (n2 + TM - 1) / TM, tc_n = (n2 + TN - 1) / TN;
+            const int ntile = tr_n * tc_n;
+            auto load_tile = [&](int t, T (&dst)[TM][TN]) {
                 const int tr = t / tc_n, tc = t - tr * tc_n;
-                const int i0 = k0 + nb + tr * TM, j0 = k0 + nb + tc * TN;
-                T acc[TM][TN];
+                const int i0 = tr * TM, j0 = tc * TN;
 #pragma unroll
                 for (int a = 0; a < TM; ++a)
 #pragma unroll
                     for (int bq = 0; bq < TN; ++bq)
-                        acc[a][bq] = (i0 + a < n && j0 + bq < n) ? L[(long)(i0 + a) * n + j0 + bq] : T(0);
+                        dst[a][bq] = (t < ntile && i0 + a < n2 && j0 + bq < n2)
+                                         ? L[(long)(k0 + nb + i0 + a) * n + k0 + nb + j0 + bq]
+                                         : T(0);
+            };
+            T nxt[TM][TN];
+            load_tile(tid, nxt);
+            for (int t = tid; t < ntile; t += NT) {
+                T acc[TM][TN];
+#pragma unroll
+                for (int a = 0; a < TM; ++a)
+#pragma unroll
+                    for (int bq = 0; bq < TN; ++bq) acc[a][bq] = nxt[a][bq];
+                load_tile(t + NT, nxt);
+                const int tr = t / tc_n, tc = t - tr * tc_n;
+                const int i0 = tr * TM, j0 = tc * TN;
                 for (int c = 0; c < nb; ++c) {
                     T av[TM], bv[TN];
 #pragma unroll
-                    for (int a = 0; a < TM; ++a)
-                        av[a] = (i0 + a < n) ? P[(nb + tr * TM + a) * NBP + c] : T(0);
+                    for (int a = 0; a < TM; ++a) av[a] = P[c * PLD + nb + i0 + a];
 #pragma unroll
-                    for (int bq = 0; bq < TN; ++bq)
-                        bv[bq] = (j0 + bq < n) ? L[(long)(k0 + c) * n + j0 + bq] : T(0);
+                    for (int bq = 0; bq < TN; ++bq) bv[bq] = U[c * PLD + j0 + bq];
 #pragma unroll
                     for (int a = 0; a < TM; ++a)
 #pragma unroll
@@ -196,82 +299,145 @@ __device__ void block_lu(T *__restrict__ L, int n, T *P, int *ipiv, double *sv, 
                 for (int a = 0; a < TM; ++a)
 #pragma unroll
                     for (int bq = 0; bq < TN; ++bq)
-                        if (i0 + a < n && j0 + bq < n) L[(long)(i0 + a) * n + j0 + bq] = acc[a][bq];
+                        if (i0 + a < n2 && j0 + bq < n2)
+                            L[(long)(k0 + nb + i0 + a) * n + k0 + nb + j0 + bq] = acc[a][bq];
             }
         }
         __syncthreads();
     }
 }
 
-// Solve L U y = y in place (y already permuted), factors row-major in L (ld n).  The
-// diagonal blocks are staged in D (NB*NB scalars of LDS) and solved by wave 0 without
-// barriers; the off-diagonal blocks are one GEMV each.
-template <typename T, int NB>
-__device__ void block_lu_solve(const T *__restrict__ L, int n, T *y, T *D)
+// Solve L U y = y in place (y in LDS, already permuted), factors row-major in L (ld n).
+// Thread i owns row i (n <= NT): it streams its row of the factor in chunks of SB columns,
+// the next chunk loaded while the current one is used.  The SB x SB diagonal block of a
+// chunk lies in one wave and is solved there with lane reads; one barrier per chunk
+// publishes the solved entries (ys) to the rows that still need them.
+template <typename T>
+__device__ void lu_solve_rows(const T *__restrict__ L, int n, T *y, T *ys)
 {
-    const int tid = threadIdx.x, lane = tid & 63;
-    // forward, unit lower
-    for (int kb = 0; kb < n; kb += NB) {
-        const int nb = min(NB, n - kb);
-        for (int idx = tid; idx < nb * nb; idx += NT) {
-            const int r = idx / nb, c = idx - r * nb;
-            D[r * NB + c] = L[(long)(kb + r) * n + kb + c];
-        }
-        __syncthreads();
-        if (tid < 64) {
-            T yr = lane < nb ? y[kb + lane] : T(0);
-            for (int c = 0; c < nb; ++c) {
-                const T yc = __shfl(yr, c);
-                if (lane > c && lane < nb) yr -= D[lane * NB + c] * yc;
+    constexpr int SB = 16;
+    const int i = threadIdx.x, w = i >> 6;
+    const bool act = i < n;
+    T yi = act ? y[i] : T(0);
+    T cur[SB], nxt[SB];
+    // forward, unit lower: row i uses columns < i
+    auto load_lo = [&](int kb, T (&dst)[SB]) {
+#pragma unroll
+        for (int c = 0; c < SB; ++c)
+            dst[c] = (act && kb + c < i) ? L[(long)i * n + kb + c] : T(0);
+    };
+    load_lo(0, nxt);
+    for (int kb = 0; kb < n; kb += SB) {
+#pragma unroll
+        for (int c = 0; c < SB; ++c) cur[c] = nxt[c];
+        load_lo(kb + SB, nxt);
+        if (w == kb / 64) {
+#pragma unroll
+            for (int c = 0; c < SB; ++c) {
+                const int r = kb + c;
+                if (r < n) {
+                    const T yc = read_lane(yi, r & 63);
+                    if (i > r && i < kb + SB) yi -= cur[c] * yc;
+                }
             }
-            if (lane < nb) y[kb + lane] = yr;
         }
+        if (act && i >= kb && i < kb + SB) ys[i] = yi;
         __syncthreads();
-        for (int i = kb + nb + tid; i < n; i += NT) {
-            T s = y[i];
-            for (int c = 0; c < nb; ++c) s -= L[(long)i * n + kb + c] * y[kb + c];
-            y[i] = s;
+        if (act && i >= kb + SB) {
+#pragma unroll
+            for (int c = 0; c < SB; ++c) yi -= cur[c] * ys[kb + c];
         }
-        __syncthreads();
     }
-    // backward, non-unit upper
-    const int last = ((n - 1) / NB) * NB;
-    for (int kb = last; kb >= 0; kb -= NB) {
-        const int nb = min(NB, n - kb);
-        for (int idx = tid; idx < nb * nb; idx += NT) {
-            const int r = idx / nb, c = idx - r * nb;
-            D[r * NB + c] = L[(long)(kb + r) * n + kb + c];
-        }
-        __syncthreads();
-        if (tid < 64) {
-            T yr = lane < nb ? y[kb + lane] : T(0);
-            for (int c = nb - 1; c >= 0; --c) {
-                if (lane == c) yr = yr / D[c * NB + c];
-                const T yc = __shfl(yr, c);
-                if (lane < c) yr -= D[lane * NB + c] * yc;
+    // backward, upper with diagonal: row i uses columns >= i
+    auto load_up = [&](int kb, T (&dst)[SB]) {
+#pragma unroll
+        for (int c = 0; c < SB; ++c)
+            dst[c] = (act && kb >= 0 && kb + c >= i && kb + c < n) ? L[(long)i * n + kb + c]
+                                                                   : T(0);
+    };
+    const int last = ((n - 1) / SB) * SB;
+    load_up(last, nxt);
+    for (int kb = last; kb >= 0; kb -= SB) {
+#pragma unroll
+        for (int c = 0; c < SB; ++c) cur[c] = nxt[c];
+        load_up(kb - SB, nxt);
+        if (w == kb / 64) {
+#pragma unroll
+            for (int c = SB - 1; c >= 0; --c) {
+                const int r = kb + c;
+                if (r < n) {
+                    if (i == r) yi = yi / cur[c];
+                    const T yc = read_lane(yi, r & 63);
+                    if (i >= kb && i < r) yi -= cur[c] * yc;
+                }
             }
-            if (lane < nb) y[kb + lane] = yr;
         }
+        if (act && i >= kb && i < kb + SB) ys[i] = yi;
         __syncthreads();
-        for (int i = tid; i < kb; i += NT) {
-            T s = y[i];
-            for (int c = 0; c < nb; ++c) s -= L[(long)i * n + kb + c] * y[kb + c];
-            y[i] = s;
+        if (act && i < kb) {
+#pragma unroll
+            for (int c = 0; c < SB; ++c)
+                if (kb + c < n) yi -= cur[c] * ys[kb + c];
         }
-        __syncthreads();
     }
+    if (act) y[i] = yi;
+    __syncthreads();
+}
+
+// r = b - A x for the FP64 n x n row-major A (global) and x, b in LDS; one wave per group
+// of RB rows, all loads of a group issued together.  With rowsum, also the largest
+// absolute row sum of A (for the backward-error test), reduced over the wave.
+template <bool ROWSUM>
+__device__ double residual(const double *__restrict__ A, int n, const double *x,
+                           const double *b, double *r)
+{
+    constexpr int RB = 8;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    double amax = 0.0;
+    for (int i0 = w * RB; i0 < n; i0 += RB * (NT / 64)) {
+        double a[RB][NMAX / 64];
+#pragma unroll
+        for (int k = 0; k < RB; ++k)
+#pragma unroll
+            for (int s = 0; s < NMAX / 64; ++s) {
+                const int i = i0 + k, j = lane + 64 * s;
+                a[k][s] = (i < n && j < n) ? A[(long)i * n + j] : 0.0;
+            }
+        double xs[NMAX / 64];
+#pragma unroll
+        for (int s = 0; s < NMAX / 64; ++s) xs[s] = lane + 64 * s < n ? x[lane + 64 * s] : 0.0;
+#pragma unroll
+        for (int k = 0; k < RB; ++k) {
+            double sum = 0.0, asum = 0.0;
+#pragma unroll
+            for (int s = 0; s < NMAX / 64; ++s) {
+                sum += a[k][s] * xs[s];
+                if (ROWSUM) asum += fabs(a[k][s]);
+            }
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) {
+                sum += __shfl_xor(sum, off);
+                if (ROWSUM) asum += __shfl_xor(asum, off);
+            }
+            if (i0 + k < n) {
+                if (lane == 0) r[i0 + k] = b[i0 + k] - sum;
+                if (ROWSUM) amax = fmax(amax, asum);
+            }
+        }
+    }
+    return amax;
 }
 
 template <typename T, bool MIXED, int NB, int TM, int TN>
-__global__ __launch_bounds__(NT, 2) void batched_solve_kernel(Args a)
+__global__ __launch_bounds__(NT, sizeof(T) * NB * PLD * 2 > 80 * 1024 ? 1 : 2) void
+batched_solve_kernel(Args a)
 {
-    constexpr int NBP = NB + 1;
-    __shared__ T P[NMAX * NBP];
+    __shared__ T P[NB * PLD];
+    __shared__ T U[NB * PLD];
     __shared__ double b64[NMAX], x64[NMAX], r64[NMAX];
-    __shared__ T yv[NMAX];
-    __shared__ int ipiv[NMAX], perm[NMAX];
+    __shared__ T yv[NMAX], ys[NMAX];
+    __shared__ PivLds pv;
     __shared__ double sv[NT / 64];
-    __shared__ int si[NT / 64];
     __shared__ int s_inst, s_zero;
 
     const int tid = threadIdx.x;
@@ -293,63 +459,60 @@ __global__ __launch_bounds__(NT, 2) void batched_solve_kernel(Args a)
         const double *th = a.theta + a.bd.toff[inst];
         const long vo = a.bd.voff[inst];
 
-        // assemble A(mu) (or its transpose)
+        // assemble A(mu) (or its transpose): 4 entries per thread, 4 terms at a time, all
+        // 16 loads issued before the sums
         const long nn = (long)n * n;
-        for (long idx = tid; idx < nn; idx += NT) {
-            const int i = (int)(idx / n), j = (int)(idx - (long)i * n);
-            const long src = a.transpose ? (long)j * n + i : idx;
-            double s = 0.0;
-            for (int q = 0; q < nq; ++q) s += th[q] * Ai[q * nn + src];
-            A64[idx] = s;
-            if (MIXED) LU[idx] = (T)s;
+        for (long i0 = tid; i0 < nn; i0 += 4 * NT) {
+            double s[4] = {0.0, 0.0, 0.0, 0.0};
+            for (int q0 = 0; q0 < nq; q0 += 4) {
+                double v[4][4];
+#pragma unroll
+                for (int qq = 0; qq < 4; ++qq)
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        const long idx = i0 + u * NT;
+                        v[qq][u] = (idx < nn && q0 + qq < nq) ? Ai[(q0 + qq) * nn + idx] : 0.0;
+                    }
+#pragma unroll
+                for (int qq = 0; qq < 4; ++qq) {
+                    const double t = q0 + qq < nq ? th[q0 + qq] : 0.0;
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) s[u] += t * v[qq][u];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const long idx = i0 + u * NT;
+                if (idx < nn) {
+                    const int i = (int)(idx / n), j = (int)(idx - (long)i * n);
+                    const long dst = a.transpose ? (long)j * n + i : idx;
+                    A64[dst] = s[u];
+                    if (MIXED) LU[dst] = (T)s[u];
+                }
+            }
         }
         for (int i = tid; i < n; i += NT) b64[i] = a.b[vo + i];
         __syncthreads();
 
-        block_lu<T, NB, TM, TN>(LU, n, P, ipiv, sv, si, &s_zero);
-        if (tid == 0) {
-            for (int i = 0; i < n; ++i) perm[i] = i;
-            for (int c = 0; c < n; ++c) {
-                const int p = ipiv[c];
-                const int t = perm[c];
-                perm[c] = perm[p];
-                perm[p] = t;
-            }
-        }
+        block_lu<T, NB, TM, TN>(LU, n, P, U, pv, &s_zero);
+        for (int i = tid; i < n; i += NT) yv[i] = (T)b64[pv.posAll[i]];
         __syncthreads();
-        for (int i = tid; i < n; i += NT) yv[i] = (T)b64[perm[i]];
-        __syncthreads();
-        block_lu_solve<T, NB>(LU, n, yv, P);
+        lu_solve_rows<T>(LU, n, yv, ys);
         for (int i = tid; i < n; i += NT) x64[i] = (double)yv[i];
         __syncthreads();
 
         int it = 0, st = s_zero ? 1 : 0;
         if (MIXED) {
-            // ||A||_inf and ||b||_inf for the backward-error test
-            double arow = 0.0;
-            const int w = tid >> 6, lane = tid & 63;
-            for (int i = w; i < n; i += NT / 64) {
-                double s = 0.0;
-                for (int j = lane; j < n; j += 64) s += fabs(A64[(long)i * n + j]);
-#pragma unroll
-                for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
-                arow = fmax(arow, s);
-            }
             double bmax = 0.0;
             for (int i = tid; i < n; i += NT) bmax = fmax(bmax, fabs(b64[i]));
-            const double anorm = block_max(arow, sv);
             const double bnorm = block_max(bmax, sv);
+            double anorm = 0.0;
             bool conv = false;
             for (; it <= a.max_iter; ++it) {
-                // r = b - A x  (one wave per row, lanes over columns)
-                for (int i = w; i < n; i += NT / 64) {
-                    double s = 0.0;
-                    for (int j = lane; j < n; j += 64) s += A64[(long)i * n + j] * x64[j];
-#pragma unroll
-                    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
-                    if (lane == 0) r64[i] = b64[i] - s;
-                }
+                const double am = it == 0 ? residual<true>(A64, n, x64, b64, r64)
+                                          : residual<false>(A64, n, x64, b64, r64);
                 __syncthreads();
+                if (it == 0) anorm = block_max(am, sv);
                 double rm = 0.0, xm = 0.0;
                 for (int i = tid; i < n; i += NT) {
                     rm = fmax(rm, fabs(r64[i]));
@@ -362,9 +525,9 @@ __global__ __launch_bounds__(NT, 2) void batched_solve_kernel(Args a)
                     break;
                 }
                 if (it == a.max_iter) break;
-                for (int i = tid; i < n; i += NT) yv[i] = (T)r64[perm[i]];
+                for (int i = tid; i < n; i += NT) yv[i] = (T)r64[pv.posAll[i]];
                 __syncthreads();
-                block_lu_solve<T, NB>(LU, n, yv, P);
+                lu_solve_rows<T>(LU, n, yv, ys);
                 for (int i = tid; i < n; i += NT) x64[i] += (double)yv[i];
                 __syncthreads();
             }

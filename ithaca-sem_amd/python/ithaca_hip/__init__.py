@@ -52,6 +52,7 @@ def lib():
         L.imf_apply.argtypes = [vp, vp, vp]
         L.imf_apply_device.argtypes = [vp, vp, vp, vp]
         L.imf_apply_transpose_device.argtypes = [vp, vp, vp, vp]
+        L.imf_apply_many_device.argtypes = [vp, i, vp, ctypes.c_long, vp, ctypes.c_long, vp]
         L.imf_last_error.restype = ctypes.c_char_p
         L.imf_basis_modified_a.argtypes = [i, i, vp, vp, vp, vp, vp]
         L.imf_geom_hex_device.argtypes = [vp, i, i, vp, i, vp, vp, vp]
@@ -191,6 +192,21 @@ class Operator:
         _check(lib().imf_apply_transpose_device(self._h, ctypes.c_void_p(inp.data_ptr()),
                                                 ctypes.c_void_p(out.data_ptr()),
                                                 _stream_ptr(stream)))
+        return out
+
+    def apply_many(self, inp, out=None, stream=None):
+        """operator()(in_v, out_v) for every row v of the 2D cuda tensor inp (rows may be
+        longer than the vector: the row stride is the leading dimension)."""
+        assert inp.is_cuda and inp.dtype == self.dtype and inp.dim() == 2 and inp.stride(1) == 1
+        assert inp.shape[1] >= self.in_size
+        if out is None:
+            out = torch.empty((inp.shape[0], self.out_size), dtype=self.dtype,
+                              device=inp.device)
+        assert out.is_cuda and out.dtype == self.dtype and out.dim() == 2 and out.stride(1) == 1
+        assert out.shape[0] == inp.shape[0] and out.shape[1] >= self.out_size
+        _check(lib().imf_apply_many_device(self._h, inp.shape[0], ctypes.c_void_p(inp.data_ptr()),
+                                           inp.stride(0), ctypes.c_void_p(out.data_ptr()),
+                                           out.stride(0), _stream_ptr(stream)))
         return out
 
     def apply_host(self, inp):

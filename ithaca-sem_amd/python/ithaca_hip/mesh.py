@@ -65,3 +65,31 @@ def workload_hex_helmholtz(n=48):
     u = rng.uniform(-1.0, 1.0, size=n ** 3 * nm ** 3)
     return dict(shape="hex", nm=nm, nq=nq, nElmt=n ** 3, verts=verts, u=u, lam=10.0,
                 deformed=True)
+
+
+def hex_quad_points(verts, z):
+    """Physical coordinates (E, Q, 3) of the nq^3 GLL points of straight-sided hexes under
+    the trilinear map of their vertices (torch, on verts' device); point index
+    (k*nq + j)*nq + i with xi = (z_i, z_j, z_k)."""
+    import torch
+    dev = verts.device
+    zt = torch.as_tensor(np.asarray(z), dtype=torch.float64, device=dev)
+    k, j, i = torch.meshgrid(zt, zt, zt, indexing="ij")
+    xi = torch.stack([i.reshape(-1), j.reshape(-1), k.reshape(-1)], dim=1)  # (Q, 3)
+    c = torch.as_tensor(HEX_CORNERS, dtype=torch.float64, device=dev)      # (8, 3)
+    shp = torch.where(c[None] > 0, (1.0 + xi[:, None]) / 2, (1.0 - xi[:, None]) / 2)
+    N = torch.prod(shp, dim=2)                                               # (Q, 8)
+    E = verts.shape[0]
+    X = N @ verts.permute(1, 0, 2).reshape(8, E * 3)                         # (Q, E*3)
+    return X.reshape(-1, E, 3).permute(1, 0, 2).contiguous()
+
+
+def pod_fields(n_fields=20, n_snap=512, seed=2):
+    """Snapshot recipe of the POD workload: field k is prod_i sin(m_{k,i} pi x_i) with
+    integer wavenumbers m ~ U{1..6}; snapshot j is sum_k a_{jk} field_k with
+    a_{jk} ~ N(0,1) 2^{-(k+1)} (k = 0..n_fields-1).  Returns (m (n_fields, 3),
+    a (n_snap, n_fields)) from numpy PCG64(seed)."""
+    rng = np.random.default_rng(seed)
+    m = rng.integers(1, 7, size=(n_fields, 3))
+    a = rng.standard_normal((n_snap, n_fields)) * 2.0 ** -np.arange(1, n_fields + 1)
+    return m, a

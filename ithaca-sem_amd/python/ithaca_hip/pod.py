@@ -17,7 +17,7 @@ import ctypes
 import numpy as np
 import torch
 
-from . import MFError
+from . import FP64, MFError, Operator
 from . import rom as _rom
 
 GRAM_FP64, GRAM_FP32X3, GRAM_BF16X3 = 0, 1, 2
@@ -106,7 +106,8 @@ def reference_rbsize(sigma, pod_tolerance):
 
 
 def mass_apply(S, bwd, ipr, out=None, phys=None):
-    """M s_j = IProduct(BwdTrans(s_j)) for every row of S (k, N)."""
+    """M s_j = IProduct(BwdTrans(s_j)) for every row of S (k, N), one pair of operator
+    applies per row (the reference's MassMatrixOp_MatFree, StdExpansion.cpp:647-661)."""
     if out is None:
         out = torch.empty_like(S)
     if phys is None:
@@ -117,8 +118,85 @@ def mass_apply(S, bwd, ipr, out=None, phys=None):
     return out
 
 
+def mass_apply_fused(S, mass, out=None, block=64):
+    """M s_j for every row of S (k, N) with the fused "Mass_Hex_*" operator: one launch
+    per block of rows, the geometry read once per launch."""
+    if out is None:
+        out = torch.empty_like(S)
+    for j0 in range(0, S.shape[0], block):
+        j1 = min(S.shape[0], j0 + block)
+        mass.apply_many(S[j0:j1], out[j0:j1])
+    return out
+
+
 def pod_basis(S, evals, evecs, r):
     """V = Psi_r^T-weighted combination of the snapshots: V (r, N) = Lambda_r^{-1/2}
     Psi_r^T S with Psi_r the leading r eigenvectors (rows of evecs)."""
     C = evecs[:r] / torch.sqrt(torch.as_tensor(evals[:r], device=S.device))[:, None]
     return C @ S
+
+
+class ElementalProjector:
+    """Elemental L2 projection onto the hex space, u_e = M_e^{-1} IProduct_e(f), for
+    straight-sided deformed hexes: the reference's per-element FwdTrans
+    (LocalRegions/HexExp.cpp:325-349, IProductWRTBase then the element's eInvMass).
+
+    M_e = B^T diag(J w) B is applied matrix-free (the fused Mass operator).  The
+    system is solved by conjugate gradients on the block-diagonal mass matrix,
+    preconditioned by the element's constant-Jacobian inverse
+    (M1^{-1} x M1^{-1} x M1^{-1}) / Jbar_e, which is exact when J is constant on the
+    element (M1 = 1D mass matrix, Jbar_e = weighted mean of J); its Kronecker product is
+    applied by the sum-factorisation BwdTrans kernel with M1^{-1} as the table."""
+
+    def __init__(self, nElmt, nm, nq, basis, jac):
+        self.E, self.nm, self.nq = nElmt, nm, nq
+        self.M, self.Q = nm ** 3, nq ** 3
+        self.bwd = Operator("BwdTrans_Hex_Regular", nm, nq, nElmt, FP64, basis)
+        self.ipr = Operator("IProduct_Hex_Deformed", nm, nq, nElmt, FP64, basis)
+        self.ipr.set_jac(jac)
+        self.mass_op = Operator("Mass_Hex_Deformed", nm, nq, nElmt, FP64, basis)
+        self.mass_op.set_jac(jac)
+        B = np.asarray(basis["bdata"]).reshape(nm, nq)
+        w = np.asarray(basis["w"])
+        Minv = np.linalg.inv((B * w) @ B.T)
+        kb = dict(bdata=np.ascontiguousarray(Minv.ravel()), dbdata=np.zeros(nm * nm),
+                  D=np.zeros(nm * nm), w=np.ones(nm), nm=nm, nq=nm)
+        self.pre = Operator("BwdTrans_Hex_Regular", nm, nm, nElmt, FP64, kb)
+        w3 = torch.as_tensor(np.einsum("k,j,i->kji", w, w, w).ravel(), device=jac.device)
+        jbar = (jac.view(nElmt, self.Q) * w3).sum(1) / w3.sum()
+        self.inv_jbar = (1.0 / jbar)[:, None].contiguous()
+        self.phys = torch.empty(nElmt * self.Q, dtype=torch.float64, device=jac.device)
+
+    def mass(self, x, out=None):
+        return self.mass_op.apply(x, out)
+
+    def precond(self, r, out=None):
+        out = self.pre.apply(r, out)
+        out.view(self.E, self.M).mul_(self.inv_jbar)
+        return out
+
+    def project(self, f, tol=1e-13, max_iter=200, check_every=4):
+        """Coefficients of the elemental L2 projection of the point values f (E*Q);
+        returns (u, iterations, relative residual ||b - M u|| / ||b||)."""
+        b = self.ipr.apply(f)
+        bn = torch.linalg.vector_norm(b)
+        x = self.precond(b)
+        r = b - self.mass(x)
+        z = self.precond(r)
+        p = z.clone()
+        rz = torch.dot(r, z)
+        Ap = torch.empty_like(b)
+        it = 0
+        for it in range(1, max_iter + 1):
+            self.mass(p, Ap)
+            alpha = rz / torch.dot(p, Ap)
+            x.add_(alpha * p)
+            r.sub_(alpha * Ap)
+            if it % check_every == 0 and float(torch.linalg.vector_norm(r) / bn) <= tol:
+                break
+            self.precond(r, z)
+            rz_new = torch.dot(r, z)
+            p.mul_(rz_new / rz).add_(z)
+            rz = rz_new
+        res = float(torch.linalg.vector_norm(b - self.mass(x)) / bn)
+        return x, it, res

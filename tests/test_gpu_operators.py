@@ -214,3 +214,50 @@ def test_quad_workload_parity():
     op.set_lambda(w["lam"])
     got = op.apply(_dev(w["u"])).cpu().numpy()
     assert rel(got, ref) < TOL64
+
+
+@pytest.mark.parametrize("deformed", [True, False])
+@pytest.mark.parametrize("nm,nq", [(2, 3), (4, 8), (6, 7), (8, 9)])
+def test_mass_parity(nm, nq, deformed):
+    """Fused Mass_Hex_* (MassMatrixOp_MatFree: BwdTrans then IProductWRTBase) against the
+    oracle's composition, for one vector and for several vectors in one launch with padded
+    rows (leading dimension > vector size); Mass^T == Mass."""
+    rng, verts, b, jac, df = _case("hex", nm, nq, deformed, 7)
+    E = verts.shape[0]
+    M = nm ** 3
+    key = f"Mass_Hex_{'Deformed' if deformed else 'Regular'}"
+    op = _op(key, nm, nq, E, b, jac=jac)
+    assert op.needs_jac() and not op.needs_df()
+    nv, ld = 3, E * M + 5
+    U = rng.uniform(-1, 1, (nv, ld))
+    refs = [O.iproduct("hex", b, E, deformed, jac, O.bwdtrans("hex", b, E, U[v, :E * M]))
+            for v in range(nv)]
+    got1 = op.apply(_dev(U[0, :E * M])).cpu().numpy()
+    assert rel(got1, refs[0]) < TOL64
+    out = torch.full((nv, ld), 7.0, dtype=torch.float64, device="cuda")
+    op.apply_many(_dev(U), out)
+    o = out.cpu().numpy()
+    for v in range(nv):
+        assert rel(o[v, :E * M], refs[v]) < TOL64
+        assert np.all(o[v, E * M:] == 7.0)  # padding untouched
+    gt = op.apply_transpose(_dev(U[0, :E * M])).cpu().numpy()
+    assert np.array_equal(gt, got1)
+    op32 = _op(key, nm, nq, E, b, jac=jac, prec=H.FP32)
+    got32 = op32.apply(_dev(U[0, :E * M], torch.float32)).double().cpu().numpy()
+    assert rel(got32, refs[0]) < TOL32
+
+
+def test_mass_quad_rejected_and_apply_many_loops():
+    with pytest.raises(H.MFError):
+        H.Operator("Mass_Quad_Regular", 4, 5, 4)
+    rng, verts, b, jac, df = _case("hex", 4, 5, True, 9)
+    E = verts.shape[0]
+    op = _op("Helmholtz_Hex_Deformed", 4, 5, E, b, jac=jac, df=df)
+    op.set_lambda(2.0)
+    U = _dev(rng.uniform(-1, 1, (2, E * 64)))
+    many = op.apply_many(U)
+    for v in range(2):
+        assert torch.equal(many[v], op.apply(U[v].contiguous()))
+    # leading dimensions shorter than the vectors are rejected before any launch
+    assert H.lib().imf_apply_many_device(op._h, 1, None, 3, None, 3, None) != 0
+    assert "leading dimension" in H.lib().imf_last_error().decode()

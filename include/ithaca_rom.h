@@ -83,9 +83,12 @@ int irom_batch_param_grad(irom_batch b, const double *d_A, const double *d_lambd
  *                    16-deep panel, FP64 across panels
  *   IROM_GRAM_BF16X3 hi/lo BF16 split, 3 BF16 MFMA products, FP32 accumulation within a
  *                    32-deep panel, FP64 across panels
- * X == Y (same pointer, m == n, ldx == ldy) computes only the upper blocks and mirrors.
+ * X == Y (same pointer, m == n, ldx == ldy) computes only the upper blocks and mirrors;
+ * so does a mode OR-ed with IROM_GRAM_SYMMETRIC_RESULT when m == n (the caller states
+ * that X^T Y is symmetric, e.g. S^T (M S) with M symmetric).
  * d_work must hold irom_gram_workspace_size(m, n, K) bytes (split-K partial blocks). */
-enum irom_gram_mode { IROM_GRAM_FP64 = 0, IROM_GRAM_FP32X3 = 1, IROM_GRAM_BF16X3 = 2 };
+enum irom_gram_mode { IROM_GRAM_FP64 = 0, IROM_GRAM_FP32X3 = 1, IROM_GRAM_BF16X3 = 2,
+                      IROM_GRAM_SYMMETRIC_RESULT = 16 };
 size_t irom_gram_workspace_size(int m, int n, long K);
 int irom_gram(int mode, int m, int n, long K, const double *d_X, long ldx, const double *d_Y,
               long ldy, double *d_C, long ldc, int accumulate, void *d_work, size_t work_bytes,

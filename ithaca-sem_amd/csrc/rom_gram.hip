@@ -32,258 +32,243 @@ typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 constexpr int GB = 128;   // C block edge per workgroup
 constexpr int GT = 256;   // threads (4 waves, each a 64 x 64 quadrant)
 
-// ---------------------------------------------------------------------------
-// FP64: K panels of 16, X panel Xs[col][k] with row stride 17 (bank spread).
-// ---------------------------------------------------------------------------
-__global__ __launch_bounds__(GT, 2) void gram_f64_kernel(int m, int n, long K, long kchunk,
-                                                         const double *__restrict__ X, long ldx,
-                                                         const double *__restrict__ Y, long ldy,
-                                                         double *__restrict__ part, int symmetric)
+struct GramArgs {
+    int m, n;
+    long K, kchunk;
+    const double *X;
+    long ldx;
+    const double *Y;
+    long ldy;
+    double *part;
+    int nbj;      // C blocks along n
+    int ntiles;   // blocks computed per K slice
+    int nsplit;   // K slices
+    int sym;      // only blocks bi <= bj (the caller's product is symmetric)
+    int vec;      // X, Y rows 16-byte aligned: 16-byte loads
+};
+
+// Block id -> (K slice, C block).  Workgroups are dispatched to the 8 XCDs round-robin by
+// linear id, so ids L with the same L % 8 share an L2: every C block of one K slice is
+// given the same L % 8 and adjacent dispatch order, and the slice's X/Y panels are read
+// from HBM once per XCD instead of once per C block.
+__device__ __forceinline__ bool gram_block(const GramArgs &g, int &split, int &bi, int &bj)
 {
-    constexpr int KB = 16, KP = KB + 1;
-    __shared__ double Xs[GB * KP], Ys[GB * KP];
-    const int bi = blockIdx.x, bj = blockIdx.y, split = blockIdx.z;
-    if (symmetric && bj < bi) return;
+    const int L = blockIdx.x, xcd = L & 7, r = L >> 3;
+    split = 8 * (r / g.ntiles) + xcd;
+    int t = r % g.ntiles;
+    if (split >= g.nsplit) return false;
+    if (!g.sym) {
+        bi = t / g.nbj;
+        bj = t - bi * g.nbj;
+        return true;
+    }
+    for (bi = 0; t >= g.nbj - bi; ++bi) t -= g.nbj - bi;  // upper blocks, row by row
+    bj = bi + t;
+    return true;
+}
+
+// Per-mode LDS panel formats and MFMA steps.
+//   MODE 0 (FP64):   KB = 16, double panels, v_mfma_f64_16x16x4f64
+//   MODE 1 (FP32x3): KB = 16, hi/lo float panels, 3 x v_mfma_f32_16x16x4f32
+//   MODE 2 (BF16x3): KB = 32, hi/lo bf16 panels, 3 x v_mfma_f32_16x16x32_bf16
+template <int MODE> struct GramMode;
+template <> struct GramMode<0> { static constexpr int KB = 16, KP = 17; using E = double; };
+template <> struct GramMode<1> { static constexpr int KB = 16, KP = 17; using E = float; };
+template <> struct GramMode<2> { static constexpr int KB = 32, KP = 40; using E = __bf16; };
+
+__device__ __forceinline__ __bf16 to_bf16(float v) { return __float2bfloat16(v); }
+__device__ __forceinline__ float from_bf16(__bf16 v) { return __bfloat162float(v); }
+
+// One KB-deep panel of GB rows of X (column-major K x m: row i of the panel is column i0+i
+// of X): KB/4 pairs of doubles per thread, consecutive threads on consecutive k of one
+// column so that a wave reads whole 128-byte lines.
+template <int KB>
+__device__ __forceinline__ void panel_load(const double *__restrict__ X, long ldx, int i0, int m,
+                                           long k0, long kend, int vec, double2 (&v)[KB / 4])
+{
+    // interior panels (the common case) take one uniform branch of unconditional loads
+    if (vec && i0 + GB <= m && k0 + KB <= kend) {
+#pragma unroll
+        for (int s = 0; s < KB / 4; ++s) {
+            const int idx = s * GT + threadIdx.x;
+            const int c = idx / (KB / 2), ko = 2 * (idx % (KB / 2));
+            v[s] = *reinterpret_cast<const double2 *>(X + (long)(i0 + c) * ldx + k0 + ko);
+        }
+        return;
+    }
+#pragma unroll
+    for (int s = 0; s < KB / 4; ++s) {
+        const int idx = s * GT + threadIdx.x;
+        const int c = idx / (KB / 2), ko = 2 * (idx % (KB / 2));
+        const int gi = i0 + c;
+        const long k = k0 + ko;
+        const double *p = X + (long)gi * ldx + k;
+        v[s].x = (gi < m && k < kend) ? p[0] : 0.0;
+        v[s].y = (gi < m && k + 1 < kend) ? p[1] : 0.0;
+    }
+}
+
+template <int MODE, int KB, int KP, typename E>
+__device__ __forceinline__ void panel_store(const double2 (&v)[KB / 4], E *hi, E *lo)
+{
+#pragma unroll
+    for (int s = 0; s < KB / 4; ++s) {
+        const int idx = s * GT + threadIdx.x;
+        const int c = idx / (KB / 2), ko = 2 * (idx % (KB / 2));
+        const double x[2] = {v[s].x, v[s].y};
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int o = c * KP + ko + h;
+            if constexpr (MODE == 0) {
+                hi[o] = x[h];
+            } else if constexpr (MODE == 1) {
+                const float xh = (float)x[h];
+                hi[o] = xh;
+                lo[o] = (float)(x[h] - (double)xh);
+            } else {
+                const __bf16 xh = to_bf16((float)x[h]);
+                hi[o] = xh;
+                lo[o] = to_bf16((float)(x[h] - (double)from_bf16(xh)));
+            }
+        }
+    }
+}
+
+// C block (bi, bj) of X^T Y over one K slice, written to part[split] (row-major m x n).
+// The next panel is loaded into registers while the current one is multiplied.
+// FP64 fits two workgroups per CU; the split modes keep FP32 panel sums next to the FP64
+// block sums and the prefetched panel, and run one workgroup per CU.
+template <int MODE>
+__global__ __launch_bounds__(GT, MODE == 0 ? 2 : 1) void gram_kernel(GramArgs g)
+{
+    using Md = GramMode<MODE>;
+    using E = typename Md::E;
+    constexpr int KB = Md::KB, KP = Md::KP;
+    constexpr int NA = MODE == 0 ? 1 : 2;  // hi (and lo) panels
+    __shared__ __attribute__((aligned(16))) E Xs[NA][GB * KP], Ys[NA][GB * KP];
+    int split, bi, bj;
+    if (!gram_block(g, split, bi, bj)) return;
     const int i0 = bi * GB, j0 = bj * GB;
-    const long kbeg = (long)split * kchunk, kend = min(K, kbeg + kchunk);
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const long kbeg = (long)split * g.kchunk, kend = min(g.K, kbeg + g.kchunk);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int wr = (wave >> 1) * 64, wc = (wave & 1) * 64;  // quadrant of this wave
 
-    d4 acc[4][4];
+    d4 tot[4][4];
 #pragma unroll
     for (int a = 0; a < 4; ++a)
 #pragma unroll
-        for (int b = 0; b < 4; ++b) acc[a][b] = d4{0, 0, 0, 0};
+        for (int b = 0; b < 4; ++b) tot[a][b] = d4{0, 0, 0, 0};
 
+    double2 xv[KB / 4], yv[KB / 4];
+    panel_load<KB>(g.X, g.ldx, i0, g.m, kbeg, kend, g.vec, xv);
+    panel_load<KB>(g.Y, g.ldy, j0, g.n, kbeg, kend, g.vec, yv);
     for (long k0 = kbeg; k0 < kend; k0 += KB) {
-        // stage: 128 columns x 16 k of X and of Y; thread -> (column, 8 consecutive k)
-        {
-            const int col = tid >> 1, kk = (tid & 1) * 8;
-#pragma unroll
-            for (int t = 0; t < 8; ++t) {
-                const long k = k0 + kk + t;
-                const bool kin = k < kend;
-                const int gi = i0 + col, gj = j0 + col;
-                Xs[col * KP + kk + t] = (kin && gi < m) ? X[k + (long)gi * ldx] : 0.0;
-                Ys[col * KP + kk + t] = (kin && gj < n) ? Y[k + (long)gj * ldy] : 0.0;
-            }
-        }
+        panel_store<MODE, KB, KP, E>(xv, Xs[0], Xs[NA - 1]);
+        panel_store<MODE, KB, KP, E>(yv, Ys[0], Ys[NA - 1]);
         __syncthreads();
+        if (k0 + KB < kend) {
+            panel_load<KB>(g.X, g.ldx, i0, g.m, k0 + KB, kend, g.vec, xv);
+            panel_load<KB>(g.Y, g.ldy, j0, g.n, k0 + KB, kend, g.vec, yv);
+        }
+        if constexpr (MODE == 0) {
 #pragma unroll
-        for (int ks = 0; ks < KB; ks += 4) {
-            double av[4], bv[4];
+            for (int ks = 0; ks < KB; ks += 4) {
+                double av[4], bv[4];
 #pragma unroll
-            for (int a = 0; a < 4; ++a) av[a] = Xs[(wr + a * 16 + (lane & 15)) * KP + ks + (lane >> 4)];
+                for (int a = 0; a < 4; ++a)
+                    av[a] = Xs[0][(wr + a * 16 + (lane & 15)) * KP + ks + (lane >> 4)];
 #pragma unroll
-            for (int b = 0; b < 4; ++b) bv[b] = Ys[(wc + b * 16 + (lane & 15)) * KP + ks + (lane >> 4)];
+                for (int b = 0; b < 4; ++b)
+                    bv[b] = Ys[0][(wc + b * 16 + (lane & 15)) * KP + ks + (lane >> 4)];
+#pragma unroll
+                for (int a = 0; a < 4; ++a)
+#pragma unroll
+                    for (int b = 0; b < 4; ++b)
+                        tot[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a], bv[b], tot[a][b],
+                                                                         0, 0, 0);
+            }
+        } else {
+            f4 acc[4][4];
+#pragma unroll
+            for (int a = 0; a < 4; ++a)
+#pragma unroll
+                for (int b = 0; b < 4; ++b) acc[a][b] = f4{0, 0, 0, 0};
+            if constexpr (MODE == 1) {
+#pragma unroll
+                for (int ks = 0; ks < KB; ks += 4) {
+                    float ah[4], al[4], bh[4], bl[4];
+#pragma unroll
+                    for (int a = 0; a < 4; ++a) {
+                        const int o = (wr + a * 16 + (lane & 15)) * KP + ks + (lane >> 4);
+                        ah[a] = Xs[0][o];
+                        al[a] = Xs[1][o];
+                    }
+#pragma unroll
+                    for (int b = 0; b < 4; ++b) {
+                        const int o = (wc + b * 16 + (lane & 15)) * KP + ks + (lane >> 4);
+                        bh[b] = Ys[0][o];
+                        bl[b] = Ys[1][o];
+                    }
+#pragma unroll
+                    for (int a = 0; a < 4; ++a)
+#pragma unroll
+                        for (int b = 0; b < 4; ++b) {
+                            acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(al[a], bh[b], acc[a][b], 0, 0, 0);
+                            acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(ah[a], bl[b], acc[a][b], 0, 0, 0);
+                            acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(ah[a], bh[b], acc[a][b], 0, 0, 0);
+                        }
+                }
+            } else {
+                bf8 ah[4], al[4], bh[4], bl[4];
+#pragma unroll
+                for (int a = 0; a < 4; ++a) {
+                    const int o = (wr + a * 16 + (lane & 15)) * KP + 8 * (lane >> 4);
+                    ah[a] = *reinterpret_cast<const bf8 *>(&Xs[0][o]);
+                    al[a] = *reinterpret_cast<const bf8 *>(&Xs[1][o]);
+                }
+#pragma unroll
+                for (int b = 0; b < 4; ++b) {
+                    const int o = (wc + b * 16 + (lane & 15)) * KP + 8 * (lane >> 4);
+                    bh[b] = *reinterpret_cast<const bf8 *>(&Ys[0][o]);
+                    bl[b] = *reinterpret_cast<const bf8 *>(&Ys[1][o]);
+                }
+#pragma unroll
+                for (int a = 0; a < 4; ++a)
+#pragma unroll
+                    for (int b = 0; b < 4; ++b) {
+                        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[a], bh[b], acc[a][b], 0, 0, 0);
+                        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], bl[b], acc[a][b], 0, 0, 0);
+                        acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], bh[b], acc[a][b], 0, 0, 0);
+                    }
+            }
 #pragma unroll
             for (int a = 0; a < 4; ++a)
 #pragma unroll
                 for (int b = 0; b < 4; ++b)
-                    acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a], bv[b], acc[a][b], 0, 0, 0);
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) tot[a][b][r] += (double)acc[a][b][r];
         }
         __syncthreads();
     }
-    // partial block -> part[split][i][j] (row-major m x n).  The f64 16x16x4 accumulator
-    // holds D[lane/16 + 4r][lane%16] in component r.
-    double *P = part + (long)split * m * n;
+    // The f64 16x16x4 accumulator holds D[lane/16 + 4r][lane%16] in component r; the f32
+    // 16x16x4 and bf16 16x16x32 ones hold D[4*(lane/16) + r][lane%16].
+    double *P = g.part + (long)split * g.m * g.n;
 #pragma unroll
     for (int a = 0; a < 4; ++a)
 #pragma unroll
         for (int b = 0; b < 4; ++b)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int i = i0 + wr + a * 16 + (lane >> 4) + 4 * r;
+                const int i = i0 + wr + a * 16 + (MODE == 0 ? (lane >> 4) + 4 * r
+                                                            : 4 * (lane >> 4) + r);
                 const int j = j0 + wc + b * 16 + (lane & 15);
-                if (i < m && j < n) P[(long)i * n + j] = acc[a][b][r];
-            }
-}
-
-// ---------------------------------------------------------------------------
-// FP32 split: x = hi + lo, three f32 MFMA products per k-step, FP64 accumulation per panel.
-// ---------------------------------------------------------------------------
-__global__ __launch_bounds__(GT, 2) void gram_f32x3_kernel(int m, int n, long K, long kchunk,
-                                                           const double *__restrict__ X, long ldx,
-                                                           const double *__restrict__ Y, long ldy,
-                                                           double *__restrict__ part, int symmetric)
-{
-    constexpr int KB = 16, KP = KB + 1;
-    __shared__ float Xh[GB * KP], Xl[GB * KP], Yh[GB * KP], Yl[GB * KP];
-    const int bi = blockIdx.x, bj = blockIdx.y, split = blockIdx.z;
-    if (symmetric && bj < bi) return;
-    const int i0 = bi * GB, j0 = bj * GB;
-    const long kbeg = (long)split * kchunk, kend = min(K, kbeg + kchunk);
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wr = (wave >> 1) * 64, wc = (wave & 1) * 64;
-
-    d4 tot[4][4];
-#pragma unroll
-    for (int a = 0; a < 4; ++a)
-#pragma unroll
-        for (int b = 0; b < 4; ++b) tot[a][b] = d4{0, 0, 0, 0};
-
-    for (long k0 = kbeg; k0 < kend; k0 += KB) {
-        {
-            const int col = tid >> 1, kk = (tid & 1) * 8;
-#pragma unroll
-            for (int t = 0; t < 8; ++t) {
-                const long k = k0 + kk + t;
-                const bool kin = k < kend;
-                const int gi = i0 + col, gj = j0 + col;
-                const double xv = (kin && gi < m) ? X[k + (long)gi * ldx] : 0.0;
-                const double yv = (kin && gj < n) ? Y[k + (long)gj * ldy] : 0.0;
-                const float xh = (float)xv, yh = (float)yv;
-                Xh[col * KP + kk + t] = xh;
-                Xl[col * KP + kk + t] = (float)(xv - (double)xh);
-                Yh[col * KP + kk + t] = yh;
-                Yl[col * KP + kk + t] = (float)(yv - (double)yh);
-            }
-        }
-        __syncthreads();
-        f4 acc[4][4];
-#pragma unroll
-        for (int a = 0; a < 4; ++a)
-#pragma unroll
-            for (int b = 0; b < 4; ++b) acc[a][b] = f4{0, 0, 0, 0};
-#pragma unroll
-        for (int ks = 0; ks < KB; ks += 4) {
-            float ah[4], al[4], bh[4], bl[4];
-#pragma unroll
-            for (int a = 0; a < 4; ++a) {
-                const int o = (wr + a * 16 + (lane & 15)) * KP + ks + (lane >> 4);
-                ah[a] = Xh[o];
-                al[a] = Xl[o];
-            }
-#pragma unroll
-            for (int b = 0; b < 4; ++b) {
-                const int o = (wc + b * 16 + (lane & 15)) * KP + ks + (lane >> 4);
-                bh[b] = Yh[o];
-                bl[b] = Yl[o];
-            }
-#pragma unroll
-            for (int a = 0; a < 4; ++a)
-#pragma unroll
-                for (int b = 0; b < 4; ++b) {
-                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(al[a], bh[b], acc[a][b], 0, 0, 0);
-                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(ah[a], bl[b], acc[a][b], 0, 0, 0);
-                    acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(ah[a], bh[b], acc[a][b], 0, 0, 0);
-                }
-        }
-#pragma unroll
-        for (int a = 0; a < 4; ++a)
-#pragma unroll
-            for (int b = 0; b < 4; ++b)
-#pragma unroll
-                for (int r = 0; r < 4; ++r) tot[a][b][r] += (double)acc[a][b][r];
-        __syncthreads();
-    }
-    double *P = part + (long)split * m * n;
-#pragma unroll
-    for (int a = 0; a < 4; ++a)
-#pragma unroll
-        for (int b = 0; b < 4; ++b)
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int i = i0 + wr + a * 16 + 4 * (lane >> 4) + r;
-                const int j = j0 + wc + b * 16 + (lane & 15);
-                if (i < m && j < n) P[(long)i * n + j] = tot[a][b][r];
-            }
-}
-
-// ---------------------------------------------------------------------------
-// BF16 split: x = hi + lo in BF16, three v_mfma_f32_16x16x32_bf16 per 32-deep step.
-// LDS holds each column's 32 k-values as bf16 (row stride 40 elements = 80 bytes).
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ __bf16 to_bf16(float v) { return __float2bfloat16(v); }
-__device__ __forceinline__ float from_bf16(__bf16 v) { return __bfloat162float(v); }
-
-__global__ __launch_bounds__(GT, 2) void gram_bf16x3_kernel(int m, int n, long K, long kchunk,
-                                                            const double *__restrict__ X, long ldx,
-                                                            const double *__restrict__ Y, long ldy,
-                                                            double *__restrict__ part, int symmetric)
-{
-    constexpr int KB = 32, KP = 40;
-    __shared__ __attribute__((aligned(16))) __bf16 Xh[GB * KP], Xl[GB * KP], Yh[GB * KP], Yl[GB * KP];
-    const int bi = blockIdx.x, bj = blockIdx.y, split = blockIdx.z;
-    if (symmetric && bj < bi) return;
-    const int i0 = bi * GB, j0 = bj * GB;
-    const long kbeg = (long)split * kchunk, kend = min(K, kbeg + kchunk);
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wr = (wave >> 1) * 64, wc = (wave & 1) * 64;
-
-    d4 tot[4][4];
-#pragma unroll
-    for (int a = 0; a < 4; ++a)
-#pragma unroll
-        for (int b = 0; b < 4; ++b) tot[a][b] = d4{0, 0, 0, 0};
-
-    for (long k0 = kbeg; k0 < kend; k0 += KB) {
-        {
-            const int col = tid >> 1, kk = (tid & 1) * 16;
-#pragma unroll
-            for (int t = 0; t < 16; ++t) {
-                const long k = k0 + kk + t;
-                const bool kin = k < kend;
-                const int gi = i0 + col, gj = j0 + col;
-                const double xv = (kin && gi < m) ? X[k + (long)gi * ldx] : 0.0;
-                const double yv = (kin && gj < n) ? Y[k + (long)gj * ldy] : 0.0;
-                const __bf16 xh = to_bf16((float)xv), yh = to_bf16((float)yv);
-                Xh[col * KP + kk + t] = xh;
-                Xl[col * KP + kk + t] = to_bf16((float)(xv - (double)from_bf16(xh)));
-                Yh[col * KP + kk + t] = yh;
-                Yl[col * KP + kk + t] = to_bf16((float)(yv - (double)from_bf16(yh)));
-            }
-        }
-        __syncthreads();
-        f4 acc[4][4];
-#pragma unroll
-        for (int a = 0; a < 4; ++a)
-#pragma unroll
-            for (int b = 0; b < 4; ++b) acc[a][b] = f4{0, 0, 0, 0};
-        bf8 ah[4], al[4], bh[4], bl[4];
-#pragma unroll
-        for (int a = 0; a < 4; ++a) {
-            const int o = (wr + a * 16 + (lane & 15)) * KP + 8 * (lane >> 4);  // 16-byte aligned
-            ah[a] = *reinterpret_cast<const bf8 *>(&Xh[o]);
-            al[a] = *reinterpret_cast<const bf8 *>(&Xl[o]);
-        }
-#pragma unroll
-        for (int b = 0; b < 4; ++b) {
-            const int o = (wc + b * 16 + (lane & 15)) * KP + 8 * (lane >> 4);
-            bh[b] = *reinterpret_cast<const bf8 *>(&Yh[o]);
-            bl[b] = *reinterpret_cast<const bf8 *>(&Yl[o]);
-        }
-#pragma unroll
-        for (int a = 0; a < 4; ++a)
-#pragma unroll
-            for (int b = 0; b < 4; ++b) {
-                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[a], bh[b], acc[a][b], 0, 0, 0);
-                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], bl[b], acc[a][b], 0, 0, 0);
-                acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[a], bh[b], acc[a][b], 0, 0, 0);
-            }
-#pragma unroll
-        for (int a = 0; a < 4; ++a)
-#pragma unroll
-            for (int b = 0; b < 4; ++b)
-#pragma unroll
-                for (int r = 0; r < 4; ++r) tot[a][b][r] += (double)acc[a][b][r];
-        __syncthreads();
-    }
-    double *P = part + (long)split * m * n;
-#pragma unroll
-    for (int a = 0; a < 4; ++a)
-#pragma unroll
-        for (int b = 0; b < 4; ++b)
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-                const int i = i0 + wr + a * 16 + 4 * (lane >> 4) + r;
-                const int j = j0 + wc + b * 16 + (lane & 15);
-                if (i < m && j < n) P[(long)i * n + j] = tot[a][b][r];
+                if (i < g.m && j < g.n) P[(long)i * g.n + j] = tot[a][b][r];
             }
 }
 
 // C (row-major m x n, ldc) (+)= sum over splits of the partial blocks; for symmetric
-// products the lower triangle is mirrored from the computed upper blocks.
+// products every entry below the diagonal is taken from its mirror above it (computed in
+// an upper or diagonal block), so the result is exactly symmetric.
 __global__ void gram_reduce_kernel(int m, int n, int nsplit, const double *__restrict__ part,
                                    double *__restrict__ C, long ldc, int accumulate, int symmetric)
 {
@@ -291,7 +276,7 @@ __global__ void gram_reduce_kernel(int m, int n, int nsplit, const double *__res
     if (g >= (long)m * n) return;
     int i = (int)(g / n), j = (int)(g - (long)i * n);
     int si = i, sj = j;
-    if (symmetric && (j / GB) < (i / GB)) {
+    if (symmetric && j < i) {
         si = j;
         sj = i;
     }
@@ -334,15 +319,34 @@ int irom_gram(int mode, int m, int n, long K, const double *d_X, long ldx, const
     long kchunk = (K + nsplit - 1) / nsplit;
     kchunk = (kchunk + 31) / 32 * 32;
     nsplit = (K + kchunk - 1) / kchunk;
-    const int symmetric = (d_X == d_Y && m == n && ldx == ldy) ? 1 : 0;
-    dim3 grid((m + irom::GB - 1) / irom::GB, (n + irom::GB - 1) / irom::GB, (unsigned)nsplit);
+    const int base = mode & 0xf;
+    const int symmetric = (m == n && ((d_X == d_Y && ldx == ldy) ||
+                                      (mode & IROM_GRAM_SYMMETRIC_RESULT))) ? 1 : 0;
+    irom::GramArgs ga;
+    ga.m = m;
+    ga.n = n;
+    ga.K = K;
+    ga.kchunk = kchunk;
+    ga.X = d_X;
+    ga.ldx = ldx;
+    ga.Y = d_Y;
+    ga.ldy = ldy;
+    ga.part = (double *)d_work;
+    const int nbi = (m + irom::GB - 1) / irom::GB;
+    ga.nbj = (n + irom::GB - 1) / irom::GB;
+    ga.ntiles = symmetric ? nbi * (nbi + 1) / 2 : nbi * ga.nbj;
+    ga.nsplit = (int)nsplit;
+    ga.sym = symmetric;
+    ga.vec = (ldx % 2 == 0 && ldy % 2 == 0 && (uintptr_t)d_X % 16 == 0 &&
+              (uintptr_t)d_Y % 16 == 0) ? 1 : 0;
+    const dim3 grid((unsigned)(8L * ga.ntiles * ((nsplit + 7) / 8)));
     double *part = (double *)d_work;
-    if (mode == IROM_GRAM_FP64)
-        irom::gram_f64_kernel<<<grid, irom::GT, 0, s>>>(m, n, K, kchunk, d_X, ldx, d_Y, ldy, part, symmetric);
-    else if (mode == IROM_GRAM_FP32X3)
-        irom::gram_f32x3_kernel<<<grid, irom::GT, 0, s>>>(m, n, K, kchunk, d_X, ldx, d_Y, ldy, part, symmetric);
-    else if (mode == IROM_GRAM_BF16X3)
-        irom::gram_bf16x3_kernel<<<grid, irom::GT, 0, s>>>(m, n, K, kchunk, d_X, ldx, d_Y, ldy, part, symmetric);
+    if (base == IROM_GRAM_FP64)
+        irom::gram_kernel<0><<<grid, irom::GT, 0, s>>>(ga);
+    else if (base == IROM_GRAM_FP32X3)
+        irom::gram_kernel<1><<<grid, irom::GT, 0, s>>>(ga);
+    else if (base == IROM_GRAM_BF16X3)
+        irom::gram_kernel<2><<<grid, irom::GT, 0, s>>>(ga);
     else {
         g_gerr = "irom_gram: unknown mode";
         return 1;

@@ -21,6 +21,7 @@ from . import FP64, MFError, Operator
 from . import rom as _rom
 
 GRAM_FP64, GRAM_FP32X3, GRAM_BF16X3 = 0, 1, 2
+GRAM_SYMMETRIC_RESULT = 16
 _SET = False
 
 
@@ -43,9 +44,14 @@ def _s(stream=None):
     return ctypes.c_void_p((stream or torch.cuda.current_stream()).cuda_stream)
 
 
-def gram(X, Y=None, mode=GRAM_FP64, out=None, accumulate=False, work=None, stream=None):
+def gram(X, Y=None, mode=GRAM_FP64, out=None, accumulate=False, work=None, stream=None,
+         symmetric_result=False):
     """C = X Y^T for row-major X (m, K) and Y (n, K) (= X^T Y of the column-major N x k
-    matrices).  X with Y=None computes the symmetric X X^T."""
+    matrices).  X with Y=None computes the symmetric X X^T; symmetric_result=True states
+    that X Y^T is symmetric (e.g. S (M S)^T with M symmetric): only the upper blocks are
+    computed and mirrored."""
+    if symmetric_result:
+        mode |= GRAM_SYMMETRIC_RESULT
     L = _lib()
     sym = Y is None
     Y = X if sym else Y

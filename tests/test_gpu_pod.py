@@ -156,3 +156,20 @@ def test_elemental_projection_vs_dense():
     f0 = proj.bwd.apply(u0)
     u1, _, _ = proj.project(f0)
     assert (torch.linalg.norm(u1 - u0) / torch.linalg.norm(u0)).item() < 10 * kmax * eps
+
+
+@pytest.mark.parametrize("m,K", [(300, 70_001), (512, 131_072), (129, 4_097)])
+def test_gram_symmetric_result(m, K):
+    """X diag(d) X^T with the symmetric-result flag (upper blocks only, mirrored) against
+    numpy FP64, for odd K (8-byte loads) and even K (16-byte loads), all three modes."""
+    print(f"[case] gram symmetric-result m={m} K={K}", flush=True)
+    rng = np.random.default_rng(m)
+    X = rng.standard_normal((m, K))
+    d = rng.uniform(0.5, 2.0, K)
+    Y = X * d
+    ref = X @ Y.T
+    Xd, Yd = _dev(X), _dev(Y)
+    for mode, tol in ((pod.GRAM_FP64, 1e-12), (pod.GRAM_FP32X3, 1e-6), (pod.GRAM_BF16X3, 1e-4)):
+        G = pod.gram(Xd, Yd, mode=mode, symmetric_result=True).cpu().numpy()
+        assert np.array_equal(G, G.T)
+        assert np.linalg.norm(G - ref) <= tol * np.linalg.norm(ref)

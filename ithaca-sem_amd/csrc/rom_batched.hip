@@ -66,14 +66,6 @@ __device__ __forceinline__ double block_max(double v, double *sv)
     return r;
 }
 
-// Orders LDS accesses of the lanes of one wave (no hardware barrier needed).
-__device__ __forceinline__ void wave_sync()
-{
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
 // Value of v in lane l of this wave (l uniform).
 __device__ __forceinline__ float read_lane(float v, int l)
 {
@@ -95,9 +87,12 @@ constexpr int PLD = NMAX + 4;
 struct PivLds {
     int posP[NMAX];    // panel-start row whose content is at row r (rows of the panel step)
     int posAll[NMAX];  // original row whose content is at row r (the final permutation)
-    int mark[NMAX];    // k0 of the panel step that put row r into list
     int list[NMAX];    // rows below the panel's top block that receive a top row
     int cnt;
+    double wv[2][NT / 64];  // per-wave pivot candidates (double-buffered by column parity)
+    int wi[2][NT / 64];     // their rows
+    int wo[2][NT / 64];     // panel-start rows of their contents
+    int corg[2];            // panel-start row of row c's content
 };
 
 // In-place LU with partial pivoting of the n x n row-major matrix L (ld n), right-looking
@@ -113,17 +108,13 @@ struct PivLds {
 //   3. A22 -= L21 U12 in TM x TN register tiles, both operands read from LDS, the next
 //      tile's global values loaded while the current one is computed.
 template <typename T, int NB, int TM, int TN>
-__device__ void block_lu(T *__restrict__ L, int n, T *P, T *U, PivLds &pv, int *zero)
+__device__ void block_lu(T *__restrict__ L, int n, T *P, T *U, T *R, PivLds &pv, int *zero)
 {
-    const int tid = threadIdx.x, lane = tid & 63;
-    for (int i = tid; i < n; i += NT) {
-        pv.posAll[i] = i;
-        pv.mark[i] = -1;
-    }
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    for (int i = tid; i < n; i += NT) pv.posAll[i] = i;
     for (int k0 = 0; k0 < n; k0 += NB) {
         const int nb = min(NB, n - k0), m = n - k0;
         // 1. panel to LDS (loads issued 8 at a time per thread)
-        for (int i = tid; i < m; i += NT) pv.posP[k0 + i] = k0 + i;
         for (int base = tid; base < m * nb; base += 8 * NT) {
             T v[8];
 #pragma unroll
@@ -140,69 +131,100 @@ __device__ void block_lu(T *__restrict__ L, int n, T *P, T *U, PivLds &pv, int *
             }
         }
         __syncthreads();
-        if (tid < 64) {
-            int cnt = 0;
-            for (int c = 0; c < nb; ++c) {
-                double v = -1.0;
-                int ix = 0x7fffffff;
+        // Panel factorisation by the whole workgroup: thread r owns panel row r (m <= NT)
+        // in registers, with the panel-start row its content came from (org).  Per column,
+        // one barrier: each wave's best candidate lane publishes its row, value and index,
+        // and the owner of row c publishes row c (LDS scratch R, double-buffered by column
+        // parity); after the barrier every thread picks the block winner, row c and the
+        // pivot row are exchanged, and every row below c is updated with the pivot row.
+        {
+            constexpr int NW = NT / 64;
+            const int r = tid;
+            const bool own = r < m;
+            T v[NB];
 #pragma unroll
-                for (int s = 0; s < NMAX / 64; ++s) {
-                    const int r = lane + 64 * s;
-                    if (r >= c && r < m) {
-                        const double a = fabs((double)P[c * PLD + r]);
-                        if (a > v) {
-                            v = a;
-                            ix = r;
+            for (int c = 0; c < NB; ++c) v[c] = (own && c < nb) ? P[c * PLD + r] : T(0);
+            int org = k0 + r;
+#pragma unroll
+            for (int c = 0; c < NB; ++c) {
+                if (c < nb) {
+                    const int par = c & 1;
+                    T *wrow = R + par * (NW + 1) * NB;  // NW candidate rows, then row c
+                    double a = (own && r >= c) ? fabs((double)v[c]) : -1.0;
+                    if (a != a) a = -1.0;  // NaN: never a pivot
+                    double wmax = a;
+#pragma unroll
+                    for (int off = 32; off > 0; off >>= 1) wmax = fmax(wmax, __shfl_xor(wmax, off));
+                    const unsigned long long hit = __ballot(a == wmax);
+                    const int wl = __builtin_ffsll((long long)hit) - 1;  // lowest lane
+                    if (lane == wl) {
+#pragma unroll
+                        for (int cc = 0; cc < NB; ++cc) wrow[wave * NB + cc] = v[cc];
+                        pv.wv[par][wave] = wmax;
+                        pv.wi[par][wave] = r;
+                        pv.wo[par][wave] = org;
+                    }
+                    if (r == c) {
+#pragma unroll
+                        for (int cc = 0; cc < NB; ++cc) wrow[NW * NB + cc] = v[cc];
+                        pv.corg[par] = org;
+                    }
+                    __syncthreads();
+                    double best = pv.wv[par][0];
+                    int p = pv.wi[par][0], pw = 0;
+#pragma unroll
+                    for (int w = 1; w < NW; ++w) {
+                        const double bw = pv.wv[par][w];
+                        const int iw = pv.wi[par][w];
+                        if (bw > best || (bw == best && iw < p)) {
+                            best = bw;
+                            p = iw;
+                            pw = w;
                         }
                     }
-                }
+                    if (tid == 0 && !(best > 0.0)) *zero = 1;
+                    const T *prow = wrow + pw * NB;
+                    if (best < 0.0) {  // no usable pivot: keep row c
+                        p = c;
+                        prow = wrow + NW * NB;
+                    }
+                    if (p != c) {
+                        if (r == c) {
 #pragma unroll
-                for (int off = 32; off > 0; off >>= 1) {
-                    const double ov = __shfl_xor(v, off);
-                    const int oi = __shfl_xor(ix, off);
-                    if (ov > v || (ov == v && oi < ix)) {
-                        v = ov;
-                        ix = oi;
-                    }
-                }
-                if (ix >= m) ix = c;  // a column of NaNs: no pivot found
-                if (lane == 0 && !(v > 0.0)) *zero = 1;
-                if (ix != c) {
-                    if (lane < nb) {
-                        const T t = P[lane * PLD + c];
-                        P[lane * PLD + c] = P[lane * PLD + ix];
-                        P[lane * PLD + ix] = t;
-                    }
-                    if (lane == 0) {
-                        const int r1 = k0 + c, r2 = k0 + ix;
-                        if (ix >= nb && pv.mark[r2] != k0) {
-                            pv.mark[r2] = k0;
-                            pv.list[cnt++] = r2;
+                            for (int cc = 0; cc < NB; ++cc) v[cc] = prow[cc];
+                            org = pv.wo[par][pw];
+                        } else if (r == p) {
+#pragma unroll
+                            for (int cc = 0; cc < NB; ++cc) v[cc] = wrow[NW * NB + cc];
+                            org = pv.corg[par];
                         }
-                        int t = pv.posP[r1];
-                        pv.posP[r1] = pv.posP[r2];
-                        pv.posP[r2] = t;
-                        t = pv.posAll[r1];
-                        pv.posAll[r1] = pv.posAll[r2];
-                        pv.posAll[r2] = t;
                     }
-                }
-                wave_sync();
-                const T piv = P[c * PLD + c];
+                    if (own && r > c) {
+                        const T l = v[c] / prow[c];
+                        v[c] = l;
 #pragma unroll
-                for (int s = 0; s < NMAX / 64; ++s) {
-                    const int r = lane + 64 * s;
-                    if (r > c && r < m) {
-                        const T l = P[c * PLD + r] / piv;
-                        P[c * PLD + r] = l;
-                        for (int cc = c + 1; cc < nb; ++cc) P[cc * PLD + r] -= l * P[cc * PLD + c];
+                        for (int cc = c + 1; cc < NB; ++cc) v[cc] -= l * prow[cc];
                     }
                 }
-                wave_sync();
             }
-            if (lane == 0) pv.cnt = cnt;
+            // factored panel back to LDS; posP from org; the final permutation advances by
+            // this panel's; rows below the top block holding a top row are listed.
+            int prev = 0;
+            if (own) {
+#pragma unroll
+                for (int c = 0; c < NB; ++c)
+                    if (c < nb) P[c * PLD + r] = v[c];
+                pv.posP[k0 + r] = org;
+                prev = pv.posAll[org];
+            }
+            if (tid == 0) pv.cnt = 0;
+            __syncthreads();
+            if (own) {
+                pv.posAll[k0 + r] = prev;
+                if (r >= nb && org != k0 + r) pv.list[atomicAdd(&pv.cnt, 1)] = k0 + r;
+            }
+            __syncthreads();
         }
-        __syncthreads();
         // panel back to global
         for (int idx = tid; idx < m * nb; idx += NT) {
             const int r = idx / nb, c = idx - r * nb;
@@ -436,6 +458,7 @@ batched_solve_kernel(Args a)
     __shared__ T U[NB * PLD];
     __shared__ double b64[NMAX], x64[NMAX], r64[NMAX];
     __shared__ T yv[NMAX], ys[NMAX];
+    __shared__ T R[2 * (NT / 64 + 1) * NB];
     __shared__ PivLds pv;
     __shared__ double sv[NT / 64];
     __shared__ int s_inst, s_zero;
@@ -494,7 +517,7 @@ batched_solve_kernel(Args a)
         for (int i = tid; i < n; i += NT) b64[i] = a.b[vo + i];
         __syncthreads();
 
-        block_lu<T, NB, TM, TN>(LU, n, P, U, pv, &s_zero);
+        block_lu<T, NB, TM, TN>(LU, n, P, U, R, pv, &s_zero);
         for (int i = tid; i < n; i += NT) yv[i] = (T)b64[pv.posAll[i]];
         __syncthreads();
         lu_solve_rows<T>(LU, n, yv, ys);

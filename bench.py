@@ -30,7 +30,8 @@ def parse():
     p.add_argument("--steps", type=int, default=50)
     p.add_argument("--warmup", type=int, default=5)
     p.add_argument("--precision", choices=["fp64", "fp32"], default="fp64")
-    p.add_argument("--op", choices=["helmholtz", "bwdtrans", "iproduct", "copy"],
+    p.add_argument("--op", choices=["helmholtz", "bwdtrans", "iproduct", "physderiv",
+                                    "iprodderiv", "copy"],
                    default="helmholtz",
                    help="copy: plain 1 GiB device copy (HBM bandwidth / counter calibration)")
     p.add_argument("--n", type=int, default=48, help="elements per direction")
@@ -49,6 +50,10 @@ def algorithmic_bytes(op, nm, nq, elt):
         return elt * (2 * M + 7 * Q)      # u_hat + out + 7 geometry values per point
     if op == "bwdtrans":
         return elt * (M + Q)              # u_hat + u
+    if op == "physderiv":
+        return elt * 13 * Q               # u + 9 derivative factors + 3 derivatives per point
+    if op == "iprodderiv":
+        return elt * (12 * Q + M)         # 3 fields + 9 w J df per point + out
     return elt * (2 * Q + M)              # f + J w + out
 
 
@@ -407,6 +412,61 @@ def pod_cpu_baseline(cpu_slice, G64, ev, r):
             "eig_rel_err_top_r_gpu_vs_cpu": float(np.max(np.abs(ev[:r] - ev_cpu[:r])
                                                          / ev_cpu[:r]))}
 
+def deriv_bench(args):
+    """PhysDeriv_Hex_Deformed and IProductWRTDerivBase_Hex_Deformed on the 48^3 mesh
+    (FP64, 50 applies each), with their algorithmic bytes per apply and the CPU oracle
+    on a 4096-element sample."""
+    import numpy as np
+    import torch
+
+    import ithaca_hip as H
+    from ithaca_hip import mesh
+
+    w = mesh.workload_hex_helmholtz(args.n)
+    nm, nq, E = w["nm"], w["nq"], w["nElmt"]
+    Q = nq ** 3
+    b = H.basis_modified_a(nm, nq)
+    verts = torch.from_numpy(w["verts"]).cuda()
+    gj, gd = H.geom_device("hex", verts, nq, b["z"], True)
+    res = {}
+    for name, key in (("physderiv", "PhysDeriv_Hex_Deformed"),
+                      ("iprodderiv", "IProductWRTDerivBase_Hex_Deformed")):
+        op = H.Operator(key, nm, nq, E, basis=b)
+        if op.needs_jac():
+            op.set_jac(gj)
+        op.set_df(gd)
+        x = torch.rand(op.in_size, dtype=torch.float64, device="cuda").mul_(2).sub_(1)
+        y = torch.empty(op.out_size, dtype=torch.float64, device="cuda")
+        ms = _event_time(lambda: op.apply(x, y), 50, 5)
+        ab = algorithmic_bytes(name, nm, nq, 8) * E
+        res[name] = {"key": key, "us_per_apply": ms * 1e3, "algorithmic_bytes": ab,
+                     "achieved_GBs": ab / (ms * 1e-3) / 1e9,
+                     "frac_of_8TBs": ab / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS}
+        del op, x, y
+    if not args.no_cpu:
+        import oracle as O
+        threads = min(16, os.cpu_count() or 1)
+        O.set_threads(threads)
+        Es = 4096
+        bo = O.basis(nm, nq)
+        jac, df = O.geom_hex(np.ascontiguousarray(w["verts"][:Es]), nq, bo["z"], True)
+        f = np.random.default_rng(0).uniform(-1, 1, 3 * Es * Q)
+        for name, fn in (("physderiv", lambda: O.physderiv("hex", bo, Es, True, df, f[:Es * Q])),
+                         ("iprodderiv", lambda: O.iprod_deriv("hex", bo, Es, True, jac, df,
+                                                              np.split(f, 3)))):
+            fn()
+            t0 = time.perf_counter()
+            for _ in range(5):
+                fn()
+            t = (time.perf_counter() - t0) / 5
+            res[name]["cpu_baseline"] = {"us_per_apply_scaled_to_full_mesh": t * E / Es * 1e6,
+                                         "cores": threads, "kind": "port",
+                                         "sample": f"{Es} elements, 5 applies, oracle"}
+    del gj, gd
+    torch.cuda.empty_cache()
+    return res
+
+
 def quad_bench(args):
     """32x32 regular quads, P=4 (5 modes), 6 GLL points, lambda=1, seed 0, 100 applies."""
     import numpy as np
@@ -491,16 +551,18 @@ def main():
     b = H.basis_modified_a(nm, nq)
     verts = torch.from_numpy(w["verts"]).to(dev)
     key = {"helmholtz": "Helmholtz_Hex_Deformed", "bwdtrans": "BwdTrans_Hex_Regular",
-           "iproduct": "IProduct_Hex_Deformed"}[args.op]
+           "iproduct": "IProduct_Hex_Deformed", "physderiv": "PhysDeriv_Hex_Deformed",
+           "iprodderiv": "IProductWRTDerivBase_Hex_Deformed"}[args.op]
     op = H.Operator(key, nm, nq, E, precision=prec, basis=b)
-    if op.needs_jac():
+    if op.needs_jac() or op.needs_df():
         gj, gd = H.geom_device("hex", verts, nq, b["z"], True)
-        op.set_jac(gj)
+        if op.needs_jac():
+            op.set_jac(gj)
         if op.needs_df():
             op.set_df(gd)
         del gj, gd
     op.set_lambda(w["lam"])
-    if args.op == "iproduct":
+    if args.op in ("iproduct", "physderiv", "iprodderiv"):
         x = torch.rand(op.in_size, dtype=torch.float64, device=dev).mul_(2).sub_(1).to(dtype)
     else:
         x = torch.from_numpy(w["u"]).to(dev, dtype)
@@ -562,6 +624,7 @@ def main():
         if not args.no_rom:
             out["batched_reduced_solves"] = rom_bench(args)
         out["quad_helmholtz"] = quad_bench(args)
+        out["deriv_operators"] = deriv_bench(args)
         if not args.no_pod:
             try:
                 out["pod_offline"] = pod_bench(args)

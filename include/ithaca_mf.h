@@ -45,7 +45,17 @@ enum imf_precision { IMF_FP64 = 0, IMF_FP32 = 1 };
  * reference kernels the operator requires homogeneous modes/points and supports
  * nm in 2..8 with nq in nm..2*nm (Helmholtz.h:669-760).  No padding restriction on
  * nElmt (the reference's vec_t::width padding is handled internally). */
-/* Also accepted, with no MatrixFreeOps counterpart: "Mass_Hex_Deformed" and
+/* Also accepted, as in the reference's MatrixFree factory:
+ *   "PhysDeriv_{Hex,Quad}_{Regular,Deformed}" (MatrixFreeOps/PhysDeriv.cpp:14-36;
+ *     PhysDeriv.h:260-642): in = nElmt*Q point values, out = dim consecutive arrays of
+ *     nElmt*Q (the reference's out[0..dim-1]), out_c = sum_a df[c*dim+a] d/dxi_a in;
+ *     needs SetDF only; any nq in 2..16 (nm is not restricted).
+ *   "IProductWRTDerivBase_{Hex,Quad}_{Regular,Deformed}" (IProductWRTDerivBaseHex.cpp,
+ *     IProductWRTDerivBaseQuad.cpp; IProductWRTDerivBase.h:371-, :1068-1343): in = dim
+ *     consecutive arrays of nElmt*Q (the reference's in[0..dim-1]), out = nElmt*M;
+ *     needs SetJac and SetDF.  Quads take 2D coordinates (dim*dim = 4 factors).
+ * Neither has a transpose (imf_apply_transpose_device reports IMF_ERR_UNSUPPORTED).
+ * Also accepted, with no MatrixFreeOps counterpart: "Mass_Hex_Deformed" and
  * "Mass_Hex_Regular", the elemental mass action out = B^T diag(w J) B in of
  * StdRegions/StdExpansion.cpp:647-661 (MassMatrixOp_MatFree: BwdTrans, then
  * IProductWRTBase) fused into one kernel; needs SetJac only; in and out are

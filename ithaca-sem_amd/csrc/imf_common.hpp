@@ -14,7 +14,14 @@
 namespace imf {
 
 // Operator kinds and element shapes handled by the library.
-enum Kind : int { kBwdTrans = 0, kIProduct = 1, kHelmholtz = 2, kMass = 3 };
+enum Kind : int {
+    kBwdTrans = 0,
+    kIProduct = 1,
+    kHelmholtz = 2,
+    kMass = 3,
+    kPhysDeriv = 4,   // quadrature values -> dim physical derivatives
+    kIProdDeriv = 5   // IProductWRTDerivBase: dim quadrature fields -> coefficients
+};
 enum Shape : int { kQuad = 2, kHex = 3 };
 
 // Geometry modes.  kUnit: no geometry (the Euclidean transposes).  kDeformed:
@@ -33,6 +40,12 @@ struct Tab {
     T B[NM * NQ];
     T D[NQ * NQ];
     T w[NQ];
+};
+
+// Collocation-derivative table of the points-only operators (PhysDeriv).
+template <int NQ, typename T>
+struct DTab {
+    T D[NQ * NQ];
 };
 
 // Launch shape of the hex kernels: NQ*NQ threads own the NQ*NQ point columns of one
@@ -58,11 +71,14 @@ struct QuadCfg {
 //   Helmholtz hex : Jw, Jw*G00, Jw*G01, Jw*G02, Jw*G11, Jw*G12, Jw*G22
 //   Helmholtz quad: Jw, Jw*G00, Jw*G01, Jw*G11
 //   IProduct, Mass: Jw
+//   PhysDeriv     : df[d]            (d = c*dim + a, the GetDerivFactors components)
+//   IProductWRTDerivBase: Jw*df[d]
 // with G^{ab} = sum_c df[c*dim+a] df[c*dim+b] (Helmholtz.h:858-966, :172-190) and
 // Jw = J w_i w_j (w_k).  Regular elements store J instead of Jw (the kernel applies w).
 constexpr int geo_comps(int kind, int shape)
 {
-    return kind == kHelmholtz ? (shape == kHex ? 7 : 4) : 1;
+    return kind == kHelmholtz ? (shape == kHex ? 7 : 4)
+         : (kind == kPhysDeriv || kind == kIProdDeriv) ? (shape == kHex ? 9 : 4) : 1;
 }
 
 // Per-element stride (in scalars) of deformed packed geometry: the nc*Q values of an

@@ -52,19 +52,28 @@ size_t df_len(const imf_op op) { return (size_t)op->dim * op->dim * jac_len(op);
 size_t geom_len(const imf_op op)
 {
     const int nc = imf::geo_comps(op->kind, op->shape);
-    if (op->kind == imf::kHelmholtz && op->shape == imf::kHex && op->deformed)
+    if ((op->kind == imf::kHelmholtz || op->kind == imf::kIProdDeriv) &&
+        op->shape == imf::kHex && op->deformed)
         return (size_t)op->nElmt * imf::geo_stride((long)nc * op->Q, (int)elt_size(op));
     return (size_t)nc * jac_len(op);
 }
 
-bool needs_jac(const imf_op op) { return op->kind != imf::kBwdTrans; }
-bool needs_df(const imf_op op) { return op->kind == imf::kHelmholtz; }
+bool needs_jac(const imf_op op)
+{
+    return op->kind != imf::kBwdTrans && op->kind != imf::kPhysDeriv;
+}
+bool needs_df(const imf_op op)
+{
+    return op->kind == imf::kHelmholtz || op->kind == imf::kPhysDeriv ||
+           op->kind == imf::kIProdDeriv;
+}
 
 // Re-pack the streamed geometry once everything it depends on has been set.
 int maybe_pack(imf_op op)
 {
     op->geom_ready = false;
-    if (!needs_jac(op) || !op->d_jac) return IMF_OK;
+    if (!needs_jac(op) && !needs_df(op)) return IMF_OK;
+    if (needs_jac(op) && !op->d_jac) return IMF_OK;
     if (needs_df(op) && !op->d_df) return IMF_OK;
     if (!op->d_geom) {
         hipError_t e = hipMalloc(&op->d_geom, geom_len(op) * elt_size(op));
@@ -101,6 +110,12 @@ void fill_tab(imf_op op, const double *B, const double *D, const double *w)
 {
     const int nm = op->nm, nq = op->nq;
     std::vector<T> img;
+    if (op->kind == imf::kPhysDeriv) {  // DTab<NQ,T>: the derivative matrix only
+        for (int i = 0; i < nq * nq; ++i) img.push_back(T(D[i]));
+        op->tab.resize(img.size() * sizeof(T));
+        std::memcpy(op->tab.data(), img.data(), op->tab.size());
+        return;
+    }
     img.reserve(nm * nq + nq * nq + nq);
     for (int i = 0; i < nm * nq; ++i) img.push_back(T(B[i]));
     for (int i = 0; i < nq * nq; ++i) img.push_back(T(D[i]));
@@ -118,9 +133,10 @@ int check_op(const imf_op op)
 int run(imf_op op, imf::LaunchFn fn, const void *in, void *out, void *stream, int nvec = 1,
         long ldin = 0, long ldout = 0)
 {
-    if (needs_jac(op) && !op->geom_ready)
-        return fail(IMF_ERR_STATE, needs_df(op) ? "Jacobian and derivative factors not set"
-                                                : "Jacobian not set");
+    if ((needs_jac(op) || needs_df(op)) && !op->geom_ready)
+        return fail(IMF_ERR_STATE, !needs_jac(op) ? "derivative factors not set"
+                                   : needs_df(op) ? "Jacobian and derivative factors not set"
+                                                  : "Jacobian not set");
     if (!in || !out) return fail(IMF_ERR_ARG, "null input/output array");
     imf::LaunchArgs a{op->tab.data(), op->nElmt, op->lambda, op->d_geom, in, out,
                       (hipStream_t)stream, nvec, ldin, ldout};
@@ -151,6 +167,8 @@ int imf_create(const char *key, int dim, const int *nm, const int *nq,
     else if (starts("IProduct_")) kind = imf::kIProduct;
     else if (starts("Helmholtz_")) kind = imf::kHelmholtz;
     else if (starts("Mass_")) kind = imf::kMass;
+    else if (starts("PhysDeriv_")) kind = imf::kPhysDeriv;
+    else if (starts("IProductWRTDerivBase_")) kind = imf::kIProdDeriv;
     else return fail(IMF_ERR_UNSUPPORTED, "No such operator key: " + k);
     const std::string rest = k.substr(k.find('_') + 1);
     if (rest == "Quad_Regular") { shape = imf::kQuad; deformed = 0; }
@@ -178,11 +196,13 @@ int imf_create(const char *key, int dim, const int *nm, const int *nq,
             return fail(IMF_ERR_UNSUPPORTED, "MatrixFree requires identical bases in all directions");
     }
     const int m = nm[0], q = nq[0];
-    if (m < 2 || m > 8 || q < m || q > 2 * m)
-        return fail(IMF_ERR_UNSUPPORTED, std::string(kind == imf::kHelmholtz ? "Helmholtz" :
-                                                     kind == imf::kIProduct ? "IProduct" :
-                                                     kind == imf::kMass ? "Mass" : "BwdTrans") +
-                                             (shape == imf::kHex ? "Hex" : "Quad") +
+    const char *oname = kind == imf::kHelmholtz ? "Helmholtz" : kind == imf::kIProduct ? "IProduct"
+                      : kind == imf::kMass ? "Mass" : kind == imf::kPhysDeriv ? "PhysDeriv"
+                      : kind == imf::kIProdDeriv ? "IProductWRTDB" : "BwdTrans";
+    const bool combo_ok = kind == imf::kPhysDeriv ? (q >= 2 && q <= 16)
+                                                 : (m >= 2 && m <= 8 && q >= m && q <= 2 * m);
+    if (!combo_ok)
+        return fail(IMF_ERR_UNSUPPORTED, std::string(oname) + (shape == imf::kHex ? "Hex" : "Quad") +
                                              ": # of modes / points combo not initialised.");
     imf_op op = new imf_op_s();
     op->kind = kind; op->shape = shape; op->deformed = deformed; op->prec = precision;
@@ -195,7 +215,10 @@ int imf_create(const char *key, int dim, const int *nm, const int *nq,
     const int geo = kind == imf::kBwdTrans ? imf::kUnit : (deformed ? imf::kDeformed : imf::kRegular);
     // BwdTrans^T uses no geometry; IProduct^T uses the IProduct geometry.
     const int geo_t = kind == imf::kBwdTrans ? imf::kUnit : geo;
-    if (shape == imf::kHex) {
+    if (kind == imf::kPhysDeriv) {
+        op->fwd = imf::lookup_physderiv(shape, q, geo, precision);
+        op->tr = nullptr;
+    } else if (shape == imf::kHex) {
         op->fwd = imf::lookup_hex(m, kind, imf::kForward, geo, q, precision);
         op->tr = imf::lookup_hex(m, kind, imf::kTranspose, geo_t, q, precision);
     } else {
@@ -230,13 +253,22 @@ int imf_needs_df(imf_op op) { return op && needs_df(op) ? 1 : 0; }
 size_t imf_in_size(imf_op op)
 {
     if (!op) return 0;
-    return (size_t)op->nElmt * (op->kind == imf::kIProduct ? op->Q : op->M);
+    switch (op->kind) {
+    case imf::kIProduct:
+    case imf::kPhysDeriv: return (size_t)op->nElmt * op->Q;
+    case imf::kIProdDeriv: return (size_t)op->dim * op->nElmt * op->Q;
+    default: return (size_t)op->nElmt * op->M;
+    }
 }
 
 size_t imf_out_size(imf_op op)
 {
     if (!op) return 0;
-    return (size_t)op->nElmt * (op->kind == imf::kBwdTrans ? op->Q : op->M);
+    switch (op->kind) {
+    case imf::kBwdTrans: return (size_t)op->nElmt * op->Q;
+    case imf::kPhysDeriv: return (size_t)op->dim * op->nElmt * op->Q;
+    default: return (size_t)op->nElmt * op->M;
+    }
 }
 
 int imf_set_jac(imf_op op, const double *jac, size_t n)

@@ -91,6 +91,47 @@ __global__ void pack_jw(long n, int Q, int nq, int deformed, Pts w,
     out[g] = T(jac[g] * wq);
 }
 
+// PhysDeriv: df converted to the operator's precision, layout unchanged ([d][n]).
+template <typename T>
+__global__ void pack_df(long n, const double *__restrict__ df, T *__restrict__ out)
+{
+    const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g < n) out[g] = T(df[g]);
+}
+
+// IProductWRTDerivBase, deformed: out[e][d][q] = J w_i w_j (w_k) df[d] (hex element blocks
+// padded to whole 128-byte lines); regular: out[e][d] = J df[d].
+template <typename T, int DIM>
+__global__ void pack_ipd_deformed(long nPts, int Q, int nq, Pts w,
+                                  const double *__restrict__ jac,
+                                  const double *__restrict__ df, T *__restrict__ out)
+{
+    constexpr int NDF = DIM * DIM;
+    const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= nPts) return;
+    const long e = g / Q;
+    const int q = (int)(g - e * Q);
+    const int i = q % nq, j = (q / nq) % nq, k = q / (nq * nq);
+    double wq = w.v[i] * w.v[j];
+    if (DIM == 3) wq *= w.v[k];
+    const double jw = jac[g] * wq;
+    const long stride = DIM == 3 ? geo_stride((long)NDF * Q, sizeof(T)) : (long)NDF * Q;
+    T *o = out + e * stride + q;
+#pragma unroll
+    for (int d = 0; d < NDF; ++d) o[(long)d * Q] = T(jw * df[(long)d * nPts + g]);
+}
+
+template <typename T, int DIM>
+__global__ void pack_ipd_regular(long nElmt, const double *__restrict__ jac,
+                                 const double *__restrict__ df, T *__restrict__ out)
+{
+    constexpr int NDF = DIM * DIM;
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nElmt) return;
+#pragma unroll
+    for (int d = 0; d < NDF; ++d) out[e * NDF + d] = T(jac[e] * df[(long)d * nElmt + e]);
+}
+
 hipError_t pack_geometry(int kind, int shape, int deformed, int prec, long nElmt, int nq,
                          const double *w, const double *d_jac, const double *d_df,
                          void *d_out, hipStream_t s)
@@ -99,6 +140,37 @@ hipError_t pack_geometry(int kind, int shape, int deformed, int prec, long nElmt
     for (int i = 0; i < nq && i < 16; ++i) pw.v[i] = w[i];
     const int Q = shape == kHex ? nq * nq * nq : nq * nq;
     const int bs = 256;
+    const int ndf = shape == kHex ? 9 : 4;
+    if (kind == kPhysDeriv) {
+        const long n = (long)ndf * (deformed ? nElmt * Q : nElmt);
+        const dim3 g((unsigned)((n + bs - 1) / bs));
+        if (prec == 0) pack_df<double><<<g, bs, 0, s>>>(n, d_df, (double *)d_out);
+        else pack_df<float><<<g, bs, 0, s>>>(n, d_df, (float *)d_out);
+        return hipGetLastError();
+    }
+    if (kind == kIProdDeriv) {
+        if (deformed) {
+            const long n = nElmt * Q;
+            const dim3 g((unsigned)((n + bs - 1) / bs));
+            if (shape == kHex) {
+                if (prec == 0) pack_ipd_deformed<double, 3><<<g, bs, 0, s>>>(n, Q, nq, pw, d_jac, d_df, (double *)d_out);
+                else pack_ipd_deformed<float, 3><<<g, bs, 0, s>>>(n, Q, nq, pw, d_jac, d_df, (float *)d_out);
+            } else {
+                if (prec == 0) pack_ipd_deformed<double, 2><<<g, bs, 0, s>>>(n, Q, nq, pw, d_jac, d_df, (double *)d_out);
+                else pack_ipd_deformed<float, 2><<<g, bs, 0, s>>>(n, Q, nq, pw, d_jac, d_df, (float *)d_out);
+            }
+        } else {
+            const dim3 g((unsigned)((nElmt + bs - 1) / bs));
+            if (shape == kHex) {
+                if (prec == 0) pack_ipd_regular<double, 3><<<g, bs, 0, s>>>(nElmt, d_jac, d_df, (double *)d_out);
+                else pack_ipd_regular<float, 3><<<g, bs, 0, s>>>(nElmt, d_jac, d_df, (float *)d_out);
+            } else {
+                if (prec == 0) pack_ipd_regular<double, 2><<<g, bs, 0, s>>>(nElmt, d_jac, d_df, (double *)d_out);
+                else pack_ipd_regular<float, 2><<<g, bs, 0, s>>>(nElmt, d_jac, d_df, (float *)d_out);
+            }
+        }
+        return hipGetLastError();
+    }
     if (kind == kHelmholtz) {
         if (deformed) {
             const long n = nElmt * Q;

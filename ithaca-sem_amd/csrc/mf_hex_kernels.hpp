@@ -499,4 +499,197 @@ __global__ __launch_bounds__(HexCfg<NQ>::BLOCK) void hex_helmholtz_kernel(
     store_modes_block<NM, NQ, T>(out + e0 * M, nv, [&](int sl) { return lds[sl][2]; });
 }
 
+// Reference-space derivatives of the element's quadrature values u (held as this thread's
+// k-column) along i and j, left in bi / bj in [k][j][i] order (PhysDerivTensor3DKernel,
+// PhysDerivKernels.hpp:219-294); bu holds u in the same order.  Ends synchronised.
+template <int NQ, typename T, typename DT>
+__device__ __forceinline__ void hex_deriv_ij(const DT &D, bool active, int t, const T *bu,
+                                             T *bi, T *bj)
+{
+    if (active) {
+        T v[NQ];
+        {
+            const int pj = t % NQ, pk = t / NQ;
+#pragma unroll
+            for (int l = 0; l < NQ; ++l) v[l] = bu[(pk * NQ + pj) * NQ + l];
+#pragma unroll
+            for (int i = 0; i < NQ; ++i) {
+                T s = v[0] * D[i];
+#pragma unroll
+                for (int l = 1; l < NQ; ++l) s = fma(v[l], D[l * NQ + i], s);
+                bi[(pk * NQ + pj) * NQ + i] = s;
+            }
+        }
+        {
+            const int pi = t % NQ, pk = t / NQ;
+#pragma unroll
+            for (int l = 0; l < NQ; ++l) v[l] = bu[(pk * NQ + l) * NQ + pi];
+#pragma unroll
+            for (int j = 0; j < NQ; ++j) {
+                T s = v[0] * D[j];
+#pragma unroll
+                for (int l = 1; l < NQ; ++l) s = fma(v[l], D[l * NQ + j], s);
+                bj[(pk * NQ + j) * NQ + pi] = s;
+            }
+        }
+    }
+    __syncthreads();
+}
+
+// PhysDeriv (PhysDeriv.h:509-600, PhysDerivKernels.hpp:296-374): out_c = sum_a
+// df[c*3+a] d_a with d_a the collocation derivatives of the quadrature values u.  in and
+// out_c (at out + c*nElmt*Q) are point arrays; df is the GetDerivFactors array in the
+// operator's precision: [d][nElmt*Q] (deformed) or [d][nElmt] (regular).
+template <int NQ, typename T, int GEO>
+__global__ __launch_bounds__(HexCfg<NQ>::BLOCK) void hex_physderiv_kernel(
+    const DTab<NQ, T> tab, int nElmt, const T *__restrict__ in, const T *__restrict__ df,
+    T *__restrict__ out)
+{
+    using C = HexCfg<NQ>;
+    constexpr int Q = NQ * NQ * NQ, TPE = C::TPE;
+    __shared__ T lds[C::EPB][3][Q];
+    const int eb = threadIdx.x / TPE;
+    const int t = threadIdx.x - eb * TPE;
+    const bool active = eb < C::EPB;
+    const int es = active ? eb : 0;
+    const long e = (long)blockIdx.x * C::EPB + eb;
+    const bool valid = active && e < nElmt;
+    T *bu = lds[es][0], *b0 = lds[es][1], *b1 = lds[es][2];
+
+    T u[NQ];
+#pragma unroll
+    for (int k = 0; k < NQ; ++k) u[k] = valid ? ld_stream(in + e * Q + k * TPE + t) : T(0);
+    if (active) {
+#pragma unroll
+        for (int k = 0; k < NQ; ++k) bu[k * TPE + t] = u[k];
+    }
+    __syncthreads();
+    hex_deriv_ij<NQ, T>(tab.D, active, t, bu, b0, b1);
+    if (!valid) return;
+    const long nt = (long)nElmt * Q;
+#pragma unroll
+    for (int k = 0; k < NQ; ++k) {
+        T d2 = u[0] * tab.D[k];
+#pragma unroll
+        for (int l = 1; l < NQ; ++l) d2 = fma(u[l], tab.D[l * NQ + k], d2);
+        const T d0 = b0[k * TPE + t], d1 = b1[k * TPE + t];
+        T f[9];
+        if (GEO == kDeformed) {
+#pragma unroll
+            for (int d = 0; d < 9; ++d) f[d] = ld_stream(df + d * nt + e * Q + k * TPE + t);
+        } else {
+#pragma unroll
+            for (int d = 0; d < 9; ++d) f[d] = df[(long)d * nElmt + e];
+        }
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+            st_stream(out + c * nt + e * Q + k * TPE + t,
+                      fma(d2, f[c * 3 + 2], fma(d1, f[c * 3 + 1], d0 * f[c * 3])));
+    }
+}
+
+// IProductWRTDerivBase (IProductWRTDerivBase.h:1231-1343): out = sum_a (D_a B)^T
+// diag(w J) tmp_a with tmp_a = sum_c df[c*3+a] in_c (in_c at in + c*nElmt*Q), the three
+// reference IProducts with dbdata collapsed into one D^T per direction and one
+// (B x B x B)^T (dbdata = D bdata).  Packed geometry: deformed [e][GS], GS = 9Q rounded
+// up to 128-byte lines, w J df[d] at d*Q; regular [e][9] = J df[d] (weights applied here).
+template <int NM, int NQ, typename T, int GEO>
+__global__ __launch_bounds__(HexCfg<NQ>::BLOCK) void hex_iprod_deriv_kernel(
+    const Tab<NM, NQ, T> tab, int nElmt, const T *__restrict__ in,
+    const T *__restrict__ geom, T *__restrict__ out)
+{
+    using C = HexCfg<NQ>;
+    constexpr int M = NM * NM * NM, Q = NQ * NQ * NQ, TPE = C::TPE;
+    constexpr long GS = geo_stride(9 * Q, sizeof(T));
+    __shared__ T lds[C::EPB][4][Q];
+    const int eb = threadIdx.x / TPE;
+    const int t = threadIdx.x - eb * TPE;
+    const bool active = eb < C::EPB;
+    const int es = active ? eb : 0;
+    const long e = (long)blockIdx.x * C::EPB + eb;
+    const bool valid = active && e < nElmt;
+    T *buf0 = lds[es][0], *buf1 = lds[es][1], *buf2 = lds[es][2], *buf3 = lds[es][3];
+    const long nt = (long)nElmt * Q;
+
+    T g0[NQ], g1[NQ], g2[NQ];
+    if (valid) {
+        const int ci = t % NQ, cj = t / NQ;
+#pragma unroll
+        for (int k = 0; k < NQ; ++k) {
+            const long o = e * Q + k * TPE + t;
+            const T x0 = ld_stream(in + o), x1 = ld_stream(in + nt + o),
+                    x2 = ld_stream(in + 2 * nt + o);
+            T f[9];
+            if (GEO == kDeformed) {
+#pragma unroll
+                for (int d = 0; d < 9; ++d) f[d] = ld_stream(geom + e * GS + d * Q + k * TPE + t);
+            } else {
+                const T wq = tab.w[ci] * tab.w[cj] * tab.w[k];
+#pragma unroll
+                for (int d = 0; d < 9; ++d) f[d] = geom[e * 9 + d] * wq;
+            }
+            g0[k] = fma(f[6], x2, fma(f[3], x1, f[0] * x0));
+            g1[k] = fma(f[7], x2, fma(f[4], x1, f[1] * x0));
+            g2[k] = fma(f[8], x2, fma(f[5], x1, f[2] * x0));
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < NQ; ++k) g0[k] = g1[k] = g2[k] = T(0);
+    }
+    if (active) {
+#pragma unroll
+        for (int k = 0; k < NQ; ++k) {
+            buf2[k * TPE + t] = g0[k];
+            buf3[k * TPE + t] = g1[k];
+        }
+    }
+    // D^T along k in registers
+    T acc[NQ];
+#pragma unroll
+    for (int k = 0; k < NQ; ++k) {
+        T s = g2[0] * tab.D[k * NQ];
+#pragma unroll
+        for (int l = 1; l < NQ; ++l) s = fma(g2[l], tab.D[k * NQ + l], s);
+        acc[k] = s;
+    }
+    __syncthreads();
+    // D^T along i (pencils of thread (j,k)) and along j (pencils of thread (i,k))
+    if (active) {
+        T v[NQ];
+        {
+            const int pj = t % NQ, pk = t / NQ;
+#pragma unroll
+            for (int l = 0; l < NQ; ++l) v[l] = buf2[(pk * NQ + pj) * NQ + l];
+#pragma unroll
+            for (int i = 0; i < NQ; ++i) {
+                T s = v[0] * tab.D[i * NQ];
+#pragma unroll
+                for (int l = 1; l < NQ; ++l) s = fma(v[l], tab.D[i * NQ + l], s);
+                buf0[(pk * NQ + pj) * NQ + i] = s;
+            }
+        }
+        {
+            const int pi = t % NQ, pk = t / NQ;
+#pragma unroll
+            for (int l = 0; l < NQ; ++l) v[l] = buf3[(pk * NQ + l) * NQ + pi];
+#pragma unroll
+            for (int j = 0; j < NQ; ++j) {
+                T s = v[0] * tab.D[j * NQ];
+#pragma unroll
+                for (int l = 1; l < NQ; ++l) s = fma(v[l], tab.D[j * NQ + l], s);
+                buf1[(pk * NQ + j) * NQ + pi] = s;
+            }
+        }
+    }
+    __syncthreads();
+    if (active) {
+#pragma unroll
+        for (int k = 0; k < NQ; ++k) acc[k] += buf0[k * TPE + t] + buf1[k * TPE + t];
+    }
+    T *dst = out + e * M + t * NM;
+    hex_columns_to_modes<NM, NQ, T>(tab, acc, active, t, buf2, buf3, [&](int p, T v) {
+        if (valid) st_stream(dst + p, v);
+    });
+}
+
 }  // namespace imf

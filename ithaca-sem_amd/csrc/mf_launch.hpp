@@ -31,6 +31,9 @@ hipError_t launch_hex(const LaunchArgs &a)
     } else if constexpr (KIND == kMass) {
         hex_mass_kernel<NM, NQ, T, GEO><<<grid, block, 0, a.stream>>>(
             tab, a.nElmt, a.nvec, in, a.ldin, geom, out, a.ldout);
+    } else if constexpr (KIND == kIProdDeriv) {
+        hex_iprod_deriv_kernel<NM, NQ, T, GEO>
+            <<<grid, block, 0, a.stream>>>(tab, a.nElmt, in, geom, out);
     } else if constexpr ((KIND == kBwdTrans && DIR == kForward) ||
                          (KIND == kIProduct && DIR == kTranspose)) {
         hex_bwdtrans_kernel<NM, NQ, T, DIR, GEO>
@@ -55,6 +58,9 @@ hipError_t launch_quad(const LaunchArgs &a)
     if constexpr (KIND == kHelmholtz) {
         quad_helmholtz_kernel<NM, NQ, T, GEO>
             <<<grid, block, 0, a.stream>>>(tab, a.nElmt, T(a.lambda), in, geom, out);
+    } else if constexpr (KIND == kIProdDeriv) {
+        quad_iprod_deriv_kernel<NM, NQ, T, GEO>
+            <<<grid, block, 0, a.stream>>>(tab, a.nElmt, in, geom, out);
     } else if constexpr ((KIND == kBwdTrans && DIR == kForward) ||
                          (KIND == kIProduct && DIR == kTranspose)) {
         quad_bwdtrans_kernel<NM, NQ, T, DIR, GEO>
@@ -71,6 +77,7 @@ hipError_t launch_quad(const LaunchArgs &a)
 //   BwdTrans  forward    -> bwdtrans<fwd>;  transpose -> iproduct<unit>
 //   IProduct  forward    -> iproduct<geo>;  transpose -> bwdtrans<transpose, geo>
 //   Mass (hex)           -> mass<geo>                     (the operator is symmetric)
+//   IProductWRTDerivBase -> iprod_deriv<geo>              (forward only)
 template <int SHAPE, int NM, int NQ, typename T>
 LaunchFn pick(int kind, int dir, int geo)
 {
@@ -87,6 +94,12 @@ LaunchFn pick(int kind, int dir, int geo)
     if (kind == kHelmholtz) {
         if (geo == kDeformed) return IMF_L(kHelmholtz, kForward, kDeformed);
         if (geo == kRegular) return IMF_L(kHelmholtz, kForward, kRegular);
+        return nullptr;
+    }
+    if (kind == kIProdDeriv) {
+        if (dir != kForward) return nullptr;
+        if (geo == kDeformed) return IMF_L(kIProdDeriv, kForward, kDeformed);
+        if (geo == kRegular) return IMF_L(kIProdDeriv, kForward, kRegular);
         return nullptr;
     }
     if (kind == kBwdTrans) {
@@ -127,5 +140,9 @@ LaunchFn lookup_nm(int kind, int dir, int geo, int nq, int prec)
 // Defined in mf_inst_*.hip, one per (shape, nm).
 LaunchFn lookup_hex(int nm, int kind, int dir, int geo, int nq, int prec);
 LaunchFn lookup_quad(int nm, int kind, int dir, int geo, int nq, int prec);
+
+// PhysDeriv depends on the points only: the reference dispatches on nq in 2..16
+// (PhysDeriv.h:291-330, :541-572).  Defined in mf_inst_deriv.hip.
+LaunchFn lookup_physderiv(int shape, int nq, int geo, int prec);
 
 }  // namespace imf

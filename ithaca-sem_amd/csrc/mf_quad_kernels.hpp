@@ -259,4 +259,142 @@ __global__ __launch_bounds__(QuadCfg<NQ>::BLOCK) void quad_helmholtz_kernel(
     quad_lines_to_modes<NM, NQ, T>(tab, acc, out + e * M, valid, active, t, buf1);
 }
 
+// PhysDeriv, quads (PhysDeriv.h:260-400, PhysDerivKernels.hpp:93-137): out_0 = df0 d0 +
+// df1 d1, out_1 = df2 d0 + df3 d1 (2D coordinates); layouts as the hex kernel.
+template <int NQ, typename T, int GEO>
+__global__ __launch_bounds__(QuadCfg<NQ>::BLOCK) void quad_physderiv_kernel(
+    const DTab<NQ, T> tab, int nElmt, const T *__restrict__ in, const T *__restrict__ df,
+    T *__restrict__ out)
+{
+    using C = QuadCfg<NQ>;
+    constexpr int Q = NQ * NQ;
+    __shared__ T lds[C::EPB][2][Q];
+    const int eb = threadIdx.x / C::TPE;
+    const int t = threadIdx.x - eb * C::TPE;
+    const bool active = eb < C::EPB;
+    const int es = active ? eb : 0;
+    const long e = (long)blockIdx.x * C::EPB + eb;
+    const bool valid = active && e < nElmt;
+    T *bu = lds[es][0], *b0 = lds[es][1];
+
+    T u[NQ];
+#pragma unroll
+    for (int j = 0; j < NQ; ++j) u[j] = valid ? ld_stream(in + e * Q + j * NQ + t) : T(0);
+    if (active) {
+#pragma unroll
+        for (int j = 0; j < NQ; ++j) bu[j * NQ + t] = u[j];
+    }
+    __syncthreads();
+    // d0 on i-lines of thread j = t
+    if (active) {
+        T v[NQ];
+#pragma unroll
+        for (int l = 0; l < NQ; ++l) v[l] = bu[t * NQ + l];
+#pragma unroll
+        for (int i = 0; i < NQ; ++i) {
+            T s = v[0] * tab.D[i];
+#pragma unroll
+            for (int l = 1; l < NQ; ++l) s = fma(v[l], tab.D[l * NQ + i], s);
+            b0[t * NQ + i] = s;
+        }
+    }
+    __syncthreads();
+    if (!valid) return;
+    const long nt = (long)nElmt * Q;
+#pragma unroll
+    for (int j = 0; j < NQ; ++j) {
+        T d1 = u[0] * tab.D[j];
+#pragma unroll
+        for (int l = 1; l < NQ; ++l) d1 = fma(u[l], tab.D[l * NQ + j], d1);
+        const T d0 = b0[j * NQ + t];
+        T f[4];
+        if (GEO == kDeformed) {
+#pragma unroll
+            for (int d = 0; d < 4; ++d) f[d] = ld_stream(df + d * nt + e * Q + j * NQ + t);
+        } else {
+#pragma unroll
+            for (int d = 0; d < 4; ++d) f[d] = df[(long)d * nElmt + e];
+        }
+        st_stream(out + e * Q + j * NQ + t, fma(d1, f[1], d0 * f[0]));
+        st_stream(out + nt + e * Q + j * NQ + t, fma(d1, f[3], d0 * f[2]));
+    }
+}
+
+// IProductWRTDerivBase, quads (IProductWRTDerivBase.h:371-, 2D coordinates): out =
+// (D_0 B)^T diag(wJ) (df0 in0 + df2 in1) + (D_1 B)^T diag(wJ) (df1 in0 + df3 in1).
+// Packed geometry: deformed [e][4][Q] = w J df[d]; regular [e][4] = J df[d].
+template <int NM, int NQ, typename T, int GEO>
+__global__ __launch_bounds__(QuadCfg<NQ>::BLOCK) void quad_iprod_deriv_kernel(
+    const Tab<NM, NQ, T> tab, int nElmt, const T *__restrict__ in,
+    const T *__restrict__ geom, T *__restrict__ out)
+{
+    using C = QuadCfg<NQ>;
+    constexpr int M = NM * NM, Q = NQ * NQ;
+    __shared__ T lds[C::EPB][2][Q];
+    const int eb = threadIdx.x / C::TPE;
+    const int t = threadIdx.x - eb * C::TPE;
+    const bool active = eb < C::EPB;
+    const int es = active ? eb : 0;
+    const long e = (long)blockIdx.x * C::EPB + eb;
+    const bool valid = active && e < nElmt;
+    T *buf0 = lds[es][0], *buf1 = lds[es][1];
+    const long nt = (long)nElmt * Q;
+
+    T g0[NQ], g1[NQ];
+    if (valid) {
+        const T wi = tab.w[t];
+#pragma unroll
+        for (int j = 0; j < NQ; ++j) {
+            const long o = e * Q + j * NQ + t;
+            const T x0 = ld_stream(in + o), x1 = ld_stream(in + nt + o);
+            T f[4];
+            if (GEO == kDeformed) {
+#pragma unroll
+                for (int d = 0; d < 4; ++d) f[d] = ld_stream(geom + e * (4 * Q) + d * Q + j * NQ + t);
+            } else {
+                const T wq = wi * tab.w[j];
+#pragma unroll
+                for (int d = 0; d < 4; ++d) f[d] = geom[e * 4 + d] * wq;
+            }
+            g0[j] = fma(f[2], x1, f[0] * x0);
+            g1[j] = fma(f[3], x1, f[1] * x0);
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < NQ; ++j) g0[j] = g1[j] = T(0);
+    }
+    if (active) {
+#pragma unroll
+        for (int j = 0; j < NQ; ++j) buf1[j * NQ + t] = g0[j];
+    }
+    T acc[NQ];
+#pragma unroll
+    for (int j = 0; j < NQ; ++j) {
+        T s = g1[0] * tab.D[j * NQ];
+#pragma unroll
+        for (int l = 1; l < NQ; ++l) s = fma(g1[l], tab.D[j * NQ + l], s);
+        acc[j] = s;
+    }
+    __syncthreads();
+    if (active) {
+        T v[NQ];
+#pragma unroll
+        for (int l = 0; l < NQ; ++l) v[l] = buf1[t * NQ + l];
+#pragma unroll
+        for (int i = 0; i < NQ; ++i) {
+            T s = v[0] * tab.D[i * NQ];
+#pragma unroll
+            for (int l = 1; l < NQ; ++l) s = fma(v[l], tab.D[i * NQ + l], s);
+            buf0[t * NQ + i] = s;
+        }
+    }
+    __syncthreads();
+    if (active) {
+#pragma unroll
+        for (int j = 0; j < NQ; ++j) acc[j] += buf0[j * NQ + t];
+    }
+    __syncthreads();
+    quad_lines_to_modes<NM, NQ, T>(tab, acc, out + e * M, valid, active, t, buf1);
+}
+
 }  // namespace imf

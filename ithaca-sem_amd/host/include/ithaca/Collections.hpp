@@ -4,7 +4,7 @@
 // Operator::operator()(in, out0, out1, out2, wsp, factors), the OperatorKey factory),
 // Collections/Collection.{h,cpp} (Initialise / ApplyOperator) and the *_MatrixFree
 // adapters (Collections/BwdTrans.cpp:140-240, IProductWRTBase.cpp:168-277,
-// Helmholtz.cpp:391-503).  One value, eMatrixFreeHip, is appended to
+// Helmholtz.cpp:391-503, PhysDeriv.cpp:270-400, IProductWRTDerivBase.cpp:274-400).  One value, eMatrixFreeHip, is appended to
 // ImplementationType; every existing value keeps its number.
 //
 // The reference operators receive vector<StdExpansionSharedPtr>; here a collection is
@@ -135,6 +135,17 @@ public:
     void ApplyOperator(const OperatorType &op, const Array<OneD, const NekDouble> &inarray,
                        Array<OneD, NekDouble> &output,
                        const StdRegions::ConstFactorMap &factors = StdRegions::NullConstFactorMap);
+    // Two-output form (2D PhysDeriv; 2D IProductWRTDerivBase as in0, in1, out).
+    void ApplyOperator(const OperatorType &op, const Array<OneD, const NekDouble> &inarray,
+                       Array<OneD, NekDouble> &output0, Array<OneD, NekDouble> &output1);
+    // Three-output form (3D PhysDeriv; 3D IProductWRTDerivBase as in0, in1, in2, out).
+    void ApplyOperator(const OperatorType &op, const Array<OneD, const NekDouble> &inarray,
+                       Array<OneD, NekDouble> &output0, Array<OneD, NekDouble> &output1,
+                       Array<OneD, NekDouble> &output2,
+                       const StdRegions::ConstFactorMap &factors = StdRegions::NullConstFactorMap);
+    // One direction (PhysDeriv).
+    void ApplyOperator(const OperatorType &op, int dir, const Array<OneD, const NekDouble> &inarray,
+                       Array<OneD, NekDouble> &output);
 
 private:
     std::unordered_map<int, OperatorSharedPtr> m_ops;

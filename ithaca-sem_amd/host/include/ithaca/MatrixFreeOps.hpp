@@ -1,8 +1,9 @@
 // Nektar::MatrixFree operator layer, implemented on the gfx950 C ABI.
 //
 // Mirrors MatrixFreeOps/Operator.hpp:21-310 and Operator.cpp:6-56: the Operator base
-// (Ndof / NeedsDF / NeedsJac / SetJac / SetDF), the BwdTrans / IProduct / Helmholtz
-// operator bases with operator()(in, out), Helmholtz::SetLambda, the string-keyed
+// (Ndof / NeedsDF / NeedsJac / SetJac / SetDF), the BwdTrans / IProduct / Helmholtz /
+// PhysDeriv / IProductWRTDerivBase operator bases with operator()(in, out),
+// Helmholtz::SetLambda, the string-keyed
 // OperatorFactory ("Helmholtz_Hex_Deformed", ...) and GetOpstring.  The registered
 // implementations run on the GPU through include/ithaca_mf.h.
 //
@@ -85,6 +86,34 @@ public:
 
 protected:
     NekDouble m_lambda = 1.0;
+};
+
+// MatrixFreeOps/PhysDeriv.h:18-48: one input of quadrature values, dim outputs
+// out[d] = d/dx_d of the input (nElmt*Q values each).
+class PhysDeriv : public Operator {
+public:
+    PhysDeriv(const std::string &key, std::vector<LibUtilities::BasisSharedPtr> basis, int nElmt)
+        : Operator(key, basis, nElmt)
+    {
+    }
+    bool NeedsDF() final { return true; }
+    virtual void operator()(const Array<OneD, const NekDouble> &input,
+                            Array<OneD, Array<OneD, NekDouble>> &output);
+};
+
+// MatrixFreeOps/IProductWRTDerivBase.h:18-48: dim inputs of quadrature values, one output
+// of coefficients.
+class IProductWRTDerivBase : public Operator {
+public:
+    IProductWRTDerivBase(const std::string &key, std::vector<LibUtilities::BasisSharedPtr> basis,
+                         int nElmt)
+        : Operator(key, basis, nElmt)
+    {
+    }
+    bool NeedsJac() final { return true; }
+    bool NeedsDF() final { return true; }
+    virtual void operator()(const Array<OneD, Array<OneD, NekDouble>> &input,
+                            Array<OneD, NekDouble> &output);
 };
 
 using OperatorFactory =

@@ -1,5 +1,6 @@
 // Implementation of the host mirror (Basis, MatrixFree operators, Collections adapters)
 // over the C ABI of include/ithaca_mf.h.
+#include <algorithm>
 #include <iostream>
 #include <sstream>
 
@@ -114,6 +115,33 @@ void Helmholtz::SetLambda(NekDouble lambda)
     check(imf_set_lambda(m_op, lambda));
 }
 
+void PhysDeriv::operator()(const Array<OneD, const NekDouble> &in,
+                           Array<OneD, Array<OneD, NekDouble>> &out)
+{
+    const size_t nq = imf_in_size(m_op), dim = m_basis.size();
+    ASSERTL0(out.size() == dim, "PhysDeriv: need one output per dimension");
+    ASSERTL0(in.size() >= nq, "PhysDeriv: array too small");
+    for (size_t d = 0; d < dim; ++d) ASSERTL0(out[d].size() >= nq, "PhysDeriv: array too small");
+    std::vector<NekDouble> tmp(dim * nq);
+    ApplyHost(in.data(), tmp.data());
+    for (size_t d = 0; d < dim; ++d)
+        std::copy(tmp.begin() + d * nq, tmp.begin() + (d + 1) * nq, out[d].data());
+}
+
+void IProductWRTDerivBase::operator()(const Array<OneD, Array<OneD, NekDouble>> &in,
+                                      Array<OneD, NekDouble> &out)
+{
+    const size_t dim = m_basis.size(), nq = imf_in_size(m_op) / dim;
+    ASSERTL0(in.size() == dim, "IProductWRTDerivBase: need one input per dimension");
+    ASSERTL0(out.size() >= imf_out_size(m_op), "IProductWRTDerivBase: array too small");
+    std::vector<NekDouble> tmp(dim * nq);
+    for (size_t d = 0; d < dim; ++d) {
+        ASSERTL0(in[d].size() >= nq, "IProductWRTDerivBase: array too small");
+        std::copy(in[d].data(), in[d].data() + nq, tmp.begin() + d * nq);
+    }
+    ApplyHost(tmp.data(), out.data());
+}
+
 OperatorFactory &GetOperatorFactory()
 {
     static OperatorFactory tmp;
@@ -146,7 +174,8 @@ OperatorSharedPtr make(const std::string &key, std::vector<LibUtilities::BasisSh
 }
 
 // Keys registered by the reference for quads and hexes: MatrixFreeOps/BwdTrans.cpp:7-33,
-// IProductQuad.cpp, IProductHex.cpp, Helmholtz.cpp:7-23.
+// IProductQuad.cpp, IProductHex.cpp, Helmholtz.cpp:7-23, PhysDeriv.cpp:14-36,
+// IProductWRTDerivBaseQuad.cpp, IProductWRTDerivBaseHex.cpp.
 const bool registered = [] {
     auto &f = GetOperatorFactory();
     for (const char *shape : {"Quad", "Hex"}) {
@@ -156,6 +185,13 @@ const bool registered = [] {
             return make<BwdTrans>(kb, b, n);
         });
         for (const char *d : {"_Regular", "_Deformed"}) {
+            const std::string kp = "PhysDeriv_" + s + d, kd = "IProductWRTDerivBase_" + s + d;
+            f.RegisterCreatorFunction(kp, [kp](std::vector<LibUtilities::BasisSharedPtr> b, int n) {
+                return make<PhysDeriv>(kp, b, n);
+            });
+            f.RegisterCreatorFunction(kd, [kd](std::vector<LibUtilities::BasisSharedPtr> b, int n) {
+                return make<IProductWRTDerivBase>(kd, b, n);
+            });
             const std::string ki = "IProduct_" + s + d, kh = "Helmholtz_" + s + d;
             f.RegisterCreatorFunction(ki, [ki](std::vector<LibUtilities::BasisSharedPtr> b, int n) {
                 return make<IProduct>(ki, b, n);
@@ -303,6 +339,75 @@ public:
     std::shared_ptr<MatrixFree::Helmholtz> m_oper;
 };
 
+// Collections/PhysDeriv.cpp:270-400 (PhysDeriv_MatrixFree), on the GPU: outputs
+// output0..output{dim-1}; the dir overload returns one direction.
+class PhysDeriv_MatrixFreeHip final : public Operator {
+public:
+    PhysDeriv_MatrixFreeHip(const CollectionExp &exp, CoalescedGeomDataSharedPtr geom)
+        : Operator(exp, geom), m_dim((int)exp.basis.size()), m_nq(exp.GetTotPoints() * exp.nElmt)
+    {
+        auto oper = create_mf("PhysDeriv", exp, m_isDeformed);
+        oper->SetDF(geom->GetDerivFactors());
+        m_oper = std::dynamic_pointer_cast<MatrixFree::PhysDeriv>(oper);
+        ASSERTL0(m_oper, "Failed to cast pointer.");
+        m_output = Array<OneD, Array<OneD, NekDouble>>(m_dim);
+        for (int d = 0; d < m_dim; ++d) m_output[d] = Array<OneD, NekDouble>(m_nq);
+    }
+    void operator()(const Array<OneD, const NekDouble> &input, Array<OneD, NekDouble> &output0,
+                    Array<OneD, NekDouble> &output1, Array<OneD, NekDouble> &output2,
+                    Array<OneD, NekDouble> &, const StdRegions::ConstFactorMap &) final
+    {
+        (*m_oper)(input, m_output);
+        Array<OneD, NekDouble> *outs[3] = {&output0, &output1, &output2};
+        for (int d = 0; d < m_dim; ++d)
+            std::copy(m_output[d].data(), m_output[d].data() + m_nq, outs[d]->data());
+    }
+    void operator()(int dir, const Array<OneD, const NekDouble> &input,
+                    Array<OneD, NekDouble> &output, Array<OneD, NekDouble> &) final
+    {
+        ASSERTL0(dir >= 0 && dir < m_dim, "PhysDeriv_MatrixFree: direction out of range");
+        (*m_oper)(input, m_output);
+        std::copy(m_output[dir].data(), m_output[dir].data() + m_nq, output.data());
+    }
+    int m_dim;
+    size_t m_nq;
+    Array<OneD, Array<OneD, NekDouble>> m_output;
+    std::shared_ptr<MatrixFree::PhysDeriv> m_oper;
+};
+
+// Collections/IProductWRTDerivBase.cpp:274-400 (IProductWRTDerivBase_MatrixFree), on the
+// GPU: inputs entry0..entry{dim-1}, output entry{dim}.
+class IProductWRTDerivBase_MatrixFreeHip final : public Operator {
+public:
+    IProductWRTDerivBase_MatrixFreeHip(const CollectionExp &exp, CoalescedGeomDataSharedPtr geom)
+        : Operator(exp, geom), m_dim((int)exp.basis.size())
+    {
+        auto oper = create_mf("IProductWRTDerivBase", exp, m_isDeformed);
+        oper->SetJac(geom->GetJac());
+        oper->SetDF(geom->GetDerivFactors());
+        m_oper = std::dynamic_pointer_cast<MatrixFree::IProductWRTDerivBase>(oper);
+        ASSERTL0(m_oper, "Failed to cast pointer.");
+    }
+    void operator()(const Array<OneD, const NekDouble> &entry0, Array<OneD, NekDouble> &entry1,
+                    Array<OneD, NekDouble> &entry2, Array<OneD, NekDouble> &entry3,
+                    Array<OneD, NekDouble> &, const StdRegions::ConstFactorMap &) final
+    {
+        Array<OneD, Array<OneD, NekDouble>> in(m_dim);
+        const size_t n = entry0.size();
+        in[0] = Array<OneD, NekDouble>(n, entry0.data());
+        in[1] = entry1;
+        if (m_dim == 3) in[2] = entry2;
+        (*m_oper)(in, m_dim == 3 ? entry3 : entry2);
+    }
+    void operator()(int, const Array<OneD, const NekDouble> &, Array<OneD, NekDouble> &,
+                    Array<OneD, NekDouble> &) final
+    {
+        NEKERROR(ErrorUtil::efatal, "Not valid for this operator.");
+    }
+    int m_dim;
+    std::shared_ptr<MatrixFree::IProductWRTDerivBase> m_oper;
+};
+
 template <class T>
 OperatorSharedPtr create(const CollectionExp &e, CoalescedGeomDataSharedPtr g)
 {
@@ -319,6 +424,11 @@ const bool registered = [] {
                                   "IProductWRTBase_MatrixFreeHip");
         f.RegisterCreatorFunction(OperatorKey(shape, eHelmholtz, eMatrixFreeHip, false),
                                   create<Helmholtz_MatrixFreeHip>, "Helmholtz_MatrixFreeHip");
+        f.RegisterCreatorFunction(OperatorKey(shape, ePhysDeriv, eMatrixFreeHip, false),
+                                  create<PhysDeriv_MatrixFreeHip>, "PhysDeriv_MatrixFreeHip");
+        f.RegisterCreatorFunction(OperatorKey(shape, eIProductWRTDerivBase, eMatrixFreeHip, false),
+                                  create<IProductWRTDerivBase_MatrixFreeHip>,
+                                  "IProductWRTDerivBase_MatrixFreeHip");
     }
     return true;
 }();
@@ -353,6 +463,30 @@ void Collection::ApplyOperator(const OperatorType &op, const Array<OneD, const N
 {
     Array<OneD, NekDouble> wsp(m_ops[op]->GetWspSize());
     (*m_ops[op])(inarray, output, NullNekDouble1DArray, NullNekDouble1DArray, wsp, factors);
+}
+
+void Collection::ApplyOperator(const OperatorType &op, const Array<OneD, const NekDouble> &inarray,
+                               Array<OneD, NekDouble> &output0, Array<OneD, NekDouble> &output1)
+{
+    Array<OneD, NekDouble> wsp(m_ops[op]->GetWspSize());
+    (*m_ops[op])(inarray, output0, output1, NullNekDouble1DArray, wsp);
+}
+
+void Collection::ApplyOperator(const OperatorType &op, const Array<OneD, const NekDouble> &inarray,
+                               Array<OneD, NekDouble> &output0, Array<OneD, NekDouble> &output1,
+                               Array<OneD, NekDouble> &output2,
+                               const StdRegions::ConstFactorMap &factors)
+{
+    Array<OneD, NekDouble> wsp(m_ops[op]->GetWspSize());
+    (*m_ops[op])(inarray, output0, output1, output2, wsp, factors);
+}
+
+void Collection::ApplyOperator(const OperatorType &op, int dir,
+                               const Array<OneD, const NekDouble> &inarray,
+                               Array<OneD, NekDouble> &output)
+{
+    Array<OneD, NekDouble> wsp(m_ops[op]->GetWspSize());
+    (*m_ops[op])(dir, inarray, output, wsp);
 }
 
 }  // namespace Collections

@@ -124,6 +124,23 @@ def physderiv(shape, b, E, deformed, df, f):
     return outs
 
 
+def iprod_deriv(shape, b, E, deformed, jac, df, ins):
+    """IProductWRTDerivBase: ins = dim arrays of E*Q point values -> E*M coefficients."""
+    nm, nq = b["nm"], b["nq"]
+    d = 3 if shape == "hex" else 2
+    ins = [np.ascontiguousarray(x, dtype=np.float64) for x in ins]
+    out = np.zeros(E * nm ** d)
+    if d == 3:
+        lib().or_iprod_deriv_hex(nm, nq, _p(b["bdata"]), _p(b["dbdata"]), _p(b["w"]), E,
+                                 int(deformed), _p(jac), _p(df), *[_p(x) for x in ins],
+                                 _p(out))
+    else:
+        lib().or_iprod_deriv_quad(nm, nq, _p(b["bdata"]), _p(b["dbdata"]), _p(b["w"]), E,
+                                  int(deformed), _p(jac), _p(df), *[_p(x) for x in ins],
+                                  _p(out))
+    return out
+
+
 def helmholtz(shape, b, E, deformed, jac, df, lam, u):
     nm, nq = b["nm"], b["nq"]
     d = 3 if shape == "hex" else 2

@@ -507,6 +507,46 @@ void or_physderiv_hex(int nq, const double *D, int E, int deformed, const double
     }
 }
 
+/* IProductWRTDerivBaseHexImpl: MatrixFreeOps/IProductWRTDerivBase.h:1231-1343.
+ * tmp_a = sum_c df[c*3+a] in_c, then IP(dB0,B1,B2; tmp0) + IP(B0,dB1,B2; tmp1)
+ * + IP(B0,B1,dB2; tmp2), each IP weighted by J w (IProductHexKernel). */
+void or_iprod_deriv_hex(int nm, int nq, const double *bdata, const double *dbdata,
+                        const double *w, int E, int deformed, const double *jac,
+                        const double *df, const double *in0, const double *in1,
+                        const double *in2, double *out)
+{
+    const int M = nm * nm * nm, Q = nq * nq * nq;
+    const size_t np = deformed ? (size_t)E * Q : (size_t)E;
+#pragma omp parallel
+    {
+        double *t[3], *wsp1 = (double *)malloc(sizeof(double) * Q);
+        double *wsp2 = (double *)malloc(sizeof(double) * Q);
+        for (int a = 0; a < 3; ++a) t[a] = (double *)malloc(sizeof(double) * Q);
+#pragma omp for schedule(static)
+        for (int e = 0; e < E; ++e) {
+            const double *jp = deformed ? jac + (size_t)e * Q : jac + e;
+            const size_t o = (size_t)e * Q;
+            for (int q = 0; q < Q; ++q) {
+                const size_t idx = deformed ? o + q : (size_t)e;
+                for (int a = 0; a < 3; ++a)
+                    t[a][q] = df[(0 * 3 + a) * np + idx] * in0[o + q] +
+                              df[(1 * 3 + a) * np + idx] * in1[o + q] +
+                              df[(2 * 3 + a) * np + idx] * in2[o + q];
+            }
+            double *op = out + (size_t)e * M;
+            iprod_hex_elmt(nm, nq, dbdata, bdata, bdata, w, deformed, jp, t[0], wsp1, wsp2,
+                           op, 0, 0, 1.0);
+            iprod_hex_elmt(nm, nq, bdata, dbdata, bdata, w, deformed, jp, t[1], wsp1, wsp2,
+                           op, 0, 1, 1.0);
+            iprod_hex_elmt(nm, nq, bdata, bdata, dbdata, w, deformed, jp, t[2], wsp1, wsp2,
+                           op, 0, 1, 1.0);
+        }
+        for (int a = 0; a < 3; ++a) free(t[a]);
+        free(wsp1);
+        free(wsp2);
+    }
+}
+
 /* HelmholtzHexImpl: MatrixFreeOps/Helmholtz.h:764-996 (both DEFORMED branches). */
 void or_helmholtz_hex(int nm, int nq, const double *bdata, const double *dbdata,
                       const double *D, const double *w, int E, int deformed,
@@ -682,6 +722,39 @@ void or_physderiv_quad(int nq, const double *D, int E, int deformed, const doubl
         }
         free(d0);
         free(d1);
+    }
+}
+
+/* IProductWRTDerivBaseQuad (2D coordinates): MatrixFreeOps/IProductWRTDerivBase.h:371-
+ * (tmp0 = df0 in0 + df2 in1, tmp1 = df1 in0 + df3 in1; IP(dB0,B1; tmp0) + IP(B0,dB1; tmp1)). */
+void or_iprod_deriv_quad(int nm, int nq, const double *bdata, const double *dbdata,
+                         const double *w, int E, int deformed, const double *jac,
+                         const double *df, const double *in0, const double *in1,
+                         double *out)
+{
+    const int M = nm * nm, Q = nq * nq;
+    const size_t np = deformed ? (size_t)E * Q : (size_t)E;
+#pragma omp parallel
+    {
+        double *t0 = (double *)malloc(sizeof(double) * Q);
+        double *t1 = (double *)malloc(sizeof(double) * Q);
+        double *sj = (double *)malloc(sizeof(double) * nq);
+#pragma omp for schedule(static)
+        for (int e = 0; e < E; ++e) {
+            const double *jp = deformed ? jac + (size_t)e * Q : jac + e;
+            const size_t o = (size_t)e * Q;
+            for (int q = 0; q < Q; ++q) {
+                const size_t idx = deformed ? o + q : (size_t)e;
+                t0[q] = df[0 * np + idx] * in0[o + q] + df[2 * np + idx] * in1[o + q];
+                t1[q] = df[1 * np + idx] * in0[o + q] + df[3 * np + idx] * in1[o + q];
+            }
+            double *op = out + (size_t)e * M;
+            iprod_quad_elmt(nm, nq, dbdata, bdata, w, deformed, jp, t0, sj, op, 0, 0, 1.0);
+            iprod_quad_elmt(nm, nq, bdata, dbdata, w, deformed, jp, t1, sj, op, 0, 1, 1.0);
+        }
+        free(t0);
+        free(t1);
+        free(sj);
     }
 }
 

@@ -51,6 +51,10 @@ void or_iproduct_hex(int nm, int nq, const double *bdata, const double *w, int E
                      int deformed, const double *jac, const double *in, double *out);
 void or_physderiv_hex(int nq, const double *D, int E, int deformed, const double *df,
                       const double *in, double *out0, double *out1, double *out2);
+void or_iprod_deriv_hex(int nm, int nq, const double *bdata, const double *dbdata,
+                        const double *w, int E, int deformed, const double *jac,
+                        const double *df, const double *in0, const double *in1,
+                        const double *in2, double *out);
 void or_helmholtz_hex(int nm, int nq, const double *bdata, const double *dbdata,
                       const double *D, const double *w, int E, int deformed,
                       const double *jac, const double *df, double lambda,
@@ -62,6 +66,10 @@ void or_iproduct_quad(int nm, int nq, const double *bdata, const double *w, int 
                       int deformed, const double *jac, const double *in, double *out);
 void or_physderiv_quad(int nq, const double *D, int E, int deformed, const double *df,
                        const double *in, double *out0, double *out1);
+void or_iprod_deriv_quad(int nm, int nq, const double *bdata, const double *dbdata,
+                         const double *w, int E, int deformed, const double *jac,
+                         const double *df, const double *in0, const double *in1,
+                         double *out);
 void or_helmholtz_quad(int nm, int nq, const double *bdata, const double *dbdata,
                        const double *D, const double *w, int E, int deformed,
                        const double *jac, const double *df, double lambda,

@@ -2,6 +2,7 @@
 // Collections API (Collection::Initialise / ApplyOperator with a ConstFactorMap, operator
 // keys, NekError on bad requests) runs the gfx950 operators and is compared with the CPU
 // oracle element by element.  Prints one line per case; exit status 0 iff all pass.
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <random>
@@ -107,6 +108,31 @@ int main()
     r = rel(out.data(), ref.data(), E * M);
     printf("Helmholtz_MatrixFreeHip hex lambda=0.5 rel=%.3e\n", r);
     fails += !(r < 1e-12);
+
+    // PhysDeriv and IProductWRTDerivBase through the multi-array ApplyOperator forms
+    {
+        Array<OneD, NekDouble> d0(E * Q), d1(E * Q), d2(E * Q), dd(E * Q), oi(E * M);
+        std::vector<double> r0(E * Q), r1(E * Q), r2(E * Q);
+        coll.Initialise(Collections::ePhysDeriv);
+        coll.ApplyOperator(Collections::ePhysDeriv, f, d0, d1, d2);
+        or_physderiv_hex(nq, B->GetD().data(), E, 1, df->data(), f.data(), r0.data(), r1.data(),
+                         r2.data());
+        r = std::max({rel(d0.data(), r0.data(), E * Q), rel(d1.data(), r1.data(), E * Q),
+                      rel(d2.data(), r2.data(), E * Q)});
+        printf("PhysDeriv_MatrixFreeHip hex rel=%.3e\n", r);
+        fails += !(r < 1e-12);
+        coll.ApplyOperator(Collections::ePhysDeriv, 1, f, dd);
+        r = rel(dd.data(), r1.data(), E * Q);
+        printf("PhysDeriv_MatrixFreeHip hex dir=1 rel=%.3e\n", r);
+        fails += !(r < 1e-12);
+        coll.Initialise(Collections::eIProductWRTDerivBase);
+        coll.ApplyOperator(Collections::eIProductWRTDerivBase, d0, d1, d2, oi);
+        or_iprod_deriv_hex(nm, nq, B->GetBdata().data(), B->GetDbdata().data(), B->GetW().data(), E,
+                           1, jac->data(), df->data(), d0.data(), d1.data(), d2.data(), ref.data());
+        r = rel(oi.data(), ref.data(), E * M);
+        printf("IProductWRTDerivBase_MatrixFreeHip hex rel=%.3e\n", r);
+        fails += !(r < 1e-12);
+    }
 
     // Regular quads through the same API
     {

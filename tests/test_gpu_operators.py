@@ -261,3 +261,74 @@ def test_mass_quad_rejected_and_apply_many_loops():
     # leading dimensions shorter than the vectors are rejected before any launch
     assert H.lib().imf_apply_many_device(op._h, 1, None, 3, None, 3, None) != 0
     assert "leading dimension" in H.lib().imf_last_error().decode()
+
+
+@pytest.mark.parametrize("deformed", [True, False])
+@pytest.mark.parametrize("shape,nq", [("hex", q) for q in (2, 3, 5, 7, 9, 12, 16)] +
+                         [("quad", q) for q in (2, 4, 6, 7, 11, 16)])
+def test_physderiv_parity(shape, nq, deformed):
+    """PhysDeriv_{Hex,Quad}_* (points only, nq 2..16) against the oracle: FP64 1e-12, FP32
+    2e-6 relative over all dim outputs."""
+    nm = min(nq, 8) if nq >= 2 else 2
+    rng, verts, b, jac, df = _case(shape, min(nm, nq), nq, deformed, 3)
+    E = verts.shape[0]
+    dim = 3 if shape == "hex" else 2
+    f = rng.uniform(-1, 1, E * nq ** dim)
+    ref = np.concatenate(O.physderiv(shape, b, E, deformed, df, f))
+    key = f"PhysDeriv_{shape.capitalize()}_{'Deformed' if deformed else 'Regular'}"
+    op = _op(key, min(nm, nq), nq, E, b, df=df)
+    assert op.needs_df() and not op.needs_jac()
+    assert op.in_size == E * nq ** dim and op.out_size == dim * E * nq ** dim
+    got = op.apply(_dev(f)).cpu().numpy()
+    assert rel(got, ref) < TOL64
+    op32 = _op(key, min(nm, nq), nq, E, b, df=df, prec=H.FP32)
+    got32 = op32.apply(_dev(f, torch.float32)).double().cpu().numpy()
+    assert rel(got32, ref) < TOL32
+    with pytest.raises(H.MFError):
+        op.apply_transpose(_dev(ref))
+
+
+@pytest.mark.parametrize("deformed", [True, False])
+@pytest.mark.parametrize("shape,nm,nq", [("hex", c[0], c[1]) for c in HEX_COMBOS] +
+                         [("quad", c[0], c[1]) for c in QUAD_COMBOS])
+def test_iprod_deriv_parity(shape, nm, nq, deformed):
+    """IProductWRTDerivBase_{Hex,Quad}_* against the oracle (the reference's three
+    IProducts with dbdata): FP64 1e-12, FP32 2e-5 relative."""
+    rng, verts, b, jac, df = _case(shape, nm, nq, deformed, 4)
+    E = verts.shape[0]
+    dim = 3 if shape == "hex" else 2
+    Q = nq ** dim
+    ins = [rng.uniform(-1, 1, E * Q) for _ in range(dim)]
+    ref = O.iprod_deriv(shape, b, E, deformed, jac, df, ins)
+    key = f"IProductWRTDerivBase_{shape.capitalize()}_{'Deformed' if deformed else 'Regular'}"
+    op = _op(key, nm, nq, E, b, jac=jac, df=df)
+    assert op.needs_jac() and op.needs_df()
+    assert op.in_size == dim * E * Q and op.out_size == E * nm ** dim
+    x = _dev(np.concatenate(ins))
+    got = op.apply(x).cpu().numpy()
+    assert rel(got, ref) < TOL64
+    op32 = _op(key, nm, nq, E, b, jac=jac, df=df, prec=H.FP32)
+    got32 = op32.apply(x.float()).double().cpu().numpy()
+    assert rel(got32, ref) < 2e-5
+
+
+@pytest.mark.parametrize("shape,nm,nq", [("hex", 6, 7), ("quad", 5, 6)])
+def test_iprod_deriv_is_weighted_transpose_of_physderiv(shape, nm, nq):
+    """<IPDB(v), u_hat> = sum_c <v_c, w J PhysDeriv(BwdTrans(u_hat))_c> on the GPU."""
+    rng, verts, b, jac, df = _case(shape, nm, nq, True, 5)
+    E = verts.shape[0]
+    dim = 3 if shape == "hex" else 2
+    M, Q = nm ** dim, nq ** dim
+    S = shape.capitalize()
+    bwd = _op(f"BwdTrans_{S}_Regular", nm, nq, E, b)
+    pd = _op(f"PhysDeriv_{S}_Deformed", nm, nq, E, b, df=df)
+    ipd = _op(f"IProductWRTDerivBase_{S}_Deformed", nm, nq, E, b, jac=jac, df=df)
+    u = _dev(rng.uniform(-1, 1, E * M))
+    v = _dev(rng.uniform(-1, 1, dim * E * Q))
+    w1 = np.asarray(b["w"])
+    wq = np.einsum("k,j,i->kji", w1, w1, w1).ravel() if dim == 3 else np.outer(w1, w1).ravel()
+    wj = _dev(np.tile(wq, E) * jac)
+    lhs = torch.dot(ipd.apply(v), u).item()
+    g = pd.apply(bwd.apply(u)).view(dim, E * Q)
+    rhs = sum(torch.dot(v.view(dim, E * Q)[c], wj * g[c]).item() for c in range(dim))
+    assert abs(lhs - rhs) <= 1e-12 * max(abs(lhs), 1.0)

@@ -173,6 +173,38 @@ def test_operators_vs_dense(shape, nm, nq):
         assert rel(hh[e * M:(e + 1) * M], ref_h) < 1e-12
 
 
+@pytest.mark.parametrize("shape,nm,nq", [("hex", 6, 7), ("hex", 3, 5), ("quad", 5, 6),
+                                         ("quad", 4, 8)])
+def test_deriv_operators_vs_dense(shape, nm, nq):
+    """PhysDeriv of a field of the space equals its exact physical gradient, and
+    IProductWRTDerivBase equals sum_c (d_c B)^T diag(W J) in_c, both with d_c B built from
+    independent numpy code (closed-form basis derivatives and the inverse map Jacobian)."""
+    rng = np.random.default_rng(13)
+    verts = _rand_hex(rng) if shape == "hex" else _rand_quad(rng)
+    E = verts.shape[0]
+    dim = 3 if shape == "hex" else 2
+    b = O.basis(nm, nq)
+    geom = O.geom_hex if shape == "hex" else O.geom_quad
+    jac, df = geom(verts, nq, b["z"], True)
+    M, Q = nm ** dim, nq ** dim
+    u = rng.uniform(-1, 1, E * M)
+    ins = [rng.uniform(-1, 1, E * Q) for _ in range(dim)]
+    grad = O.physderiv(shape, b, E, True, df, O.bwdtrans(shape, b, E, u))
+    ipd = O.iprod_deriv(shape, b, E, True, jac, df, ins)
+    rel = lambda a, r: np.linalg.norm(a - r) / np.linalg.norm(r)
+    for e in range(E):
+        m = R.element_mats(shape, nm, nq, verts[e])
+        dc = [sum(m["Ainv"][:, a, c][:, None] * m["G"][a] for a in range(dim))
+              for c in range(dim)]
+        ue = u[e * M:(e + 1) * M]
+        sl = slice(e * Q, (e + 1) * Q)
+        for c in range(dim):
+            assert rel(grad[c][sl], dc[c] @ ue) < 1e-12
+        WJ = m["W"] * m["J"]
+        ref = sum(dc[c].T @ (WJ * ins[c][sl]) for c in range(dim))
+        assert rel(ipd[e * M:(e + 1) * M], ref) < 1e-12
+
+
 def test_regular_equals_deformed_on_affine_elements():
     # a parallelepiped is "regular": per-element factors must give the same operator
     b = O.basis(6, 7)

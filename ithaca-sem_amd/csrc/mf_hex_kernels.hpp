@@ -567,6 +567,9 @@ __global__ __launch_bounds__(HexCfg<NQ>::BLOCK) void hex_physderiv_kernel(
     hex_deriv_ij<NQ, T>(tab.D, active, t, bu, b0, b1);
     if (!valid) return;
     const long nt = (long)nElmt * Q;
+    // all loads before any store: gfx9 counts both in one in-order counter, so a load
+    // issued after a store would wait for it
+    T o[3][NQ];
 #pragma unroll
     for (int k = 0; k < NQ; ++k) {
         T d2 = u[0] * tab.D[k];
@@ -582,10 +585,12 @@ __global__ __launch_bounds__(HexCfg<NQ>::BLOCK) void hex_physderiv_kernel(
             for (int d = 0; d < 9; ++d) f[d] = df[(long)d * nElmt + e];
         }
 #pragma unroll
-        for (int c = 0; c < 3; ++c)
-            st_stream(out + c * nt + e * Q + k * TPE + t,
-                      fma(d2, f[c * 3 + 2], fma(d1, f[c * 3 + 1], d0 * f[c * 3])));
+        for (int c = 0; c < 3; ++c) o[c][k] = fma(d2, f[c * 3 + 2], fma(d1, f[c * 3 + 1], d0 * f[c * 3]));
     }
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int k = 0; k < NQ; ++k) st_stream(out + c * nt + e * Q + k * TPE + t, o[c][k]);
 }
 
 // IProductWRTDerivBase (IProductWRTDerivBase.h:1231-1343): out = sum_a (D_a B)^T

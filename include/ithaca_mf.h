@@ -117,6 +117,54 @@ int imf_apply_many_device(imf_op op, int nvec, const void *d_in, long ld_in, voi
 /* Message of the last failing call on this thread ("" if none). */
 const char *imf_last_error(void);
 
+/* ---- C0 global assembly and the matrix-free Helmholtz solve (hexahedra) -------------
+ * Global numbering of the modified (C0) basis built from the elements' vertex ids
+ * (vertex_ids[e*8 + v], any 64-bit ids, vertex order of the reference hex): vertex modes
+ * shared by id, edge-interior modes by the vertex pair with sign (-1)^m against the global
+ * edge direction, face-interior modes by the 4 vertex ids in a frame fixed by the ids
+ * (sign per reversed axis, index swap for swapped axes).  Local arrays are element-major
+ * (e*nm^3 + (r*nm+q)*nm + p), global arrays have imf_assembly_nglobal entries. */
+typedef struct imf_asm_s *imf_asm;
+
+/* Host only (no device): the map itself.  l2g, sign, on_boundary (1 if the local mode lies
+ * on a face owned by one element) each nElmt*nm^3 entries or NULL; *nglobal out. */
+int imf_assembly_map_hex(int nElmt, int nm, const long long *vertex_ids, int *l2g,
+                         signed char *sign, unsigned char *on_boundary, long *nglobal);
+
+/* Device assembly map.  dirichlet = 1: homogeneous Dirichlet conditions on every mode of
+ * the boundary faces (those modes are excluded from the solve and set to zero);
+ * dirichlet = 0: natural (Neumann) conditions. */
+int imf_assembly_create_hex(int nElmt, int nm, const long long *vertex_ids, int dirichlet,
+                            imf_asm *assembly);
+long imf_assembly_nglobal(imf_asm assembly);
+int imf_assembly_destroy(imf_asm assembly);
+
+/* AssemblyMapCG::GlobalToLocal (MultiRegions/AssemblyMap/AssemblyMapCG.cpp:2853-2876):
+ * local = sign * global[l2g].  Device arrays, asynchronous. */
+int imf_global_to_local_device(imf_asm assembly, const double *d_global, double *d_local,
+                               void *stream);
+
+/* AssemblyMapCG::Assemble (AssemblyMapCG.cpp:2885-2910): global = sum of sign * local over
+ * each global mode's local copies (overwrites global; deterministic order).  Asynchronous. */
+int imf_assemble_device(imf_asm assembly, const double *d_local, double *d_global, void *stream);
+
+/* ContField::HelmSolve's iterative path with a matrix-free operator: conjugate gradients
+ * of NekLinSysIterCG::DoConjugateGradient (LibUtilities/LinearAlgebra/NekLinSysIterCG.cpp:
+ * 104-260, one fused reduction per iteration) on y = Assemble(H(GlobalToLocal(x)))
+ * (GlobalLinSysIterativeFull::v_DoMatrixMultiply, GlobalLinSysIterativeFull.cpp:215-232)
+ * with the assembled diagonal as preconditioner; stops when (r,r) < tol^2 (b,b)
+ * (Set_Rhs_Magnitude, NekLinSysIter.cpp:128-153; the reference's default tol is 1e-9).
+ * helmholtz: an FP64 "Helmholtz_Hex_*" operator on the same elements and modes, with
+ * SetJac/SetDF/SetLambda done.  d_rhs, d_x: global arrays.  Synchronous (reads the three
+ * dot products every iteration); allocates its workspace on the first call.  Returns
+ * IMF_ERR_STATE with *iterations = max_iter + 1 when the tolerance is not reached. */
+int imf_helmsolve_cg_device(imf_asm assembly, imf_op helmholtz, const double *d_rhs,
+                            double *d_x, double tol, int max_iter, int *iterations,
+                            double *residual, void *stream);
+
+/* Message of the last failing assembly call on this thread. */
+const char *imf_assembly_last_error(void);
+
 /* ---- set-up helpers (no reference FFI counterpart; host-side restatements) ---- */
 
 /* LibUtilities::Basis::GenBasis for eModified_A on nq Gauss-Lobatto-Legendre points

@@ -1,0 +1,588 @@
+// C0 global assembly of hexahedral expansions and the matrix-free conjugate-gradient
+// Helmholtz solve on the GPU.
+//
+// Reference (paths relative to the reference root):
+//   AssemblyMapCG::v_GlobalToLocal / v_Assemble   MultiRegions/AssemblyMap/AssemblyMapCG.cpp:2853-2910
+//   GlobalLinSysIterativeFull::v_DoMatrixMultiply MultiRegions/GlobalLinSysIterativeFull.cpp:215-232
+//   NekLinSysIterCG::DoConjugateGradient          LibUtilities/LinearAlgebra/NekLinSysIterCG.cpp:104-260
+//   NekLinSysIter::Set_Rhs_Magnitude              LibUtilities/LinearAlgebra/NekLinSysIter.cpp:128-153
+// The global numbering is built here from the elements' vertex ids: vertex modes are
+// shared through the vertex id, edge-interior modes through the (unordered) vertex pair with
+// a sign (-1)^m where an element traverses the edge against the global direction, and
+// face-interior modes through the face's 4 vertex ids in a frame fixed by those ids (origin
+// at the smallest id, first axis towards its smaller neighbour), with a sign for each
+// reversed axis and an index swap when the element's face axes are swapped.  This is the
+// sign/permutation structure of the modified (C0) basis; the reference derives the same
+// from its mesh graph.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/ithaca_mf.h"
+#include "mf_op.hpp"
+
+namespace {
+
+thread_local std::string g_aerr;
+
+int afail(int code, const std::string &m)
+{
+    g_aerr = m;
+    return code;
+}
+
+// Local vertex of the reference hex with coordinate bits (b0, b1, b2): xi0 = +1 for v%4 in
+// {1,2}, xi1 = +1 for v%4 in {2,3}, xi2 = +1 for v >= 4 (StdRegions/StdHexExp.cpp:1042-1082).
+inline int vert_of(int b0, int b1, int b2)
+{
+    const int v = b1 ? (b0 ? 2 : 3) : (b0 ? 1 : 0);
+    return v + 4 * b2;
+}
+
+struct H2 {
+    size_t operator()(const std::array<long long, 2> &a) const
+    {
+        return std::hash<long long>()(a[0] * 1000003LL ^ a[1]);
+    }
+};
+struct H4 {
+    size_t operator()(const std::array<long long, 4> &a) const
+    {
+        size_t h = 0;
+        for (long long v : a) h = h * 1000003ULL ^ std::hash<long long>()(v);
+        return h;
+    }
+};
+
+// l2g[e*M + m] = global index, sgn = +-1, bnd[e*M + m] = 1 if the local mode lies on a face
+// owned by one element only.
+int build_map_hex(int E, int nm, const long long *vid, std::vector<int> &l2g,
+                  std::vector<signed char> &sgn, std::vector<unsigned char> &bnd, long &nglobal)
+{
+    if (E <= 0 || nm < 2 || !vid) return afail(IMF_ERR_ARG, "assembly: bad argument");
+    const int M = nm * nm * nm, ni = nm - 2;
+    std::unordered_map<long long, int> vmap;
+    std::unordered_map<std::array<long long, 2>, int, H2> emap;
+    std::unordered_map<std::array<long long, 4>, std::array<int, 2>, H4> fmap;  // id, uses
+    for (long e = 0; e < E; ++e) {
+        const long long *v = vid + e * 8;
+        for (int a = 0; a < 8; ++a) vmap.emplace(v[a], (int)vmap.size());
+        for (int d = 0; d < 3; ++d)
+            for (int c = 0; c < 4; ++c) {  // the 4 edges along direction d
+                int bits[3], bb[3];
+                const int o1 = (d + 1) % 3, o2 = (d + 2) % 3;
+                bits[o1] = c & 1;
+                bits[o2] = c >> 1;
+                bits[d] = 0;
+                std::copy(bits, bits + 3, bb);
+                bb[d] = 1;
+                long long a0 = v[vert_of(bits[0], bits[1], bits[2])];
+                long long a1 = v[vert_of(bb[0], bb[1], bb[2])];
+                if (a0 == a1) return afail(IMF_ERR_ARG, "assembly: degenerate edge");
+                emap.emplace(std::array<long long, 2>{std::min(a0, a1), std::max(a0, a1)},
+                             (int)emap.size());
+            }
+        for (int df = 0; df < 3; ++df)
+            for (int s = 0; s < 2; ++s) {
+                std::array<long long, 4> k;
+                int n = 0;
+                for (int c = 0; c < 4; ++c) {
+                    int bits[3];
+                    bits[df] = s;
+                    bits[(df + 1) % 3] = c & 1;
+                    bits[(df + 2) % 3] = c >> 1;
+                    k[n++] = v[vert_of(bits[0], bits[1], bits[2])];
+                }
+                std::sort(k.begin(), k.end());
+                auto it = fmap.find(k);
+                if (it == fmap.end()) fmap.emplace(k, std::array<int, 2>{(int)fmap.size(), 1});
+                else if (++it->second[1] > 2)
+                    return afail(IMF_ERR_ARG, "assembly: a face is shared by more than two elements");
+            }
+    }
+    const long nV = (long)vmap.size(), nE = (long)emap.size(), nF = (long)fmap.size();
+    const long baseE = nV, baseF = baseE + nE * ni, baseI = baseF + nF * (long)ni * ni;
+    nglobal = baseI + (long)E * ni * ni * ni;
+    if (nglobal > 2147483647L) return afail(IMF_ERR_ARG, "assembly: more than 2^31 global modes");
+    l2g.assign((size_t)E * M, 0);
+    sgn.assign((size_t)E * M, 1);
+    bnd.assign((size_t)E * M, 0);
+    for (long e = 0; e < E; ++e) {
+        const long long *v = vid + e * 8;
+        for (int r = 0; r < nm; ++r)
+            for (int q = 0; q < nm; ++q)
+                for (int p = 0; p < nm; ++p) {
+                    const int m[3] = {p, q, r};
+                    const size_t li = (size_t)e * M + (r * nm + q) * nm + p;
+                    int nv = 0;
+                    for (int d = 0; d < 3; ++d) nv += m[d] < 2;
+                    long gi = 0;
+                    int sg = 1;
+                    if (nv == 3) {
+                        gi = vmap[v[vert_of(p, q, r)]];
+                    } else if (nv == 2) {
+                        int di = 0;
+                        while (m[di] < 2) ++di;
+                        int b0[3] = {m[0], m[1], m[2]}, b1[3] = {m[0], m[1], m[2]};
+                        b0[di] = 0;
+                        b1[di] = 1;
+                        const long long g0 = v[vert_of(b0[0], b0[1], b0[2])];
+                        const long long g1 = v[vert_of(b1[0], b1[1], b1[2])];
+                        const int idx = emap[{std::min(g0, g1), std::max(g0, g1)}];
+                        gi = baseE + (long)idx * ni + (m[di] - 2);
+                        if (g0 > g1 && (m[di] & 1)) sg = -1;
+                    } else if (nv == 1) {
+                        int df = 0;
+                        while (m[df] >= 2) ++df;
+                        const int d1 = df == 0 ? 1 : 0, d2 = df == 2 ? 1 : 2;  // d1 < d2
+                        long long g[2][2];
+                        std::array<long long, 4> k;
+                        for (int c1 = 0; c1 < 2; ++c1)
+                            for (int c2 = 0; c2 < 2; ++c2) {
+                                int bits[3];
+                                bits[df] = m[df];
+                                bits[d1] = c1;
+                                bits[d2] = c2;
+                                g[c1][c2] = v[vert_of(bits[0], bits[1], bits[2])];
+                                k[c1 * 2 + c2] = g[c1][c2];
+                            }
+                        std::sort(k.begin(), k.end());
+                        const int idx = fmap[k][0];
+                        int o1 = 0, o2 = 0;
+                        for (int c1 = 0; c1 < 2; ++c1)
+                            for (int c2 = 0; c2 < 2; ++c2)
+                                if (g[c1][c2] < g[o1][o2]) {
+                                    o1 = c1;
+                                    o2 = c2;
+                                }
+                        const bool tr = g[o1][1 - o2] < g[1 - o1][o2];  // first axis along d2
+                        const int mA = tr ? m[d2] : m[d1], mB = tr ? m[d1] : m[d2];
+                        const bool fA = tr ? o2 == 1 : o1 == 1, fB = tr ? o1 == 1 : o2 == 1;
+                        gi = baseF + (long)idx * ni * ni + (long)(mA - 2) * ni + (mB - 2);
+                        if (fA && (mA & 1)) sg = -sg;
+                        if (fB && (mB & 1)) sg = -sg;
+                    } else {
+                        gi = baseI + ((e * ni + (r - 2)) * (long)ni + (q - 2)) * ni + (p - 2);
+                    }
+                    l2g[li] = (int)gi;
+                    sgn[li] = (signed char)sg;
+                }
+        // modes on boundary faces (faces with one user): the vertex-type index of the
+        // face's normal direction equals the face side
+        for (int df = 0; df < 3; ++df)
+            for (int s = 0; s < 2; ++s) {
+                std::array<long long, 4> k;
+                for (int c = 0; c < 4; ++c) {
+                    int bits[3];
+                    bits[df] = s;
+                    bits[(df + 1) % 3] = c & 1;
+                    bits[(df + 2) % 3] = c >> 1;
+                    k[c] = v[vert_of(bits[0], bits[1], bits[2])];
+                }
+                std::sort(k.begin(), k.end());
+                if (fmap[k][1] != 1) continue;
+                for (int r = 0; r < nm; ++r)
+                    for (int q = 0; q < nm; ++q)
+                        for (int p = 0; p < nm; ++p) {
+                            const int m[3] = {p, q, r};
+                            if (m[df] == s) bnd[(size_t)e * M + (r * nm + q) * nm + p] = 1;
+                        }
+            }
+    }
+    return IMF_OK;
+}
+
+// --------------------------------------------------------------------------- kernels
+// local = sign * global[l2g]   (Vmath::Gathr with signs)
+__global__ void g2l_kernel(long n, const int *__restrict__ l2g, const signed char *__restrict__ sg,
+                           const double *__restrict__ glob, double *__restrict__ loc)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) loc[i] = sg[i] * glob[l2g[i]];
+}
+
+// global[g] = sum over the local modes mapped to g of sign * local (Vmath::Assmb), as a
+// deterministic gather over the inverse map (CSR: off[g]..off[g+1] into inv).
+// With sg == NULL the contributions are added unsigned (the assembled diagonal:
+// sign^2 = 1).
+__global__ void assemble_kernel(long ng, const long *__restrict__ off, const long *__restrict__ inv,
+                                const signed char *__restrict__ sg, const double *__restrict__ loc,
+                                const unsigned char *__restrict__ mask, double *__restrict__ glob)
+{
+    const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= ng) return;
+    double s = 0.0;
+    for (long k = off[g]; k < off[g + 1]; ++k) {
+        const long i = inv[k];
+        s += sg ? sg[i] * loc[i] : loc[i];
+    }
+    glob[g] = (mask && mask[g]) ? 0.0 : s;
+}
+
+__global__ void zero_kernel(long n, double *__restrict__ x)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] = 0.0;
+}
+
+struct DiagTab {
+    double B[128], dB[128], w[16];
+};
+
+// Local diagonal of the Helmholtz operator lambda M + sum_ab (D_a B)^T diag(w J G^ab) (D_b B)
+// from the operator's packed geometry (deformed [e][GS], GS = 7Q rounded to 128-byte lines;
+// regular [e][7] with the weights applied here); one thread per (element, mode).
+__global__ void helm_diag_hex(DiagTab t, int nm, int nq, long nElmt, int deformed, double lambda,
+                              const double *__restrict__ geom, double *__restrict__ diag)
+{
+    const long M = (long)nm * nm * nm, Q = (long)nq * nq * nq;
+    const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= nElmt * M) return;
+    const long e = g / M;
+    const int md = (int)(g - e * M), p = md % nm, q = (md / nm) % nm, r = md / (nm * nm);
+    const long GS = imf::geo_stride(7 * Q, sizeof(double));
+    double s = 0.0;
+    for (int k = 0; k < nq; ++k)
+        for (int j = 0; j < nq; ++j)
+            for (int i = 0; i < nq; ++i) {
+                const long pt = ((long)k * nq + j) * nq + i;
+                const double bp = t.B[p * nq + i], bq = t.B[q * nq + j], br = t.B[r * nq + k];
+                const double phi = bp * bq * br;
+                const double d0 = t.dB[p * nq + i] * bq * br, d1 = bp * t.dB[q * nq + j] * br,
+                             d2 = bp * bq * t.dB[r * nq + k];
+                double c[7];
+                if (deformed) {
+                    for (int a = 0; a < 7; ++a) c[a] = geom[e * GS + a * Q + pt];
+                } else {
+                    const double wq = t.w[i] * t.w[j] * t.w[k];
+                    for (int a = 0; a < 7; ++a) c[a] = geom[e * 7 + a] * wq;
+                }
+                s += lambda * c[0] * phi * phi + c[1] * d0 * d0 + c[4] * d1 * d1 + c[6] * d2 * d2 +
+                     2.0 * (c[2] * d0 * d1 + c[3] * d0 * d2 + c[5] * d1 * d2);
+            }
+    diag[g] = s;
+}
+
+__global__ void inv_masked(long n, const unsigned char *__restrict__ mask, double *__restrict__ d)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) d[i] = (mask && mask[i]) ? 0.0 : 1.0 / d[i];
+}
+
+// r = mask0(b), x = 0
+__global__ void cg_init(long n, const unsigned char *__restrict__ mask, const double *__restrict__ b,
+                        double *__restrict__ r, double *__restrict__ x)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    r[i] = (mask && mask[i]) ? 0.0 : b[i];
+    x[i] = 0.0;
+}
+
+// w = dinv * r
+__global__ void cg_precon(long n, const double *__restrict__ dinv, const double *__restrict__ r,
+                          double *__restrict__ w)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) w[i] = dinv[i] * r[i];
+}
+
+// p = w + beta p; q = s + beta q; x += alpha p; r -= alpha q; w = dinv r
+// (NekLinSysIterCG.cpp:186-204, with the Jacobi preconditioner fused)
+__global__ void cg_update(long n, double alpha, double beta, const double *__restrict__ dinv,
+                          double *__restrict__ p, double *__restrict__ q, double *__restrict__ w,
+                          const double *__restrict__ s, double *__restrict__ x, double *__restrict__ r)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double pi = w[i] + beta * p[i], qi = s[i] + beta * q[i];
+    p[i] = pi;
+    q[i] = qi;
+    x[i] += alpha * pi;
+    const double ri = r[i] - alpha * qi;
+    r[i] = ri;
+    w[i] = dinv[i] * ri;
+}
+
+constexpr int RB = 256;
+
+// partial sums of (a.b, c.b, a.a) per block (Dirichlet entries are zero in all vectors)
+__global__ void dot3_partial(long n, const double *__restrict__ a, const double *__restrict__ b,
+                             const double *__restrict__ c, double *__restrict__ part)
+{
+    __shared__ double sh[3][RB / 64];
+    double s0 = 0, s1 = 0, s2 = 0;
+    for (long i = (long)blockIdx.x * RB + threadIdx.x; i < n; i += (long)gridDim.x * RB) {
+        const double ai = a[i], bi = b[i];
+        s0 += ai * bi;
+        s1 += c[i] * bi;
+        s2 += ai * ai;
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        s0 += __shfl_xor(s0, off);
+        s1 += __shfl_xor(s1, off);
+        s2 += __shfl_xor(s2, off);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        sh[0][threadIdx.x >> 6] = s0;
+        sh[1][threadIdx.x >> 6] = s1;
+        sh[2][threadIdx.x >> 6] = s2;
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        double t = 0;
+        for (int w = 0; w < RB / 64; ++w) t += sh[threadIdx.x][w];
+        part[(long)threadIdx.x * gridDim.x + blockIdx.x] = t;
+    }
+}
+
+__global__ void dot3_final(int nb, const double *__restrict__ part, double *__restrict__ out)
+{
+    __shared__ double sh[RB / 64];
+    for (int c = 0; c < 3; ++c) {
+        double s = 0;
+        for (int i = threadIdx.x; i < nb; i += RB) s += part[(long)c * nb + i];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off);
+        if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double t = 0;
+            for (int w = 0; w < RB / 64; ++w) t += sh[w];
+            out[c] = t;
+        }
+        __syncthreads();
+    }
+}
+
+inline unsigned nblk(long n) { return (unsigned)((n + RB - 1) / RB); }
+
+}  // namespace
+
+struct imf_asm_s {
+    int nElmt = 0, nm = 0, dirichlet = 0;
+    long nloc = 0, ng = 0;
+    int *d_l2g = nullptr;
+    signed char *d_sg = nullptr;
+    long *d_off = nullptr, *d_inv = nullptr;
+    unsigned char *d_mask = nullptr;  // global Dirichlet mask (NULL without Dirichlet)
+    // solver workspace (allocated on the first solve)
+    double *d_ws = nullptr, *d_loc = nullptr, *d_part = nullptr, *d_dots = nullptr;
+    int nparts = 0;
+    const void *diag_op = nullptr;  // operator and lambda the cached diagonal belongs to
+    double diag_lambda = 0.0;
+};
+
+namespace {
+
+#define ACHECK(call, what)                                                                 \
+    do {                                                                                   \
+        hipError_t e_ = (call);                                                            \
+        if (e_ != hipSuccess) return afail(IMF_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+// out = mask0(Assemble(H(GlobalToLocal(in))))  (GlobalLinSysIterativeFull::v_DoMatrixMultiply)
+int apply_global(imf_asm a, imf_op op, const double *in, double *out, hipStream_t s)
+{
+    double *l0 = a->d_loc, *l1 = a->d_loc + a->nloc;
+    g2l_kernel<<<nblk(a->nloc), RB, 0, s>>>(a->nloc, a->d_l2g, a->d_sg, in, l0);
+    if (int st = imf_apply_device(op, l0, l1, s)) return st;
+    assemble_kernel<<<nblk(a->ng), RB, 0, s>>>(a->ng, a->d_off, a->d_inv, a->d_sg, l1, a->d_mask, out);
+    ACHECK(hipGetLastError(), "assembly kernels");
+    return IMF_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *imf_assembly_last_error(void) { return g_aerr.c_str(); }
+
+int imf_assembly_map_hex(int nElmt, int nm, const long long *vertex_ids, int *l2g,
+                         signed char *sign, unsigned char *on_boundary, long *nglobal)
+{
+    std::vector<int> m;
+    std::vector<signed char> sg;
+    std::vector<unsigned char> bd;
+    long ng = 0;
+    if (int st = build_map_hex(nElmt, nm, vertex_ids, m, sg, bd, ng)) return st;
+    if (l2g) std::memcpy(l2g, m.data(), m.size() * sizeof(int));
+    if (sign) std::memcpy(sign, sg.data(), sg.size());
+    if (on_boundary) std::memcpy(on_boundary, bd.data(), bd.size());
+    if (nglobal) *nglobal = ng;
+    return IMF_OK;
+}
+
+int imf_assembly_create_hex(int nElmt, int nm, const long long *vertex_ids, int dirichlet,
+                            imf_asm *out)
+{
+    if (!out) return afail(IMF_ERR_ARG, "imf_assembly_create_hex: null output");
+    std::vector<int> m;
+    std::vector<signed char> sg;
+    std::vector<unsigned char> bd;
+    long ng = 0;
+    if (int st = build_map_hex(nElmt, nm, vertex_ids, m, sg, bd, ng)) return st;
+    auto *a = new imf_asm_s();
+    a->nElmt = nElmt;
+    a->nm = nm;
+    a->dirichlet = dirichlet ? 1 : 0;
+    a->nloc = (long)m.size();
+    a->ng = ng;
+    // inverse map (CSR) for the deterministic assembly
+    std::vector<long> off(ng + 1, 0), inv(a->nloc);
+    for (long i = 0; i < a->nloc; ++i) ++off[m[i] + 1];
+    for (long g = 0; g < ng; ++g) off[g + 1] += off[g];
+    std::vector<long> fill(off.begin(), off.end() - 1);
+    for (long i = 0; i < a->nloc; ++i) inv[fill[m[i]]++] = i;
+    std::vector<unsigned char> gmask;
+    if (a->dirichlet) {
+        gmask.assign(ng, 0);
+        for (long i = 0; i < a->nloc; ++i)
+            if (bd[i]) gmask[m[i]] = 1;
+    }
+    auto cleanup = [&](int st) {
+        imf_assembly_destroy(a);
+        return st;
+    };
+    hipError_t e;
+    if ((e = hipMalloc(&a->d_l2g, a->nloc * sizeof(int))) != hipSuccess ||
+        (e = hipMalloc(&a->d_sg, a->nloc)) != hipSuccess ||
+        (e = hipMalloc(&a->d_off, (ng + 1) * sizeof(long))) != hipSuccess ||
+        (e = hipMalloc(&a->d_inv, a->nloc * sizeof(long))) != hipSuccess ||
+        (a->dirichlet && (e = hipMalloc(&a->d_mask, ng)) != hipSuccess))
+        return cleanup(afail(IMF_ERR_HIP, std::string("hipMalloc(assembly): ") + hipGetErrorString(e)));
+    if ((e = hipMemcpy(a->d_l2g, m.data(), a->nloc * sizeof(int), hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMemcpy(a->d_sg, sg.data(), a->nloc, hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMemcpy(a->d_off, off.data(), (ng + 1) * sizeof(long), hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMemcpy(a->d_inv, inv.data(), a->nloc * sizeof(long), hipMemcpyHostToDevice)) != hipSuccess ||
+        (a->dirichlet && (e = hipMemcpy(a->d_mask, gmask.data(), ng, hipMemcpyHostToDevice)) != hipSuccess))
+        return cleanup(afail(IMF_ERR_HIP, std::string("hipMemcpy(assembly): ") + hipGetErrorString(e)));
+    *out = a;
+    return IMF_OK;
+}
+
+long imf_assembly_nglobal(imf_asm a) { return a ? a->ng : 0; }
+
+int imf_assembly_destroy(imf_asm a)
+{
+    if (!a) return IMF_OK;
+    for (void *p : {(void *)a->d_l2g, (void *)a->d_sg, (void *)a->d_off, (void *)a->d_inv,
+                    (void *)a->d_mask, (void *)a->d_ws, (void *)a->d_loc, (void *)a->d_part,
+                    (void *)a->d_dots})
+        if (p) (void)hipFree(p);
+    delete a;
+    return IMF_OK;
+}
+
+int imf_global_to_local_device(imf_asm a, const double *d_global, double *d_local, void *stream)
+{
+    if (!a || !d_global || !d_local) return afail(IMF_ERR_ARG, "imf_global_to_local_device: null argument");
+    g2l_kernel<<<nblk(a->nloc), RB, 0, (hipStream_t)stream>>>(a->nloc, a->d_l2g, a->d_sg, d_global, d_local);
+    ACHECK(hipGetLastError(), "g2l_kernel");
+    return IMF_OK;
+}
+
+int imf_assemble_device(imf_asm a, const double *d_local, double *d_global, void *stream)
+{
+    if (!a || !d_global || !d_local) return afail(IMF_ERR_ARG, "imf_assemble_device: null argument");
+    assemble_kernel<<<nblk(a->ng), RB, 0, (hipStream_t)stream>>>(a->ng, a->d_off, a->d_inv, a->d_sg,
+                                                                d_local, nullptr, d_global);
+    ACHECK(hipGetLastError(), "assemble_kernel");
+    return IMF_OK;
+}
+
+int imf_helmsolve_cg_device(imf_asm a, imf_op op, const double *d_rhs, double *d_x, double tol,
+                            int max_iter, int *iterations, double *residual, void *stream)
+{
+    if (!a || !op || !d_rhs || !d_x) return afail(IMF_ERR_ARG, "imf_helmsolve_cg_device: null argument");
+    if (op->kind != imf::kHelmholtz || op->shape != imf::kHex || op->prec != IMF_FP64 ||
+        op->nElmt != a->nElmt || op->nm != a->nm)
+        return afail(IMF_ERR_ARG, "imf_helmsolve_cg_device: needs an FP64 Helmholtz_Hex operator "
+                                  "on the assembly's elements and modes");
+    if (!op->geom_ready) return afail(IMF_ERR_STATE, "Jacobian and derivative factors not set");
+    if (op->dbdata.size() != (size_t)op->nm * op->nq || op->nq > 16)
+        return afail(IMF_ERR_ARG, "imf_helmsolve_cg_device: operator created without dbdata");
+    hipStream_t s = (hipStream_t)stream;
+    const long ng = a->ng;
+    if (!a->d_ws) {
+        a->nparts = (int)std::min<long>(1024, (ng + RB - 1) / RB);
+        ACHECK(hipMalloc(&a->d_ws, 6 * ng * sizeof(double)), "hipMalloc(cg)");
+        ACHECK(hipMalloc(&a->d_loc, 2 * a->nloc * sizeof(double)), "hipMalloc(cg)");
+        ACHECK(hipMalloc(&a->d_part, 3 * (long)a->nparts * sizeof(double)), "hipMalloc(cg)");
+        ACHECK(hipMalloc(&a->d_dots, 4 * sizeof(double)), "hipMalloc(cg)");
+        a->diag_op = nullptr;
+    }
+    double *r = a->d_ws, *p = r + ng, *q = p + ng, *w = q + ng, *sv = w + ng, *dinv = sv + ng;
+    // assembled diagonal (Jacobi preconditioner), cached per operator and lambda
+    if (a->diag_op != (const void *)op || a->diag_lambda != op->lambda) {
+        DiagTab t{};
+        const int nm = op->nm, nq = op->nq;
+        std::copy(op->bdata.begin(), op->bdata.end(), t.B);
+        std::copy(op->dbdata.begin(), op->dbdata.end(), t.dB);
+        std::copy(op->w.begin(), op->w.end(), t.w);
+        helm_diag_hex<<<nblk(a->nloc), RB, 0, s>>>(t, nm, nq, a->nElmt, op->deformed, op->lambda,
+                                                  (const double *)op->d_geom, a->d_loc);
+        assemble_kernel<<<nblk(ng), RB, 0, s>>>(ng, a->d_off, a->d_inv, nullptr, a->d_loc, nullptr, dinv);
+        inv_masked<<<nblk(ng), RB, 0, s>>>(ng, a->d_mask, dinv);
+        ACHECK(hipGetLastError(), "diagonal kernels");
+        a->diag_op = op;
+        a->diag_lambda = op->lambda;
+    }
+    double h[4];
+    auto dots = [&](const double *x0, const double *x1, const double *x2) -> int {
+        dot3_partial<<<a->nparts, RB, 0, s>>>(ng, x0, x1, x2, a->d_part);
+        dot3_final<<<1, RB, 0, s>>>(a->nparts, a->d_part, a->d_dots);
+        ACHECK(hipGetLastError(), "dot kernels");
+        ACHECK(hipMemcpyAsync(h, a->d_dots, 3 * sizeof(double), hipMemcpyDeviceToHost, s), "copy");
+        ACHECK(hipStreamSynchronize(s), "sync");
+        return IMF_OK;
+    };
+    // rhs magnitude over all global modes (Set_Rhs_Magnitude)
+    if (int st = dots(d_rhs, d_rhs, d_rhs)) return st;
+    const double rhs_mag = h[0] > 1e-6 ? h[0] : 1.0;
+    cg_init<<<nblk(ng), RB, 0, s>>>(ng, a->d_mask, d_rhs, r, d_x);
+    ACHECK(hipGetLastError(), "cg_init");
+    if (int st = dots(r, r, r)) return st;
+    double eps = h[2];
+    const double thr = tol * tol * rhs_mag;
+    int it = 0;
+    if (eps >= thr) {
+        cg_precon<<<nblk(ng), RB, 0, s>>>(ng, dinv, r, w);
+        if (int st = apply_global(a, op, w, sv, s)) return st;
+        if (int st = dots(r, w, sv)) return st;  // (r,w), (s,w), (r,r)
+        double rho = h[0], mu = h[1];
+        double beta = 0.0, alpha = rho / mu;
+        it = 1;
+        zero_kernel<<<nblk(2 * ng), RB, 0, s>>>(2 * ng, p);  // p and q
+        for (;;) {
+            if (it > max_iter) {
+                if (iterations) *iterations = it;
+                if (residual) *residual = std::sqrt(eps / rhs_mag);
+                return afail(IMF_ERR_STATE, "Exceeded maximum number of iterations");
+            }
+            cg_update<<<nblk(ng), RB, 0, s>>>(ng, alpha, beta, dinv, p, q, w, sv, d_x, r);
+            if (int st = apply_global(a, op, w, sv, s)) return st;
+            if (int st = dots(r, w, sv)) return st;
+            const double rho_new = h[0];
+            mu = h[1];
+            eps = h[2];
+            ++it;
+            if (eps < thr) break;
+            beta = rho_new / rho;
+            alpha = rho_new / (mu - rho_new * beta / alpha);
+            rho = rho_new;
+        }
+    }
+    if (iterations) *iterations = it;
+    if (residual) *residual = std::sqrt(eps / rhs_mag);
+    return IMF_OK;
+}
+
+}  // extern "C"

@@ -13,21 +13,7 @@
 #include "imf_common.hpp"
 #include "mf_geom.hpp"
 #include "mf_launch.hpp"
-
-struct imf_op_s {
-    int kind = 0, shape = 0, deformed = 0, prec = 0, dim = 0, nm = 0, nq = 0;
-    int nElmt = 0;
-    size_t M = 0, Q = 0;
-    std::vector<double> w;                 // direction-0 weights (host)
-    std::vector<unsigned char> tab;        // Tab<NM,NQ,T> image
-    double lambda = 1.0;                   // Helmholtz default m_lambda(1.0)
-    double *d_jac = nullptr, *d_df = nullptr;
-    size_t n_jac = 0, n_df = 0;
-    void *d_geom = nullptr;
-    bool geom_ready = false;
-    void *d_in = nullptr, *d_out = nullptr;  // staging for host applies
-    imf::LaunchFn fwd = nullptr, tr = nullptr;
-};
+#include "mf_op.hpp"
 
 namespace {
 
@@ -158,7 +144,8 @@ int imf_create(const char *key, int dim, const int *nm, const int *nq,
 {
     if (!key || !nm || !nq || !bdata || !D || !w || !out)
         return fail(IMF_ERR_ARG, "imf_create: null argument");
-    (void)dbdata;  // D bdata is applied in factored form by the kernels
+    // dbdata = D bdata is applied in factored form by the kernels; a copy is kept for
+    // the assembled diagonal (mf_assembly.hip)
     *out = nullptr;
     const std::string k(key);
     int kind, shape, deformed;
@@ -210,6 +197,8 @@ int imf_create(const char *key, int dim, const int *nm, const int *nq,
     op->M = 1; op->Q = 1;
     for (int d = 0; d < dim; ++d) { op->M *= m; op->Q *= q; }
     op->w.assign(w[0], w[0] + q);
+    op->bdata.assign(bdata[0], bdata[0] + (size_t)m * q);
+    if (dbdata && dbdata[0]) op->dbdata.assign(dbdata[0], dbdata[0] + (size_t)m * q);
     if (precision == IMF_FP64) fill_tab<double>(op, bdata[0], D[0], w[0]);
     else fill_tab<float>(op, bdata[0], D[0], w[0]);
     const int geo = kind == imf::kBwdTrans ? imf::kUnit : (deformed ? imf::kDeformed : imf::kRegular);

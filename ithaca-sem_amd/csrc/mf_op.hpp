@@ -1,0 +1,23 @@
+// Internal definition of an operator handle, shared by the C-ABI translation units.
+#pragma once
+
+#include <cstddef>
+#include <vector>
+
+#include "imf_common.hpp"
+
+struct imf_op_s {
+    int kind = 0, shape = 0, deformed = 0, prec = 0, dim = 0, nm = 0, nq = 0;
+    int nElmt = 0;
+    size_t M = 0, Q = 0;
+    std::vector<double> w;                 // direction-0 weights (host)
+    std::vector<double> bdata, dbdata;     // direction-0 tables (host, nm*nq)
+    std::vector<unsigned char> tab;        // Tab<NM,NQ,T> image
+    double lambda = 1.0;                   // Helmholtz default m_lambda(1.0)
+    double *d_jac = nullptr, *d_df = nullptr;
+    size_t n_jac = 0, n_df = 0;
+    void *d_geom = nullptr;
+    bool geom_ready = false;
+    void *d_in = nullptr, *d_out = nullptr;  // staging for host applies
+    imf::LaunchFn fwd = nullptr, tr = nullptr;
+};

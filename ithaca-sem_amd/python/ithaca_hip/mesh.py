@@ -93,3 +93,42 @@ def pod_fields(n_fields=20, n_snap=512, seed=2):
     m = rng.integers(1, 7, size=(n_fields, 3))
     a = rng.standard_normal((n_snap, n_fields)) * 2.0 ** -np.arange(1, n_fields + 1)
     return m, a
+
+
+def hex_box_ids(n):
+    """Global vertex ids (n^3, 8) of hex_box(n): vertex (ix, iy, iz) of the (n+1)^3 grid has
+    id (iz*(n+1) + iy)*(n+1) + ix; same element and vertex order as hex_box."""
+    ez, ey, ex = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    ex, ey, ez = ex.ravel(), ey.ravel(), ez.ravel()
+    ids = np.empty((n ** 3, 8), dtype=np.int64)
+    for v, (cx, cy, cz) in enumerate(HEX_CORNERS):
+        ids[:, v] = ((ez + cz) * (n + 1) + (ey + cy)) * (n + 1) + (ex + cx)
+    return ids
+
+
+def cube_rotations():
+    """The 24 proper rotations of the reference cube as vertex permutations: element vertex
+    k of the rotated frame is vertex perm[k] of the original frame."""
+    import itertools
+    corners = HEX_CORNERS * 2 - 1
+    index = {tuple(c): v for v, c in enumerate(corners)}
+    perms = []
+    for axes in itertools.permutations(range(3)):
+        for signs in itertools.product((1, -1), repeat=3):
+            R = np.zeros((3, 3), dtype=int)
+            for i, (a, s) in enumerate(zip(axes, signs)):
+                R[i, a] = s
+            if round(np.linalg.det(R)) != 1:
+                continue
+            perms.append([index[tuple(R.T @ c)] for c in corners])
+    return np.array(perms)
+
+
+def rotate_elements(verts, ids, rng):
+    """Each element re-listed in a random proper rotation of its reference frame (same
+    physical element, different local orientation)."""
+    rots = cube_rotations()
+    pick = rng.integers(0, len(rots), size=len(verts))
+    p = rots[pick]
+    return (np.take_along_axis(verts, p[:, :, None], axis=1),
+            np.take_along_axis(ids, p, axis=1))

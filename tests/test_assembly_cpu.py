@@ -1,0 +1,126 @@
+"""C0 global numbering (host part of the library) checked on the CPU.
+
+* counts: a box of n^3 hexes with nm modes has (n(nm-1)+1)^3 global modes, for any local
+  orientation of the elements;
+* continuity: a random global vector, scattered with the map and evaluated with the oracle
+  BwdTrans, takes one value at every physical quadrature point shared by several elements,
+  with every element listed in a random rotation of its reference frame (edge reversals,
+  face flips and face transposes all occur);
+* the discrete Helmholtz problem assembled with the map from independent dense element
+  matrices (tests/dense_ref.py) converges spectrally to a manufactured solution.
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+
+import dense_ref as DR
+import oracle as O
+from ithaca_hip import assembly, mesh
+
+
+def _mesh(n, rotate, seed=0, jitter=0.0):
+    rng = np.random.default_rng(seed)
+    verts = mesh.hex_box(n, jitter, rng)
+    ids = mesh.hex_box_ids(n)
+    if rotate:
+        verts, ids = mesh.rotate_elements(verts, ids, rng)
+    return verts, ids
+
+
+@pytest.mark.parametrize("n,nm,rotate", [(2, 4, False), (3, 5, True), (2, 8, True)])
+def test_global_count(n, nm, rotate):
+    verts, ids = _mesh(n, rotate)
+    l2g, sign, bnd, ng = assembly.assembly_map_hex(ids, nm)
+    assert ng == (n * (nm - 1) + 1) ** 3
+    assert set(np.unique(l2g)) == set(range(ng))
+    if not rotate:
+        assert np.all(sign == 1)
+    # boundary modes: everything except the interior (n(nm-1)-1)^3 block
+    assert len(np.unique(l2g[bnd == 1])) == ng - (n * (nm - 1) - 1) ** 3
+
+
+@pytest.mark.parametrize("nm,nq", [(4, 5), (6, 7)])
+def test_continuity_rotated_elements(nm, nq):
+    verts, ids = _mesh(3, True, seed=nm, jitter=0.1)
+    E = len(ids)
+    l2g, sign, _, ng = assembly.assembly_map_hex(ids, nm)
+    assert (sign < 0).any()
+    x = np.random.default_rng(1).uniform(-1, 1, ng)
+    loc = sign * x[l2g]
+    b = O.basis(nm, nq)
+    vals = O.bwdtrans("hex", b, E, loc)
+    xyz = O.coords_hex(verts, nq, b["z"]).T  # (E*Q, 3)
+    key = np.round(xyz * 1e9).astype(np.int64)
+    _, inv = np.unique(key, axis=0, return_inverse=True)
+    inv = inv.ravel()
+    lo = np.full(inv.max() + 1, np.inf)
+    hi = np.full(inv.max() + 1, -np.inf)
+    np.minimum.at(lo, inv, vals)
+    np.maximum.at(hi, inv, vals)
+    shared = np.bincount(inv) > 1
+    assert shared.sum() > 0
+    assert np.max(hi[shared] - lo[shared]) < 1e-12
+
+
+def helmholtz_system(verts, ids, nm, nq, lam, f_exact):
+    """Global CSR matrix and rhs (Dirichlet modes removed) from dense element matrices."""
+    E = len(ids)
+    M = nm ** 3
+    l2g, sign, bnd, ng = assembly.assembly_map_hex(ids, nm)
+    rows, cols, vals = [], [], []
+    rhs = np.zeros(ng)
+    for e in range(E):
+        m = DR.element_mats("hex", nm, nq, verts[e])
+        He = DR.helmholtz_matrix(m, lam)
+        g = l2g[e * M:(e + 1) * M]
+        s = sign[e * M:(e + 1) * M].astype(float)
+        Hs = He * np.outer(s, s)
+        rows.append(np.repeat(g, M))
+        cols.append(np.tile(g, M))
+        vals.append(Hs.ravel())
+        fe = f_exact(m["X"])
+        np.add.at(rhs, g, s * (m["B"].T @ (m["W"] * m["J"] * fe)))
+    A = sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))),
+                      shape=(ng, ng))
+    dir_mask = np.zeros(ng, dtype=bool)
+    dir_mask[np.unique(l2g[bnd == 1])] = True
+    return A, rhs, dir_mask, l2g, sign
+
+
+def manufactured(lam):
+    u = lambda X: np.sin(np.pi * X[:, 0]) * np.sin(np.pi * X[:, 1]) * np.sin(np.pi * X[:, 2])
+    f = lambda X: (3 * np.pi ** 2 + lam) * u(X)
+    return u, f
+
+
+def cpu_solve(verts, ids, nm, nq, lam):
+    u, f = manufactured(lam)
+    A, rhs, dm, l2g, sign = helmholtz_system(verts, ids, nm, nq, lam, f)
+    free = ~dm
+    x = np.zeros(len(rhs))
+    x[free] = spla.spsolve(A[free][:, free].tocsc(), rhs[free])
+    return x, l2g, sign, u
+
+
+def l2_error(verts, ids, nm, nq, x, l2g, sign, u):
+    M = nm ** 3
+    num = den = 0.0
+    for e in range(len(ids)):
+        m = DR.element_mats("hex", nm, nq, verts[e])
+        ue = m["B"] @ (sign[e * M:(e + 1) * M] * x[l2g[e * M:(e + 1) * M]])
+        ex = u(m["X"])
+        num += np.sum(m["W"] * m["J"] * (ue - ex) ** 2)
+        den += np.sum(m["W"] * m["J"] * ex ** 2)
+    return np.sqrt(num / den)
+
+
+def test_manufactured_solution_converges():
+    verts, ids = _mesh(2, True, seed=3, jitter=0.05)
+    errs = []
+    for nm in (4, 5, 6, 7):
+        x, l2g, sign, u = cpu_solve(verts, ids, nm, nm + 1, 10.0)
+        errs.append(l2_error(verts, ids, nm, nm + 1, x, l2g, sign, u))
+    print("L2 errors", errs)
+    assert errs[-1] < 1e-4
+    assert all(errs[i + 1] < 0.5 * errs[i] for i in range(len(errs) - 1))

@@ -1,0 +1,88 @@
+"""GPU C0 assembly and matrix-free Helmholtz solve against the CPU checks of
+tests/test_assembly_cpu.py (the same map, dense element matrices, scipy sparse solve)."""
+import numpy as np
+import pytest
+import torch
+
+import ithaca_hip as H
+from ithaca_hip import assembly, mesh
+from test_assembly_cpu import _mesh, cpu_solve, l2_error, manufactured
+
+pytestmark = pytest.mark.gpu
+
+
+def _dev(a, dtype=torch.float64):
+    return torch.from_numpy(np.ascontiguousarray(a)).to("cuda", dtype=dtype)
+
+
+def _gpu_problem(verts, ids, nm, nq, lam, dirichlet=True):
+    E = len(ids)
+    b = H.basis_modified_a(nm, nq)
+    vd = _dev(verts)
+    jac, df = H.geom_device("hex", vd, nq, b["z"], True)
+    Hop = H.Operator("Helmholtz_Hex_Deformed", nm, nq, E, basis=b)
+    Hop.set_jac(jac)
+    Hop.set_df(df)
+    Hop.set_lambda(lam)
+    ipr = H.Operator("IProduct_Hex_Deformed", nm, nq, E, basis=b)
+    ipr.set_jac(jac)
+    u, f = manufactured(lam)
+    X = mesh.hex_quad_points(vd, b["z"]).reshape(-1, 3)
+    fx = ((3 * np.pi ** 2 + lam) * torch.sin(np.pi * X[:, 0]) * torch.sin(np.pi * X[:, 1])
+          * torch.sin(np.pi * X[:, 2]))
+    A = assembly.Assembly(ids, nm, dirichlet=dirichlet)
+    rhs = A.assemble(ipr.apply(fx.contiguous()))
+    return A, Hop, rhs, b
+
+
+def test_gather_and_assemble_match_the_map():
+    verts, ids = _mesh(3, True, seed=7, jitter=0.1)
+    nm = 5
+    l2g, sign, _, ng = assembly.assembly_map_hex(ids, nm)
+    A = assembly.Assembly(ids, nm, dirichlet=False)
+    assert A.nglobal == ng
+    rng = np.random.default_rng(2)
+    g = rng.uniform(-1, 1, ng)
+    loc = A.global_to_local(_dev(g)).cpu().numpy()
+    assert np.array_equal(loc, sign * g[l2g])
+    lv = rng.uniform(-1, 1, len(l2g))
+    ref = np.zeros(ng)
+    np.add.at(ref, l2g, sign * lv)
+    got = A.assemble(_dev(lv)).cpu().numpy()
+    assert np.abs(got - ref).max() < 1e-13
+
+
+@pytest.mark.parametrize("nm", [4, 6])
+def test_helmsolve_matches_cpu_sparse_solve(nm):
+    nq = nm + 1
+    verts, ids = _mesh(2, True, seed=3, jitter=0.05)
+    print(f"[case] helmsolve rotated 2^3 mesh nm={nm}", flush=True)
+    x_cpu, l2g, sign, u = cpu_solve(verts, ids, nm, nq, 10.0)
+    A, Hop, rhs, _ = _gpu_problem(verts, ids, nm, nq, 10.0)
+    x, it, res = A.helmsolve(Hop, rhs, tol=1e-13, max_iter=5000)
+    xg = x.cpu().numpy()
+    print(f"iterations {it}, residual {res:.2e}")
+    assert res < 1e-13
+    assert np.linalg.norm(xg - x_cpu) <= 1e-9 * np.linalg.norm(x_cpu)
+    e_gpu = l2_error(verts, ids, nm, nq, xg, l2g, sign, u)
+    e_cpu = l2_error(verts, ids, nm, nq, x_cpu, l2g, sign, u)
+    assert abs(e_gpu - e_cpu) <= 1e-6 * e_cpu
+
+
+def test_helmsolve_neumann_residual():
+    """Natural boundary conditions (lambda > 0: SPD): the returned x satisfies
+    Assemble(H(GlobalToLocal(x))) = b to the requested tolerance."""
+    verts, ids = _mesh(3, True, seed=11, jitter=0.1)
+    A, Hop, rhs, _ = _gpu_problem(verts, ids, 5, 6, 2.0, dirichlet=False)
+    x, it, res = A.helmsolve(Hop, rhs, tol=1e-11)
+    y = A.assemble(Hop.apply(A.global_to_local(x)))
+    rel = (torch.linalg.vector_norm(y - rhs) / torch.linalg.vector_norm(rhs)).item()
+    print(f"iterations {it}, residual {res:.2e}, check {rel:.2e}")
+    assert rel < 1e-10
+
+
+def test_helmsolve_reports_non_convergence():
+    verts, ids = _mesh(2, False)
+    A, Hop, rhs, _ = _gpu_problem(verts, ids, 5, 6, 1.0)
+    with pytest.raises(H.MFError, match="maximum number of iterations"):
+        A.helmsolve(Hop, rhs, tol=1e-14, max_iter=2)

@@ -53,19 +53,6 @@ struct Args {
     float *ws32;    // per workgroup NMAX*NMAX (FP32 factors, MIXED only)
 };
 
-__device__ __forceinline__ double block_max(double v, double *sv)
-{
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off));
-    if ((threadIdx.x & 63) == 0) sv[threadIdx.x >> 6] = v;
-    __syncthreads();
-    double r = sv[0];
-#pragma unroll
-    for (int k = 1; k < NT / 64; ++k) r = fmax(r, sv[k]);
-    __syncthreads();
-    return r;
-}
-
 // Value of v in lane l of this wave (l uniform).
 __device__ __forceinline__ float read_lane(float v, int l)
 {
@@ -77,6 +64,58 @@ __device__ __forceinline__ double read_lane(double v, int l)
     const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffLL), l);
     const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
     return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned int)lo);
+}
+
+// v of another lane of the same 16-lane row, by a DPP lane pattern (every lane active).
+template <int CTRL>
+__device__ __forceinline__ float dpp(float v)
+{
+    return __builtin_bit_cast(float,
+                              __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), CTRL, 0xf, 0xf, false));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp(double v)
+{
+    const long long b = __builtin_bit_cast(long long, v);
+    const int lo = __builtin_amdgcn_mov_dpp((int)(b & 0xffffffffLL), CTRL, 0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), CTRL, 0xf, 0xf, false);
+    return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned int)lo);
+}
+
+// Reduction over the 64 lanes of a wave (all active), result in every lane: within each
+// row of 16 lanes by DPP exchanges (lane xor 1, xor 2, half-row mirror, row mirror: each
+// step combines two disjoint groups, so every lane ends with its row's total), then the
+// four row totals read as scalars.  No LDS round trip, unlike a shuffle.
+template <typename V, typename Op>
+__device__ __forceinline__ V wave_reduce(V v, Op op)
+{
+    v = op(v, dpp<0xB1>(v));   // quad_perm [1,0,3,2]
+    v = op(v, dpp<0x4E>(v));   // quad_perm [2,3,0,1]
+    v = op(v, dpp<0x141>(v));  // row_half_mirror
+    v = op(v, dpp<0x140>(v));  // row_mirror
+    return op(op(read_lane(v, 0), read_lane(v, 16)), op(read_lane(v, 32), read_lane(v, 48)));
+}
+template <typename V>
+__device__ __forceinline__ V wave_max(V v)
+{
+    return wave_reduce(v, [](V x, V y) { return x > y ? x : y; });
+}
+template <typename V>
+__device__ __forceinline__ V wave_sum(V v)
+{
+    return wave_reduce(v, [](V x, V y) { return x + y; });
+}
+
+__device__ __forceinline__ double block_max(double v, double *sv)
+{
+    v = wave_max(v);
+    if ((threadIdx.x & 63) == 0) sv[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double r = sv[0];
+#pragma unroll
+    for (int k = 1; k < NT / 64; ++k) r = fmax(r, sv[k]);
+    __syncthreads();
+    return r;
 }
 
 // LDS column stride of the panel and of U12: reads of a 4-wide register tile may run up
@@ -108,7 +147,7 @@ struct PivLds {
 //   3. A22 -= L21 U12 in TM x TN register tiles, both operands read from LDS, the next
 //      tile's global values loaded while the current one is computed.
 template <typename T, int NB, int TM, int TN>
-__device__ void block_lu(T *__restrict__ L, int n, T *P, T *U, T *R, PivLds &pv, int *zero)
+__device__ __forceinline__ void block_lu(T *__restrict__ L, int n, T *P, T *U, T *R, PivLds &pv, int *zero)
 {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     for (int i = tid; i < n; i += NT) pv.posAll[i] = i;
@@ -150,11 +189,9 @@ __device__ void block_lu(T *__restrict__ L, int n, T *P, T *U, T *R, PivLds &pv,
                 if (c < nb) {
                     const int par = c & 1;
                     T *wrow = R + par * (NW + 1) * NB;  // NW candidate rows, then row c
-                    double a = (own && r >= c) ? fabs((double)v[c]) : -1.0;
-                    if (a != a) a = -1.0;  // NaN: never a pivot
-                    double wmax = a;
-#pragma unroll
-                    for (int off = 32; off > 0; off >>= 1) wmax = fmax(wmax, __shfl_xor(wmax, off));
+                    T a = (own && r >= c) ? fabs(v[c]) : T(-1);
+                    if (a != a) a = T(-1);  // NaN: never a pivot
+                    const T wmax = wave_max(a);
                     const unsigned long long hit = __ballot(a == wmax);
                     const int wl = __builtin_ffsll((long long)hit) - 1;  // lowest lane
                     if (lane == wl) {
@@ -335,7 +372,7 @@ __device__ void block_lu(T *__restrict__ L, int n, T *P, T *U, T *R, PivLds &pv,
 // chunk lies in one wave and is solved there with lane reads; one barrier per chunk
 // publishes the solved entries (ys) to the rows that still need them.
 template <typename T>
-__device__ void lu_solve_rows(const T *__restrict__ L, int n, T *y, T *ys)
+__device__ __forceinline__ void lu_solve_rows(const T *__restrict__ L, int n, T *y, T *ys)
 {
     constexpr int SB = 16;
     const int i = threadIdx.x, w = i >> 6;
@@ -410,10 +447,10 @@ __device__ void lu_solve_rows(const T *__restrict__ L, int n, T *y, T *ys)
 // of RB rows, all loads of a group issued together.  With rowsum, also the largest
 // absolute row sum of A (for the backward-error test), reduced over the wave.
 template <bool ROWSUM>
-__device__ double residual(const double *__restrict__ A, int n, const double *x,
+__device__ __forceinline__ double residual(const double *__restrict__ A, int n, const double *x,
                            const double *b, double *r)
 {
-    constexpr int RB = 8;
+    constexpr int RB = 4;
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     double amax = 0.0;
     for (int i0 = w * RB; i0 < n; i0 += RB * (NT / 64)) {
@@ -436,11 +473,8 @@ __device__ double residual(const double *__restrict__ A, int n, const double *x,
                 sum += a[k][s] * xs[s];
                 if (ROWSUM) asum += fabs(a[k][s]);
             }
-#pragma unroll
-            for (int off = 32; off > 0; off >>= 1) {
-                sum += __shfl_xor(sum, off);
-                if (ROWSUM) asum += __shfl_xor(asum, off);
-            }
+            sum = wave_sum(sum);
+            if (ROWSUM) asum = wave_sum(asum);
             if (i0 + k < n) {
                 if (lane == 0) r[i0 + k] = b[i0 + k] - sum;
                 if (ROWSUM) amax = fmax(amax, asum);
@@ -450,19 +484,35 @@ __device__ double residual(const double *__restrict__ A, int n, const double *x,
     return amax;
 }
 
+// Workgroups per CU the static LDS allows (P and U plus about 5 KB of pivot scratch, of
+// the 160 KB of a CU), at most 4; the launch bound lets the compiler size registers for it.
+template <typename T, int NB>
+constexpr int min_blocks()
+{
+    const int b = 160 * 1024 / (2 * (int)sizeof(T) * NB * PLD + 5 * 1024);
+    return b < 1 ? 1 : (b > 4 ? 4 : b);
+}
+
 template <typename T, bool MIXED, int NB, int TM, int TN>
-__global__ __launch_bounds__(NT, sizeof(T) * NB * PLD * 2 > 80 * 1024 ? 1 : 2) void
+__global__ __launch_bounds__(NT, (min_blocks<T, NB>())) void
 batched_solve_kernel(Args a)
 {
-    __shared__ T P[NB * PLD];
-    __shared__ T U[NB * PLD];
-    __shared__ double b64[NMAX], x64[NMAX], r64[NMAX];
-    __shared__ T yv[NMAX], ys[NMAX];
+    __shared__ __attribute__((aligned(16))) T P[NB * PLD];
+    __shared__ __attribute__((aligned(16))) T U[NB * PLD];
+    // P and U are free once the factorisation is done: the solve and refinement vectors
+    // live there, which keeps the static LDS (and so the co-resident workgroups per CU)
+    // at what the factorisation needs
+    static_assert(sizeof(T) * NB * PLD >= 3 * NMAX * sizeof(double) && NB * PLD >= 2 * NMAX,
+                  "solve vectors must fit in the panel buffers");
+    double *const b64 = reinterpret_cast<double *>(P), *const x64 = b64 + NMAX,
+                  *const r64 = x64 + NMAX;
+    T *const yv = U, *const ys = U + NMAX;
     __shared__ T R[2 * (NT / 64 + 1) * NB];
     __shared__ PivLds pv;
     __shared__ double sv[NT / 64];
     __shared__ int s_inst, s_zero;
 
+    constexpr int QB = 4;
     const int tid = threadIdx.x;
     double *A64 = a.ws64 + (long)blockIdx.x * NMAX * NMAX;
     T *LU = MIXED ? (T *)(a.ws32 + (long)blockIdx.x * NMAX * NMAX) : (T *)A64;
@@ -482,22 +532,22 @@ batched_solve_kernel(Args a)
         const double *th = a.theta + a.bd.toff[inst];
         const long vo = a.bd.voff[inst];
 
-        // assemble A(mu) (or its transpose): 4 entries per thread, 4 terms at a time, all
-        // 16 loads issued before the sums
+        // assemble A(mu) (or its transpose): 4 entries per thread, QB terms at a time, all
+        // 4*QB loads issued before the sums (one round trip for up to QB terms)
         const long nn = (long)n * n;
         for (long i0 = tid; i0 < nn; i0 += 4 * NT) {
             double s[4] = {0.0, 0.0, 0.0, 0.0};
-            for (int q0 = 0; q0 < nq; q0 += 4) {
-                double v[4][4];
+            for (int q0 = 0; q0 < nq; q0 += QB) {
+                double v[QB][4];
 #pragma unroll
-                for (int qq = 0; qq < 4; ++qq)
+                for (int qq = 0; qq < QB; ++qq)
 #pragma unroll
                     for (int u = 0; u < 4; ++u) {
                         const long idx = i0 + u * NT;
                         v[qq][u] = (idx < nn && q0 + qq < nq) ? Ai[(q0 + qq) * nn + idx] : 0.0;
                     }
 #pragma unroll
-                for (int qq = 0; qq < 4; ++qq) {
+                for (int qq = 0; qq < QB; ++qq) {
                     const double t = q0 + qq < nq ? th[q0 + qq] : 0.0;
 #pragma unroll
                     for (int u = 0; u < 4; ++u) s[u] += t * v[qq][u];
@@ -514,11 +564,13 @@ batched_solve_kernel(Args a)
                 }
             }
         }
-        for (int i = tid; i < n; i += NT) b64[i] = a.b[vo + i];
         __syncthreads();
 
-        block_lu<T, NB, TM, TN>(LU, n, P, U, R, pv, &s_zero);
-        for (int i = tid; i < n; i += NT) yv[i] = (T)b64[pv.posAll[i]];
+        block_lu<T, NB, TM, TN>(LU, n, P, U, R, pv, &s_zero);  // ends with a barrier
+        for (int i = tid; i < n; i += NT) {
+            b64[i] = a.b[vo + i];
+            yv[i] = (T)a.b[vo + pv.posAll[i]];
+        }
         __syncthreads();
         lu_solve_rows<T>(LU, n, yv, ys);
         for (int i = tid; i < n; i += NT) x64[i] = (double)yv[i];
@@ -618,7 +670,8 @@ struct Variant {
             &batched_solve_kernel<float, true, NB, TM, TN>}
 
 const Variant kVariants[] = {IROM_V(16, 4, 4), IROM_V(32, 4, 4), IROM_V(16, 2, 4),
-                             IROM_V(32, 2, 4), IROM_V(16, 4, 2), IROM_V(32, 2, 2)};
+                             IROM_V(32, 2, 4), IROM_V(16, 4, 2), IROM_V(32, 2, 2),
+                             IROM_V(8, 4, 4),  IROM_V(8, 2, 4)};
 constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
 
 }  // namespace irom
@@ -740,8 +793,8 @@ int irom_batch_create(int nInst, const int *N, const int *Q, irom_batch *out)
     IROM_CHECK(hipMemcpy(b->d_aoff, b->aoff.data(), nInst * sizeof(long), hipMemcpyHostToDevice), "copy");
     IROM_CHECK(hipMemcpy(b->d_toff, b->toff.data(), nInst * sizeof(long), hipMemcpyHostToDevice), "copy");
     IROM_CHECK(hipMemcpy(b->d_voff, b->voff.data(), nInst * sizeof(long), hipMemcpyHostToDevice), "copy");
-    b->variant[0] = 1;  // NB32 4x4
-    b->variant[1] = 1;
+    b->variant[0] = 0;  // NB16 4x4: the fastest on the 4096-instance N ~ U{16..256} batch
+    b->variant[1] = 0;
     *out = b;
     return 0;
 }

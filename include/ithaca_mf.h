@@ -117,7 +117,7 @@ int imf_apply_many_device(imf_op op, int nvec, const void *d_in, long ld_in, voi
 /* Message of the last failing call on this thread ("" if none). */
 const char *imf_last_error(void);
 
-/* ---- C0 global assembly and the matrix-free Helmholtz solve (hexahedra) -------------
+/* ---- C0 global assembly and the matrix-free Helmholtz solve (hexahedra, quads) -------
  * Global numbering of the modified (C0) basis built from the elements' vertex ids
  * (vertex_ids[e*8 + v], any 64-bit ids, vertex order of the reference hex): vertex modes
  * shared by id, edge-interior modes by the vertex pair with sign (-1)^m against the global
@@ -131,11 +131,20 @@ typedef struct imf_asm_s *imf_asm;
 int imf_assembly_map_hex(int nElmt, int nm, const long long *vertex_ids, int *l2g,
                          signed char *sign, unsigned char *on_boundary, long *nglobal);
 
+/* The same for quadrilaterals: vertex_ids[e*4 + v] counter-clockwise from (-1,-1) (the
+ * reference quad's vertex order), local arrays e*nm^2 + q*nm + p, edge-interior modes
+ * signed (-1)^m against the global edge direction, on_boundary = on an edge owned by one
+ * element. */
+int imf_assembly_map_quad(int nElmt, int nm, const long long *vertex_ids, int *l2g,
+                          signed char *sign, unsigned char *on_boundary, long *nglobal);
+
 /* Device assembly map.  dirichlet = 1: homogeneous Dirichlet conditions on every mode of
  * the boundary faces (those modes are excluded from the solve and set to zero);
  * dirichlet = 0: natural (Neumann) conditions. */
 int imf_assembly_create_hex(int nElmt, int nm, const long long *vertex_ids, int dirichlet,
                             imf_asm *assembly);
+int imf_assembly_create_quad(int nElmt, int nm, const long long *vertex_ids, int dirichlet,
+                             imf_asm *assembly);
 long imf_assembly_nglobal(imf_asm assembly);
 int imf_assembly_destroy(imf_asm assembly);
 
@@ -154,7 +163,8 @@ int imf_assemble_device(imf_asm assembly, const double *d_local, double *d_globa
  * (GlobalLinSysIterativeFull::v_DoMatrixMultiply, GlobalLinSysIterativeFull.cpp:215-232)
  * with the assembled diagonal as preconditioner; stops when (r,r) < tol^2 (b,b)
  * (Set_Rhs_Magnitude, NekLinSysIter.cpp:128-153; the reference's default tol is 1e-9).
- * helmholtz: an FP64 "Helmholtz_Hex_*" operator on the same elements and modes, with
+ * helmholtz: an FP64 "Helmholtz_Hex_*" (or, for a quad assembly, "Helmholtz_Quad_*")
+ * operator on the same elements and modes, with
  * SetJac/SetDF/SetLambda done.  d_rhs, d_x: global arrays.  Synchronous (reads the three
  * dot products every iteration); allocates its workspace on the first call.  Returns
  * IMF_ERR_STATE with *iterations = max_iter + 1 when the tolerance is not reached. */

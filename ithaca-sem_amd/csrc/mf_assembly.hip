@@ -198,6 +198,95 @@ int build_map_hex(int E, int nm, const long long *vid, std::vector<int> &l2g,
     return IMF_OK;
 }
 
+// Local vertex of the reference quad with coordinate bits (b0, b1), counter-clockwise from
+// (-1,-1) (StdRegions/StdQuadExp.cpp vertex order, the bottom face of the hex above).
+inline int vert2(int b0, int b1) { return vert_of(b0, b1, 0); }
+
+// The quad analogue of build_map_hex: vertex, edge ((-1)^m sign when the element runs the
+// edge against the min -> max id direction) and interior modes; boundary = edges with one
+// user.
+int build_map_quad(int E, int nm, const long long *vid, std::vector<int> &l2g,
+                   std::vector<signed char> &sgn, std::vector<unsigned char> &bnd, long &nglobal)
+{
+    if (E <= 0 || nm < 2 || !vid) return afail(IMF_ERR_ARG, "assembly: bad argument");
+    const int M = nm * nm, ni = nm - 2;
+    // end points of the edge where coordinate dn is fixed at s (running along 1 - dn)
+    auto ends = [&](const long long *v, int dn, int s, long long &g0, long long &g1) {
+        int b0[2], b1[2];
+        b0[dn] = b1[dn] = s;
+        b0[1 - dn] = 0;
+        b1[1 - dn] = 1;
+        g0 = v[vert2(b0[0], b0[1])];
+        g1 = v[vert2(b1[0], b1[1])];
+    };
+    std::unordered_map<long long, int> vmap;
+    std::unordered_map<std::array<long long, 2>, std::array<int, 2>, H2> emap;  // id, uses
+    for (long e = 0; e < E; ++e) {
+        const long long *v = vid + e * 4;
+        for (int a = 0; a < 4; ++a) vmap.emplace(v[a], (int)vmap.size());
+        for (int dn = 0; dn < 2; ++dn)
+            for (int sd = 0; sd < 2; ++sd) {
+                long long g0, g1;
+                ends(v, dn, sd, g0, g1);
+                if (g0 == g1) return afail(IMF_ERR_ARG, "assembly: degenerate edge");
+                const std::array<long long, 2> k{std::min(g0, g1), std::max(g0, g1)};
+                auto it = emap.find(k);
+                if (it == emap.end()) emap.emplace(k, std::array<int, 2>{(int)emap.size(), 1});
+                else if (++it->second[1] > 2)
+                    return afail(IMF_ERR_ARG, "assembly: an edge is shared by more than two elements");
+            }
+    }
+    const long nV = (long)vmap.size(), nE = (long)emap.size();
+    const long baseE = nV, baseI = baseE + nE * ni;
+    nglobal = baseI + (long)E * ni * ni;
+    if (nglobal > 2147483647L) return afail(IMF_ERR_ARG, "assembly: more than 2^31 global modes");
+    l2g.assign((size_t)E * M, 0);
+    sgn.assign((size_t)E * M, 1);
+    bnd.assign((size_t)E * M, 0);
+    for (long e = 0; e < E; ++e) {
+        const long long *v = vid + e * 4;
+        for (int q = 0; q < nm; ++q)
+            for (int p = 0; p < nm; ++p) {
+                const int m[2] = {p, q};
+                const size_t li = (size_t)e * M + q * nm + p;
+                const int nv = (p < 2) + (q < 2);
+                long gi = 0;
+                int sg = 1;
+                if (nv == 2) {
+                    gi = vmap[v[vert2(p, q)]];
+                } else if (nv == 1) {
+                    const int di = p >= 2 ? 0 : 1, dn = 1 - di;
+                    long long g0, g1;
+                    ends(v, dn, m[dn], g0, g1);
+                    const int idx = emap[{std::min(g0, g1), std::max(g0, g1)}][0];
+                    gi = baseE + (long)idx * ni + (m[di] - 2);
+                    if (g0 > g1 && (m[di] & 1)) sg = -1;
+                } else {
+                    gi = baseI + (e * ni + (q - 2)) * (long)ni + (p - 2);
+                }
+                l2g[li] = (int)gi;
+                sgn[li] = (signed char)sg;
+            }
+        for (int dn = 0; dn < 2; ++dn)
+            for (int sd = 0; sd < 2; ++sd) {
+                long long g0, g1;
+                ends(v, dn, sd, g0, g1);
+                if (emap[{std::min(g0, g1), std::max(g0, g1)}][1] != 1) continue;
+                for (int q = 0; q < nm; ++q)
+                    for (int p = 0; p < nm; ++p)
+                        if ((dn == 0 ? p : q) == sd) bnd[(size_t)e * M + q * nm + p] = 1;
+            }
+    }
+    return IMF_OK;
+}
+
+int build_map(int shape, int E, int nm, const long long *vid, std::vector<int> &l2g,
+              std::vector<signed char> &sgn, std::vector<unsigned char> &bnd, long &nglobal)
+{
+    return shape == imf::kHex ? build_map_hex(E, nm, vid, l2g, sgn, bnd, nglobal)
+                              : build_map_quad(E, nm, vid, l2g, sgn, bnd, nglobal);
+}
+
 // --------------------------------------------------------------------------- kernels
 // local = sign * global[l2g]   (Vmath::Gathr with signs)
 __global__ void g2l_kernel(long n, const int *__restrict__ l2g, const signed char *__restrict__ sg,
@@ -266,6 +355,35 @@ __global__ void helm_diag_hex(DiagTab t, int nm, int nq, long nElmt, int deforme
                 s += lambda * c[0] * phi * phi + c[1] * d0 * d0 + c[4] * d1 * d1 + c[6] * d2 * d2 +
                      2.0 * (c[2] * d0 * d1 + c[3] * d0 * d2 + c[5] * d1 * d2);
             }
+    diag[g] = s;
+}
+
+// The quad analogue (deformed [e][4Q]: Jw, Jw G00, Jw G01, Jw G11; regular [e][4]).
+__global__ void helm_diag_quad(DiagTab t, int nm, int nq, long nElmt, int deformed,
+                               double lambda, const double *__restrict__ geom,
+                               double *__restrict__ diag)
+{
+    const long M = (long)nm * nm, Q = (long)nq * nq;
+    const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= nElmt * M) return;
+    const long e = g / M;
+    const int md = (int)(g - e * M), p = md % nm, q = md / nm;
+    double s = 0.0;
+    for (int j = 0; j < nq; ++j)
+        for (int i = 0; i < nq; ++i) {
+            const long pt = (long)j * nq + i;
+            const double bp = t.B[p * nq + i], bq = t.B[q * nq + j];
+            const double phi = bp * bq, d0 = t.dB[p * nq + i] * bq, d1 = bp * t.dB[q * nq + j];
+            double c[4];
+            if (deformed) {
+                for (int a = 0; a < 4; ++a) c[a] = geom[e * 4 * Q + a * Q + pt];
+            } else {
+                const double wq = t.w[i] * t.w[j];
+                for (int a = 0; a < 4; ++a) c[a] = geom[e * 4 + a] * wq;
+            }
+            s += lambda * c[0] * phi * phi + c[1] * d0 * d0 + c[3] * d1 * d1 +
+                 2.0 * c[2] * d0 * d1;
+        }
     diag[g] = s;
 }
 
@@ -367,7 +485,7 @@ inline unsigned nblk(long n) { return (unsigned)((n + RB - 1) / RB); }
 }  // namespace
 
 struct imf_asm_s {
-    int nElmt = 0, nm = 0, dirichlet = 0;
+    int shape = 0, nElmt = 0, nm = 0, dirichlet = 0;
     long nloc = 0, ng = 0;
     int *d_l2g = nullptr;
     signed char *d_sg = nullptr;
@@ -405,14 +523,14 @@ extern "C" {
 
 const char *imf_assembly_last_error(void) { return g_aerr.c_str(); }
 
-int imf_assembly_map_hex(int nElmt, int nm, const long long *vertex_ids, int *l2g,
-                         signed char *sign, unsigned char *on_boundary, long *nglobal)
+static int map_any(int shape, int nElmt, int nm, const long long *vertex_ids, int *l2g,
+                   signed char *sign, unsigned char *on_boundary, long *nglobal)
 {
     std::vector<int> m;
     std::vector<signed char> sg;
     std::vector<unsigned char> bd;
     long ng = 0;
-    if (int st = build_map_hex(nElmt, nm, vertex_ids, m, sg, bd, ng)) return st;
+    if (int st = build_map(shape, nElmt, nm, vertex_ids, m, sg, bd, ng)) return st;
     if (l2g) std::memcpy(l2g, m.data(), m.size() * sizeof(int));
     if (sign) std::memcpy(sign, sg.data(), sg.size());
     if (on_boundary) std::memcpy(on_boundary, bd.data(), bd.size());
@@ -420,16 +538,17 @@ int imf_assembly_map_hex(int nElmt, int nm, const long long *vertex_ids, int *l2
     return IMF_OK;
 }
 
-int imf_assembly_create_hex(int nElmt, int nm, const long long *vertex_ids, int dirichlet,
-                            imf_asm *out)
+static int create_any(int shape, int nElmt, int nm, const long long *vertex_ids, int dirichlet,
+                      imf_asm *out)
 {
-    if (!out) return afail(IMF_ERR_ARG, "imf_assembly_create_hex: null output");
+    if (!out) return afail(IMF_ERR_ARG, "imf_assembly_create: null output");
     std::vector<int> m;
     std::vector<signed char> sg;
     std::vector<unsigned char> bd;
     long ng = 0;
-    if (int st = build_map_hex(nElmt, nm, vertex_ids, m, sg, bd, ng)) return st;
+    if (int st = build_map(shape, nElmt, nm, vertex_ids, m, sg, bd, ng)) return st;
     auto *a = new imf_asm_s();
+    a->shape = shape;
     a->nElmt = nElmt;
     a->nm = nm;
     a->dirichlet = dirichlet ? 1 : 0;
@@ -468,6 +587,30 @@ int imf_assembly_create_hex(int nElmt, int nm, const long long *vertex_ids, int 
     return IMF_OK;
 }
 
+int imf_assembly_map_hex(int nElmt, int nm, const long long *vertex_ids, int *l2g,
+                         signed char *sign, unsigned char *on_boundary, long *nglobal)
+{
+    return map_any(imf::kHex, nElmt, nm, vertex_ids, l2g, sign, on_boundary, nglobal);
+}
+
+int imf_assembly_map_quad(int nElmt, int nm, const long long *vertex_ids, int *l2g,
+                          signed char *sign, unsigned char *on_boundary, long *nglobal)
+{
+    return map_any(imf::kQuad, nElmt, nm, vertex_ids, l2g, sign, on_boundary, nglobal);
+}
+
+int imf_assembly_create_hex(int nElmt, int nm, const long long *vertex_ids, int dirichlet,
+                            imf_asm *out)
+{
+    return create_any(imf::kHex, nElmt, nm, vertex_ids, dirichlet, out);
+}
+
+int imf_assembly_create_quad(int nElmt, int nm, const long long *vertex_ids, int dirichlet,
+                             imf_asm *out)
+{
+    return create_any(imf::kQuad, nElmt, nm, vertex_ids, dirichlet, out);
+}
+
 long imf_assembly_nglobal(imf_asm a) { return a ? a->ng : 0; }
 
 int imf_assembly_destroy(imf_asm a)
@@ -502,10 +645,10 @@ int imf_helmsolve_cg_device(imf_asm a, imf_op op, const double *d_rhs, double *d
                             int max_iter, int *iterations, double *residual, void *stream)
 {
     if (!a || !op || !d_rhs || !d_x) return afail(IMF_ERR_ARG, "imf_helmsolve_cg_device: null argument");
-    if (op->kind != imf::kHelmholtz || op->shape != imf::kHex || op->prec != IMF_FP64 ||
+    if (op->kind != imf::kHelmholtz || op->shape != a->shape || op->prec != IMF_FP64 ||
         op->nElmt != a->nElmt || op->nm != a->nm)
-        return afail(IMF_ERR_ARG, "imf_helmsolve_cg_device: needs an FP64 Helmholtz_Hex operator "
-                                  "on the assembly's elements and modes");
+        return afail(IMF_ERR_ARG, "imf_helmsolve_cg_device: needs an FP64 Helmholtz operator "
+                                  "of the assembly's shape on its elements and modes");
     if (!op->geom_ready) return afail(IMF_ERR_STATE, "Jacobian and derivative factors not set");
     if (op->dbdata.size() != (size_t)op->nm * op->nq || op->nq > 16)
         return afail(IMF_ERR_ARG, "imf_helmsolve_cg_device: operator created without dbdata");
@@ -527,8 +670,14 @@ int imf_helmsolve_cg_device(imf_asm a, imf_op op, const double *d_rhs, double *d
         std::copy(op->bdata.begin(), op->bdata.end(), t.B);
         std::copy(op->dbdata.begin(), op->dbdata.end(), t.dB);
         std::copy(op->w.begin(), op->w.end(), t.w);
-        helm_diag_hex<<<nblk(a->nloc), RB, 0, s>>>(t, nm, nq, a->nElmt, op->deformed, op->lambda,
-                                                  (const double *)op->d_geom, a->d_loc);
+        if (a->shape == imf::kHex)
+            helm_diag_hex<<<nblk(a->nloc), RB, 0, s>>>(t, nm, nq, a->nElmt, op->deformed,
+                                                      op->lambda, (const double *)op->d_geom,
+                                                      a->d_loc);
+        else
+            helm_diag_quad<<<nblk(a->nloc), RB, 0, s>>>(t, nm, nq, a->nElmt, op->deformed,
+                                                       op->lambda, (const double *)op->d_geom,
+                                                       a->d_loc);
         assemble_kernel<<<nblk(ng), RB, 0, s>>>(ng, a->d_off, a->d_inv, nullptr, a->d_loc, nullptr, dinv);
         inv_masked<<<nblk(ng), RB, 0, s>>>(ng, a->d_mask, dinv);
         ACHECK(hipGetLastError(), "diagonal kernels");

@@ -20,8 +20,11 @@ def _lib():
     L = lib()
     if not _SET:
         vp, i, l_, d = ctypes.c_void_p, ctypes.c_int, ctypes.c_long, ctypes.c_double
-        L.imf_assembly_map_hex.argtypes = [i, i, vp, vp, vp, vp, ctypes.POINTER(l_)]
-        L.imf_assembly_create_hex.argtypes = [i, i, vp, i, ctypes.POINTER(vp)]
+        for shape in ("hex", "quad"):
+            getattr(L, f"imf_assembly_map_{shape}").argtypes = [i, i, vp, vp, vp, vp,
+                                                                 ctypes.POINTER(l_)]
+            getattr(L, f"imf_assembly_create_{shape}").argtypes = [i, i, vp, i,
+                                                                    ctypes.POINTER(vp)]
         L.imf_assembly_nglobal.argtypes = [vp]
         L.imf_assembly_nglobal.restype = l_
         L.imf_assembly_destroy.argtypes = [vp]
@@ -40,35 +43,54 @@ def _check(st):
         raise MFError(msg)
 
 
-def assembly_map_hex(vertex_ids, nm):
+def _nverts(shape):
+    if shape not in ("hex", "quad"):
+        raise MFError(f"assembly: unknown shape {shape!r} (hex or quad)")
+    return 8 if shape == "hex" else 4
+
+
+def assembly_map(vertex_ids, nm, shape):
     """(l2g, sign, on_boundary, nglobal) of the C0 numbering (host only, no device)."""
     ids = np.ascontiguousarray(vertex_ids, dtype=np.int64)
+    if ids.ndim != 2 or ids.shape[1] != _nverts(shape):
+        raise MFError(f"assembly: vertex ids must be (nElmt, {_nverts(shape)}) for {shape}")
     E = ids.shape[0]
-    n = E * nm ** 3
+    n = E * nm ** (3 if shape == "hex" else 2)
     l2g = np.empty(n, dtype=np.int32)
     sign = np.empty(n, dtype=np.int8)
     bnd = np.empty(n, dtype=np.uint8)
     ng = ctypes.c_long()
-    _check(_lib().imf_assembly_map_hex(E, nm, ids.ctypes.data_as(ctypes.c_void_p),
-                                       l2g.ctypes.data_as(ctypes.c_void_p),
-                                       sign.ctypes.data_as(ctypes.c_void_p),
-                                       bnd.ctypes.data_as(ctypes.c_void_p), ctypes.byref(ng)))
+    _check(getattr(_lib(), f"imf_assembly_map_{shape}")(
+        E, nm, ids.ctypes.data_as(ctypes.c_void_p), l2g.ctypes.data_as(ctypes.c_void_p),
+        sign.ctypes.data_as(ctypes.c_void_p), bnd.ctypes.data_as(ctypes.c_void_p),
+        ctypes.byref(ng)))
     return l2g, sign, bnd, ng.value
 
 
-class Assembly:
-    """Device assembly map of a hex mesh (vertex_ids (E, 8)); dirichlet=True fixes every
-    mode on the boundary faces to zero."""
+def assembly_map_hex(vertex_ids, nm):
+    return assembly_map(vertex_ids, nm, "hex")
 
-    def __init__(self, vertex_ids, nm, dirichlet=True):
+
+def assembly_map_quad(vertex_ids, nm):
+    return assembly_map(vertex_ids, nm, "quad")
+
+
+class Assembly:
+    """Device assembly map of a hex mesh (vertex_ids (E, 8)) or a quad mesh (shape="quad",
+    vertex_ids (E, 4)); dirichlet=True fixes every mode on the boundary faces (edges) to
+    zero."""
+
+    def __init__(self, vertex_ids, nm, dirichlet=True, shape="hex"):
         ids = np.ascontiguousarray(vertex_ids, dtype=np.int64)
-        self.nElmt, self.nm = ids.shape[0], nm
+        if ids.ndim != 2 or ids.shape[1] != _nverts(shape):
+            raise MFError(f"assembly: vertex ids must be (nElmt, {_nverts(shape)}) for {shape}")
+        self.shape, self.nElmt, self.nm = shape, ids.shape[0], nm
         h = ctypes.c_void_p()
-        _check(_lib().imf_assembly_create_hex(self.nElmt, nm, ids.ctypes.data_as(ctypes.c_void_p),
-                                              int(dirichlet), ctypes.byref(h)))
+        _check(getattr(_lib(), f"imf_assembly_create_{shape}")(
+            self.nElmt, nm, ids.ctypes.data_as(ctypes.c_void_p), int(dirichlet), ctypes.byref(h)))
         self._h = h
         self.nglobal = _lib().imf_assembly_nglobal(h)
-        self.nlocal = self.nElmt * nm ** 3
+        self.nlocal = self.nElmt * nm ** (3 if shape == "hex" else 2)
 
     def __del__(self):
         h = getattr(self, "_h", None)

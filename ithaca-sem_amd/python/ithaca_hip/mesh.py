@@ -84,6 +84,22 @@ def hex_quad_points(verts, z):
     return X.reshape(-1, E, 3).permute(1, 0, 2).contiguous()
 
 
+def quad_points(verts, z):
+    """Physical coordinates (E, Q, 2) of the nq^2 GLL points of straight-sided quads under
+    the bilinear map of their vertices (torch, on verts' device); point index j*nq + i."""
+    import torch
+    dev = verts.device
+    zt = torch.as_tensor(np.asarray(z), dtype=torch.float64, device=dev)
+    j, i = torch.meshgrid(zt, zt, indexing="ij")
+    xi = torch.stack([i.reshape(-1), j.reshape(-1)], dim=1)                  # (Q, 2)
+    c = torch.as_tensor(QUAD_CORNERS, dtype=torch.float64, device=dev)      # (4, 2)
+    shp = torch.where(c[None] > 0, (1.0 + xi[:, None]) / 2, (1.0 - xi[:, None]) / 2)
+    N = torch.prod(shp, dim=2)                                               # (Q, 4)
+    E = verts.shape[0]
+    X = N @ verts.permute(1, 0, 2).reshape(4, E * 2)                         # (Q, E*2)
+    return X.reshape(-1, E, 2).permute(1, 0, 2).contiguous()
+
+
 def pod_fields(n_fields=20, n_snap=512, seed=2):
     """Snapshot recipe of the POD workload: field k is prod_i sin(m_{k,i} pi x_i) with
     integer wavenumbers m ~ U{1..6}; snapshot j is sum_k a_{jk} field_k with
@@ -104,6 +120,26 @@ def hex_box_ids(n):
     for v, (cx, cy, cz) in enumerate(HEX_CORNERS):
         ids[:, v] = ((ez + cz) * (n + 1) + (ey + cy)) * (n + 1) + (ex + cx)
     return ids
+
+
+def quad_box_ids(n):
+    """Global vertex ids (n^2, 4) of quad_box(n): vertex (ix, iy) of the (n+1)^2 grid has id
+    iy*(n+1) + ix; same element and vertex order as quad_box."""
+    ey, ex = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    ex, ey = ex.ravel(), ey.ravel()
+    ids = np.empty((n ** 2, 4), dtype=np.int64)
+    for v, (cx, cy) in enumerate(QUAD_CORNERS):
+        ids[:, v] = (ey + cy) * (n + 1) + (ex + cx)
+    return ids
+
+
+def rotate_quads(verts, ids, rng):
+    """Each quad re-listed from a random starting vertex (the 4 proper rotations of the
+    reference square): same physical element, different local orientation."""
+    shift = rng.integers(0, 4, size=len(verts))
+    p = (np.arange(4)[None, :] + shift[:, None]) % 4
+    return (np.take_along_axis(verts, p[:, :, None], axis=1),
+            np.take_along_axis(ids, p, axis=1))
 
 
 def cube_rotations():

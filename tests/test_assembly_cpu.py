@@ -63,15 +63,15 @@ def test_continuity_rotated_elements(nm, nq):
     assert np.max(hi[shared] - lo[shared]) < 1e-12
 
 
-def helmholtz_system(verts, ids, nm, nq, lam, f_exact):
+def helmholtz_system(verts, ids, nm, nq, lam, f_exact, shape="hex"):
     """Global CSR matrix and rhs (Dirichlet modes removed) from dense element matrices."""
     E = len(ids)
-    M = nm ** 3
-    l2g, sign, bnd, ng = assembly.assembly_map_hex(ids, nm)
+    M = nm ** (3 if shape == "hex" else 2)
+    l2g, sign, bnd, ng = assembly.assembly_map(ids, nm, shape)
     rows, cols, vals = [], [], []
     rhs = np.zeros(ng)
     for e in range(E):
-        m = DR.element_mats("hex", nm, nq, verts[e])
+        m = DR.element_mats(shape, nm, nq, verts[e])
         He = DR.helmholtz_matrix(m, lam)
         g = l2g[e * M:(e + 1) * M]
         s = sign[e * M:(e + 1) * M].astype(float)
@@ -88,26 +88,27 @@ def helmholtz_system(verts, ids, nm, nq, lam, f_exact):
     return A, rhs, dir_mask, l2g, sign
 
 
-def manufactured(lam):
-    u = lambda X: np.sin(np.pi * X[:, 0]) * np.sin(np.pi * X[:, 1]) * np.sin(np.pi * X[:, 2])
-    f = lambda X: (3 * np.pi ** 2 + lam) * u(X)
+def manufactured(lam, dim=3):
+    """u = prod_c sin(pi x_c) (zero on the boundary of the unit box), f = -lap u + lam u."""
+    u = lambda X: np.prod(np.sin(np.pi * X[:, :dim]), axis=1)
+    f = lambda X: (dim * np.pi ** 2 + lam) * u(X)
     return u, f
 
 
-def cpu_solve(verts, ids, nm, nq, lam):
-    u, f = manufactured(lam)
-    A, rhs, dm, l2g, sign = helmholtz_system(verts, ids, nm, nq, lam, f)
+def cpu_solve(verts, ids, nm, nq, lam, shape="hex"):
+    u, f = manufactured(lam, 3 if shape == "hex" else 2)
+    A, rhs, dm, l2g, sign = helmholtz_system(verts, ids, nm, nq, lam, f, shape)
     free = ~dm
     x = np.zeros(len(rhs))
     x[free] = spla.spsolve(A[free][:, free].tocsc(), rhs[free])
     return x, l2g, sign, u
 
 
-def l2_error(verts, ids, nm, nq, x, l2g, sign, u):
-    M = nm ** 3
+def l2_error(verts, ids, nm, nq, x, l2g, sign, u, shape="hex"):
+    M = nm ** (3 if shape == "hex" else 2)
     num = den = 0.0
     for e in range(len(ids)):
-        m = DR.element_mats("hex", nm, nq, verts[e])
+        m = DR.element_mats(shape, nm, nq, verts[e])
         ue = m["B"] @ (sign[e * M:(e + 1) * M] * x[l2g[e * M:(e + 1) * M]])
         ex = u(m["X"])
         num += np.sum(m["W"] * m["J"] * (ue - ex) ** 2)
@@ -123,4 +124,68 @@ def test_manufactured_solution_converges():
         errs.append(l2_error(verts, ids, nm, nm + 1, x, l2g, sign, u))
     print("L2 errors", errs)
     assert errs[-1] < 1e-4
+    assert all(errs[i + 1] < 0.5 * errs[i] for i in range(len(errs) - 1))
+
+
+# ---------------------------------------------------------------- quadrilaterals
+
+def _qmesh(n, rotate, seed=0, jitter=0.0):
+    rng = np.random.default_rng(seed)
+    verts = mesh.quad_box(n, jitter, rng)
+    ids = mesh.quad_box_ids(n)
+    if rotate:
+        verts, ids = mesh.rotate_quads(verts, ids, rng)
+    return verts, ids
+
+
+@pytest.mark.parametrize("n,nm,rotate", [(3, 4, False), (4, 5, True), (2, 8, True)])
+def test_quad_global_count(n, nm, rotate):
+    verts, ids = _qmesh(n, rotate)
+    l2g, sign, bnd, ng = assembly.assembly_map_quad(ids, nm)
+    assert ng == (n * (nm - 1) + 1) ** 2
+    assert set(np.unique(l2g)) == set(range(ng))
+    if not rotate:
+        assert np.all(sign == 1)
+    assert len(np.unique(l2g[bnd == 1])) == ng - (n * (nm - 1) - 1) ** 2
+
+
+@pytest.mark.parametrize("nm,nq", [(4, 5), (7, 8)])
+def test_quad_continuity_rotated_elements(nm, nq):
+    verts, ids = _qmesh(4, True, seed=nm, jitter=0.15)
+    E = len(ids)
+    l2g, sign, _, ng = assembly.assembly_map_quad(ids, nm)
+    assert (sign < 0).any()
+    x = np.random.default_rng(1).uniform(-1, 1, ng)
+    vals = O.bwdtrans("quad", O.basis(nm, nq), E, sign * x[l2g])
+    xy = O.coords_quad(verts, nq, O.basis(nm, nq)["z"]).T
+    _, inv = np.unique(np.round(xy * 1e9).astype(np.int64), axis=0, return_inverse=True)
+    inv = inv.ravel()
+    lo = np.full(inv.max() + 1, np.inf)
+    hi = np.full(inv.max() + 1, -np.inf)
+    np.minimum.at(lo, inv, vals)
+    np.maximum.at(hi, inv, vals)
+    shared = np.bincount(inv) > 1
+    assert shared.sum() > 0
+    assert np.max(hi[shared] - lo[shared]) < 1e-12
+
+
+def test_quad_assembly_rejects_bad_meshes():
+    ids = mesh.quad_box_ids(2)
+    bad = ids.copy()
+    bad[0, 1] = bad[0, 0]
+    with pytest.raises(Exception, match="degenerate edge"):
+        assembly.assembly_map_quad(bad, 4)
+    triple = np.concatenate([ids, ids[:1], ids[:1]])
+    with pytest.raises(Exception, match="more than two elements"):
+        assembly.assembly_map_quad(triple, 4)
+
+
+def test_quad_manufactured_solution_converges():
+    verts, ids = _qmesh(3, True, seed=5, jitter=0.1)
+    errs = []
+    for nm in (4, 5, 6, 7, 8):
+        x, l2g, sign, u = cpu_solve(verts, ids, nm, nm + 1, 1.0, "quad")
+        errs.append(l2_error(verts, ids, nm, nm + 1, x, l2g, sign, u, "quad"))
+    print("L2 errors", errs)
+    assert errs[-1] < 1e-6
     assert all(errs[i + 1] < 0.5 * errs[i] for i in range(len(errs) - 1))

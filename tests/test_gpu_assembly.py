@@ -6,7 +6,8 @@ import torch
 
 import ithaca_hip as H
 from ithaca_hip import assembly, mesh
-from test_assembly_cpu import _mesh, cpu_solve, l2_error, manufactured
+import oracle as O
+from test_assembly_cpu import _mesh, _qmesh, cpu_solve, l2_error, manufactured
 
 pytestmark = pytest.mark.gpu
 
@@ -86,3 +87,69 @@ def test_helmsolve_reports_non_convergence():
     A, Hop, rhs, _ = _gpu_problem(verts, ids, 5, 6, 1.0)
     with pytest.raises(H.MFError, match="maximum number of iterations"):
         A.helmsolve(Hop, rhs, tol=1e-14, max_iter=2)
+
+
+# ---------------------------------------------------------------- quadrilaterals
+
+def _gpu_quad_problem(verts, ids, nm, nq, lam, deformed=True, dirichlet=True):
+    E = len(ids)
+    b = H.basis_modified_a(nm, nq)
+    geo = "Deformed" if deformed else "Regular"
+    jac, df = H.geom_device("quad", _dev(verts), nq, b["z"], deformed)
+    Hop = H.Operator(f"Helmholtz_Quad_{geo}", nm, nq, E, basis=b)
+    Hop.set_jac(jac)
+    Hop.set_df(df)
+    Hop.set_lambda(lam)
+    ipr = H.Operator(f"IProduct_Quad_{geo}", nm, nq, E, basis=b)
+    ipr.set_jac(jac)
+    _, f = manufactured(lam, 2)
+    fx = f(O.coords_quad(verts, nq, b["z"]).T)  # host points of the quadrature grid
+    A = assembly.Assembly(ids, nm, dirichlet=dirichlet, shape="quad")
+    rhs = A.assemble(ipr.apply(_dev(fx)))
+    return A, Hop, rhs
+
+
+def test_quad_gather_and_assemble_match_the_map():
+    verts, ids = _qmesh(4, True, seed=7, jitter=0.1)
+    nm = 6
+    l2g, sign, _, ng = assembly.assembly_map_quad(ids, nm)
+    A = assembly.Assembly(ids, nm, dirichlet=False, shape="quad")
+    assert A.nglobal == ng and A.nlocal == len(ids) * nm * nm
+    rng = np.random.default_rng(3)
+    g = rng.uniform(-1, 1, ng)
+    assert np.array_equal(A.global_to_local(_dev(g)).cpu().numpy(), sign * g[l2g])
+    lv = rng.uniform(-1, 1, len(l2g))
+    ref = np.zeros(ng)
+    np.add.at(ref, l2g, sign * lv)
+    assert np.abs(A.assemble(_dev(lv)).cpu().numpy() - ref).max() < 1e-13
+
+
+@pytest.mark.parametrize("nm,deformed", [(5, True), (8, True), (5, False)])
+def test_quad_helmsolve_matches_cpu_sparse_solve(nm, deformed):
+    nq = nm + 1
+    if deformed:
+        verts, ids = _qmesh(4, True, seed=3, jitter=0.1)
+    else:  # the regular 32x32 quads of the measured quad workload
+        verts, ids = _qmesh(32, False)
+    print(f"[case] quad helmsolve nm={nm} deformed={deformed} E={len(ids)}", flush=True)
+    x_cpu, l2g, sign, u = cpu_solve(verts, ids, nm, nq, 1.0, "quad")
+    A, Hop, rhs = _gpu_quad_problem(verts, ids, nm, nq, 1.0, deformed)
+    x, it, res = A.helmsolve(Hop, rhs, tol=1e-13, max_iter=5000)
+    xg = x.cpu().numpy()
+    print(f"iterations {it}, residual {res:.2e}")
+    assert res < 1e-13
+    assert np.linalg.norm(xg - x_cpu) <= 1e-9 * np.linalg.norm(x_cpu)
+    e_gpu = l2_error(verts, ids, nm, nq, xg, l2g, sign, u, "quad")
+    e_cpu = l2_error(verts, ids, nm, nq, x_cpu, l2g, sign, u, "quad")
+    # relative L2 errors; at nm = 8 they are ~3e-10, so allow round-off in absolute terms
+    assert abs(e_gpu - e_cpu) <= 1e-6 * e_cpu + 1e-12
+
+
+def test_quad_helmsolve_rejects_a_hex_operator():
+    verts, ids = _qmesh(2, False)
+    A = assembly.Assembly(ids, 4, shape="quad")
+    b = H.basis_modified_a(4, 5)
+    Hop = H.Operator("Helmholtz_Hex_Regular", 4, 5, len(ids), basis=b)
+    rhs = torch.zeros(A.nglobal, dtype=torch.float64, device="cuda")
+    with pytest.raises(H.MFError, match="Helmholtz operator"):
+        A.helmsolve(Hop, rhs)

@@ -1,0 +1,315 @@
+"""Readers for Nektar++ XML meshes and .fld field files (host side, no device work).
+
+What is read, restated from the reference:
+
+* the GEOMETRY section of a session/mesh file (SpatialDomains/MeshGraphXml.cpp,
+  MeshGraphXmlCompressed.cpp): VERTEX, EDGE, FACE (quadrilateral faces ``Q``) and ELEMENT
+  (``Q`` in 2D, ``H`` in 3D), either as text (``<V ID="i"> x y z </V>``) or compressed
+  ("B64Z-LittleEndian": base64 of a zlib stream of the little-endian entity records of
+  SpatialDomains/MeshEntities.hpp:54-103: int64 ids, float64 coordinates);
+* the local vertex order that the reference derives from those entities: QuadGeom's
+  (SpatialDomains/QuadGeom.cpp:56-86: vertex j is the end of edge j not shared with edge
+  j+1) and HexGeom's (SpatialDomains/HexGeom.cpp:183-421: bottom edges from face 0, the
+  vertical edges between faces 1-4, top edges from face 5, vertices chained along them);
+* the EXPANSIONS section for TYPE="MODIFIED" (Modified_A with nm+1 Gauss-Lobatto-Legendre
+  points per direction);
+* the ELEMENTS blocks of a .fld file (LibUtilities/BasicUtils/FieldIOXml.cpp:1108-1180):
+  field names, shape, basis, UNIORDER mode counts, element id ranges, and the data, one
+  field after the other, each element's coefficients in its own local frame.
+
+Only straight-sided quadrilateral and hexahedral meshes are read (a non-empty CURVED
+section is rejected), which is the element family the operators here implement.
+"""
+import base64
+import re
+import xml.etree.ElementTree as ET
+import zlib
+
+import numpy as np
+
+_VERTEX = np.dtype([("id", "<i8"), ("x", "<f8"), ("y", "<f8"), ("z", "<f8")])
+_EDGE = np.dtype([("id", "<i8"), ("v", "<i8", (2,))])
+_QUAD = np.dtype([("id", "<i8"), ("e", "<i8", (4,))])
+_HEX = np.dtype([("id", "<i8"), ("f", "<i8", (6,))])
+
+
+class NektarIOError(ValueError):
+    pass
+
+
+def _b64z(text, dtype):
+    raw = zlib.decompress(base64.b64decode("".join(text.split())))
+    if len(raw) % dtype.itemsize:
+        raise NektarIOError("compressed block size is not a multiple of its record size")
+    return np.frombuffer(raw, dtype=dtype)
+
+
+def _section(text, tag):
+    """The <TAG ...>...</TAG> block of a file as its own XML document (the rest of a
+    session file is not parsed, so unrelated sections need not be well formed)."""
+    m = re.search(rf"<{tag}\b.*?</{tag}>", text, re.S)
+    return ET.fromstring(m.group(0)) if m else None
+
+
+def _records(parent, child, dtype, ncols):
+    """Entity records of a VERTEX/EDGE/FACE/ELEMENT section.  Compressed sections hold one
+    node per entity type (the section itself for VERTEX/EDGE, its Q/H children for
+    FACE/ELEMENT) whose text is the B64Z stream; text sections hold one tag per entity."""
+    if parent is None:
+        return np.zeros(0, dtype=dtype)
+    holders = [parent] if child is None else [c for c in parent if c.tag == child]
+    packed = [h for h in holders if h.get("COMPRESSED")]
+    if packed:
+        out = []
+        for h in packed:
+            if h.get("COMPRESSED").upper() != "B64Z-LITTLEENDIAN" or h.get("BITSIZE", "64") != "64":
+                raise NektarIOError(f"unsupported compression {h.get('COMPRESSED')!r}")
+            out.append(_b64z(h.text, dtype))
+        return np.concatenate(out) if out else np.zeros(0, dtype=dtype)
+    tags = list(parent) if child is None else holders
+    out = np.zeros(len(tags), dtype=dtype)
+    names = dtype.names[1:]
+    for i, el in enumerate(tags):
+        vals = el.text.split()
+        if len(vals) != ncols:
+            raise NektarIOError(f"{el.tag} {el.get('ID')}: expected {ncols} values")
+        out[i][0] = int(el.get("ID"))
+        if len(names) == 1:  # one vector of ids
+            out[i][1] = [int(v) for v in vals]
+        else:
+            for n, v in zip(names, vals):
+                out[i][n] = float(v)
+    return out
+
+
+def _id_list(s):
+    """Element id list in the reference's range syntax ("0-7,9,12-15")."""
+    ids = []
+    for part in s.replace(" ", "").split(","):
+        if not part:
+            continue
+        if "-" in part:
+            a, b = part.split("-")
+            ids.extend(range(int(a), int(b) + 1))
+        else:
+            ids.append(int(part))
+    return np.array(ids, dtype=np.int64)
+
+
+class Mesh:
+    """Straight-sided quad (2D) or hex (3D) mesh.
+
+    elements: int64 ids in file order; element_vertex_ids[e]: the element's vertex ids in
+    the reference's local order (4 or 8); coords[v]: vertex coordinates by id (dim values);
+    expansions: list of (element ids, nummodes, type, fields)."""
+
+    def __init__(self, dim, vertices, edges, faces, elements, composites, expansions):
+        self.dim = dim
+        self.coords = {int(v["id"]): np.array([v["x"], v["y"], v["z"]][:dim]) for v in vertices}
+        self.edges = {int(e["id"]): tuple(int(x) for x in e["v"]) for e in edges}
+        self.faces = {int(f["id"]): tuple(int(x) for x in f["e"]) for f in faces}
+        self.composites = composites
+        self.expansions = expansions
+        if dim == 3:
+            self.elements = np.array([int(h["id"]) for h in elements], dtype=np.int64)
+            self.element_vertex_ids = np.array(
+                [self._hex_vertices(tuple(int(x) for x in h["f"])) for h in elements],
+                dtype=np.int64).reshape(-1, 8)
+        else:
+            self.elements = np.array([int(q["id"]) for q in elements], dtype=np.int64)
+            self.element_vertex_ids = np.array(
+                [self._quad_vertices(tuple(int(x) for x in q["e"])) for q in elements],
+                dtype=np.int64).reshape(-1, 4)
+
+    # -- local vertex order --------------------------------------------------------
+    def _quad_vertices(self, eids):
+        verts = []
+        for j in range(4):
+            a, b = self.edges[eids[j]], self.edges[eids[(j + 1) % 4]]
+            if a[0] in b:      # backwards: vertex 0 is shared with the next edge
+                verts.append(a[1])
+            elif a[1] in b:
+                verts.append(a[0])
+            else:
+                raise NektarIOError(f"quad edges {eids[j]}, {eids[(j + 1) % 4]} share no vertex")
+        return verts
+
+    def _hex_vertices(self, fids):
+        F = [self.faces[f] for f in fids]
+
+        def shared(fa, fb):
+            s = [e for e in F[fa] if e in F[fb]]
+            if len(s) != 1:
+                raise NektarIOError(f"hex faces {fids[fa]}, {fids[fb]} share {len(s)} edges")
+            return s[0]
+
+        E = [shared(0, f) for f in range(1, 5)]                 # bottom edges 0-3
+        E += [shared(1, 4)] + [shared(f, f + 1) for f in range(1, 4)]  # vertical 4-7
+        E += [shared(5, f) for f in range(1, 5)]                # top edges 8-11
+        ed = [self.edges[e] for e in E]
+
+        def first_two(a, b):
+            if ed[a][0] in ed[b]:
+                return [ed[a][1], ed[a][0]]
+            if ed[a][1] in ed[b]:
+                return [ed[a][0], ed[a][1]]
+            raise NektarIOError("hex edges share no vertex")
+
+        def chain(verts, edges):
+            for i in edges:
+                last = verts[-1]
+                if ed[i][0] == last:
+                    verts.append(ed[i][1])
+                elif ed[i][1] == last:
+                    verts.append(ed[i][0])
+                else:
+                    raise NektarIOError("hex edges share no vertex")
+            return verts
+
+        bottom = chain(first_two(0, 1), [1, 2])
+        top = chain(first_two(8, 9), [9, 10])
+        return bottom + top
+
+    # -- helpers -------------------------------------------------------------------
+    def element_coords(self, element_ids=None):
+        """(E, 2^dim, dim) vertex coordinates in local order, for the given ids (default:
+        file order)."""
+        rows = self._rows(element_ids)
+        ids = self.element_vertex_ids[rows]
+        return np.array([[self.coords[int(v)] for v in r] for r in ids])
+
+    def vertex_ids(self, element_ids=None):
+        return self.element_vertex_ids[self._rows(element_ids)]
+
+    def _rows(self, element_ids):
+        if element_ids is None:
+            return np.arange(len(self.elements))
+        pos = {int(e): i for i, e in enumerate(self.elements)}
+        try:
+            return np.array([pos[int(e)] for e in element_ids])
+        except KeyError as exc:
+            raise NektarIOError(f"element {exc.args[0]} is not in the mesh") from None
+
+
+def read_mesh(path):
+    """Read the GEOMETRY and EXPANSIONS sections of a Nektar++ XML file."""
+    with open(path) as fh:
+        text = fh.read()
+    g = _section(text, "GEOMETRY")
+    if g is None:
+        raise NektarIOError(f"{path}: no GEOMETRY section")
+    dim = int(g.get("DIM"))
+    if dim not in (2, 3):
+        raise NektarIOError(f"{path}: DIM={dim} (2 or 3 supported)")
+    curved = g.find("CURVED")
+    if curved is not None and len(list(curved)) > 0:
+        raise NektarIOError(f"{path}: curved elements are not supported")
+    vertices = _records(g.find("VERTEX"), None, _VERTEX, 3)
+    edges = _records(g.find("EDGE"), None, _EDGE, 2)
+    el = g.find("ELEMENT")
+    if dim == 3:
+        faces = _records(g.find("FACE"), "Q", _QUAD, 4)
+        other = [c.tag for c in (list(g.find("FACE")) if g.find("FACE") is not None else [])
+                 if c.tag != "Q"]
+        if other:
+            raise NektarIOError(f"{path}: face types {sorted(set(other))} are not supported")
+        elements = _records(el, "H", _HEX, 6)
+        other = [c.tag for c in el if c.tag != "H"]
+    else:
+        faces = np.zeros(0, dtype=_QUAD)
+        elements = _records(el, "Q", _QUAD, 4)
+        other = [c.tag for c in el if c.tag != "Q"]
+    if other:
+        raise NektarIOError(f"{path}: element types {sorted(set(other))} are not supported")
+    composites = {}
+    comp = g.find("COMPOSITE")
+    if comp is not None:
+        for c in comp:
+            composites[int(c.get("ID"))] = c.text.strip()
+    expansions = []
+    ex = _section(text, "EXPANSIONS")
+    if ex is not None:
+        for e in ex:
+            expansions.append(dict(composite=e.get("COMPOSITE"), nummodes=int(e.get("NUMMODES", 0)),
+                                   type=e.get("TYPE"), fields=(e.get("FIELDS") or "").split(",")))
+    return Mesh(dim, vertices, edges, faces, elements, composites, expansions)
+
+
+class FieldBlock:
+    """One ELEMENTS block of a .fld file: fields, shape, basis, modes per direction, element
+    ids, and data[f] = (len(ids), ncoeffs) coefficients of field f."""
+
+    def __init__(self, attrs, data):
+        self.fields = attrs["FIELDS"].split(",")
+        self.shape = attrs["SHAPE"]
+        self.basis = attrs["BASIS"].split(",")
+        nm = attrs["NUMMODESPERDIR"]
+        if not nm.startswith("UNIORDER:"):
+            raise NektarIOError(f"NUMMODESPERDIR {nm!r}: only UNIORDER is supported")
+        self.nummodes = [int(x) for x in nm.split(":", 1)[1].split(",")]
+        self.ids = _id_list(attrs["ID"])
+        if self.shape not in ("Hexahedron", "Quadrilateral"):
+            raise NektarIOError(f"shape {self.shape!r} is not supported")
+        if any(b != "Modified_A" for b in self.basis):
+            raise NektarIOError(f"basis {self.basis}: only Modified_A is supported")
+        ncoef = int(np.prod(self.nummodes))
+        if data.size != len(self.fields) * len(self.ids) * ncoef:
+            raise NektarIOError("Input data is not the same length as header information")
+        self.data = {f: data[i * len(self.ids) * ncoef:(i + 1) * len(self.ids) * ncoef]
+                     .reshape(len(self.ids), ncoef) for i, f in enumerate(self.fields)}
+
+
+def read_fld(path):
+    """ELEMENTS blocks of a Nektar++ XML field file (.fld / .chk)."""
+    with open(path) as fh:
+        text = fh.read()
+    blocks = []
+    for m in re.finditer(r"<ELEMENTS\b([^>]*)>(.*?)</ELEMENTS>", text, re.S):
+        attrs = dict(re.findall(r'(\w+)="([^"]*)"', m.group(1)))
+        comp = attrs.get("COMPRESSED")
+        if comp:
+            if comp.upper() != "B64Z-LITTLEENDIAN" or attrs.get("BITSIZE", "64") != "64":
+                raise NektarIOError(f"unsupported compression {comp!r}")
+            data = _b64z(m.group(2), np.dtype("<f8"))
+        else:
+            data = np.array(m.group(2).split(), dtype=np.float64)
+        blocks.append(FieldBlock(attrs, data))
+    if not blocks:
+        raise NektarIOError(f"{path}: no ELEMENTS block")
+    return blocks
+
+
+def field_coefficients(blocks, field, element_ids):
+    """Element-major coefficients (len(element_ids) * ncoeffs) of one field, gathered from
+    the blocks by element id (the reference's ExpList::ExtractDataToCoeffs for a file
+    whose modes match the expansion)."""
+    rows = {}
+    for b in blocks:
+        if field not in b.data:
+            continue
+        for i, e in enumerate(b.ids):
+            rows[int(e)] = b.data[field][i]
+    try:
+        return np.concatenate([rows[int(e)] for e in element_ids])
+    except KeyError as exc:
+        raise NektarIOError(f"field {field!r} has no data for element {exc.args[0]}") from None
+
+
+def expansion_arrays(mesh, blocks, field):
+    """Inputs for the operators and the assembly of one field, in the element order of
+    the file: (coords (E, 2^dim, dim) in local vertex order, vertex_ids (E, 2^dim),
+    coefficients (E * nm^dim, element-major), nm, nq).  The basis is Modified_A with
+    nm + 1 Gauss-Lobatto-Legendre points per direction (the reference's TYPE="MODIFIED")."""
+    blk = [b for b in blocks if field in b.data]
+    if not blk:
+        raise NektarIOError(f"no data for field {field!r}")
+    nms = {tuple(b.nummodes) for b in blk}
+    if len(nms) != 1 or len(set(next(iter(nms)))) != 1:
+        raise NektarIOError(f"field {field!r}: mode counts {sorted(nms)} are not uniform")
+    nm = blk[0].nummodes[0]
+    want = "Hexahedron" if mesh.dim == 3 else "Quadrilateral"
+    if any(b.shape != want for b in blk):
+        raise NektarIOError(f"field {field!r} is not on the mesh's {want} elements")
+    ids = np.concatenate([b.ids for b in blk])
+    return (mesh.element_coords(ids), mesh.vertex_ids(ids),
+            field_coefficients(blocks, field, ids), nm, nm + 1)

@@ -7,6 +7,11 @@ import torch
 import ithaca_hip as H
 from ithaca_hip import assembly, mesh
 import oracle as O
+import os
+import scipy.sparse as sp
+import scipy.sparse.linalg as spla
+import dense_ref as DR
+from ithaca_hip import nektar_io as N
 from test_assembly_cpu import _mesh, _qmesh, cpu_solve, l2_error, manufactured
 
 pytestmark = pytest.mark.gpu
@@ -153,3 +158,45 @@ def test_quad_helmsolve_rejects_a_hex_operator():
     rhs = torch.zeros(A.nglobal, dtype=torch.float64, device="cuda")
     with pytest.raises(H.MFError, match="Helmholtz operator"):
         A.helmsolve(Hop, rhs)
+
+
+def test_smoothing_a_nektar_field():
+    """The reference's FieldConvert Helmholtz smoother (ProcessC0Projection.cpp:196-224:
+    lambda = (2 pi / L)^2, forcing -lambda u, natural boundary conditions) on the mesh and
+    field of its test Hex_channel_C0helmsmoothing (read with nektar_io): GPU BwdTrans,
+    IProduct, assembly and CG against a dense-element sparse direct solve."""
+    gold = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "nektar")
+    mesh = N.read_mesh(os.path.join(gold, "Hex_channel_C0helmsmoothing.xml"))
+    blocks = N.read_fld(os.path.join(gold, "Hex_channel_C0helmsmoothing.fld"))
+    X, ids, c, nm, nq = N.expansion_arrays(mesh, blocks, "u")
+    lam = (2 * np.pi / 0.6) ** 2
+    E, M = len(ids), nm ** 3
+    # CPU: dense element matrices assembled with the same map, every mode free
+    l2g, sign, _, ng = assembly.assembly_map_hex(ids, nm)
+    rows, cols, vals, rhs = [], [], [], np.zeros(ng)
+    for e in range(E):
+        m = DR.element_mats("hex", nm, nq, X[e])
+        g, s_ = l2g[e * M:(e + 1) * M], sign[e * M:(e + 1) * M].astype(float)
+        rows.append(np.repeat(g, M))
+        cols.append(np.tile(g, M))
+        vals.append((DR.helmholtz_matrix(m, lam) * np.outer(s_, s_)).ravel())
+        phys = m["B"] @ c[e * M:(e + 1) * M]
+        np.add.at(rhs, g, s_ * (m["B"].T @ (m["W"] * m["J"] * lam * phys)))
+    Ag = sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))),
+                       shape=(ng, ng))
+    x_cpu = spla.spsolve(Ag.tocsc(), rhs)
+    # GPU
+    b = H.basis_modified_a(nm, nq)
+    jac, df = H.geom_device("hex", _dev(X), nq, b["z"], True)
+    Hop = H.Operator("Helmholtz_Hex_Deformed", nm, nq, E, basis=b)
+    Hop.set_jac(jac)
+    Hop.set_df(df)
+    Hop.set_lambda(lam)
+    ipr = H.Operator("IProduct_Hex_Deformed", nm, nq, E, basis=b)
+    ipr.set_jac(jac)
+    bwd = H.Operator("BwdTrans_Hex_Regular", nm, nq, E, basis=b)
+    A = assembly.Assembly(ids, nm, dirichlet=False)
+    phys = bwd.apply(_dev(c))
+    x, it, res = A.helmsolve(Hop, A.assemble(ipr.apply(lam * phys)), tol=1e-13)
+    print(f"iterations {it}, residual {res:.2e}")
+    assert np.linalg.norm(x.cpu().numpy() - x_cpu) <= 1e-10 * np.linalg.norm(x_cpu)

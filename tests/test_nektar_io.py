@@ -1,0 +1,165 @@
+"""Nektar++ XML mesh and .fld readers (host only) on the reference's own FieldConvert
+fixtures (tests/golden/nektar) and on small synthetic files.
+
+What pins the readers:
+* the local vertex order restated from QuadGeom/HexGeom gives right-handed elements;
+* the field coefficients, read in that local frame and evaluated with the dense
+  Modified_A basis of tests/dense_ref.py, form a continuous field (a wrong frame or a
+  wrong data layout breaks continuity across element boundaries by O(1));
+* the hex fixture's fields are the session's initial conditions
+  (u = sin 2 pi y + cos 6 pi y, v and w the same in x and z): the field read as u
+  correlates with the y function only.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import dense_ref as DR
+from ithaca_hip import nektar_io as N
+
+HERE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "nektar")
+
+
+def _fields_at_points(shape, nm, nq, X, coefs):
+    pts, vals = [], []
+    M = nm ** (3 if shape == "hex" else 2)
+    for e in range(len(X)):
+        m = DR.element_mats(shape, nm, nq, X[e])
+        assert m["J"].min() > 0
+        pts.append(m["X"])
+        vals.append(m["B"] @ coefs[e * M:(e + 1) * M])
+    return np.concatenate(pts), np.concatenate(vals)
+
+
+def _max_jump(P, V):
+    _, inv = np.unique(np.round(P * 1e9).astype(np.int64), axis=0, return_inverse=True)
+    inv = inv.ravel()
+    lo = np.full(inv.max() + 1, np.inf)
+    hi = np.full(inv.max() + 1, -np.inf)
+    np.minimum.at(lo, inv, V)
+    np.maximum.at(hi, inv, V)
+    shared = np.bincount(inv) > 1
+    assert shared.sum() > 0
+    return np.max(hi[shared] - lo[shared])
+
+
+def _right_handed(X, dim):
+    """Signed Jacobian of the multilinear map at the reference centre, per element."""
+    if dim == 3:
+        a = (X[:, 1] - X[:, 0] + X[:, 2] - X[:, 3] + X[:, 5] - X[:, 4] + X[:, 6] - X[:, 7])
+        b = (X[:, 3] - X[:, 0] + X[:, 2] - X[:, 1] + X[:, 7] - X[:, 4] + X[:, 6] - X[:, 5])
+        c = X[:, 4:].sum(1) - X[:, :4].sum(1)
+        return np.einsum("ei,ei->e", a, np.cross(b, c))
+    a = X[:, 1] - X[:, 0] + X[:, 2] - X[:, 3]
+    b = X[:, 3] - X[:, 0] + X[:, 2] - X[:, 1]
+    return a[:, 0] * b[:, 1] - a[:, 1] * b[:, 0]
+
+
+def test_hex_mesh_and_fields():
+    m = N.read_mesh(os.path.join(HERE, "Hex_channel_C0helmsmoothing.xml"))
+    assert m.dim == 3
+    assert (len(m.coords), len(m.edges), len(m.faces), len(m.elements)) == (27, 54, 36, 8)
+    assert m.expansions[0]["nummodes"] == 4 and m.expansions[0]["type"] == "MODIFIED"
+    blocks = N.read_fld(os.path.join(HERE, "Hex_channel_C0helmsmoothing.fld"))
+    assert len(blocks) == 1
+    b = blocks[0]
+    assert b.fields == ["u", "v", "w", "p"] and b.nummodes == [4, 4, 4]
+    assert list(b.ids) == list(range(8))
+    X = m.element_coords(b.ids)
+    assert np.all(_right_handed(X, 3) > 0)
+    # every vertex of the unit cube's 3^3 grid, each element an axis-aligned 0.5-cube
+    assert np.allclose(X.max(1) - X.min(1), 0.5)
+    nm, nq = 4, 5
+    f1 = lambda s: np.sin(2 * np.pi * s) + np.cos(6 * np.pi * s)
+    for name, axis in (("u", 1), ("v", 0), ("w", 2)):
+        P, V = _fields_at_points("hex", nm, nq, X, N.field_coefficients(blocks, name, b.ids))
+        assert _max_jump(P, V) < 1e-12
+        assert np.corrcoef(V, f1(P[:, axis]))[0, 1] > 0.99
+        for other in {0, 1, 2} - {axis}:
+            assert abs(np.corrcoef(V, f1(P[:, other]))[0, 1]) < 1e-6
+    # p = u + v + w (linear in the data)
+    cs = [N.field_coefficients(blocks, f, b.ids) for f in "uvwp"]
+    assert np.allclose(cs[0] + cs[1] + cs[2], cs[3], atol=1e-10)
+
+
+def test_quad_mesh_with_offset_ids_and_fields():
+    m = N.read_mesh(os.path.join(HERE, "interpfield_4quad.xml"))
+    blocks = N.read_fld(os.path.join(HERE, "interpfield_4quad.fld"))
+    b = blocks[0]
+    assert m.dim == 2 and list(b.ids) == [1655, 1656, 1657, 1658]
+    assert b.fields == ["rho"] and b.nummodes == [5, 5]
+    X = m.element_coords(b.ids)
+    assert np.all(_right_handed(X, 2) > 0)
+    ids = m.vertex_ids(b.ids)
+    assert len(np.unique(ids)) == 10  # a strip of 4 quads
+    P, V = _fields_at_points("quad", 5, 6, X, N.field_coefficients(blocks, "rho", b.ids))
+    assert _max_jump(P, V) < 1e-9 * np.abs(V).max()
+
+
+def test_text_format_mesh_and_field(tmp_path):
+    """Uncompressed entities and data: 2 quads, listed with edges in both directions."""
+    mesh = """<?xml version="1.0"?>
+<NEKTAR><GEOMETRY DIM="2" SPACE="2">
+<VERTEX><V ID="0">0 0 0</V><V ID="1">1 0 0</V><V ID="2">2 0 0</V>
+<V ID="3">0 1 0</V><V ID="4">1 1 0</V><V ID="5">2 1 0</V></VERTEX>
+<EDGE><E ID="0">0 1</E><E ID="1">1 4</E><E ID="2">4 3</E><E ID="3">3 0</E>
+<E ID="4">2 1</E><E ID="5">5 2</E><E ID="6">4 5</E></EDGE>
+<ELEMENT><Q ID="0">0 1 2 3</Q><Q ID="7">4 5 6 1</Q></ELEMENT>
+<COMPOSITE><C ID="0"> Q[0,7] </C></COMPOSITE><DOMAIN> C[0] </DOMAIN>
+</GEOMETRY>
+<EXPANSIONS><E COMPOSITE="C[0]" NUMMODES="2" FIELDS="u" TYPE="MODIFIED" /></EXPANSIONS>
+</NEKTAR>"""
+    p = tmp_path / "m.xml"
+    p.write_text(mesh)
+    m = N.read_mesh(str(p))
+    assert list(m.elements) == [0, 7]
+    assert m.vertex_ids().tolist() == [[0, 1, 4, 3], [1, 2, 5, 4]]
+    assert np.all(_right_handed(m.element_coords(), 2) > 0)
+    # u = x + 2y with 2 modes (the vertex modes carry the vertex values)
+    X = m.element_coords()
+    coef = np.concatenate([X[e, :, 0] + 2 * X[e, :, 1] for e in range(2)])
+    # local mode order p + 2q: (0,0) (1,0) (0,1) (1,1) = vertices 0, 1, 3, 2
+    coef = coef.reshape(2, 4)[:, [0, 1, 3, 2]].ravel()
+    fld = tmp_path / "f.fld"
+    fld.write_text('<?xml version="1.0"?><NEKTAR><ELEMENTS FIELDS="u" SHAPE="Quadrilateral" '
+                   'BASIS="Modified_A,Modified_A" NUMMODESPERDIR="UNIORDER:2,2" ID="0,7">'
+                   + " ".join(repr(float(v)) for v in coef) + "</ELEMENTS></NEKTAR>")
+    blocks = N.read_fld(str(fld))
+    P, V = _fields_at_points("quad", 2, 3, X, N.field_coefficients(blocks, "u", [0, 7]))
+    assert np.allclose(V, P[:, 0] + 2 * P[:, 1], atol=1e-14)
+
+
+@pytest.mark.parametrize("attrs,msg", [
+    ('FIELDS="u" SHAPE="Triangle" BASIS="Modified_A,Modified_B" NUMMODESPERDIR="UNIORDER:2,2" ID="0"', "shape"),
+    ('FIELDS="u" SHAPE="Quadrilateral" BASIS="Modified_A,Modified_A" NUMMODESPERDIR="MIXORDER:2,2" ID="0"', "UNIORDER"),
+    ('FIELDS="u" SHAPE="Quadrilateral" BASIS="Modified_A,Modified_A" NUMMODESPERDIR="UNIORDER:2,2" ID="0-1"', "length"),
+])
+def test_fld_rejects_what_it_cannot_read(tmp_path, attrs, msg):
+    f = tmp_path / "bad.fld"
+    f.write_text(f"<NEKTAR><ELEMENTS {attrs}>1 2 3 4</ELEMENTS></NEKTAR>")
+    with pytest.raises(N.NektarIOError, match=msg):
+        N.read_fld(str(f))
+
+
+def test_mesh_rejects_curved_elements_and_unknown_ids(tmp_path):
+    base = os.path.join(HERE, "interpfield_4quad.xml")
+    text = open(base).read()
+    curved = text.replace("</GEOMETRY>", '<CURVED><E ID="0" EDGEID="0" TYPE="PolyEvenlySpaced" '
+                                         'NUMPOINTS="3">0 0 0 0.5 0.1 0 1 0 0</E></CURVED></GEOMETRY>')
+    p = tmp_path / "c.xml"
+    p.write_text(curved)
+    with pytest.raises(N.NektarIOError, match="curved"):
+        N.read_mesh(str(p))
+    with pytest.raises(N.NektarIOError, match="not in the mesh"):
+        N.read_mesh(base).element_coords([1])
+
+
+def test_expansion_arrays():
+    m = N.read_mesh(os.path.join(HERE, "Hex_channel_C0helmsmoothing.xml"))
+    blocks = N.read_fld(os.path.join(HERE, "Hex_channel_C0helmsmoothing.fld"))
+    X, ids, c, nm, nq = N.expansion_arrays(m, blocks, "p")
+    assert X.shape == (8, 8, 3) and ids.shape == (8, 8) and c.shape == (8 * 64,)
+    assert (nm, nq) == (4, 5)
+    with pytest.raises(N.NektarIOError, match="no data"):
+        N.expansion_arrays(m, blocks, "T")

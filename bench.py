@@ -40,6 +40,8 @@ def parse():
     p.add_argument("--no-rom", action="store_true", help="skip the batched reduced solves")
     p.add_argument("--rom-instances", type=int, default=4096)
     p.add_argument("--no-pod", action="store_true", help="skip the full-size POD offline stage")
+    p.add_argument("--headline-only", action="store_true",
+                   help="time the Helmholtz apply only (no further sections; for counter runs)")
     p.add_argument("--pod-snapshots", type=int, default=512)
     return p.parse_args()
 
@@ -714,18 +716,24 @@ def main():
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(w, args.cpu_seconds)
         out["gpu_over_cpu"] = out["value"] / out["cpu_baseline"]["value"]
-    if args.op == "helmholtz" and args.precision == "fp64" and args.n == 48:
+    if args.op == "helmholtz" and args.precision == "fp64" and args.n == 48 and not args.headline_only:
         del op, x, y
         torch.cuda.empty_cache()
+        def section(name, fn):
+            t0 = time.perf_counter()
+            out[name] = fn(args)
+            print(f"[bench] {name}: {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+
+        print(f"[bench] headline: {out['value']:.2f} GDOF/s", file=sys.stderr, flush=True)
         if not args.no_rom:
-            out["batched_reduced_solves"] = rom_bench(args)
-        out["quad_helmholtz"] = quad_bench(args)
-        out["quad_helmsolve"] = quad_helmsolve_bench(args)
-        out["deriv_operators"] = deriv_bench(args)
-        out["helmsolve"] = helmsolve_bench(args)
+            section("batched_reduced_solves", rom_bench)
+        section("quad_helmholtz", quad_bench)
+        section("quad_helmsolve", quad_helmsolve_bench)
+        section("deriv_operators", deriv_bench)
+        section("helmsolve", helmsolve_bench)
         if not args.no_pod:
             try:
-                out["pod_offline"] = pod_bench(args)
+                section("pod_offline", pod_bench)
             except Exception as exc:  # noqa: BLE001 -- reported, the main line stands
                 out["pod_offline"] = {"error": f"{type(exc).__name__}: {exc}"}
     print(json.dumps(out))

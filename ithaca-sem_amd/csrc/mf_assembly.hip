@@ -300,18 +300,26 @@ __global__ void g2l_kernel(long n, const int *__restrict__ l2g, const signed cha
 // deterministic gather over the inverse map (CSR: off[g]..off[g+1] into inv).
 // With sg == NULL the contributions are added unsigned (the assembled diagonal:
 // sign^2 = 1).
-__global__ void assemble_kernel(long ng, const long *__restrict__ off, const long *__restrict__ inv,
+__device__ __forceinline__ double assemble_one(long g, const int *__restrict__ off,
+                                               const int *__restrict__ inv,
+                                               const signed char *__restrict__ sg,
+                                               const double *__restrict__ loc,
+                                               const unsigned char *__restrict__ mask)
+{
+    double s = 0.0;
+    for (int k = off[g]; k < off[g + 1]; ++k) {
+        const int i = inv[k];
+        s += sg ? sg[i] * loc[i] : loc[i];
+    }
+    return (mask && mask[g]) ? 0.0 : s;
+}
+
+__global__ void assemble_kernel(long ng, const int *__restrict__ off, const int *__restrict__ inv,
                                 const signed char *__restrict__ sg, const double *__restrict__ loc,
                                 const unsigned char *__restrict__ mask, double *__restrict__ glob)
 {
     const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (g >= ng) return;
-    double s = 0.0;
-    for (long k = off[g]; k < off[g + 1]; ++k) {
-        const long i = inv[k];
-        s += sg ? sg[i] * loc[i] : loc[i];
-    }
-    glob[g] = (mask && mask[g]) ? 0.0 : s;
+    if (g < ng) glob[g] = assemble_one(g, off, inv, sg, loc, mask);
 }
 
 __global__ void zero_kernel(long n, double *__restrict__ x)
@@ -461,6 +469,46 @@ __global__ void dot3_partial(long n, const double *__restrict__ a, const double 
     }
 }
 
+// s = Assemble(loc) as assemble_kernel, and in the same pass the block partial sums of
+// (r,w), (s,w), (r,r) for the conjugate-gradient step (grid-stride over the global modes,
+// part laid out as in dot3_partial, gridDim.x blocks: a fixed order, so reproducible).
+__global__ void assemble_dot3_kernel(long ng, const int *__restrict__ off,
+                                     const int *__restrict__ inv,
+                                     const signed char *__restrict__ sg,
+                                     const double *__restrict__ loc,
+                                     const unsigned char *__restrict__ mask,
+                                     const double *__restrict__ r, const double *__restrict__ w,
+                                     double *__restrict__ s_out, double *__restrict__ part)
+{
+    __shared__ double sh[3][RB / 64];
+    double s0 = 0, s1 = 0, s2 = 0;
+    for (long g = (long)blockIdx.x * RB + threadIdx.x; g < ng; g += (long)gridDim.x * RB) {
+        const double sv = assemble_one(g, off, inv, sg, loc, mask);
+        s_out[g] = sv;
+        const double rg = r[g], wg = w[g];
+        s0 += rg * wg;
+        s1 += sv * wg;
+        s2 += rg * rg;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        s0 += __shfl_xor(s0, o);
+        s1 += __shfl_xor(s1, o);
+        s2 += __shfl_xor(s2, o);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        sh[0][threadIdx.x >> 6] = s0;
+        sh[1][threadIdx.x >> 6] = s1;
+        sh[2][threadIdx.x >> 6] = s2;
+    }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        double t = 0;
+        for (int k = 0; k < RB / 64; ++k) t += sh[threadIdx.x][k];
+        part[(long)threadIdx.x * gridDim.x + blockIdx.x] = t;
+    }
+}
+
 __global__ void dot3_final(int nb, const double *__restrict__ part, double *__restrict__ out)
 {
     __shared__ double sh[RB / 64];
@@ -482,6 +530,9 @@ __global__ void dot3_final(int nb, const double *__restrict__ part, double *__re
 
 inline unsigned nblk(long n) { return (unsigned)((n + RB - 1) / RB); }
 
+// blocks of the fused assembly + dot pass (16 per CU on a 256-CU device)
+constexpr long kAsmParts = 4096;
+
 }  // namespace
 
 struct imf_asm_s {
@@ -489,7 +540,7 @@ struct imf_asm_s {
     long nloc = 0, ng = 0;
     int *d_l2g = nullptr;
     signed char *d_sg = nullptr;
-    long *d_off = nullptr, *d_inv = nullptr;
+    int *d_off = nullptr, *d_inv = nullptr;  // inverse map (CSR), 32-bit
     unsigned char *d_mask = nullptr;  // global Dirichlet mask (NULL without Dirichlet)
     // solver workspace (allocated on the first solve)
     double *d_ws = nullptr, *d_loc = nullptr, *d_part = nullptr, *d_dots = nullptr;
@@ -506,13 +557,18 @@ namespace {
         if (e_ != hipSuccess) return afail(IMF_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
-// out = mask0(Assemble(H(GlobalToLocal(in))))  (GlobalLinSysIterativeFull::v_DoMatrixMultiply)
-int apply_global(imf_asm a, imf_op op, const double *in, double *out, hipStream_t s)
+// out = mask0(Assemble(H(GlobalToLocal(w))))  (GlobalLinSysIterativeFull::v_DoMatrixMultiply)
+// with the partial sums of (r,w), (out,w), (r,r) taken in the assembly pass.
+int apply_global_dots(imf_asm a, imf_op op, const double *w, const double *r, double *out,
+                      hipStream_t s)
 {
     double *l0 = a->d_loc, *l1 = a->d_loc + a->nloc;
-    g2l_kernel<<<nblk(a->nloc), RB, 0, s>>>(a->nloc, a->d_l2g, a->d_sg, in, l0);
+    g2l_kernel<<<nblk(a->nloc), RB, 0, s>>>(a->nloc, a->d_l2g, a->d_sg, w, l0);
     if (int st = imf_apply_device(op, l0, l1, s)) return st;
-    assemble_kernel<<<nblk(a->ng), RB, 0, s>>>(a->ng, a->d_off, a->d_inv, a->d_sg, l1, a->d_mask, out);
+    const int nb = (int)std::min<long>(kAsmParts, nblk(a->ng));
+    assemble_dot3_kernel<<<nb, RB, 0, s>>>(a->ng, a->d_off, a->d_inv, a->d_sg, l1, a->d_mask, r,
+                                           w, out, a->d_part);
+    dot3_final<<<1, RB, 0, s>>>(nb, a->d_part, a->d_dots);
     ACHECK(hipGetLastError(), "assembly kernels");
     return IMF_OK;
 }
@@ -547,6 +603,8 @@ static int create_any(int shape, int nElmt, int nm, const long long *vertex_ids,
     std::vector<unsigned char> bd;
     long ng = 0;
     if (int st = build_map(shape, nElmt, nm, vertex_ids, m, sg, bd, ng)) return st;
+    if (m.size() > 2147483647UL)
+        return afail(IMF_ERR_ARG, "imf_assembly_create: more than 2^31 local modes");
     auto *a = new imf_asm_s();
     a->shape = shape;
     a->nElmt = nElmt;
@@ -555,11 +613,11 @@ static int create_any(int shape, int nElmt, int nm, const long long *vertex_ids,
     a->nloc = (long)m.size();
     a->ng = ng;
     // inverse map (CSR) for the deterministic assembly
-    std::vector<long> off(ng + 1, 0), inv(a->nloc);
+    std::vector<int> off(ng + 1, 0), inv(a->nloc);
     for (long i = 0; i < a->nloc; ++i) ++off[m[i] + 1];
     for (long g = 0; g < ng; ++g) off[g + 1] += off[g];
-    std::vector<long> fill(off.begin(), off.end() - 1);
-    for (long i = 0; i < a->nloc; ++i) inv[fill[m[i]]++] = i;
+    std::vector<int> fill(off.begin(), off.end() - 1);
+    for (long i = 0; i < a->nloc; ++i) inv[fill[m[i]]++] = (int)i;
     std::vector<unsigned char> gmask;
     if (a->dirichlet) {
         gmask.assign(ng, 0);
@@ -573,14 +631,14 @@ static int create_any(int shape, int nElmt, int nm, const long long *vertex_ids,
     hipError_t e;
     if ((e = hipMalloc(&a->d_l2g, a->nloc * sizeof(int))) != hipSuccess ||
         (e = hipMalloc(&a->d_sg, a->nloc)) != hipSuccess ||
-        (e = hipMalloc(&a->d_off, (ng + 1) * sizeof(long))) != hipSuccess ||
-        (e = hipMalloc(&a->d_inv, a->nloc * sizeof(long))) != hipSuccess ||
+        (e = hipMalloc(&a->d_off, (ng + 1) * sizeof(int))) != hipSuccess ||
+        (e = hipMalloc(&a->d_inv, a->nloc * sizeof(int))) != hipSuccess ||
         (a->dirichlet && (e = hipMalloc(&a->d_mask, ng)) != hipSuccess))
         return cleanup(afail(IMF_ERR_HIP, std::string("hipMalloc(assembly): ") + hipGetErrorString(e)));
     if ((e = hipMemcpy(a->d_l2g, m.data(), a->nloc * sizeof(int), hipMemcpyHostToDevice)) != hipSuccess ||
         (e = hipMemcpy(a->d_sg, sg.data(), a->nloc, hipMemcpyHostToDevice)) != hipSuccess ||
-        (e = hipMemcpy(a->d_off, off.data(), (ng + 1) * sizeof(long), hipMemcpyHostToDevice)) != hipSuccess ||
-        (e = hipMemcpy(a->d_inv, inv.data(), a->nloc * sizeof(long), hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMemcpy(a->d_off, off.data(), (ng + 1) * sizeof(int), hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMemcpy(a->d_inv, inv.data(), a->nloc * sizeof(int), hipMemcpyHostToDevice)) != hipSuccess ||
         (a->dirichlet && (e = hipMemcpy(a->d_mask, gmask.data(), ng, hipMemcpyHostToDevice)) != hipSuccess))
         return cleanup(afail(IMF_ERR_HIP, std::string("hipMemcpy(assembly): ") + hipGetErrorString(e)));
     *out = a;
@@ -658,7 +716,9 @@ int imf_helmsolve_cg_device(imf_asm a, imf_op op, const double *d_rhs, double *d
         a->nparts = (int)std::min<long>(1024, (ng + RB - 1) / RB);
         ACHECK(hipMalloc(&a->d_ws, 6 * ng * sizeof(double)), "hipMalloc(cg)");
         ACHECK(hipMalloc(&a->d_loc, 2 * a->nloc * sizeof(double)), "hipMalloc(cg)");
-        ACHECK(hipMalloc(&a->d_part, 3 * (long)a->nparts * sizeof(double)), "hipMalloc(cg)");
+        // block partial sums: dot3_partial's or the fused assembly pass's blocks
+        const long np = std::max<long>(a->nparts, std::min<long>(kAsmParts, nblk(ng)));
+        ACHECK(hipMalloc(&a->d_part, 3 * np * sizeof(double)), "hipMalloc(cg)");
         ACHECK(hipMalloc(&a->d_dots, 4 * sizeof(double)), "hipMalloc(cg)");
         a->diag_op = nullptr;
     }
@@ -685,13 +745,21 @@ int imf_helmsolve_cg_device(imf_asm a, imf_op op, const double *d_rhs, double *d
         a->diag_lambda = op->lambda;
     }
     double h[4];
+    auto read_dots = [&]() -> int {
+        ACHECK(hipMemcpyAsync(h, a->d_dots, 3 * sizeof(double), hipMemcpyDeviceToHost, s), "copy");
+        ACHECK(hipStreamSynchronize(s), "sync");
+        return IMF_OK;
+    };
     auto dots = [&](const double *x0, const double *x1, const double *x2) -> int {
         dot3_partial<<<a->nparts, RB, 0, s>>>(ng, x0, x1, x2, a->d_part);
         dot3_final<<<1, RB, 0, s>>>(a->nparts, a->d_part, a->d_dots);
         ACHECK(hipGetLastError(), "dot kernels");
-        ACHECK(hipMemcpyAsync(h, a->d_dots, 3 * sizeof(double), hipMemcpyDeviceToHost, s), "copy");
-        ACHECK(hipStreamSynchronize(s), "sync");
-        return IMF_OK;
+        return read_dots();
+    };
+    // s = A w with (r,w), (s,w), (r,r) in the same pass
+    auto apply_dots = [&]() -> int {
+        if (int st = apply_global_dots(a, op, w, r, sv, s)) return st;
+        return read_dots();
     };
     // rhs magnitude over all global modes (Set_Rhs_Magnitude)
     if (int st = dots(d_rhs, d_rhs, d_rhs)) return st;
@@ -704,8 +772,7 @@ int imf_helmsolve_cg_device(imf_asm a, imf_op op, const double *d_rhs, double *d
     int it = 0;
     if (eps >= thr) {
         cg_precon<<<nblk(ng), RB, 0, s>>>(ng, dinv, r, w);
-        if (int st = apply_global(a, op, w, sv, s)) return st;
-        if (int st = dots(r, w, sv)) return st;  // (r,w), (s,w), (r,r)
+        if (int st = apply_dots()) return st;  // (r,w), (s,w), (r,r)
         double rho = h[0], mu = h[1];
         double beta = 0.0, alpha = rho / mu;
         it = 1;
@@ -717,8 +784,7 @@ int imf_helmsolve_cg_device(imf_asm a, imf_op op, const double *d_rhs, double *d
                 return afail(IMF_ERR_STATE, "Exceeded maximum number of iterations");
             }
             cg_update<<<nblk(ng), RB, 0, s>>>(ng, alpha, beta, dinv, p, q, w, sv, d_x, r);
-            if (int st = apply_global(a, op, w, sv, s)) return st;
-            if (int st = dots(r, w, sv)) return st;
+            if (int st = apply_dots()) return st;
             const double rho_new = h[0];
             mu = h[1];
             eps = h[2];

@@ -30,7 +30,6 @@ typedef float f4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 
 constexpr int GB = 128;   // C block edge per workgroup
-constexpr int GT = 256;   // threads (4 waves, each a 64 x 64 quadrant)
 
 struct GramArgs {
     int m, n;
@@ -67,38 +66,44 @@ __device__ __forceinline__ bool gram_block(const GramArgs &g, int &split, int &b
     return true;
 }
 
-// Per-mode LDS panel formats and MFMA steps.
-//   MODE 0 (FP64):   KB = 16, double panels, v_mfma_f64_16x16x4f64
+// Per-mode LDS panel formats, MFMA steps and workgroup shapes.
+//   MODE 0 (FP64):   KB = 16, double panels, v_mfma_f64_16x16x4f64; 4 waves, each a
+//                    64 x 64 quadrant of the C block, two workgroups per CU
 //   MODE 1 (FP32x3): KB = 16, hi/lo float panels, 3 x v_mfma_f32_16x16x4f32
 //   MODE 2 (BF16x3): KB = 32, hi/lo bf16 panels, 3 x v_mfma_f32_16x16x32_bf16
+// The split modes convert every panel element to two parts (VALU work between the MFMA
+// steps) and keep FP32 panel sums next to FP64 block sums, so they run 8 waves, each a
+// 32 x 64 strip: two waves per SIMD, one converting while the other multiplies.
 template <int MODE> struct GramMode;
-template <> struct GramMode<0> { static constexpr int KB = 16, KP = 17; using E = double; };
-template <> struct GramMode<1> { static constexpr int KB = 16, KP = 17; using E = float; };
-template <> struct GramMode<2> { static constexpr int KB = 32, KP = 40; using E = __bf16; };
+template <> struct GramMode<0> { static constexpr int KB = 16, KP = 17, NW = 4; using E = double; };
+template <> struct GramMode<1> { static constexpr int KB = 16, KP = 17, NW = 8; using E = float; };
+template <> struct GramMode<2> { static constexpr int KB = 32, KP = 40, NW = 8; using E = __bf16; };
 
 __device__ __forceinline__ __bf16 to_bf16(float v) { return __float2bfloat16(v); }
 __device__ __forceinline__ float from_bf16(__bf16 v) { return __bfloat162float(v); }
 
 // One KB-deep panel of GB rows of X (column-major K x m: row i of the panel is column i0+i
-// of X): KB/4 pairs of doubles per thread, consecutive threads on consecutive k of one
-// column so that a wave reads whole 128-byte lines.
-template <int KB>
+// of X): NL = KB*GB/(2T) pairs of doubles per thread of a T-thread workgroup, consecutive
+// threads on consecutive k of one column so that a wave reads whole 128-byte lines.
+template <int KB, int T>
 __device__ __forceinline__ void panel_load(const double *__restrict__ X, long ldx, int i0, int m,
-                                           long k0, long kend, int vec, double2 (&v)[KB / 4])
+                                           long k0, long kend, int vec,
+                                           double2 (&v)[KB * GB / (2 * T)])
 {
+    constexpr int NL = KB * GB / (2 * T);
     // interior panels (the common case) take one uniform branch of unconditional loads
     if (vec && i0 + GB <= m && k0 + KB <= kend) {
 #pragma unroll
-        for (int s = 0; s < KB / 4; ++s) {
-            const int idx = s * GT + threadIdx.x;
+        for (int s = 0; s < NL; ++s) {
+            const int idx = s * T + threadIdx.x;
             const int c = idx / (KB / 2), ko = 2 * (idx % (KB / 2));
             v[s] = *reinterpret_cast<const double2 *>(X + (long)(i0 + c) * ldx + k0 + ko);
         }
         return;
     }
 #pragma unroll
-    for (int s = 0; s < KB / 4; ++s) {
-        const int idx = s * GT + threadIdx.x;
+    for (int s = 0; s < NL; ++s) {
+        const int idx = s * T + threadIdx.x;
         const int c = idx / (KB / 2), ko = 2 * (idx % (KB / 2));
         const int gi = i0 + c;
         const long k = k0 + ko;
@@ -108,42 +113,53 @@ __device__ __forceinline__ void panel_load(const double *__restrict__ X, long ld
     }
 }
 
-template <int MODE, int KB, int KP, typename E>
-__device__ __forceinline__ void panel_store(const double2 (&v)[KB / 4], E *hi, E *lo)
+typedef __bf16 bf2 __attribute__((ext_vector_type(2)));
+
+template <int MODE, int KB, int KP, int T, typename E>
+__device__ __forceinline__ void panel_store(const double2 (&v)[KB * GB / (2 * T)], E *hi, E *lo)
 {
+    constexpr int NL = KB * GB / (2 * T);
 #pragma unroll
-    for (int s = 0; s < KB / 4; ++s) {
-        const int idx = s * GT + threadIdx.x;
+    for (int s = 0; s < NL; ++s) {
+        const int idx = s * T + threadIdx.x;
         const int c = idx / (KB / 2), ko = 2 * (idx % (KB / 2));
         const double x[2] = {v[s].x, v[s].y};
+        if constexpr (MODE == 2) {
+            // x ~ hi + lo: hi = bf16(x), lo = bf16(x - hi), the difference taken in FP32
+            // (exact: x rounded to FP32 minus its leading 8 bits); the pair is stored as one
+            // 32-bit word (KP and ko even)
+            const float x0 = (float)x[0], x1 = (float)x[1];
+            const __bf16 h0 = to_bf16(x0), h1 = to_bf16(x1);
+            const int o = c * KP + ko;
+            *reinterpret_cast<bf2 *>(&hi[o]) = bf2{h0, h1};
+            *reinterpret_cast<bf2 *>(&lo[o]) =
+                bf2{to_bf16(x0 - from_bf16(h0)), to_bf16(x1 - from_bf16(h1))};
+        } else {
 #pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            const int o = c * KP + ko + h;
-            if constexpr (MODE == 0) {
-                hi[o] = x[h];
-            } else if constexpr (MODE == 1) {
-                const float xh = (float)x[h];
-                hi[o] = xh;
-                lo[o] = (float)(x[h] - (double)xh);
-            } else {
-                const __bf16 xh = to_bf16((float)x[h]);
-                hi[o] = xh;
-                lo[o] = to_bf16((float)(x[h] - (double)from_bf16(xh)));
+            for (int h = 0; h < 2; ++h) {
+                const int o = c * KP + ko + h;
+                if constexpr (MODE == 0) {
+                    hi[o] = x[h];
+                } else {
+                    const float xh = (float)x[h];
+                    hi[o] = xh;
+                    lo[o] = (float)(x[h] - (double)xh);
+                }
             }
         }
     }
 }
 
 // C block (bi, bj) of X^T Y over one K slice, written to part[split] (row-major m x n).
-// The next panel is loaded into registers while the current one is multiplied.
-// FP64 fits two workgroups per CU; the split modes keep FP32 panel sums next to the FP64
-// block sums and the prefetched panel, and run one workgroup per CU.
+// The next panel is loaded into registers while the current one is multiplied.  Wave w
+// owns rows (w/2)*WR .. +WR and columns (w%2)*64 .. +64 of the block (TA x 4 MFMA tiles).
 template <int MODE>
-__global__ __launch_bounds__(GT, MODE == 0 ? 2 : 1) void gram_kernel(GramArgs g)
+__global__ __launch_bounds__(64 * GramMode<MODE>::NW, 2) void gram_kernel(GramArgs g)
 {
     using Md = GramMode<MODE>;
     using E = typename Md::E;
-    constexpr int KB = Md::KB, KP = Md::KP;
+    constexpr int KB = Md::KB, KP = Md::KP, T = 64 * Md::NW;
+    constexpr int WR = GB / (Md::NW / 2), TA = WR / 16, NL = KB * GB / (2 * T);
     constexpr int NA = MODE == 0 ? 1 : 2;  // hi (and lo) panels
     __shared__ __attribute__((aligned(16))) E Xs[NA][GB * KP], Ys[NA][GB * KP];
     int split, bi, bj;
@@ -151,54 +167,54 @@ __global__ __launch_bounds__(GT, MODE == 0 ? 2 : 1) void gram_kernel(GramArgs g)
     const int i0 = bi * GB, j0 = bj * GB;
     const long kbeg = (long)split * g.kchunk, kend = min(g.K, kbeg + g.kchunk);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int wr = (wave >> 1) * 64, wc = (wave & 1) * 64;  // quadrant of this wave
+    const int wr = (wave >> 1) * WR, wc = (wave & 1) * 64;  // this wave's strip
 
-    d4 tot[4][4];
+    d4 tot[TA][4];
 #pragma unroll
-    for (int a = 0; a < 4; ++a)
+    for (int a = 0; a < TA; ++a)
 #pragma unroll
         for (int b = 0; b < 4; ++b) tot[a][b] = d4{0, 0, 0, 0};
 
-    double2 xv[KB / 4], yv[KB / 4];
-    panel_load<KB>(g.X, g.ldx, i0, g.m, kbeg, kend, g.vec, xv);
-    panel_load<KB>(g.Y, g.ldy, j0, g.n, kbeg, kend, g.vec, yv);
+    double2 xv[NL], yv[NL];
+    panel_load<KB, T>(g.X, g.ldx, i0, g.m, kbeg, kend, g.vec, xv);
+    panel_load<KB, T>(g.Y, g.ldy, j0, g.n, kbeg, kend, g.vec, yv);
     for (long k0 = kbeg; k0 < kend; k0 += KB) {
-        panel_store<MODE, KB, KP, E>(xv, Xs[0], Xs[NA - 1]);
-        panel_store<MODE, KB, KP, E>(yv, Ys[0], Ys[NA - 1]);
+        panel_store<MODE, KB, KP, T, E>(xv, Xs[0], Xs[NA - 1]);
+        panel_store<MODE, KB, KP, T, E>(yv, Ys[0], Ys[NA - 1]);
         __syncthreads();
         if (k0 + KB < kend) {
-            panel_load<KB>(g.X, g.ldx, i0, g.m, k0 + KB, kend, g.vec, xv);
-            panel_load<KB>(g.Y, g.ldy, j0, g.n, k0 + KB, kend, g.vec, yv);
+            panel_load<KB, T>(g.X, g.ldx, i0, g.m, k0 + KB, kend, g.vec, xv);
+            panel_load<KB, T>(g.Y, g.ldy, j0, g.n, k0 + KB, kend, g.vec, yv);
         }
         if constexpr (MODE == 0) {
 #pragma unroll
             for (int ks = 0; ks < KB; ks += 4) {
-                double av[4], bv[4];
+                double av[TA], bv[4];
 #pragma unroll
-                for (int a = 0; a < 4; ++a)
+                for (int a = 0; a < TA; ++a)
                     av[a] = Xs[0][(wr + a * 16 + (lane & 15)) * KP + ks + (lane >> 4)];
 #pragma unroll
                 for (int b = 0; b < 4; ++b)
                     bv[b] = Ys[0][(wc + b * 16 + (lane & 15)) * KP + ks + (lane >> 4)];
 #pragma unroll
-                for (int a = 0; a < 4; ++a)
+                for (int a = 0; a < TA; ++a)
 #pragma unroll
                     for (int b = 0; b < 4; ++b)
                         tot[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a], bv[b], tot[a][b],
                                                                          0, 0, 0);
             }
         } else {
-            f4 acc[4][4];
+            f4 acc[TA][4];
 #pragma unroll
-            for (int a = 0; a < 4; ++a)
+            for (int a = 0; a < TA; ++a)
 #pragma unroll
                 for (int b = 0; b < 4; ++b) acc[a][b] = f4{0, 0, 0, 0};
             if constexpr (MODE == 1) {
 #pragma unroll
                 for (int ks = 0; ks < KB; ks += 4) {
-                    float ah[4], al[4], bh[4], bl[4];
+                    float ah[TA], al[TA], bh[4], bl[4];
 #pragma unroll
-                    for (int a = 0; a < 4; ++a) {
+                    for (int a = 0; a < TA; ++a) {
                         const int o = (wr + a * 16 + (lane & 15)) * KP + ks + (lane >> 4);
                         ah[a] = Xs[0][o];
                         al[a] = Xs[1][o];
@@ -210,7 +226,7 @@ __global__ __launch_bounds__(GT, MODE == 0 ? 2 : 1) void gram_kernel(GramArgs g)
                         bl[b] = Ys[1][o];
                     }
 #pragma unroll
-                    for (int a = 0; a < 4; ++a)
+                    for (int a = 0; a < TA; ++a)
 #pragma unroll
                         for (int b = 0; b < 4; ++b) {
                             acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(al[a], bh[b], acc[a][b], 0, 0, 0);
@@ -219,9 +235,9 @@ __global__ __launch_bounds__(GT, MODE == 0 ? 2 : 1) void gram_kernel(GramArgs g)
                         }
                 }
             } else {
-                bf8 ah[4], al[4], bh[4], bl[4];
+                bf8 ah[TA], al[TA], bh[4], bl[4];
 #pragma unroll
-                for (int a = 0; a < 4; ++a) {
+                for (int a = 0; a < TA; ++a) {
                     const int o = (wr + a * 16 + (lane & 15)) * KP + 8 * (lane >> 4);
                     ah[a] = *reinterpret_cast<const bf8 *>(&Xs[0][o]);
                     al[a] = *reinterpret_cast<const bf8 *>(&Xs[1][o]);
@@ -233,7 +249,7 @@ __global__ __launch_bounds__(GT, MODE == 0 ? 2 : 1) void gram_kernel(GramArgs g)
                     bl[b] = *reinterpret_cast<const bf8 *>(&Ys[1][o]);
                 }
 #pragma unroll
-                for (int a = 0; a < 4; ++a)
+                for (int a = 0; a < TA; ++a)
 #pragma unroll
                     for (int b = 0; b < 4; ++b) {
                         acc[a][b] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[a], bh[b], acc[a][b], 0, 0, 0);
@@ -242,7 +258,7 @@ __global__ __launch_bounds__(GT, MODE == 0 ? 2 : 1) void gram_kernel(GramArgs g)
                     }
             }
 #pragma unroll
-            for (int a = 0; a < 4; ++a)
+            for (int a = 0; a < TA; ++a)
 #pragma unroll
                 for (int b = 0; b < 4; ++b)
 #pragma unroll
@@ -254,7 +270,7 @@ __global__ __launch_bounds__(GT, MODE == 0 ? 2 : 1) void gram_kernel(GramArgs g)
     // 16x16x4 and bf16 16x16x32 ones hold D[4*(lane/16) + r][lane%16].
     double *P = g.part + (long)split * g.m * g.n;
 #pragma unroll
-    for (int a = 0; a < 4; ++a)
+    for (int a = 0; a < TA; ++a)
 #pragma unroll
         for (int b = 0; b < 4; ++b)
 #pragma unroll
@@ -342,11 +358,11 @@ int irom_gram(int mode, int m, int n, long K, const double *d_X, long ldx, const
     const dim3 grid((unsigned)(8L * ga.ntiles * ((nsplit + 7) / 8)));
     double *part = (double *)d_work;
     if (base == IROM_GRAM_FP64)
-        irom::gram_kernel<0><<<grid, irom::GT, 0, s>>>(ga);
+        irom::gram_kernel<0><<<grid, 64 * irom::GramMode<0>::NW, 0, s>>>(ga);
     else if (base == IROM_GRAM_FP32X3)
-        irom::gram_kernel<1><<<grid, irom::GT, 0, s>>>(ga);
+        irom::gram_kernel<1><<<grid, 64 * irom::GramMode<1>::NW, 0, s>>>(ga);
     else if (base == IROM_GRAM_BF16X3)
-        irom::gram_kernel<2><<<grid, irom::GT, 0, s>>>(ga);
+        irom::gram_kernel<2><<<grid, 64 * irom::GramMode<2>::NW, 0, s>>>(ga);
     else {
         g_gerr = "irom_gram: unknown mode";
         return 1;

@@ -82,6 +82,17 @@ template <> struct GramMode<2> { static constexpr int KB = 32, KP = 40, NW = 8; 
 __device__ __forceinline__ __bf16 to_bf16(float v) { return __float2bfloat16(v); }
 __device__ __forceinline__ float from_bf16(__bf16 v) { return __bfloat162float(v); }
 
+// Panel row of the q-th group of KB/2 consecutive threads.  For the BF16 panels (KB = 32,
+// 80-byte LDS rows, 16 threads per row) the 4 rows one wave stores are spread 4 apart,
+// which puts them in 4 disjoint 16-bank ranges (consecutive rows would overlap the first
+// and the fourth); loads stay whole-row and coalesced.
+template <int KB>
+__device__ __forceinline__ int panel_row(int q)
+{
+    if constexpr (KB == 32) return (q & ~15) | ((q & 3) << 2) | ((q >> 2) & 3);
+    return q;
+}
+
 // One KB-deep panel of GB rows of X (column-major K x m: row i of the panel is column i0+i
 // of X): NL = KB*GB/(2T) pairs of doubles per thread of a T-thread workgroup, consecutive
 // threads on consecutive k of one column so that a wave reads whole 128-byte lines.
@@ -96,7 +107,7 @@ __device__ __forceinline__ void panel_load(const double *__restrict__ X, long ld
 #pragma unroll
         for (int s = 0; s < NL; ++s) {
             const int idx = s * T + threadIdx.x;
-            const int c = idx / (KB / 2), ko = 2 * (idx % (KB / 2));
+            const int c = panel_row<KB>(idx / (KB / 2)), ko = 2 * (idx % (KB / 2));
             v[s] = *reinterpret_cast<const double2 *>(X + (long)(i0 + c) * ldx + k0 + ko);
         }
         return;
@@ -104,7 +115,7 @@ __device__ __forceinline__ void panel_load(const double *__restrict__ X, long ld
 #pragma unroll
     for (int s = 0; s < NL; ++s) {
         const int idx = s * T + threadIdx.x;
-        const int c = idx / (KB / 2), ko = 2 * (idx % (KB / 2));
+        const int c = panel_row<KB>(idx / (KB / 2)), ko = 2 * (idx % (KB / 2));
         const int gi = i0 + c;
         const long k = k0 + ko;
         const double *p = X + (long)gi * ldx + k;
@@ -122,18 +133,20 @@ __device__ __forceinline__ void panel_store(const double2 (&v)[KB * GB / (2 * T)
 #pragma unroll
     for (int s = 0; s < NL; ++s) {
         const int idx = s * T + threadIdx.x;
-        const int c = idx / (KB / 2), ko = 2 * (idx % (KB / 2));
+        const int c = panel_row<KB>(idx / (KB / 2)), ko = 2 * (idx % (KB / 2));
         const double x[2] = {v[s].x, v[s].y};
         if constexpr (MODE == 2) {
-            // x ~ hi + lo: hi = bf16(x), lo = bf16(x - hi), the difference taken in FP32
-            // (exact: x rounded to FP32 minus its leading 8 bits); the pair is stored as one
-            // 32-bit word (KP and ko even)
+            // x ~ hi + lo: hi = the leading 8 bits of x rounded to FP32 (half-up on the bit
+            // pattern: an add and a mask, no correctly rounded FP64 -> BF16 sequence, which
+            // hi does not need), lo = bf16(xf - hi), the difference exact in FP32; each pair
+            // is stored as one 32-bit word (KP and ko even)
             const float x0 = (float)x[0], x1 = (float)x[1];
-            const __bf16 h0 = to_bf16(x0), h1 = to_bf16(x1);
+            const unsigned b0 = (__float_as_uint(x0) + 0x8000u) & 0xffff0000u;
+            const unsigned b1 = (__float_as_uint(x1) + 0x8000u) & 0xffff0000u;
             const int o = c * KP + ko;
-            *reinterpret_cast<bf2 *>(&hi[o]) = bf2{h0, h1};
+            *reinterpret_cast<unsigned *>(&hi[o]) = (b0 >> 16) | b1;
             *reinterpret_cast<bf2 *>(&lo[o]) =
-                bf2{to_bf16(x0 - from_bf16(h0)), to_bf16(x1 - from_bf16(h1))};
+                bf2{to_bf16(x0 - __uint_as_float(b0)), to_bf16(x1 - __uint_as_float(b1))};
         } else {
 #pragma unroll
             for (int h = 0; h < 2; ++h) {
@@ -161,7 +174,11 @@ __global__ __launch_bounds__(64 * GramMode<MODE>::NW, 2) void gram_kernel(GramAr
     constexpr int KB = Md::KB, KP = Md::KP, T = 64 * Md::NW;
     constexpr int WR = GB / (Md::NW / 2), TA = WR / 16, NL = KB * GB / (2 * T);
     constexpr int NA = MODE == 0 ? 1 : 2;  // hi (and lo) panels
-    __shared__ __attribute__((aligned(16))) E Xs[NA][GB * KP], Ys[NA][GB * KP];
+    // FP32x3: two LDS panel buffers, the next panel converted and stored while the current
+    // one is multiplied, one barrier per panel (measured faster there, slower for the
+    // other modes, which keep one buffer and two barriers per panel)
+    constexpr int NBUF = MODE == 1 ? 2 : 1;
+    __shared__ __attribute__((aligned(16))) E Xs[NBUF][NA][GB * KP], Ys[NBUF][NA][GB * KP];
     int split, bi, bj;
     if (!gram_block(g, split, bi, bj)) return;
     const int i0 = bi * GB, j0 = bj * GB;
@@ -175,27 +192,17 @@ __global__ __launch_bounds__(64 * GramMode<MODE>::NW, 2) void gram_kernel(GramAr
 #pragma unroll
         for (int b = 0; b < 4; ++b) tot[a][b] = d4{0, 0, 0, 0};
 
-    double2 xv[NL], yv[NL];
-    panel_load<KB, T>(g.X, g.ldx, i0, g.m, kbeg, kend, g.vec, xv);
-    panel_load<KB, T>(g.Y, g.ldy, j0, g.n, kbeg, kend, g.vec, yv);
-    for (long k0 = kbeg; k0 < kend; k0 += KB) {
-        panel_store<MODE, KB, KP, T, E>(xv, Xs[0], Xs[NA - 1]);
-        panel_store<MODE, KB, KP, T, E>(yv, Ys[0], Ys[NA - 1]);
-        __syncthreads();
-        if (k0 + KB < kend) {
-            panel_load<KB, T>(g.X, g.ldx, i0, g.m, k0 + KB, kend, g.vec, xv);
-            panel_load<KB, T>(g.Y, g.ldy, j0, g.n, k0 + KB, kend, g.vec, yv);
-        }
+    auto multiply = [&](const E *xh, const E *xl, const E *yh, const E *yl) {
         if constexpr (MODE == 0) {
 #pragma unroll
             for (int ks = 0; ks < KB; ks += 4) {
                 double av[TA], bv[4];
 #pragma unroll
                 for (int a = 0; a < TA; ++a)
-                    av[a] = Xs[0][(wr + a * 16 + (lane & 15)) * KP + ks + (lane >> 4)];
+                    av[a] = xh[(wr + a * 16 + (lane & 15)) * KP + ks + (lane >> 4)];
 #pragma unroll
                 for (int b = 0; b < 4; ++b)
-                    bv[b] = Ys[0][(wc + b * 16 + (lane & 15)) * KP + ks + (lane >> 4)];
+                    bv[b] = yh[(wc + b * 16 + (lane & 15)) * KP + ks + (lane >> 4)];
 #pragma unroll
                 for (int a = 0; a < TA; ++a)
 #pragma unroll
@@ -216,14 +223,14 @@ __global__ __launch_bounds__(64 * GramMode<MODE>::NW, 2) void gram_kernel(GramAr
 #pragma unroll
                     for (int a = 0; a < TA; ++a) {
                         const int o = (wr + a * 16 + (lane & 15)) * KP + ks + (lane >> 4);
-                        ah[a] = Xs[0][o];
-                        al[a] = Xs[1][o];
+                        ah[a] = xh[o];
+                        al[a] = xl[o];
                     }
 #pragma unroll
                     for (int b = 0; b < 4; ++b) {
                         const int o = (wc + b * 16 + (lane & 15)) * KP + ks + (lane >> 4);
-                        bh[b] = Ys[0][o];
-                        bl[b] = Ys[1][o];
+                        bh[b] = yh[o];
+                        bl[b] = yl[o];
                     }
 #pragma unroll
                     for (int a = 0; a < TA; ++a)
@@ -239,14 +246,14 @@ __global__ __launch_bounds__(64 * GramMode<MODE>::NW, 2) void gram_kernel(GramAr
 #pragma unroll
                 for (int a = 0; a < TA; ++a) {
                     const int o = (wr + a * 16 + (lane & 15)) * KP + 8 * (lane >> 4);
-                    ah[a] = *reinterpret_cast<const bf8 *>(&Xs[0][o]);
-                    al[a] = *reinterpret_cast<const bf8 *>(&Xs[1][o]);
+                    ah[a] = *reinterpret_cast<const bf8 *>(&xh[o]);
+                    al[a] = *reinterpret_cast<const bf8 *>(&xl[o]);
                 }
 #pragma unroll
                 for (int b = 0; b < 4; ++b) {
                     const int o = (wc + b * 16 + (lane & 15)) * KP + 8 * (lane >> 4);
-                    bh[b] = *reinterpret_cast<const bf8 *>(&Ys[0][o]);
-                    bl[b] = *reinterpret_cast<const bf8 *>(&Ys[1][o]);
+                    bh[b] = *reinterpret_cast<const bf8 *>(&yh[o]);
+                    bl[b] = *reinterpret_cast<const bf8 *>(&yl[o]);
                 }
 #pragma unroll
                 for (int a = 0; a < TA; ++a)
@@ -264,7 +271,44 @@ __global__ __launch_bounds__(64 * GramMode<MODE>::NW, 2) void gram_kernel(GramAr
 #pragma unroll
                     for (int r = 0; r < 4; ++r) tot[a][b][r] += (double)acc[a][b][r];
         }
+    };
+
+    double2 xv[NL], yv[NL];
+    panel_load<KB, T>(g.X, g.ldx, i0, g.m, kbeg, kend, g.vec, xv);
+    panel_load<KB, T>(g.Y, g.ldy, j0, g.n, kbeg, kend, g.vec, yv);
+    if constexpr (NBUF == 2) {
+        panel_store<MODE, KB, KP, T, E>(xv, Xs[0][0], Xs[0][NA - 1]);
+        panel_store<MODE, KB, KP, T, E>(yv, Ys[0][0], Ys[0][NA - 1]);
+        if (kbeg + KB < kend) {
+            panel_load<KB, T>(g.X, g.ldx, i0, g.m, kbeg + KB, kend, g.vec, xv);
+            panel_load<KB, T>(g.Y, g.ldy, j0, g.n, kbeg + KB, kend, g.vec, yv);
+        }
         __syncthreads();
+        int cur = 0;
+        for (long k0 = kbeg; k0 < kend; k0 += KB, cur ^= 1) {
+            if (k0 + KB < kend) {  // the other buffer was last read before the last barrier
+                panel_store<MODE, KB, KP, T, E>(xv, Xs[cur ^ 1][0], Xs[cur ^ 1][NA - 1]);
+                panel_store<MODE, KB, KP, T, E>(yv, Ys[cur ^ 1][0], Ys[cur ^ 1][NA - 1]);
+                if (k0 + 2 * KB < kend) {
+                    panel_load<KB, T>(g.X, g.ldx, i0, g.m, k0 + 2 * KB, kend, g.vec, xv);
+                    panel_load<KB, T>(g.Y, g.ldy, j0, g.n, k0 + 2 * KB, kend, g.vec, yv);
+                }
+            }
+            multiply(Xs[cur][0], Xs[cur][NA - 1], Ys[cur][0], Ys[cur][NA - 1]);
+            __syncthreads();
+        }
+    } else {
+        for (long k0 = kbeg; k0 < kend; k0 += KB) {
+            panel_store<MODE, KB, KP, T, E>(xv, Xs[0][0], Xs[0][NA - 1]);
+            panel_store<MODE, KB, KP, T, E>(yv, Ys[0][0], Ys[0][NA - 1]);
+            __syncthreads();
+            if (k0 + KB < kend) {
+                panel_load<KB, T>(g.X, g.ldx, i0, g.m, k0 + KB, kend, g.vec, xv);
+                panel_load<KB, T>(g.Y, g.ldy, j0, g.n, k0 + KB, kend, g.vec, yv);
+            }
+            multiply(Xs[0][0], Xs[0][NA - 1], Ys[0][0], Ys[0][NA - 1]);
+            __syncthreads();
+        }
     }
     // The f64 16x16x4 accumulator holds D[lane/16 + 4r][lane%16] in component r; the f32
     // 16x16x4 and bf16 16x16x32 ones hold D[4*(lane/16) + r][lane%16].

@@ -100,6 +100,10 @@ struct LaunchArgs {
     hipStream_t stream;
     int nvec = 1;      // vectors per launch (Mass), vector v at in + v*ldin, out + v*ldout
     long ldin = 0, ldout = 0;
+    // FP64 hex Helmholtz only: the input is a global (assembled) vector, gathered per local
+    // mode as gsign[i] * in[gmap[i]] (GlobalToLocal fused into the operator's load)
+    const int *gmap = nullptr;
+    const signed char *gsign = nullptr;
 };
 using LaunchFn = hipError_t (*)(const LaunchArgs &);
 

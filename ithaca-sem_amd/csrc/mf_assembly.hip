@@ -563,8 +563,12 @@ int apply_global_dots(imf_asm a, imf_op op, const double *w, const double *r, do
                       hipStream_t s)
 {
     double *l0 = a->d_loc, *l1 = a->d_loc + a->nloc;
-    g2l_kernel<<<nblk(a->nloc), RB, 0, s>>>(a->nloc, a->d_l2g, a->d_sg, w, l0);
-    if (int st = imf_apply_device(op, l0, l1, s)) return st;
+    if (a->shape == imf::kHex) {  // GlobalToLocal fused into the Helmholtz kernel's load
+        if (int st = imf_apply_gather_device(op, w, a->d_l2g, a->d_sg, l1, s)) return st;
+    } else {
+        g2l_kernel<<<nblk(a->nloc), RB, 0, s>>>(a->nloc, a->d_l2g, a->d_sg, w, l0);
+        if (int st = imf_apply_device(op, l0, l1, s)) return st;
+    }
     const int nb = (int)std::min<long>(kAsmParts, nblk(a->ng));
     assemble_dot3_kernel<<<nb, RB, 0, s>>>(a->ng, a->d_off, a->d_inv, a->d_sg, l1, a->d_mask, r,
                                            w, out, a->d_part);

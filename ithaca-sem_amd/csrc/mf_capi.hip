@@ -117,7 +117,8 @@ int check_op(const imf_op op)
 }
 
 int run(imf_op op, imf::LaunchFn fn, const void *in, void *out, void *stream, int nvec = 1,
-        long ldin = 0, long ldout = 0)
+        long ldin = 0, long ldout = 0, const int *gmap = nullptr,
+        const signed char *gsign = nullptr)
 {
     if ((needs_jac(op) || needs_df(op)) && !op->geom_ready)
         return fail(IMF_ERR_STATE, !needs_jac(op) ? "derivative factors not set"
@@ -125,13 +126,23 @@ int run(imf_op op, imf::LaunchFn fn, const void *in, void *out, void *stream, in
                                                   : "Jacobian not set");
     if (!in || !out) return fail(IMF_ERR_ARG, "null input/output array");
     imf::LaunchArgs a{op->tab.data(), op->nElmt, op->lambda, op->d_geom, in, out,
-                      (hipStream_t)stream, nvec, ldin, ldout};
+                      (hipStream_t)stream, nvec, ldin, ldout, gmap, gsign};
     hipError_t e = fn(a);
     if (e != hipSuccess) return hip_fail(e, "kernel launch");
     return IMF_OK;
 }
 
 }  // namespace
+
+int imf_apply_gather_device(imf_op_s *op, const double *glob, const int *gmap,
+                            const signed char *gsign, double *out, hipStream_t stream)
+{
+    if (int s = check_op(op)) return s;
+    if (op->kind != imf::kHelmholtz || op->shape != imf::kHex || op->prec != IMF_FP64 ||
+        !gmap || !gsign)
+        return fail(IMF_ERR_UNSUPPORTED, "gathered input: FP64 Helmholtz_Hex only");
+    return run(op, op->fwd, glob, out, stream, 1, 0, 0, gmap, gsign);
+}
 
 extern "C" {
 

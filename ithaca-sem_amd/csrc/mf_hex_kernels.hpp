@@ -61,6 +61,23 @@ __device__ __forceinline__ void load_modes_block(const T *__restrict__ src, int 
     }
 }
 
+// The same from a global vector: coefficient f of the block is gsign[f] * glob[gmap[f]]
+// (gmap, gsign offset to the block's first element; the map reads are coalesced, the
+// gather follows the global numbering).
+template <int NM, int NQ, typename T, typename BufFn>
+__device__ __forceinline__ void load_modes_block_gather(const T *__restrict__ glob,
+                                                        const int *__restrict__ gmap,
+                                                        const signed char *__restrict__ gsign,
+                                                        int nv, BufFn buf)
+{
+    constexpr int M = NM * NM * NM;
+    for (int f = threadIdx.x; f < nv * M; f += blockDim.x) {
+        const int sl = f / M, o = f - sl * M;
+        const int tp = o / NM, p = o - tp * NM;
+        buf(sl)[tp * (NM + 1) + p] = T(gsign[f]) * glob[gmap[f]];
+    }
+}
+
 // Block-cooperative, fully coalesced store of nv elements' coefficients from the padded
 // pencil buffers buf(slot) to dst.
 template <int NM, int NQ, typename T, typename BufFn>
@@ -336,10 +353,13 @@ __global__ __launch_bounds__(HexCfg<NQ>::BLOCK) void hex_mass_kernel(
 // Packed geometry: deformed [e][GS] with GS = 7Q rounded up to 128-byte lines and the
 // components (Jw, JwG00, JwG01, JwG02, JwG11, JwG12, JwG22) at c*Q; regular [e][7]
 // (J, J G00, ...), weights applied in the kernel.
-template <int NM, int NQ, typename T, int GEO>
+// GATHER: `in` is a global vector read through (gmap, gsign), as GlobalToLocal would
+// expand it, so the solver's operator needs no local copy of its input.
+template <int NM, int NQ, typename T, int GEO, bool GATHER = false>
 __global__ __launch_bounds__(HexCfg<NQ>::BLOCK) void hex_helmholtz_kernel(
     const Tab<NM, NQ, T> tab, int nElmt, T lambda, const T *__restrict__ in,
-    const T *__restrict__ geom, T *__restrict__ out)
+    const T *__restrict__ geom, T *__restrict__ out, const int *__restrict__ gmap = nullptr,
+    const signed char *__restrict__ gsign = nullptr)
 {
     using C = HexCfg<NQ>;
     using Lb = HexLds<NM, NQ>;
@@ -357,7 +377,11 @@ __global__ __launch_bounds__(HexCfg<NQ>::BLOCK) void hex_helmholtz_kernel(
     const int ci = t % NQ, cj = t / NQ;  // this thread's column (i, j)
 
     // 1. BwdTrans into k-columns; keep a copy of u in LDS for the cross derivatives.
-    load_modes_block<NM, NQ, T>(in + e0 * M, nv, [&](int sl) { return lds[sl][2]; });
+    if constexpr (GATHER)
+        load_modes_block_gather<NM, NQ, T>(in, gmap + e0 * M, gsign + e0 * M, nv,
+                                           [&](int sl) { return lds[sl][2]; });
+    else
+        load_modes_block<NM, NQ, T>(in + e0 * M, nv, [&](int sl) { return lds[sl][2]; });
     __syncthreads();
     T u[NQ];
     hex_bwd_to_columns<NM, NQ, T>(

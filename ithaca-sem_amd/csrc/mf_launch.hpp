@@ -26,6 +26,14 @@ hipError_t launch_hex(const LaunchArgs &a)
     const T *geom = static_cast<const T *>(a.geom);
     T *out = static_cast<T *>(a.out);
     if constexpr (KIND == kHelmholtz) {
+        if constexpr (std::is_same_v<T, double>) {
+            if (a.gmap) {
+                hex_helmholtz_kernel<NM, NQ, T, GEO, true><<<grid, block, 0, a.stream>>>(
+                    tab, a.nElmt, T(a.lambda), in, geom, out, a.gmap, a.gsign);
+                return hipGetLastError();
+            }
+        }
+        if (a.gmap) return hipErrorInvalidValue;  // gathered input: FP64 only
         hex_helmholtz_kernel<NM, NQ, T, GEO>
             <<<grid, block, 0, a.stream>>>(tab, a.nElmt, T(a.lambda), in, geom, out);
     } else if constexpr (KIND == kMass) {

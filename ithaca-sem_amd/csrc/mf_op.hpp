@@ -21,3 +21,9 @@ struct imf_op_s {
     void *d_in = nullptr, *d_out = nullptr;  // staging for host applies
     imf::LaunchFn fwd = nullptr, tr = nullptr;
 };
+
+// FP64 hex Helmholtz with its input gathered from a global vector: out = H(local), where
+// local[i] = gsign[i] * glob[gmap[i]] (GlobalToLocal fused into the kernel's load).
+// Library-internal (used by the assembled solver); defined in mf_capi.hip.
+int imf_apply_gather_device(imf_op_s *op, const double *glob, const int *gmap,
+                            const signed char *gsign, double *out, hipStream_t stream);

@@ -118,6 +118,10 @@ int imf_apply_many_device(imf_op op, int nvec, const void *d_in, long ld_in, voi
 const char *imf_last_error(void);
 
 /* ---- C0 global assembly and the matrix-free Helmholtz solve (hexahedra, quads) -------
+ * Replaces AssemblyMapCG's local-to-global map and signs (MultiRegions/AssemblyMap/
+ * AssemblyMapCG.cpp:1332-1900; edge and face signs at :1786, :1832).  Its global order
+ * (graph-based, bandwidth-reducing) differs; the map's contract is the same: continuity
+ * across elements, with the signs of the modes' orientation.
  * Global numbering of the modified (C0) basis built from the elements' vertex ids
  * (vertex_ids[e*8 + v], any 64-bit ids, vertex order of the reference hex): vertex modes
  * shared by id, edge-interior modes by the vertex pair with sign (-1)^m against the global

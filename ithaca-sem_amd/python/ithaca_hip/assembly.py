@@ -75,6 +75,112 @@ def assembly_map_quad(vertex_ids, nm):
     return assembly_map(vertex_ids, nm, "quad")
 
 
+# ---------------------------------------------------------------- Dirichlet data
+
+def _tables(nm, nq):
+    from . import basis_modified_a
+    b = basis_modified_a(nm, nq)
+    return np.asarray(b["z"]), np.asarray(b["w"]), np.asarray(b["bdata"]).reshape(nm, nq).T
+
+
+def _constrained_1d(vals, B, w):
+    """StdSegExp::v_FwdTrans_BndConstrained (StdRegions/StdSegExp.cpp:319-390): vertex
+    modes = end values, interior modes by the interior block of the mass matrix with the
+    vertex part removed."""
+    nm = B.shape[1]
+    c = np.zeros(nm)
+    c[0], c[1] = vals[0], vals[-1]
+    if nm > 2:
+        M1 = B.T @ (w[:, None] * B)
+        rhs = B.T @ (w * vals) - M1[:, :2] @ c[:2]
+        c[2:] = np.linalg.solve(M1[2:, 2:], rhs[2:])
+    return c
+
+
+def _constrained_2d(F, B, w):
+    """StdQuadExp::v_FwdTrans_BndConstrained (StdRegions/StdQuadExp.cpp:260-354) on a face
+    grid F[j, i] (i along the first tangential direction): vertices, then each edge by the
+    1D constrained projection, then the interior by the interior mass block.  Returns
+    c[q, p].  The face Jacobian of a straight-sided (parallelogram) face is constant and
+    cancels."""
+    nm = B.shape[1]
+    c = np.zeros((nm, nm))
+    c[0, :] = _constrained_1d(F[0, :], B, w)
+    c[1, :] = _constrained_1d(F[-1, :], B, w)
+    c[:, 0] = _constrained_1d(F[:, 0], B, w)
+    c[:, 1] = _constrained_1d(F[:, -1], B, w)
+    if nm > 2:
+        B2 = np.kron(B, B)
+        W2 = np.kron(w, w)
+        M2 = B2.T @ (W2[:, None] * B2)
+        it = np.zeros((nm, nm), dtype=bool)
+        it[2:, 2:] = True
+        it = it.ravel()
+        cb = c.ravel().copy()
+        cb[it] = 0.0
+        rhs = B2.T @ (W2 * F.ravel()) - M2 @ cb
+        cb[it] = np.linalg.solve(M2[np.ix_(it, it)], rhs[it])
+        c = cb.reshape(nm, nm)
+    return c
+
+
+def _points(coords, z):
+    """Physical GLL points of straight-sided elements: (E, nq, nq[, nq], dim), index
+    [k][j][i] (hex) or [j][i] (quad), from the vertices in local order."""
+    coords = np.asarray(coords, dtype=np.float64)
+    nv, dim = coords.shape[1], coords.shape[2]
+    lo, hi = 0.5 * (1.0 - z), 0.5 * (1.0 + z)
+    if nv == 8:
+        corners = [(0, 0, 0), (1, 0, 0), (1, 1, 0), (0, 1, 0),
+                   (0, 0, 1), (1, 0, 1), (1, 1, 1), (0, 1, 1)]
+        P = 0.0
+        for v, (a, b, c) in enumerate(corners):
+            f = np.einsum("k,j,i->kji", hi if c else lo, hi if b else lo, hi if a else lo)
+            P = P + f[None, ..., None] * coords[:, v, None, None, None, :]
+        return P
+    corners = [(0, 0), (1, 0), (1, 1), (0, 1)]
+    P = 0.0
+    for v, (a, b) in enumerate(corners):
+        f = np.einsum("j,i->ji", hi if b else lo, hi if a else lo)
+        P = P + f[None, ..., None] * coords[:, v, None, None, :]
+    return P
+
+
+def dirichlet_lift(coords, vertex_ids, nm, fn, nq=None):
+    """Global vector of Dirichlet values for the boundary modes of a straight-sided hex
+    (coords (E, 8, 3)) or quad (coords (E, 4, 2)) mesh, zero elsewhere: fn(points (n, dim))
+    -> values, projected on every boundary face (edge) as the reference projects Dirichlet
+    conditions (DisContField.cpp:3630-3663: FwdTrans_BndConstrained of the boundary
+    expansion), on the element's own GLL points (nq = nm + 1 by default, the reference's
+    TYPE="MODIFIED").  Host side (numpy); pass the result as `lift` to helmsolve."""
+    coords = np.asarray(coords, dtype=np.float64)
+    nq = nm + 1 if nq is None else nq
+    shape = "hex" if coords.shape[1] == 8 else "quad"
+    z, w, B = _tables(nm, nq)
+    l2g, sign, bnd, ng = assembly_map(vertex_ids, nm, shape)
+    g = np.zeros(ng)
+    P = _points(coords, z)
+    E = coords.shape[0]
+    dim = 3 if shape == "hex" else 2
+    M = nm ** dim
+    loc = np.arange(M).reshape((nm,) * dim)  # [r][q][p] or [q][p]
+    for e in range(E):
+        U = fn(P[e].reshape(-1, coords.shape[2])).reshape((nq,) * dim)
+        for d in range(dim):              # normal direction of the face (edge)
+            for side in range(2):
+                sl = [slice(None)] * dim
+                sl[dim - 1 - d] = 0 if side == 0 else nq - 1
+                ml = [slice(None)] * dim
+                ml[dim - 1 - d] = side
+                li = e * M + loc[tuple(ml)].ravel()
+                if not np.all(bnd[li]):
+                    continue
+                F = U[tuple(sl)]
+                c = _constrained_2d(F, B, w) if dim == 3 else _constrained_1d(F, B, w)
+                g[l2g[li]] = sign[li] * c.ravel()
+    return g
+
+
 class Assembly:
     """Device assembly map of a hex mesh (vertex_ids (E, 8)) or a quad mesh (shape="quad",
     vertex_ids (E, 4)); dirichlet=True fixes every mode on the boundary faces (edges) to
@@ -111,13 +217,21 @@ class Assembly:
                                           ctypes.c_void_p(out.data_ptr()), _stream_ptr(stream)))
         return out
 
-    def helmsolve(self, op, rhs, x=None, tol=1e-9, max_iter=10000, stream=None):
-        """x with Assemble(H(GlobalToLocal(x))) = rhs (Dirichlet modes zero); returns
-        (x, iterations, relative residual)."""
+    def helmsolve(self, op, rhs, x=None, tol=1e-9, max_iter=10000, stream=None, lift=None):
+        """x with Assemble(H(GlobalToLocal(x))) = rhs on the free modes; returns
+        (x, iterations, relative residual).  Dirichlet modes are zero, or, with `lift` (a
+        global vector holding the Dirichlet values, e.g. from dirichlet_lift), equal to
+        lift: the solve is then for x - lift with rhs - A lift, as the reference's
+        ContField::HelmSolve treats inhomogeneous Dirichlet conditions."""
+        if lift is not None:
+            lift = torch.as_tensor(lift, dtype=torch.float64, device=rhs.device)
+            rhs = rhs - self.assemble(op.apply(self.global_to_local(lift)))
         x = torch.empty_like(rhs) if x is None else x
         it, res = ctypes.c_int(), ctypes.c_double()
         _check(_lib().imf_helmsolve_cg_device(self._h, op._h, ctypes.c_void_p(rhs.data_ptr()),
                                               ctypes.c_void_p(x.data_ptr()), float(tol),
                                               int(max_iter), ctypes.byref(it), ctypes.byref(res),
                                               _stream_ptr(stream)))
+        if lift is not None:
+            x += lift
         return x, it.value, res.value

@@ -189,3 +189,49 @@ def test_quad_manufactured_solution_converges():
     print("L2 errors", errs)
     assert errs[-1] < 1e-6
     assert all(errs[i + 1] < 0.5 * errs[i] for i in range(len(errs) - 1))
+
+
+# ------------------------------------------------ the reference's solver-level golden
+
+def test_helmholtz3d_hex_golden():
+    """Demos/MultiRegions/Tests/Helmholtz3D_CG_Hex.tst: the session
+    Helmholtz3D_Hex_P6.xml (8 hexes on the unit cube, NUMMODES 6, lambda = 1, Dirichlet
+    u = sin(pi x/2) sin(pi y/2) sin(pi z/2) on every boundary face, forcing
+    -(lambda + 3 pi^2/4) u) solved by the reference's Helmholtz3D gives L2 error 1.11818e-07
+    and Linf error 5.66374e-07 (tolerance 1e-12).  Here: the mesh read with nektar_io, the
+    C0 map, the Dirichlet values by the boundary-constrained projection of
+    assembly.dirichlet_lift, dense element matrices and a sparse direct solve."""
+    import os
+    from ithaca_hip import nektar_io
+    gold = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "nektar")
+    mesh = nektar_io.read_mesh(os.path.join(gold, "Helmholtz3D_Hex_P6.xml"))
+    X, ids = mesh.element_coords(), mesh.vertex_ids()
+    nm = mesh.expansions[0]["nummodes"]
+    nq, M, lam = nm + 1, nm ** 3, 1.0
+    uex = lambda P: np.sin(np.pi / 2 * P[:, 0]) * np.sin(np.pi / 2 * P[:, 1]) * np.sin(np.pi / 2 * P[:, 2])
+    g = assembly.dirichlet_lift(X, ids, nm, uex)
+    l2g, sign, bnd, ng = assembly.assembly_map_hex(ids, nm)
+    dmask = np.zeros(ng, dtype=bool)
+    dmask[np.unique(l2g[bnd == 1])] = True
+    assert np.all(g[~dmask] == 0)
+    mats = [DR.element_mats("hex", nm, nq, X[e]) for e in range(len(ids))]
+    rows, cols, vals, rhs = [], [], [], np.zeros(ng)
+    for e, m in enumerate(mats):
+        gi, s = l2g[e * M:(e + 1) * M], sign[e * M:(e + 1) * M].astype(float)
+        rows.append(np.repeat(gi, M))
+        cols.append(np.tile(gi, M))
+        vals.append((DR.helmholtz_matrix(m, lam) * np.outer(s, s)).ravel())
+        np.add.at(rhs, gi, s * (m["B"].T @ (m["W"] * m["J"] * (lam + 3 * np.pi ** 2 / 4) * uex(m["X"]))))
+    A = sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))),
+                      shape=(ng, ng))
+    x = g.copy()
+    free = ~dmask
+    x[free] = spla.spsolve(A[free][:, free].tocsc(), rhs[free] - A[free][:, dmask] @ g[dmask])
+    l2 = linf = 0.0
+    for e, m in enumerate(mats):
+        d = m["B"] @ (sign[e * M:(e + 1) * M] * x[l2g[e * M:(e + 1) * M]]) - uex(m["X"])
+        l2 += np.sum(m["W"] * m["J"] * d * d)
+        linf = max(linf, np.abs(d).max())
+    print(f"L2 {np.sqrt(l2):.6e}  Linf {linf:.6e}")
+    assert abs(np.sqrt(l2) - 1.11818e-07) <= 1e-12
+    assert abs(linf - 5.66374e-07) <= 1e-12

@@ -200,3 +200,37 @@ def test_smoothing_a_nektar_field():
     x, it, res = A.helmsolve(Hop, A.assemble(ipr.apply(lam * phys)), tol=1e-13)
     print(f"iterations {it}, residual {res:.2e}")
     assert np.linalg.norm(x.cpu().numpy() - x_cpu) <= 1e-10 * np.linalg.norm(x_cpu)
+
+
+def test_helmholtz3d_hex_golden_on_the_gpu():
+    """The reference's Demos/MultiRegions/Tests/Helmholtz3D_CG_Hex.tst (L2 error
+    1.11818e-07, Linf 5.66374e-07, tolerance 1e-12) through the GPU path: geometry,
+    IProduct of the forcing, assembly, the inhomogeneous Dirichlet lift and CG on the GPU,
+    errors at the GLL points from the GPU BwdTrans."""
+    gold = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "nektar")
+    msh = N.read_mesh(os.path.join(gold, "Helmholtz3D_Hex_P6.xml"))
+    X, ids = msh.element_coords(), msh.vertex_ids()
+    nm = msh.expansions[0]["nummodes"]
+    nq, E, lam = nm + 1, len(ids), 1.0
+    uex = lambda P: np.sin(np.pi / 2 * P[:, 0]) * np.sin(np.pi / 2 * P[:, 1]) * np.sin(np.pi / 2 * P[:, 2])
+    b = H.basis_modified_a(nm, nq)
+    jac, df = H.geom_device("hex", _dev(X), nq, b["z"], True)
+    Hop = H.Operator("Helmholtz_Hex_Deformed", nm, nq, E, basis=b)
+    Hop.set_jac(jac)
+    Hop.set_df(df)
+    Hop.set_lambda(lam)
+    ipr = H.Operator("IProduct_Hex_Deformed", nm, nq, E, basis=b)
+    ipr.set_jac(jac)
+    bwd = H.Operator("BwdTrans_Hex_Regular", nm, nq, E, basis=b)
+    P = mesh.hex_quad_points(_dev(X), b["z"]).reshape(-1, 3).cpu().numpy()
+    A = assembly.Assembly(ids, nm, dirichlet=True)
+    rhs = A.assemble(ipr.apply(_dev((lam + 3 * np.pi ** 2 / 4) * uex(P))))
+    g = assembly.dirichlet_lift(X, ids, nm, uex)
+    x, it, res = A.helmsolve(Hop, rhs, tol=1e-14, max_iter=5000, lift=g)
+    d = bwd.apply(A.global_to_local(x)).cpu().numpy() - uex(P)
+    wq = np.einsum("k,j,i->kji", b["w"], b["w"], b["w"]).ravel()
+    wj = jac.cpu().numpy() * np.tile(wq, E)
+    l2, linf = np.sqrt(np.sum(wj * d * d)), np.abs(d).max()
+    print(f"iterations {it}, residual {res:.1e}, L2 {l2:.6e}, Linf {linf:.6e}")
+    assert abs(l2 - 1.11818e-07) <= 1e-12
+    assert abs(linf - 5.66374e-07) <= 1e-12

@@ -149,6 +149,19 @@ int imf_assembly_create_hex(int nElmt, int nm, const long long *vertex_ids, int 
                             imf_asm *assembly);
 int imf_assembly_create_quad(int nElmt, int nm, const long long *vertex_ids, int dirichlet,
                              imf_asm *assembly);
+
+/* Variable order (dim = 3: hexahedra, dim = 2: quadrilaterals): element e has nm[e] modes
+ * per direction (nm[e] >= 2) and its local coefficients start at sum_{f<e} nm[f]^dim (the
+ * reference's ExpList coefficient offsets).  As AssemblyMapCG.cpp:1380-1440 and :1771-1791
+ * do for modal expansions of different order, a shared edge carries the fewest interior
+ * modes of the elements around it and a shared face the fewest of its two elements; an
+ * element's modes beyond those get l2g = 0 and sign = 0, so they take no part in the global
+ * system (GlobalToLocal sets them to zero, Assemble skips them, on_boundary is 0 for them).
+ * With every nm[e] equal these are the maps of the uniform calls above. */
+int imf_assembly_map_varp(int dim, int nElmt, const int *nm, const long long *vertex_ids,
+                          int *l2g, signed char *sign, unsigned char *on_boundary, long *nglobal);
+int imf_assembly_create_varp(int dim, int nElmt, const int *nm, const long long *vertex_ids,
+                             int dirichlet, imf_asm *assembly);
 long imf_assembly_nglobal(imf_asm assembly);
 int imf_assembly_destroy(imf_asm assembly);
 
@@ -175,6 +188,15 @@ int imf_assemble_device(imf_asm assembly, const double *d_local, double *d_globa
 int imf_helmsolve_cg_device(imf_asm assembly, imf_op helmholtz, const double *d_rhs,
                             double *d_x, double tol, int max_iter, int *iterations,
                             double *residual, void *stream);
+
+/* The same solve with one operator per run of equal-order elements, as the reference starts
+ * a new Collection where the element's coefficient or point count changes
+ * (ExpList::CreateCollections, MultiRegions/ExpList.cpp:5005-5110): helmholtz[g] acts on the
+ * next helmholtz[g]'s nElmt elements, in element order, and must have their modes; together
+ * they cover the assembly's elements. */
+int imf_helmsolve_cg_groups_device(imf_asm assembly, int nops, const imf_op *helmholtz,
+                                   const double *d_rhs, double *d_x, double tol, int max_iter,
+                                   int *iterations, double *residual, void *stream);
 
 /* Message of the last failing assembly call on this thread. */
 const char *imf_assembly_last_error(void);

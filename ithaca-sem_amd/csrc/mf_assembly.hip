@@ -60,18 +60,29 @@ struct H4 {
     }
 };
 
-// l2g[e*M + m] = global index, sgn = +-1, bnd[e*M + m] = 1 if the local mode lies on a face
-// owned by one element only.
-int build_map_hex(int E, int nm, const long long *vid, std::vector<int> &l2g,
+// l2g[loff[e] + m] = global index, sgn = +-1, bnd = 1 if the local mode lies on a face owned
+// by one element only.  Element e has nmv[e] modes per direction and its coefficients start
+// at loff[e] = sum_{f<e} nmv[f]^3 (ExpList's coefficient offsets).  Variable order follows
+// AssemblyMapCG.cpp:1380-1440 and :1771-1791: an edge carries the fewest interior modes of
+// the elements that share it, a face the fewest of its two elements, and an element's modes
+// beyond those map to global 0 with sign 0 (they take no part in the global system and are
+// not marked as boundary modes).
+int build_map_hex(int E, const int *nmv, const long long *vid, std::vector<int> &l2g,
                   std::vector<signed char> &sgn, std::vector<unsigned char> &bnd, long &nglobal)
 {
-    if (E <= 0 || nm < 2 || !vid) return afail(IMF_ERR_ARG, "assembly: bad argument");
-    const int M = nm * nm * nm, ni = nm - 2;
+    if (E <= 0 || !nmv || !vid) return afail(IMF_ERR_ARG, "assembly: bad argument");
+    std::vector<long> loff(E + 1, 0);
+    for (long e = 0; e < E; ++e) {
+        if (nmv[e] < 2) return afail(IMF_ERR_ARG, "assembly: fewer than 2 modes on an element");
+        loff[e + 1] = loff[e] + (long)nmv[e] * nmv[e] * nmv[e];
+    }
     std::unordered_map<long long, int> vmap;
     std::unordered_map<std::array<long long, 2>, int, H2> emap;
-    std::unordered_map<std::array<long long, 4>, std::array<int, 2>, H4> fmap;  // id, uses
+    std::unordered_map<std::array<long long, 4>, int, H4> fmap;
+    std::vector<int> ecnt, fcnt, fuse;  // per id: interior modes (per direction), users
     for (long e = 0; e < E; ++e) {
         const long long *v = vid + e * 8;
+        const int ni = nmv[e] - 2;
         for (int a = 0; a < 8; ++a) vmap.emplace(v[a], (int)vmap.size());
         for (int d = 0; d < 3; ++d)
             for (int c = 0; c < 4; ++c) {  // the 4 edges along direction d
@@ -85,8 +96,10 @@ int build_map_hex(int E, int nm, const long long *vid, std::vector<int> &l2g,
                 long long a0 = v[vert_of(bits[0], bits[1], bits[2])];
                 long long a1 = v[vert_of(bb[0], bb[1], bb[2])];
                 if (a0 == a1) return afail(IMF_ERR_ARG, "assembly: degenerate edge");
-                emap.emplace(std::array<long long, 2>{std::min(a0, a1), std::max(a0, a1)},
-                             (int)emap.size());
+                auto ins = emap.emplace(std::array<long long, 2>{std::min(a0, a1), std::max(a0, a1)},
+                                        (int)emap.size());
+                if (ins.second) ecnt.push_back(ni);
+                else ecnt[ins.first->second] = std::min(ecnt[ins.first->second], ni);
             }
         for (int df = 0; df < 3; ++df)
             for (int s = 0; s < 2; ++s) {
@@ -100,26 +113,42 @@ int build_map_hex(int E, int nm, const long long *vid, std::vector<int> &l2g,
                     k[n++] = v[vert_of(bits[0], bits[1], bits[2])];
                 }
                 std::sort(k.begin(), k.end());
-                auto it = fmap.find(k);
-                if (it == fmap.end()) fmap.emplace(k, std::array<int, 2>{(int)fmap.size(), 1});
-                else if (++it->second[1] > 2)
-                    return afail(IMF_ERR_ARG, "assembly: a face is shared by more than two elements");
+                auto ins = fmap.emplace(k, (int)fmap.size());
+                if (ins.second) {
+                    fcnt.push_back(ni);
+                    fuse.push_back(1);
+                } else {
+                    const int id = ins.first->second;
+                    fcnt[id] = std::min(fcnt[id], ni);
+                    if (++fuse[id] > 2)
+                        return afail(IMF_ERR_ARG, "assembly: a face is shared by more than two elements");
+                }
             }
     }
     const long nV = (long)vmap.size(), nE = (long)emap.size(), nF = (long)fmap.size();
-    const long baseE = nV, baseF = baseE + nE * ni, baseI = baseF + nF * (long)ni * ni;
-    nglobal = baseI + (long)E * ni * ni * ni;
+    std::vector<long> eoff(nE + 1), foff(nF + 1), ioff(E + 1);
+    eoff[0] = nV;
+    for (long i = 0; i < nE; ++i) eoff[i + 1] = eoff[i] + ecnt[i];
+    foff[0] = eoff[nE];
+    for (long i = 0; i < nF; ++i) foff[i + 1] = foff[i] + (long)fcnt[i] * fcnt[i];
+    ioff[0] = foff[nF];
+    for (long e = 0; e < E; ++e) {
+        const long ni = nmv[e] - 2;
+        ioff[e + 1] = ioff[e] + ni * ni * ni;
+    }
+    nglobal = ioff[E];
     if (nglobal > 2147483647L) return afail(IMF_ERR_ARG, "assembly: more than 2^31 global modes");
-    l2g.assign((size_t)E * M, 0);
-    sgn.assign((size_t)E * M, 1);
-    bnd.assign((size_t)E * M, 0);
+    l2g.assign((size_t)loff[E], 0);
+    sgn.assign((size_t)loff[E], 1);
+    bnd.assign((size_t)loff[E], 0);
     for (long e = 0; e < E; ++e) {
         const long long *v = vid + e * 8;
+        const int nm = nmv[e], ni = nm - 2;
         for (int r = 0; r < nm; ++r)
             for (int q = 0; q < nm; ++q)
                 for (int p = 0; p < nm; ++p) {
                     const int m[3] = {p, q, r};
-                    const size_t li = (size_t)e * M + (r * nm + q) * nm + p;
+                    const size_t li = (size_t)loff[e] + (r * nm + q) * nm + p;
                     int nv = 0;
                     for (int d = 0; d < 3; ++d) nv += m[d] < 2;
                     long gi = 0;
@@ -135,8 +164,12 @@ int build_map_hex(int E, int nm, const long long *vid, std::vector<int> &l2g,
                         const long long g0 = v[vert_of(b0[0], b0[1], b0[2])];
                         const long long g1 = v[vert_of(b1[0], b1[1], b1[2])];
                         const int idx = emap[{std::min(g0, g1), std::max(g0, g1)}];
-                        gi = baseE + (long)idx * ni + (m[di] - 2);
-                        if (g0 > g1 && (m[di] & 1)) sg = -1;
+                        if (m[di] - 2 < ecnt[idx]) {
+                            gi = eoff[idx] + (m[di] - 2);
+                            if (g0 > g1 && (m[di] & 1)) sg = -1;
+                        } else {
+                            sg = 0;
+                        }
                     } else if (nv == 1) {
                         int df = 0;
                         while (m[df] >= 2) ++df;
@@ -153,7 +186,7 @@ int build_map_hex(int E, int nm, const long long *vid, std::vector<int> &l2g,
                                 k[c1 * 2 + c2] = g[c1][c2];
                             }
                         std::sort(k.begin(), k.end());
-                        const int idx = fmap[k][0];
+                        const int idx = fmap[k], nf = fcnt[idx];
                         int o1 = 0, o2 = 0;
                         for (int c1 = 0; c1 < 2; ++c1)
                             for (int c2 = 0; c2 < 2; ++c2)
@@ -164,11 +197,15 @@ int build_map_hex(int E, int nm, const long long *vid, std::vector<int> &l2g,
                         const bool tr = g[o1][1 - o2] < g[1 - o1][o2];  // first axis along d2
                         const int mA = tr ? m[d2] : m[d1], mB = tr ? m[d1] : m[d2];
                         const bool fA = tr ? o2 == 1 : o1 == 1, fB = tr ? o1 == 1 : o2 == 1;
-                        gi = baseF + (long)idx * ni * ni + (long)(mA - 2) * ni + (mB - 2);
-                        if (fA && (mA & 1)) sg = -sg;
-                        if (fB && (mB & 1)) sg = -sg;
+                        if (mA - 2 < nf && mB - 2 < nf) {
+                            gi = foff[idx] + (long)(mA - 2) * nf + (mB - 2);
+                            if (fA && (mA & 1)) sg = -sg;
+                            if (fB && (mB & 1)) sg = -sg;
+                        } else {
+                            sg = 0;
+                        }
                     } else {
-                        gi = baseI + ((e * ni + (r - 2)) * (long)ni + (q - 2)) * ni + (p - 2);
+                        gi = ioff[e] + ((long)(r - 2) * ni + (q - 2)) * ni + (p - 2);
                     }
                     l2g[li] = (int)gi;
                     sgn[li] = (signed char)sg;
@@ -186,12 +223,13 @@ int build_map_hex(int E, int nm, const long long *vid, std::vector<int> &l2g,
                     k[c] = v[vert_of(bits[0], bits[1], bits[2])];
                 }
                 std::sort(k.begin(), k.end());
-                if (fmap[k][1] != 1) continue;
+                if (fuse[fmap[k]] != 1) continue;
                 for (int r = 0; r < nm; ++r)
                     for (int q = 0; q < nm; ++q)
                         for (int p = 0; p < nm; ++p) {
                             const int m[3] = {p, q, r};
-                            if (m[df] == s) bnd[(size_t)e * M + (r * nm + q) * nm + p] = 1;
+                            const size_t li = (size_t)loff[e] + (r * nm + q) * nm + p;
+                            if (m[df] == s && sgn[li] != 0) bnd[li] = 1;
                         }
             }
     }
@@ -204,12 +242,17 @@ inline int vert2(int b0, int b1) { return vert_of(b0, b1, 0); }
 
 // The quad analogue of build_map_hex: vertex, edge ((-1)^m sign when the element runs the
 // edge against the min -> max id direction) and interior modes; boundary = edges with one
-// user.
-int build_map_quad(int E, int nm, const long long *vid, std::vector<int> &l2g,
+// user; variable order as above (an edge carries the fewest interior modes of its two
+// elements).
+int build_map_quad(int E, const int *nmv, const long long *vid, std::vector<int> &l2g,
                    std::vector<signed char> &sgn, std::vector<unsigned char> &bnd, long &nglobal)
 {
-    if (E <= 0 || nm < 2 || !vid) return afail(IMF_ERR_ARG, "assembly: bad argument");
-    const int M = nm * nm, ni = nm - 2;
+    if (E <= 0 || !nmv || !vid) return afail(IMF_ERR_ARG, "assembly: bad argument");
+    std::vector<long> loff(E + 1, 0);
+    for (long e = 0; e < E; ++e) {
+        if (nmv[e] < 2) return afail(IMF_ERR_ARG, "assembly: fewer than 2 modes on an element");
+        loff[e + 1] = loff[e] + (long)nmv[e] * nmv[e];
+    }
     // end points of the edge where coordinate dn is fixed at s (running along 1 - dn)
     auto ends = [&](const long long *v, int dn, int s, long long &g0, long long &g1) {
         int b0[2], b1[2];
@@ -220,7 +263,8 @@ int build_map_quad(int E, int nm, const long long *vid, std::vector<int> &l2g,
         g1 = v[vert2(b1[0], b1[1])];
     };
     std::unordered_map<long long, int> vmap;
-    std::unordered_map<std::array<long long, 2>, std::array<int, 2>, H2> emap;  // id, uses
+    std::unordered_map<std::array<long long, 2>, int, H2> emap;
+    std::vector<int> ecnt, euse;
     for (long e = 0; e < E; ++e) {
         const long long *v = vid + e * 4;
         for (int a = 0; a < 4; ++a) vmap.emplace(v[a], (int)vmap.size());
@@ -230,25 +274,36 @@ int build_map_quad(int E, int nm, const long long *vid, std::vector<int> &l2g,
                 ends(v, dn, sd, g0, g1);
                 if (g0 == g1) return afail(IMF_ERR_ARG, "assembly: degenerate edge");
                 const std::array<long long, 2> k{std::min(g0, g1), std::max(g0, g1)};
-                auto it = emap.find(k);
-                if (it == emap.end()) emap.emplace(k, std::array<int, 2>{(int)emap.size(), 1});
-                else if (++it->second[1] > 2)
-                    return afail(IMF_ERR_ARG, "assembly: an edge is shared by more than two elements");
+                auto ins = emap.emplace(k, (int)emap.size());
+                if (ins.second) {
+                    ecnt.push_back(nmv[e] - 2);
+                    euse.push_back(1);
+                } else {
+                    const int id = ins.first->second;
+                    ecnt[id] = std::min(ecnt[id], nmv[e] - 2);
+                    if (++euse[id] > 2)
+                        return afail(IMF_ERR_ARG, "assembly: an edge is shared by more than two elements");
+                }
             }
     }
     const long nV = (long)vmap.size(), nE = (long)emap.size();
-    const long baseE = nV, baseI = baseE + nE * ni;
-    nglobal = baseI + (long)E * ni * ni;
+    std::vector<long> eoff(nE + 1), ioff(E + 1);
+    eoff[0] = nV;
+    for (long i = 0; i < nE; ++i) eoff[i + 1] = eoff[i] + ecnt[i];
+    ioff[0] = eoff[nE];
+    for (long e = 0; e < E; ++e) ioff[e + 1] = ioff[e] + (long)(nmv[e] - 2) * (nmv[e] - 2);
+    nglobal = ioff[E];
     if (nglobal > 2147483647L) return afail(IMF_ERR_ARG, "assembly: more than 2^31 global modes");
-    l2g.assign((size_t)E * M, 0);
-    sgn.assign((size_t)E * M, 1);
-    bnd.assign((size_t)E * M, 0);
+    l2g.assign((size_t)loff[E], 0);
+    sgn.assign((size_t)loff[E], 1);
+    bnd.assign((size_t)loff[E], 0);
     for (long e = 0; e < E; ++e) {
         const long long *v = vid + e * 4;
+        const int nm = nmv[e], ni = nm - 2;
         for (int q = 0; q < nm; ++q)
             for (int p = 0; p < nm; ++p) {
                 const int m[2] = {p, q};
-                const size_t li = (size_t)e * M + q * nm + p;
+                const size_t li = (size_t)loff[e] + q * nm + p;
                 const int nv = (p < 2) + (q < 2);
                 long gi = 0;
                 int sg = 1;
@@ -258,11 +313,15 @@ int build_map_quad(int E, int nm, const long long *vid, std::vector<int> &l2g,
                     const int di = p >= 2 ? 0 : 1, dn = 1 - di;
                     long long g0, g1;
                     ends(v, dn, m[dn], g0, g1);
-                    const int idx = emap[{std::min(g0, g1), std::max(g0, g1)}][0];
-                    gi = baseE + (long)idx * ni + (m[di] - 2);
-                    if (g0 > g1 && (m[di] & 1)) sg = -1;
+                    const int idx = emap[{std::min(g0, g1), std::max(g0, g1)}];
+                    if (m[di] - 2 < ecnt[idx]) {
+                        gi = eoff[idx] + (m[di] - 2);
+                        if (g0 > g1 && (m[di] & 1)) sg = -1;
+                    } else {
+                        sg = 0;
+                    }
                 } else {
-                    gi = baseI + (e * ni + (q - 2)) * (long)ni + (p - 2);
+                    gi = ioff[e] + (long)(q - 2) * ni + (p - 2);
                 }
                 l2g[li] = (int)gi;
                 sgn[li] = (signed char)sg;
@@ -271,20 +330,22 @@ int build_map_quad(int E, int nm, const long long *vid, std::vector<int> &l2g,
             for (int sd = 0; sd < 2; ++sd) {
                 long long g0, g1;
                 ends(v, dn, sd, g0, g1);
-                if (emap[{std::min(g0, g1), std::max(g0, g1)}][1] != 1) continue;
+                if (euse[emap[{std::min(g0, g1), std::max(g0, g1)}]] != 1) continue;
                 for (int q = 0; q < nm; ++q)
-                    for (int p = 0; p < nm; ++p)
-                        if ((dn == 0 ? p : q) == sd) bnd[(size_t)e * M + q * nm + p] = 1;
+                    for (int p = 0; p < nm; ++p) {
+                        const size_t li = (size_t)loff[e] + q * nm + p;
+                        if ((dn == 0 ? p : q) == sd && sgn[li] != 0) bnd[li] = 1;
+                    }
             }
     }
     return IMF_OK;
 }
 
-int build_map(int shape, int E, int nm, const long long *vid, std::vector<int> &l2g,
+int build_map(int shape, int E, const int *nmv, const long long *vid, std::vector<int> &l2g,
               std::vector<signed char> &sgn, std::vector<unsigned char> &bnd, long &nglobal)
 {
-    return shape == imf::kHex ? build_map_hex(E, nm, vid, l2g, sgn, bnd, nglobal)
-                              : build_map_quad(E, nm, vid, l2g, sgn, bnd, nglobal);
+    return shape == imf::kHex ? build_map_hex(E, nmv, vid, l2g, sgn, bnd, nglobal)
+                              : build_map_quad(E, nmv, vid, l2g, sgn, bnd, nglobal);
 }
 
 // --------------------------------------------------------------------------- kernels
@@ -536,8 +597,10 @@ constexpr long kAsmParts = 4096;
 }  // namespace
 
 struct imf_asm_s {
-    int shape = 0, nElmt = 0, nm = 0, dirichlet = 0;
+    int shape = 0, nElmt = 0, nm = 0, dirichlet = 0;  // nm = 0: orders vary by element
     long nloc = 0, ng = 0;
+    std::vector<int> nmv;    // modes per direction of each element
+    std::vector<long> loff;  // first local coefficient of each element (nElmt + 1)
     int *d_l2g = nullptr;
     signed char *d_sg = nullptr;
     int *d_off = nullptr, *d_inv = nullptr;  // inverse map (CSR), 32-bit
@@ -545,8 +608,8 @@ struct imf_asm_s {
     // solver workspace (allocated on the first solve)
     double *d_ws = nullptr, *d_loc = nullptr, *d_part = nullptr, *d_dots = nullptr;
     int nparts = 0;
-    const void *diag_op = nullptr;  // operator and lambda the cached diagonal belongs to
-    double diag_lambda = 0.0;
+    // operators and lambdas the cached diagonal belongs to
+    std::vector<std::pair<const void *, double>> diag_key;
 };
 
 namespace {
@@ -558,16 +621,23 @@ namespace {
     } while (0)
 
 // out = mask0(Assemble(H(GlobalToLocal(w))))  (GlobalLinSysIterativeFull::v_DoMatrixMultiply)
-// with the partial sums of (r,w), (out,w), (r,r) taken in the assembly pass.
-int apply_global_dots(imf_asm a, imf_op op, const double *w, const double *r, double *out,
-                      hipStream_t s)
+// with the partial sums of (r,w), (out,w), (r,r) taken in the assembly pass.  ops[g] acts on
+// the next ops[g]->nElmt elements (one operator per order, as the reference keeps one
+// collection per expansion type).
+int apply_global_dots(imf_asm a, int nops, const imf_op *ops, const double *w, const double *r,
+                      double *out, hipStream_t s)
 {
     double *l0 = a->d_loc, *l1 = a->d_loc + a->nloc;
-    if (a->shape == imf::kHex) {  // GlobalToLocal fused into the Helmholtz kernel's load
-        if (int st = imf_apply_gather_device(op, w, a->d_l2g, a->d_sg, l1, s)) return st;
+    if (nops == 1 && a->shape == imf::kHex) {  // GlobalToLocal fused into the kernel's load
+        if (int st = imf_apply_gather_device(ops[0], w, a->d_l2g, a->d_sg, l1, s)) return st;
     } else {
         g2l_kernel<<<nblk(a->nloc), RB, 0, s>>>(a->nloc, a->d_l2g, a->d_sg, w, l0);
-        if (int st = imf_apply_device(op, l0, l1, s)) return st;
+        long e0 = 0;
+        for (int g = 0; g < nops; ++g) {
+            const long o = a->loff[e0];
+            if (int st = imf_apply_device(ops[g], l0 + o, l1 + o, s)) return st;
+            e0 += ops[g]->nElmt;
+        }
     }
     const int nb = (int)std::min<long>(kAsmParts, nblk(a->ng));
     assemble_dot3_kernel<<<nb, RB, 0, s>>>(a->ng, a->d_off, a->d_inv, a->d_sg, l1, a->d_mask, r,
@@ -583,7 +653,7 @@ extern "C" {
 
 const char *imf_assembly_last_error(void) { return g_aerr.c_str(); }
 
-static int map_any(int shape, int nElmt, int nm, const long long *vertex_ids, int *l2g,
+static int map_any(int shape, int nElmt, const int *nm, const long long *vertex_ids, int *l2g,
                    signed char *sign, unsigned char *on_boundary, long *nglobal)
 {
     std::vector<int> m;
@@ -598,8 +668,8 @@ static int map_any(int shape, int nElmt, int nm, const long long *vertex_ids, in
     return IMF_OK;
 }
 
-static int create_any(int shape, int nElmt, int nm, const long long *vertex_ids, int dirichlet,
-                      imf_asm *out)
+static int create_any(int shape, int nElmt, const int *nm, const long long *vertex_ids,
+                      int dirichlet, imf_asm *out)
 {
     if (!out) return afail(IMF_ERR_ARG, "imf_assembly_create: null output");
     std::vector<int> m;
@@ -612,21 +682,31 @@ static int create_any(int shape, int nElmt, int nm, const long long *vertex_ids,
     auto *a = new imf_asm_s();
     a->shape = shape;
     a->nElmt = nElmt;
-    a->nm = nm;
+    a->nmv.assign(nm, nm + nElmt);
+    a->nm = std::all_of(nm, nm + nElmt, [&](int v) { return v == nm[0]; }) ? nm[0] : 0;
+    a->loff.assign(nElmt + 1, 0);
+    const int dim = shape == imf::kHex ? 3 : 2;
+    for (int e = 0; e < nElmt; ++e)
+        a->loff[e + 1] = a->loff[e] + (long)(dim == 3 ? nm[e] * nm[e] * nm[e] : nm[e] * nm[e]);
     a->dirichlet = dirichlet ? 1 : 0;
     a->nloc = (long)m.size();
     a->ng = ng;
-    // inverse map (CSR) for the deterministic assembly
-    std::vector<int> off(ng + 1, 0), inv(a->nloc);
-    for (long i = 0; i < a->nloc; ++i) ++off[m[i] + 1];
+    // inverse map (CSR) for the deterministic assembly; modes with sign 0 (beyond the order
+    // of a shared edge or face) are left out
+    long nused = 0;
+    std::vector<int> off(ng + 1, 0);
+    for (long i = 0; i < a->nloc; ++i)
+        if (sg[i]) ++off[m[i] + 1], ++nused;
+    std::vector<int> inv(std::max<long>(nused, 1));
     for (long g = 0; g < ng; ++g) off[g + 1] += off[g];
     std::vector<int> fill(off.begin(), off.end() - 1);
-    for (long i = 0; i < a->nloc; ++i) inv[fill[m[i]]++] = (int)i;
+    for (long i = 0; i < a->nloc; ++i)
+        if (sg[i]) inv[fill[m[i]]++] = (int)i;
     std::vector<unsigned char> gmask;
     if (a->dirichlet) {
         gmask.assign(ng, 0);
         for (long i = 0; i < a->nloc; ++i)
-            if (bd[i]) gmask[m[i]] = 1;
+            if (bd[i] && sg[i]) gmask[m[i]] = 1;
     }
     auto cleanup = [&](int st) {
         imf_assembly_destroy(a);
@@ -636,41 +716,71 @@ static int create_any(int shape, int nElmt, int nm, const long long *vertex_ids,
     if ((e = hipMalloc(&a->d_l2g, a->nloc * sizeof(int))) != hipSuccess ||
         (e = hipMalloc(&a->d_sg, a->nloc)) != hipSuccess ||
         (e = hipMalloc(&a->d_off, (ng + 1) * sizeof(int))) != hipSuccess ||
-        (e = hipMalloc(&a->d_inv, a->nloc * sizeof(int))) != hipSuccess ||
+        (e = hipMalloc(&a->d_inv, inv.size() * sizeof(int))) != hipSuccess ||
         (a->dirichlet && (e = hipMalloc(&a->d_mask, ng)) != hipSuccess))
         return cleanup(afail(IMF_ERR_HIP, std::string("hipMalloc(assembly): ") + hipGetErrorString(e)));
     if ((e = hipMemcpy(a->d_l2g, m.data(), a->nloc * sizeof(int), hipMemcpyHostToDevice)) != hipSuccess ||
         (e = hipMemcpy(a->d_sg, sg.data(), a->nloc, hipMemcpyHostToDevice)) != hipSuccess ||
         (e = hipMemcpy(a->d_off, off.data(), (ng + 1) * sizeof(int), hipMemcpyHostToDevice)) != hipSuccess ||
-        (e = hipMemcpy(a->d_inv, inv.data(), a->nloc * sizeof(int), hipMemcpyHostToDevice)) != hipSuccess ||
+        (e = hipMemcpy(a->d_inv, inv.data(), inv.size() * sizeof(int), hipMemcpyHostToDevice)) != hipSuccess ||
         (a->dirichlet && (e = hipMemcpy(a->d_mask, gmask.data(), ng, hipMemcpyHostToDevice)) != hipSuccess))
         return cleanup(afail(IMF_ERR_HIP, std::string("hipMemcpy(assembly): ") + hipGetErrorString(e)));
     *out = a;
     return IMF_OK;
 }
 
+static std::vector<int> uniform(int nElmt, int nm)
+{
+    return std::vector<int>(nElmt > 0 ? nElmt : 0, nm);
+}
+
+static int shape_of_dim(int dim, int &shape)
+{
+    if (dim != 2 && dim != 3) return afail(IMF_ERR_ARG, "assembly: dim must be 2 (quad) or 3 (hex)");
+    shape = dim == 3 ? imf::kHex : imf::kQuad;
+    return IMF_OK;
+}
+
 int imf_assembly_map_hex(int nElmt, int nm, const long long *vertex_ids, int *l2g,
                          signed char *sign, unsigned char *on_boundary, long *nglobal)
 {
-    return map_any(imf::kHex, nElmt, nm, vertex_ids, l2g, sign, on_boundary, nglobal);
+    return map_any(imf::kHex, nElmt, uniform(nElmt, nm).data(), vertex_ids, l2g, sign,
+                   on_boundary, nglobal);
 }
 
 int imf_assembly_map_quad(int nElmt, int nm, const long long *vertex_ids, int *l2g,
                           signed char *sign, unsigned char *on_boundary, long *nglobal)
 {
-    return map_any(imf::kQuad, nElmt, nm, vertex_ids, l2g, sign, on_boundary, nglobal);
+    return map_any(imf::kQuad, nElmt, uniform(nElmt, nm).data(), vertex_ids, l2g, sign,
+                   on_boundary, nglobal);
+}
+
+int imf_assembly_map_varp(int dim, int nElmt, const int *nm, const long long *vertex_ids,
+                          int *l2g, signed char *sign, unsigned char *on_boundary, long *nglobal)
+{
+    int shape;
+    if (int st = shape_of_dim(dim, shape)) return st;
+    return map_any(shape, nElmt, nm, vertex_ids, l2g, sign, on_boundary, nglobal);
 }
 
 int imf_assembly_create_hex(int nElmt, int nm, const long long *vertex_ids, int dirichlet,
                             imf_asm *out)
 {
-    return create_any(imf::kHex, nElmt, nm, vertex_ids, dirichlet, out);
+    return create_any(imf::kHex, nElmt, uniform(nElmt, nm).data(), vertex_ids, dirichlet, out);
 }
 
 int imf_assembly_create_quad(int nElmt, int nm, const long long *vertex_ids, int dirichlet,
                              imf_asm *out)
 {
-    return create_any(imf::kQuad, nElmt, nm, vertex_ids, dirichlet, out);
+    return create_any(imf::kQuad, nElmt, uniform(nElmt, nm).data(), vertex_ids, dirichlet, out);
+}
+
+int imf_assembly_create_varp(int dim, int nElmt, const int *nm, const long long *vertex_ids,
+                             int dirichlet, imf_asm *out)
+{
+    int shape;
+    if (int st = shape_of_dim(dim, shape)) return st;
+    return create_any(shape, nElmt, nm, vertex_ids, dirichlet, out);
 }
 
 long imf_assembly_nglobal(imf_asm a) { return a ? a->ng : 0; }
@@ -703,17 +813,30 @@ int imf_assemble_device(imf_asm a, const double *d_local, double *d_global, void
     return IMF_OK;
 }
 
-int imf_helmsolve_cg_device(imf_asm a, imf_op op, const double *d_rhs, double *d_x, double tol,
-                            int max_iter, int *iterations, double *residual, void *stream)
+int imf_helmsolve_cg_groups_device(imf_asm a, int nops, const imf_op *ops, const double *d_rhs,
+                                   double *d_x, double tol, int max_iter, int *iterations,
+                                   double *residual, void *stream)
 {
-    if (!a || !op || !d_rhs || !d_x) return afail(IMF_ERR_ARG, "imf_helmsolve_cg_device: null argument");
-    if (op->kind != imf::kHelmholtz || op->shape != a->shape || op->prec != IMF_FP64 ||
-        op->nElmt != a->nElmt || op->nm != a->nm)
-        return afail(IMF_ERR_ARG, "imf_helmsolve_cg_device: needs an FP64 Helmholtz operator "
-                                  "of the assembly's shape on its elements and modes");
-    if (!op->geom_ready) return afail(IMF_ERR_STATE, "Jacobian and derivative factors not set");
-    if (op->dbdata.size() != (size_t)op->nm * op->nq || op->nq > 16)
-        return afail(IMF_ERR_ARG, "imf_helmsolve_cg_device: operator created without dbdata");
+    if (!a || !ops || nops < 1 || !d_rhs || !d_x)
+        return afail(IMF_ERR_ARG, "imf_helmsolve_cg_device: null argument");
+    long e0 = 0;
+    for (int g = 0; g < nops; ++g) {
+        const imf_op op = ops[g];
+        if (!op) return afail(IMF_ERR_ARG, "imf_helmsolve_cg_device: null operator");
+        if (op->kind != imf::kHelmholtz || op->shape != a->shape || op->prec != IMF_FP64 ||
+            e0 + op->nElmt > a->nElmt ||
+            !std::all_of(a->nmv.begin() + e0, a->nmv.begin() + e0 + op->nElmt,
+                         [&](int v) { return v == op->nm; }))
+            return afail(IMF_ERR_ARG, "imf_helmsolve_cg_device: needs FP64 Helmholtz operators "
+                                      "of the assembly's shape on its elements and modes");
+        if (!op->geom_ready) return afail(IMF_ERR_STATE, "Jacobian and derivative factors not set");
+        if (op->dbdata.size() != (size_t)op->nm * op->nq || op->nq > 16)
+            return afail(IMF_ERR_ARG, "imf_helmsolve_cg_device: operator created without dbdata");
+        e0 += op->nElmt;
+    }
+    if (e0 != a->nElmt)
+        return afail(IMF_ERR_ARG, "imf_helmsolve_cg_device: the operators do not cover the "
+                                  "assembly's elements");
     hipStream_t s = (hipStream_t)stream;
     const long ng = a->ng;
     if (!a->d_ws) {
@@ -724,29 +847,35 @@ int imf_helmsolve_cg_device(imf_asm a, imf_op op, const double *d_rhs, double *d
         const long np = std::max<long>(a->nparts, std::min<long>(kAsmParts, nblk(ng)));
         ACHECK(hipMalloc(&a->d_part, 3 * np * sizeof(double)), "hipMalloc(cg)");
         ACHECK(hipMalloc(&a->d_dots, 4 * sizeof(double)), "hipMalloc(cg)");
-        a->diag_op = nullptr;
+        a->diag_key.clear();
     }
     double *r = a->d_ws, *p = r + ng, *q = p + ng, *w = q + ng, *sv = w + ng, *dinv = sv + ng;
-    // assembled diagonal (Jacobi preconditioner), cached per operator and lambda
-    if (a->diag_op != (const void *)op || a->diag_lambda != op->lambda) {
-        DiagTab t{};
-        const int nm = op->nm, nq = op->nq;
-        std::copy(op->bdata.begin(), op->bdata.end(), t.B);
-        std::copy(op->dbdata.begin(), op->dbdata.end(), t.dB);
-        std::copy(op->w.begin(), op->w.end(), t.w);
-        if (a->shape == imf::kHex)
-            helm_diag_hex<<<nblk(a->nloc), RB, 0, s>>>(t, nm, nq, a->nElmt, op->deformed,
-                                                      op->lambda, (const double *)op->d_geom,
-                                                      a->d_loc);
-        else
-            helm_diag_quad<<<nblk(a->nloc), RB, 0, s>>>(t, nm, nq, a->nElmt, op->deformed,
-                                                       op->lambda, (const double *)op->d_geom,
-                                                       a->d_loc);
+    // assembled diagonal (Jacobi preconditioner), cached per operators and lambdas
+    std::vector<std::pair<const void *, double>> key;
+    for (int g = 0; g < nops; ++g) key.emplace_back((const void *)ops[g], ops[g]->lambda);
+    if (a->diag_key != key) {
+        e0 = 0;
+        for (int g = 0; g < nops; ++g) {
+            const imf_op op = ops[g];
+            DiagTab t{};
+            const int nm = op->nm, nq = op->nq;
+            std::copy(op->bdata.begin(), op->bdata.end(), t.B);
+            std::copy(op->dbdata.begin(), op->dbdata.end(), t.dB);
+            std::copy(op->w.begin(), op->w.end(), t.w);
+            double *dl = a->d_loc + a->loff[e0];
+            const long n = a->loff[e0 + op->nElmt] - a->loff[e0];
+            if (a->shape == imf::kHex)
+                helm_diag_hex<<<nblk(n), RB, 0, s>>>(t, nm, nq, op->nElmt, op->deformed, op->lambda,
+                                                    (const double *)op->d_geom, dl);
+            else
+                helm_diag_quad<<<nblk(n), RB, 0, s>>>(t, nm, nq, op->nElmt, op->deformed, op->lambda,
+                                                     (const double *)op->d_geom, dl);
+            e0 += op->nElmt;
+        }
         assemble_kernel<<<nblk(ng), RB, 0, s>>>(ng, a->d_off, a->d_inv, nullptr, a->d_loc, nullptr, dinv);
         inv_masked<<<nblk(ng), RB, 0, s>>>(ng, a->d_mask, dinv);
         ACHECK(hipGetLastError(), "diagonal kernels");
-        a->diag_op = op;
-        a->diag_lambda = op->lambda;
+        a->diag_key = key;
     }
     double h[4];
     auto read_dots = [&]() -> int {
@@ -762,7 +891,7 @@ int imf_helmsolve_cg_device(imf_asm a, imf_op op, const double *d_rhs, double *d
     };
     // s = A w with (r,w), (s,w), (r,r) in the same pass
     auto apply_dots = [&]() -> int {
-        if (int st = apply_global_dots(a, op, w, r, sv, s)) return st;
+        if (int st = apply_global_dots(a, nops, ops, w, r, sv, s)) return st;
         return read_dots();
     };
     // rhs magnitude over all global modes (Set_Rhs_Magnitude)
@@ -802,6 +931,13 @@ int imf_helmsolve_cg_device(imf_asm a, imf_op op, const double *d_rhs, double *d
     if (iterations) *iterations = it;
     if (residual) *residual = std::sqrt(eps / rhs_mag);
     return IMF_OK;
+}
+
+int imf_helmsolve_cg_device(imf_asm a, imf_op op, const double *d_rhs, double *d_x, double tol,
+                            int max_iter, int *iterations, double *residual, void *stream)
+{
+    return imf_helmsolve_cg_groups_device(a, 1, &op, d_rhs, d_x, tol, max_iter, iterations,
+                                          residual, stream);
 }
 
 }  // extern "C"

@@ -25,6 +25,10 @@ def _lib():
                                                                  ctypes.POINTER(l_)]
             getattr(L, f"imf_assembly_create_{shape}").argtypes = [i, i, vp, i,
                                                                     ctypes.POINTER(vp)]
+        L.imf_assembly_map_varp.argtypes = [i, i, vp, vp, vp, vp, vp, ctypes.POINTER(l_)]
+        L.imf_assembly_create_varp.argtypes = [i, i, vp, vp, i, ctypes.POINTER(vp)]
+        L.imf_helmsolve_cg_groups_device.argtypes = [vp, i, vp, vp, vp, d, i, ctypes.POINTER(i),
+                                                     ctypes.POINTER(d), vp]
         L.imf_assembly_nglobal.argtypes = [vp]
         L.imf_assembly_nglobal.restype = l_
         L.imf_assembly_destroy.argtypes = [vp]
@@ -49,21 +53,38 @@ def _nverts(shape):
     return 8 if shape == "hex" else 4
 
 
+def _orders(nm, E):
+    """Per-element orders as int32 (None for a uniform int order)."""
+    if np.ndim(nm) == 0:
+        return None
+    nmv = np.ascontiguousarray(nm, dtype=np.int32)
+    if nmv.shape != (E,):
+        raise MFError(f"assembly: {E} per-element orders expected, got shape {nmv.shape}")
+    return nmv
+
+
 def assembly_map(vertex_ids, nm, shape):
-    """(l2g, sign, on_boundary, nglobal) of the C0 numbering (host only, no device)."""
+    """(l2g, sign, on_boundary, nglobal) of the C0 numbering (host only, no device).  nm is
+    the order of every element, or an array of per-element orders (variable order: modes
+    beyond a shared edge's or face's order get l2g 0 and sign 0)."""
     ids = np.ascontiguousarray(vertex_ids, dtype=np.int64)
     if ids.ndim != 2 or ids.shape[1] != _nverts(shape):
         raise MFError(f"assembly: vertex ids must be (nElmt, {_nverts(shape)}) for {shape}")
     E = ids.shape[0]
-    n = E * nm ** (3 if shape == "hex" else 2)
+    dim = 3 if shape == "hex" else 2
+    nmv = _orders(nm, E)
+    n = int(np.sum(nmv.astype(np.int64) ** dim)) if nmv is not None else E * nm ** dim
     l2g = np.empty(n, dtype=np.int32)
     sign = np.empty(n, dtype=np.int8)
     bnd = np.empty(n, dtype=np.uint8)
     ng = ctypes.c_long()
-    _check(getattr(_lib(), f"imf_assembly_map_{shape}")(
-        E, nm, ids.ctypes.data_as(ctypes.c_void_p), l2g.ctypes.data_as(ctypes.c_void_p),
-        sign.ctypes.data_as(ctypes.c_void_p), bnd.ctypes.data_as(ctypes.c_void_p),
-        ctypes.byref(ng)))
+    ptrs = [ids.ctypes.data_as(ctypes.c_void_p), l2g.ctypes.data_as(ctypes.c_void_p),
+            sign.ctypes.data_as(ctypes.c_void_p), bnd.ctypes.data_as(ctypes.c_void_p),
+            ctypes.byref(ng)]
+    if nmv is None:
+        _check(getattr(_lib(), f"imf_assembly_map_{shape}")(E, nm, *ptrs))
+    else:
+        _check(_lib().imf_assembly_map_varp(dim, E, nmv.ctypes.data_as(ctypes.c_void_p), *ptrs))
     return l2g, sign, bnd, ng.value
 
 
@@ -184,19 +205,29 @@ def dirichlet_lift(coords, vertex_ids, nm, fn, nq=None):
 class Assembly:
     """Device assembly map of a hex mesh (vertex_ids (E, 8)) or a quad mesh (shape="quad",
     vertex_ids (E, 4)); dirichlet=True fixes every mode on the boundary faces (edges) to
-    zero."""
+    zero.  nm: the order of every element, or per-element orders (variable order; local
+    arrays then hold element e's nm[e]^dim coefficients after those of elements < e)."""
 
     def __init__(self, vertex_ids, nm, dirichlet=True, shape="hex"):
         ids = np.ascontiguousarray(vertex_ids, dtype=np.int64)
         if ids.ndim != 2 or ids.shape[1] != _nverts(shape):
             raise MFError(f"assembly: vertex ids must be (nElmt, {_nverts(shape)}) for {shape}")
         self.shape, self.nElmt, self.nm = shape, ids.shape[0], nm
+        dim = 3 if shape == "hex" else 2
+        nmv = _orders(nm, self.nElmt)
         h = ctypes.c_void_p()
-        _check(getattr(_lib(), f"imf_assembly_create_{shape}")(
-            self.nElmt, nm, ids.ctypes.data_as(ctypes.c_void_p), int(dirichlet), ctypes.byref(h)))
+        if nmv is None:
+            _check(getattr(_lib(), f"imf_assembly_create_{shape}")(
+                self.nElmt, nm, ids.ctypes.data_as(ctypes.c_void_p), int(dirichlet),
+                ctypes.byref(h)))
+            self.nlocal = self.nElmt * nm ** dim
+        else:
+            _check(_lib().imf_assembly_create_varp(
+                dim, self.nElmt, nmv.ctypes.data_as(ctypes.c_void_p),
+                ids.ctypes.data_as(ctypes.c_void_p), int(dirichlet), ctypes.byref(h)))
+            self.nlocal = int(np.sum(nmv.astype(np.int64) ** dim))
         self._h = h
         self.nglobal = _lib().imf_assembly_nglobal(h)
-        self.nlocal = self.nElmt * nm ** (3 if shape == "hex" else 2)
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -217,21 +248,36 @@ class Assembly:
                                           ctypes.c_void_p(out.data_ptr()), _stream_ptr(stream)))
         return out
 
+    def apply_local(self, ops, loc):
+        """H applied to a local vector by one operator per run of equal-order elements."""
+        if not isinstance(ops, (list, tuple)):
+            return ops.apply(loc)
+        out, o = torch.empty_like(loc), 0
+        for op in ops:
+            n = int(op.in_size)
+            out[o:o + n] = op.apply(loc[o:o + n])
+            o += n
+        return out
+
     def helmsolve(self, op, rhs, x=None, tol=1e-9, max_iter=10000, stream=None, lift=None):
         """x with Assemble(H(GlobalToLocal(x))) = rhs on the free modes; returns
-        (x, iterations, relative residual).  Dirichlet modes are zero, or, with `lift` (a
-        global vector holding the Dirichlet values, e.g. from dirichlet_lift), equal to
-        lift: the solve is then for x - lift with rhs - A lift, as the reference's
-        ContField::HelmSolve treats inhomogeneous Dirichlet conditions."""
+        (x, iterations, relative residual).  op: a Helmholtz operator on all elements, or a
+        list of operators, one per run of equal-order elements in element order (variable
+        order).  Dirichlet modes are zero, or, with `lift` (a global vector holding the
+        Dirichlet values, e.g. from dirichlet_lift), equal to lift: the solve is then for
+        x - lift with rhs - A lift, as the reference's ContField::HelmSolve treats
+        inhomogeneous Dirichlet conditions."""
         if lift is not None:
             lift = torch.as_tensor(lift, dtype=torch.float64, device=rhs.device)
-            rhs = rhs - self.assemble(op.apply(self.global_to_local(lift)))
+            rhs = rhs - self.assemble(self.apply_local(op, self.global_to_local(lift)))
         x = torch.empty_like(rhs) if x is None else x
         it, res = ctypes.c_int(), ctypes.c_double()
-        _check(_lib().imf_helmsolve_cg_device(self._h, op._h, ctypes.c_void_p(rhs.data_ptr()),
-                                              ctypes.c_void_p(x.data_ptr()), float(tol),
-                                              int(max_iter), ctypes.byref(it), ctypes.byref(res),
-                                              _stream_ptr(stream)))
+        ops = list(op) if isinstance(op, (list, tuple)) else [op]
+        hs = (ctypes.c_void_p * len(ops))(*[o._h.value for o in ops])
+        _check(_lib().imf_helmsolve_cg_groups_device(
+            self._h, len(ops), ctypes.cast(hs, ctypes.c_void_p), ctypes.c_void_p(rhs.data_ptr()),
+            ctypes.c_void_p(x.data_ptr()), float(tol), int(max_iter), ctypes.byref(it),
+            ctypes.byref(res), _stream_ptr(stream)))
         if lift is not None:
             x += lift
         return x, it.value, res.value

@@ -234,3 +234,88 @@ def test_helmholtz3d_hex_golden_on_the_gpu():
     print(f"iterations {it}, residual {res:.1e}, L2 {l2:.6e}, Linf {linf:.6e}")
     assert abs(l2 - 1.11818e-07) <= 1e-12
     assert abs(linf - 5.66374e-07) <= 1e-12
+
+
+# ---------------------------------------------------------------- variable order
+
+def _runs(nmv):
+    """(first, end, order) of each run of equal-order elements, in element order."""
+    runs, e0 = [], 0
+    for e in range(1, len(nmv) + 1):
+        if e == len(nmv) or nmv[e] != nmv[e0]:
+            runs.append((e0, e, int(nmv[e0])))
+            e0 = e
+    return runs
+
+
+def _varp_gpu_problem(verts, ids, nmv, lam, shape):
+    dim = 3 if shape == "hex" else 2
+    S = "Hex" if shape == "hex" else "Quad"
+    _, f = manufactured(lam, dim)
+    ops, loc = [], []
+    for e0, e1, nm in _runs(nmv):
+        nq = nm + 1
+        b = H.basis_modified_a(nm, nq)
+        v = verts[e0:e1]
+        jac, df = H.geom_device(shape, _dev(v), nq, b["z"], True)
+        Hop = H.Operator(f"Helmholtz_{S}_Deformed", nm, nq, e1 - e0, basis=b)
+        Hop.set_jac(jac)
+        Hop.set_df(df)
+        Hop.set_lambda(lam)
+        ipr = H.Operator(f"IProduct_{S}_Deformed", nm, nq, e1 - e0, basis=b)
+        ipr.set_jac(jac)
+        pts = (O.coords_hex if shape == "hex" else O.coords_quad)(v, nq, b["z"]).T
+        loc.append(ipr.apply(_dev(f(pts))))
+        ops.append(Hop)
+    A = assembly.Assembly(ids, nmv, dirichlet=True, shape=shape)
+    return A, ops, A.assemble(torch.cat(loc).contiguous())
+
+
+@pytest.mark.parametrize("shape", ["hex", "quad"])
+def test_variable_order_gather_and_assemble(shape):
+    from test_assembly_varp import _offsets  # noqa: F401  (same layout as the CPU tests)
+    verts, ids = (_mesh if shape == "hex" else _qmesh)(3, True, seed=8, jitter=0.1)
+    nmv = np.random.default_rng(9).integers(3, 7, len(ids))
+    l2g, sign, _, ng = assembly.assembly_map(ids, nmv, shape)
+    assert (sign == 0).any()
+    A = assembly.Assembly(ids, nmv, dirichlet=False, shape=shape)
+    assert A.nglobal == ng and A.nlocal == len(l2g)
+    rng = np.random.default_rng(4)
+    g = rng.uniform(-1, 1, ng)
+    assert np.array_equal(A.global_to_local(_dev(g)).cpu().numpy(), sign * g[l2g])
+    lv = rng.uniform(-1, 1, len(l2g))
+    ref = np.zeros(ng)
+    np.add.at(ref, l2g, sign * lv)
+    assert np.abs(A.assemble(_dev(lv)).cpu().numpy() - ref).max() < 1e-13
+
+
+@pytest.mark.parametrize("shape", ["hex", "quad"])
+def test_variable_order_helmsolve_matches_cpu_sparse_solve(shape):
+    from test_assembly_varp import varp_solve, varp_error
+    if shape == "hex":
+        verts, ids = _mesh(3, True, seed=3, jitter=0.05)
+        nmv = np.repeat([4, 6, 5], 9)  # one order per z layer (elements are z-major)
+    else:
+        verts, ids = _qmesh(6, True, seed=3, jitter=0.05)
+        nmv = np.repeat([3, 7, 5, 5, 8, 4], 6)  # one order per row
+    print(f"[case] variable-order helmsolve {shape} orders {sorted(set(nmv.tolist()))}", flush=True)
+    x_cpu, l2g, sign, u = varp_solve(verts, ids, nmv, 10.0, shape)
+    A, ops, rhs = _varp_gpu_problem(verts, ids, nmv, 10.0, shape)
+    x, it, res = A.helmsolve(ops, rhs, tol=1e-13, max_iter=5000)
+    xg = x.cpu().numpy()
+    print(f"operators {len(ops)}, iterations {it}, residual {res:.2e}")
+    assert res < 1e-13
+    assert np.linalg.norm(xg - x_cpu) <= 1e-9 * np.linalg.norm(x_cpu)
+    e_gpu = varp_error(verts, ids, nmv, xg, l2g, sign, u, shape)
+    e_cpu = varp_error(verts, ids, nmv, x_cpu, l2g, sign, u, shape)
+    assert abs(e_gpu - e_cpu) <= 1e-6 * e_cpu
+
+
+def test_variable_order_rejects_mismatched_operators():
+    verts, ids = _mesh(2, False)
+    nmv = np.repeat([4, 5], 4)
+    A, ops, rhs = _varp_gpu_problem(verts, ids, nmv, 1.0, "hex")
+    with pytest.raises(H.MFError, match="modes"):
+        A.helmsolve(ops[::-1], rhs)
+    with pytest.raises(H.MFError, match="cover"):
+        A.helmsolve(ops[:1], rhs)

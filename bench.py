@@ -519,6 +519,74 @@ def helmsolve_bench(args):
     return res_d
 
 
+def varp_helmsolve_bench(args):
+    """The solve of helmsolve_bench on the same 48^3 jittered mesh with two orders: P=4
+    (5 modes, 6 GLL points) in the lower half of the box, P=6 (7 modes, 8 points) in the
+    upper half; one Helmholtz operator per half, the variable-order C0 numbering between
+    them."""
+    import numpy as np
+    import torch
+
+    import ithaca_hip as H
+    from ithaca_hip import assembly, mesh
+
+    w = mesh.workload_hex_helmholtz(args.n)
+    E, lam = w["nElmt"], w["lam"]
+    half = E // 2  # elements are z-major: the first half is the lower half of the box
+    groups = [(0, half, 5), (half, E, 7)]
+    nmv = np.concatenate([np.full(e1 - e0, nm, dtype=np.int32) for e0, e1, nm in groups])
+    A = assembly.Assembly(mesh.hex_box_ids(args.n), nmv, dirichlet=True)
+    ops, loc, parts = [], [], []
+    for e0, e1, nm in groups:
+        nq = nm + 1
+        b = H.basis_modified_a(nm, nq)
+        vd = torch.from_numpy(w["verts"][e0:e1]).cuda()
+        jac, df = H.geom_device("hex", vd, nq, b["z"], True)
+        Hop = H.Operator("Helmholtz_Hex_Deformed", nm, nq, e1 - e0, basis=b)
+        Hop.set_jac(jac)
+        Hop.set_df(df)
+        Hop.set_lambda(lam)
+        ipr = H.Operator("IProduct_Hex_Deformed", nm, nq, e1 - e0, basis=b)
+        ipr.set_jac(jac)
+        X = mesh.hex_quad_points(vd, b["z"]).reshape(-1, 3)
+        uex = torch.sin(np.pi * X[:, 0]) * torch.sin(np.pi * X[:, 1]) * torch.sin(np.pi * X[:, 2])
+        del X, df
+        loc.append(ipr.apply(((3 * np.pi ** 2 + lam) * uex).contiguous()))
+        wq = np.einsum("k,j,i->kji", b["w"], b["w"], b["w"]).ravel()
+        parts.append((H.Operator("BwdTrans_Hex_Regular", nm, nq, e1 - e0, basis=b), uex,
+                      jac * torch.from_numpy(np.tile(wq, e1 - e0)).cuda()))
+        ops.append(Hop)
+        del ipr
+    rhs = A.assemble(torch.cat(loc))
+    del loc
+    x = torch.empty_like(rhs)
+    A.helmsolve(ops, rhs, x=x, tol=1e-9, max_iter=20000)  # warm-up (builds the diagonal)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    x, it, res = A.helmsolve(ops, rhs, x=x, tol=1e-9, max_iter=20000)
+    torch.cuda.synchronize()
+    t = time.perf_counter() - t0
+    xl = A.global_to_local(x)
+    num = den = 0.0
+    o = 0
+    for bwd, uex, wj in parts:
+        n = int(bwd.in_size)
+        uh = bwd.apply(xl[o:o + n].contiguous())
+        num += torch.sum(wj * (uh - uex) ** 2).item()
+        den += torch.sum(wj * uex ** 2).item()
+        o += n
+    res_d = {"workload": f"Dirichlet Helmholtz solve, {E} deformed hexes, P=4 in the lower "
+                         "half and P=6 in the upper half (variable order), lambda=10, "
+                         "manufactured u = sin(pi x) sin(pi y) sin(pi z)",
+             "operators": len(ops), "local_modes": A.nlocal, "global_modes": A.nglobal,
+             "cg_iterations": it, "relative_residual": res, "solve_s": t,
+             "ms_per_iteration": t / max(it, 1) * 1e3,
+             "l2_error_vs_exact": float(np.sqrt(num / den)), "tolerance": 1e-9}
+    del A, ops, parts, x, xl, rhs
+    torch.cuda.empty_cache()
+    return res_d
+
+
 def quad_helmsolve_bench(args):
     """C0-assembled Helmholtz solve on the 32x32 regular quads of the quad workload (P=4,
     6 GLL points, lambda = 1): -lap u + u = f on [0,1]^2, u = 0 on the boundary, f from
@@ -731,6 +799,7 @@ def main():
         section("quad_helmsolve", quad_helmsolve_bench)
         section("deriv_operators", deriv_bench)
         section("helmsolve", helmsolve_bench)
+        section("helmsolve_varp", varp_helmsolve_bench)
         if not args.no_pod:
             try:
                 section("pod_offline", pod_bench)

@@ -174,6 +174,9 @@ def dirichlet_lift(coords, vertex_ids, nm, fn, nq=None):
     conditions (DisContField.cpp:3630-3663: FwdTrans_BndConstrained of the boundary
     expansion), on the element's own GLL points (nq = nm + 1 by default, the reference's
     TYPE="MODIFIED").  Host side (numpy); pass the result as `lift` to helmsolve."""
+    if np.ndim(nm) != 0:
+        raise MFError("dirichlet_lift: one order for every element (variable order is not "
+                      "supported here)")
     coords = np.asarray(coords, dtype=np.float64)
     nq = nm + 1 if nq is None else nq
     shape = "hex" if coords.shape[1] == 8 else "quad"

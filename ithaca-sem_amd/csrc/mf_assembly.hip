@@ -628,8 +628,14 @@ int apply_global_dots(imf_asm a, int nops, const imf_op *ops, const double *w, c
                       double *out, hipStream_t s)
 {
     double *l0 = a->d_loc, *l1 = a->d_loc + a->nloc;
-    if (nops == 1 && a->shape == imf::kHex) {  // GlobalToLocal fused into the kernel's load
-        if (int st = imf_apply_gather_device(ops[0], w, a->d_l2g, a->d_sg, l1, s)) return st;
+    if (a->shape == imf::kHex) {  // GlobalToLocal fused into each operator's coefficient load
+        long e0 = 0;
+        for (int g = 0; g < nops; ++g) {
+            const long o = a->loff[e0];
+            if (int st = imf_apply_gather_device(ops[g], w, a->d_l2g + o, a->d_sg + o, l1 + o, s))
+                return st;
+            e0 += ops[g]->nElmt;
+        }
     } else {
         g2l_kernel<<<nblk(a->nloc), RB, 0, s>>>(a->nloc, a->d_l2g, a->d_sg, w, l0);
         long e0 = 0;

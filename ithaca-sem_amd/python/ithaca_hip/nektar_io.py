@@ -17,13 +17,20 @@ What is read, restated from the reference:
   field names, shape, basis, UNIORDER mode counts, element id ranges, and the data, one
   field after the other, each element's coefficients in its own local frame.
 
+Written: .fld files in the reference's XML layout (FieldIOXml::v_Write,
+FieldIOXml.cpp:86-330), so that fields computed here (reduced-order reconstructions,
+solutions) can be post-processed with the reference's tools.
+
 Only straight-sided quadrilateral and hexahedral meshes are read (a non-empty CURVED
 section is rejected), which is the element family the operators here implement.
 """
 import base64
 import re
+import socket
+import time
 import xml.etree.ElementTree as ET
 import zlib
+from xml.sax.saxutils import escape
 
 import numpy as np
 
@@ -313,3 +320,103 @@ def expansion_arrays(mesh, blocks, field):
     ids = np.concatenate([b.ids for b in blk])
     return (mesh.element_coords(ids), mesh.vertex_ids(ids),
             field_coefficients(blocks, field, ids), nm, nm + 1)
+
+
+# ---------------------------------------------------------------- writing
+
+_SHAPE_NAME = {"hex": ("Hexahedron", 3), "quad": ("Quadrilateral", 2)}
+
+
+def seq_string(ids):
+    """Element id list in the reference's range syntax, the inverse of _id_list
+    (ParseUtils::GenerateSeqString, LibUtilities/BasicUtils/ParseUtils.h:71-106): runs of
+    consecutive ids become "a-b", a repeated id is written once."""
+    ids = [int(i) for i in ids]
+    if not ids:
+        return ""
+    out = [str(ids[0])]
+    first = last = ids[0]
+    for i in ids:
+        if i != last + 1 and i != last:
+            if last != first:
+                out.append(f"-{last}")
+            out.append(f",{i}")
+            first = i
+        last = i
+    if last != first:
+        out.append(f"-{last}")
+    return "".join(out)
+
+
+def b64z_encode(data):
+    """"B64Z-LittleEndian" text of float64 data: a zlib stream of the little-endian values
+    (default compression level), padded with zero bytes to a multiple of 3, in base64
+    (CompressData::ZlibEncodeToBase64Str and BinaryStrToBase64Str,
+    LibUtilities/BasicUtils/CompressData.cpp:108-133)."""
+    comp = zlib.compress(np.ascontiguousarray(data, dtype="<f8").tobytes())
+    comp += b"\0" * (-len(comp) % 3)
+    return base64.b64encode(comp).decode("ascii")
+
+
+def write_fld(path, fields, element_ids, nm, shape="hex", metadata=None, compressed=True):
+    """Write one ELEMENTS block in the reference's .fld XML layout (FieldIOXml::v_Write,
+    FieldIOXml.cpp:86-330, with FieldIO::AddInfoTag's Metadata section, FieldIO.cpp:
+    348-393): fields = {name: element-major coefficients, len(element_ids) * nm^dim values
+    in each element's local frame}, in the file's field order; modified basis, UNIORDER nm
+    modes per direction.  metadata: extra {key: value} entries of the Metadata section
+    (e.g. Time, Kinvis).  compressed=False writes the data as text, which the reference
+    reads as well."""
+    if shape not in _SHAPE_NAME:
+        raise NektarIOError(f"shape {shape!r}: hex or quad")
+    sname, dim = _SHAPE_NAME[shape]
+    ids = [int(i) for i in element_ids]
+    if not ids or not fields:
+        raise NektarIOError("write_fld: no elements or no fields")
+    if not all(re.fullmatch(r"[A-Za-z_][\w]*", k) for k in list(fields) + list(metadata or {})):
+        raise NektarIOError("write_fld: field and metadata names must be XML names")
+    n = len(ids) * nm ** dim
+    data = []
+    for name, v in fields.items():
+        v = np.asarray(v, dtype=np.float64).ravel()
+        if v.size != n:
+            raise NektarIOError(f"field {name!r}: {v.size} values, {n} expected")
+        data.append(v)
+    data = np.concatenate(data)
+    prov = {"Hostname": socket.gethostname(),
+            "Timestamp": time.strftime("%d-%b-%Y %H:%M:%S")}
+    lines = ['<?xml version="1.0" encoding="utf-8" ?>', "<NEKTAR>", "    <Metadata>",
+             "        <Provenance>"]
+    lines += [f"            <{k}>{escape(v)}</{k}>" for k, v in sorted(prov.items())]
+    lines.append("        </Provenance>")
+    lines += [f"        <{k}>{escape(str(v))}</{k}>" for k, v in sorted((metadata or {}).items())]
+    lines.append("    </Metadata>")
+    attrs = (f'FIELDS="{",".join(fields)}" SHAPE="{sname}" '
+             f'BASIS="{",".join(["Modified_A"] * dim)}" '
+             f'NUMMODESPERDIR="UNIORDER:{",".join([str(nm)] * dim)}" ID="{seq_string(ids)}"')
+    if compressed:
+        body = b64z_encode(data)
+        attrs += ' COMPRESSED="B64Z-LittleEndian" BITSIZE="64"'
+    else:
+        body = " ".join(repr(float(x)) for x in data)
+    lines.append(f"    <ELEMENTS {attrs}>{body}</ELEMENTS>")
+    lines.append("</NEKTAR>")
+    with open(path, "w") as fh:
+        fh.write("\n".join(lines) + "\n")
+
+
+def read_fld_metadata(path):
+    """The Metadata section of a .fld file as {key: text} (Provenance entries under
+    "Provenance/<key>")."""
+    with open(path) as fh:
+        text = fh.read()
+    md = _section(text, "Metadata")
+    out = {}
+    if md is None:
+        return out
+    for child in md:
+        if len(child):
+            for g in child:
+                out[f"{child.tag}/{g.tag}"] = (g.text or "").strip()
+        else:
+            out[child.tag] = (child.text or "").strip()
+    return out

@@ -163,3 +163,68 @@ def test_expansion_arrays():
     assert (nm, nq) == (4, 5)
     with pytest.raises(N.NektarIOError, match="no data"):
         N.expansion_arrays(m, blocks, "T")
+
+
+# ---------------------------------------------------------------- writing
+
+def _elements(path):
+    import re
+    with open(path) as fh:
+        m = re.search(r"<ELEMENTS\b([^>]*)>(.*?)</ELEMENTS>", fh.read(), re.S)
+    return m.group(1).strip(), "".join(m.group(2).split())
+
+
+@pytest.mark.parametrize("name,shape", [("Hex_channel_C0helmsmoothing", "hex"),
+                                        ("interpfield_4quad", "quad")])
+def test_writer_reproduces_the_reference_files(tmp_path, name, shape):
+    """The reference's own .fld files, read and written back: the ELEMENTS attributes and
+    the compressed data text are identical to the reference's, byte for byte (zlib at the
+    default level, zero padding to a multiple of 3, base64, the id range syntax)."""
+    src = os.path.join(HERE, name + ".fld")
+    b = N.read_fld(src)[0]
+    out = tmp_path / "w.fld"
+    N.write_fld(str(out), {f: b.data[f] for f in b.fields}, b.ids, b.nummodes[0], shape,
+                metadata={"Time": 0})
+    assert _elements(str(out)) == _elements(src)
+    c = N.read_fld(str(out))[0]
+    assert c.fields == b.fields and list(c.ids) == list(b.ids)
+    for f in b.fields:
+        assert np.array_equal(c.data[f], b.data[f])
+
+
+def test_writer_text_round_trip_and_metadata(tmp_path):
+    rng = np.random.default_rng(0)
+    ids = [3, 4, 5, 9, 2]
+    u, v = rng.standard_normal(5 * 27), rng.standard_normal(5 * 27)
+    out = tmp_path / "t.fld"
+    N.write_fld(str(out), {"u": u, "v": v}, ids, 3, "hex", metadata={"Time": 1.5, "Kinvis": 0.01},
+                compressed=False)
+    b = N.read_fld(str(out))[0]
+    assert list(b.ids) == ids and b.nummodes == [3, 3, 3]
+    assert np.array_equal(N.field_coefficients([b], "u", ids), u)
+    assert np.array_equal(N.field_coefficients([b], "v", [9]), v[3 * 27:4 * 27])
+    md = N.read_fld_metadata(str(out))
+    assert md["Time"] == "1.5" and md["Kinvis"] == "0.01" and "Provenance/Timestamp" in md
+    ref = N.read_fld_metadata(os.path.join(HERE, "Hex_channel_C0helmsmoothing.fld"))
+    assert ref["Provenance/NektarVersion"] == "5.0.0" and ref["Time"] == "0"
+    assert ref["SessionName0"] == "Hex_channel_m3.xml"
+
+
+def test_seq_string_inverts_the_range_syntax():
+    # the example of ParseUtils::GenerateSeqVector's documentation, and edge cases
+    seq = [1, 2, 3, 4, 6, 7, 8, 5, 2, 3]
+    s = N.seq_string(seq)
+    assert s == "1-4,6-8,5,2-3"
+    assert list(N._id_list(s)) == seq
+    assert N.seq_string([7]) == "7" and N.seq_string([1, 1, 2]) == "1-2"
+    assert N.seq_string([]) == ""
+
+
+def test_writer_rejects_bad_input(tmp_path):
+    out = str(tmp_path / "x.fld")
+    with pytest.raises(N.NektarIOError, match="expected"):
+        N.write_fld(out, {"u": np.zeros(10)}, [0], 2, "hex")
+    with pytest.raises(N.NektarIOError, match="shape"):
+        N.write_fld(out, {"u": np.zeros(8)}, [0], 2, "tet")
+    with pytest.raises(N.NektarIOError, match="XML names"):
+        N.write_fld(out, {"u v": np.zeros(8)}, [0], 2, "hex")

@@ -243,27 +243,45 @@ def read_mesh(path):
 
 
 class FieldBlock:
-    """One ELEMENTS block of a .fld file: fields, shape, basis, modes per direction, element
-    ids, and data[f] = (len(ids), ncoeffs) coefficients of field f."""
+    """One ELEMENTS block of a .fld file: fields, shape, basis, element ids, modes[e] = the
+    mode counts per direction of element e (nummodes = the common ones for UNIORDER, None
+    for MIXORDER), and data[f][e] = the coefficients of field f on element e (a
+    (len(ids), ncoeffs) array for UNIORDER, a list of arrays for MIXORDER).  The data are
+    field after field, each field element after element with nm_0 * nm_1 [* nm_2]
+    coefficients per element (FieldIO::CheckFieldDefinition, FieldIO.cpp:570-760)."""
 
     def __init__(self, attrs, data):
         self.fields = attrs["FIELDS"].split(",")
         self.shape = attrs["SHAPE"]
         self.basis = attrs["BASIS"].split(",")
-        nm = attrs["NUMMODESPERDIR"]
-        if not nm.startswith("UNIORDER:"):
-            raise NektarIOError(f"NUMMODESPERDIR {nm!r}: only UNIORDER is supported")
-        self.nummodes = [int(x) for x in nm.split(":", 1)[1].split(",")]
         self.ids = _id_list(attrs["ID"])
         if self.shape not in ("Hexahedron", "Quadrilateral"):
             raise NektarIOError(f"shape {self.shape!r} is not supported")
         if any(b != "Modified_A" for b in self.basis):
             raise NektarIOError(f"basis {self.basis}: only Modified_A is supported")
-        ncoef = int(np.prod(self.nummodes))
-        if data.size != len(self.fields) * len(self.ids) * ncoef:
+        spec = attrs["NUMMODESPERDIR"]
+        kind, _, vals = spec.partition(":")
+        vals = [int(x) for x in vals.split(",") if x.strip()]
+        dim, E = len(self.basis), len(self.ids)
+        if kind == "UNIORDER" and len(vals) == dim:
+            self.nummodes = vals
+            self.modes = np.tile(np.array(vals, dtype=np.int64), (E, 1))
+        elif kind == "MIXORDER" and len(vals) == E * dim:
+            self.nummodes = None
+            self.modes = np.array(vals, dtype=np.int64).reshape(E, dim)
+        else:
+            raise NektarIOError(f"NUMMODESPERDIR {spec[:40]!r}: UNIORDER with {dim} counts or "
+                                f"MIXORDER with {dim} counts per element expected")
+        ncoef = self.modes.prod(axis=1)
+        off = np.concatenate([[0], np.cumsum(ncoef)])
+        total = int(off[-1])
+        if data.size != len(self.fields) * total:
             raise NektarIOError("Input data is not the same length as header information")
-        self.data = {f: data[i * len(self.ids) * ncoef:(i + 1) * len(self.ids) * ncoef]
-                     .reshape(len(self.ids), ncoef) for i, f in enumerate(self.fields)}
+        self.data = {}
+        for i, f in enumerate(self.fields):
+            d = data[i * total:(i + 1) * total]
+            self.data[f] = (d.reshape(E, int(ncoef[0])) if self.nummodes is not None
+                            else [d[off[e]:off[e + 1]] for e in range(E)])
 
 
 def read_fld(path):
@@ -305,19 +323,23 @@ def field_coefficients(blocks, field, element_ids):
 def expansion_arrays(mesh, blocks, field):
     """Inputs for the operators and the assembly of one field, in the element order of
     the file: (coords (E, 2^dim, dim) in local vertex order, vertex_ids (E, 2^dim),
-    coefficients (E * nm^dim, element-major), nm, nq).  The basis is Modified_A with
+    coefficients (element-major), nm, nq), with nm and nq ints when every element has the
+    same order and per-element arrays otherwise (MIXORDER files of variable-order runs).  The basis is Modified_A with
     nm + 1 Gauss-Lobatto-Legendre points per direction (the reference's TYPE="MODIFIED")."""
     blk = [b for b in blocks if field in b.data]
     if not blk:
         raise NektarIOError(f"no data for field {field!r}")
-    nms = {tuple(b.nummodes) for b in blk}
-    if len(nms) != 1 or len(set(next(iter(nms)))) != 1:
-        raise NektarIOError(f"field {field!r}: mode counts {sorted(nms)} are not uniform")
-    nm = blk[0].nummodes[0]
+    modes = np.concatenate([b.modes for b in blk])
+    if np.any(modes != modes[:, :1]):
+        raise NektarIOError(f"field {field!r}: elements with different mode counts per "
+                            "direction are not supported")
     want = "Hexahedron" if mesh.dim == 3 else "Quadrilateral"
     if any(b.shape != want for b in blk):
         raise NektarIOError(f"field {field!r} is not on the mesh's {want} elements")
     ids = np.concatenate([b.ids for b in blk])
+    nmv = modes[:, 0]
+    # one order everywhere: ints; variable order: per-element arrays (Assembly takes either)
+    nm = int(nmv[0]) if np.all(nmv == nmv[0]) else nmv.astype(np.int32)
     return (mesh.element_coords(ids), mesh.vertex_ids(ids),
             field_coefficients(blocks, field, ids), nm, nm + 1)
 
@@ -348,6 +370,14 @@ def seq_string(ids):
     return "".join(out)
 
 
+def _modes_string(nmv, dim, mixed):
+    """NUMMODESPERDIR value (FieldIOXml.cpp:289-330): the common counts, or every element's
+    counts in element order."""
+    if not mixed:
+        return "UNIORDER:" + ",".join([str(int(nmv[0]))] * dim)
+    return "MIXORDER:" + ",".join(str(int(m)) for m in nmv for _ in range(dim))
+
+
 def b64z_encode(data):
     """"B64Z-LittleEndian" text of float64 data: a zlib stream of the little-endian values
     (default compression level), padded with zero bytes to a multiple of 3, in base64
@@ -361,9 +391,9 @@ def b64z_encode(data):
 def write_fld(path, fields, element_ids, nm, shape="hex", metadata=None, compressed=True):
     """Write one ELEMENTS block in the reference's .fld XML layout (FieldIOXml::v_Write,
     FieldIOXml.cpp:86-330, with FieldIO::AddInfoTag's Metadata section, FieldIO.cpp:
-    348-393): fields = {name: element-major coefficients, len(element_ids) * nm^dim values
-    in each element's local frame}, in the file's field order; modified basis, UNIORDER nm
-    modes per direction.  metadata: extra {key: value} entries of the Metadata section
+    348-393): fields = {name: element-major coefficients in each element's local frame (an
+    array, or a list of per-element arrays)}, in the file's field order; modified basis
+    with nm modes per direction: an int (UNIORDER) or one order per element (MIXORDER).  metadata: extra {key: value} entries of the Metadata section
     (e.g. Time, Kinvis).  compressed=False writes the data as text, which the reference
     reads as well."""
     if shape not in _SHAPE_NAME:
@@ -374,9 +404,15 @@ def write_fld(path, fields, element_ids, nm, shape="hex", metadata=None, compres
         raise NektarIOError("write_fld: no elements or no fields")
     if not all(re.fullmatch(r"[A-Za-z_][\w]*", k) for k in list(fields) + list(metadata or {})):
         raise NektarIOError("write_fld: field and metadata names must be XML names")
-    n = len(ids) * nm ** dim
+    mixed = np.ndim(nm) != 0
+    nmv = np.asarray(nm, dtype=np.int64).ravel() if mixed else np.full(len(ids), int(nm))
+    if nmv.size != len(ids) or np.any(nmv < 1):
+        raise NektarIOError(f"write_fld: {len(ids)} per-element orders expected")
+    n = int(np.sum(nmv ** dim))
     data = []
     for name, v in fields.items():
+        if isinstance(v, (list, tuple)):
+            v = np.concatenate([np.ravel(x) for x in v]) if v else np.zeros(0)
         v = np.asarray(v, dtype=np.float64).ravel()
         if v.size != n:
             raise NektarIOError(f"field {name!r}: {v.size} values, {n} expected")
@@ -392,7 +428,7 @@ def write_fld(path, fields, element_ids, nm, shape="hex", metadata=None, compres
     lines.append("    </Metadata>")
     attrs = (f'FIELDS="{",".join(fields)}" SHAPE="{sname}" '
              f'BASIS="{",".join(["Modified_A"] * dim)}" '
-             f'NUMMODESPERDIR="UNIORDER:{",".join([str(nm)] * dim)}" ID="{seq_string(ids)}"')
+             f'NUMMODESPERDIR="{_modes_string(nmv, dim, mixed)}" ID="{seq_string(ids)}"')
     if compressed:
         body = b64z_encode(data)
         attrs += ' COMPRESSED="B64Z-LittleEndian" BITSIZE="64"'

@@ -132,7 +132,8 @@ def test_text_format_mesh_and_field(tmp_path):
 
 @pytest.mark.parametrize("attrs,msg", [
     ('FIELDS="u" SHAPE="Triangle" BASIS="Modified_A,Modified_B" NUMMODESPERDIR="UNIORDER:2,2" ID="0"', "shape"),
-    ('FIELDS="u" SHAPE="Quadrilateral" BASIS="Modified_A,Modified_A" NUMMODESPERDIR="MIXORDER:2,2" ID="0"', "UNIORDER"),
+    ('FIELDS="u" SHAPE="Quadrilateral" BASIS="Modified_A,Modified_A" NUMMODESPERDIR="MIXORDER:2,2,3" ID="0"', "MIXORDER"),
+    ('FIELDS="u" SHAPE="Quadrilateral" BASIS="Modified_A,Modified_A" NUMMODESPERDIR="MIXORDER:2,2" ID="0-1"', "MIXORDER"),
     ('FIELDS="u" SHAPE="Quadrilateral" BASIS="Modified_A,Modified_A" NUMMODESPERDIR="UNIORDER:2,2" ID="0-1"', "length"),
 ])
 def test_fld_rejects_what_it_cannot_read(tmp_path, attrs, msg):
@@ -228,3 +229,48 @@ def test_writer_rejects_bad_input(tmp_path):
         N.write_fld(out, {"u": np.zeros(8)}, [0], 2, "tet")
     with pytest.raises(N.NektarIOError, match="XML names"):
         N.write_fld(out, {"u v": np.zeros(8)}, [0], 2, "hex")
+
+
+def test_mixorder_file_of_the_reference(tmp_path):
+    """A MIXORDER block of the reference (surface fields of a variable-order run, quads of 2
+    and 6 modes): the data split by each element's counts fills the block exactly, and the
+    block written back is byte-identical to the reference's (NUMMODESPERDIR list, ids,
+    compressed data)."""
+    src = os.path.join(HERE, "naca0012_3D_bnd_P0000015.fld")
+    b = N.read_fld(src)[0]
+    assert b.nummodes is None and b.shape == "Quadrilateral"
+    assert set(np.unique(b.modes)) == {2, 6} and np.all(b.modes[:, 0] == b.modes[:, 1])
+    assert len(b.fields) == 7
+    for f in b.fields:
+        assert [len(r) for r in b.data[f]] == list(b.modes.prod(axis=1))
+    # Shear_mag is the magnitude of the shear vector at the vertex modes (which carry the
+    # vertex values of the modified basis)
+    # vertex modes (p, q) in {0, 1}^2 at p + q nm
+    vm = [np.array([0, 1, n, n + 1]) for n in b.modes[:, 0]]
+    sx, sy, sz, sm = (np.array([r[v] for r, v in zip(b.data[f], vm)]) for f in
+                      ("Shear_x", "Shear_y", "Shear_z", "Shear_mag"))
+    assert np.allclose(sm, np.sqrt(sx ** 2 + sy ** 2 + sz ** 2), rtol=1e-8, atol=1e-12)
+    out = tmp_path / "m.fld"
+    N.write_fld(str(out), {f: b.data[f] for f in b.fields}, b.ids, b.modes[:, 0], "quad")
+    assert _elements(str(out)) == _elements(src)
+    c = N.read_fld(str(out))[0]
+    for f in b.fields:
+        assert all(np.array_equal(x, y) for x, y in zip(c.data[f], b.data[f]))
+
+
+def test_mixorder_round_trip_into_variable_order_arrays(tmp_path):
+    """A hex field of mixed order written and read back gives per-element orders, which the
+    variable-order assembly takes as they are."""
+    m = N.read_mesh(os.path.join(HERE, "Hex_channel_C0helmsmoothing.xml"))
+    nmv = np.array([3, 4, 5, 3, 4, 5, 6, 2])
+    rng = np.random.default_rng(2)
+    u = [rng.standard_normal(n ** 3) for n in nmv]
+    out = tmp_path / "v.fld"
+    N.write_fld(str(out), {"u": u}, range(8), nmv, "hex")
+    blocks = N.read_fld(str(out))
+    X, ids, c, nm, nq = N.expansion_arrays(m, blocks, "u")
+    assert list(nm) == list(nmv) and list(nq) == list(nmv + 1)
+    assert np.array_equal(c, np.concatenate(u))
+    from ithaca_hip import assembly
+    l2g, sign, bnd, ng = assembly.assembly_map(ids, nm, "hex")
+    assert len(l2g) == len(c)

@@ -1,6 +1,6 @@
 """GPU parity of the batched online reduced solves against the FP64 CPU oracle.
 
-Bar (north star): every solution within 1e-10 relative error of the FP64 oracle, for the
+Bar: every solution within 1e-10 relative error of the FP64 oracle, for the
 FP64 LU path and for FP32 LU + FP64 iterative refinement."""
 import numpy as np
 import pytest

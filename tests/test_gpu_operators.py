@@ -332,3 +332,33 @@ def test_iprod_deriv_is_weighted_transpose_of_physderiv(shape, nm, nq):
     g = pd.apply(bwd.apply(u)).view(dim, E * Q)
     rhs = sum(torch.dot(v.view(dim, E * Q)[c], wj * g[c]).item() for c in range(dim))
     assert abs(lhs - rhs) <= 1e-12 * max(abs(lhs), 1.0)
+
+
+@pytest.mark.parametrize("shape,nm,nq", [("hex", 6, 7), ("hex", 3, 5), ("quad", 5, 6),
+                                         ("quad", 8, 16)])
+def test_element_counts(shape, nm, nq):
+    """One element, a few, and counts that leave the last workgroup partly filled: every
+    operator equals the oracle on the first E elements of a deformed mesh."""
+    rng = np.random.default_rng(21)
+    full = mesh.hex_box(6, 0.1, rng) if shape == "hex" else mesh.quad_box(12, 0.1, rng)
+    geom = O.geom_hex if shape == "hex" else O.geom_quad
+    b = O.basis(nm, nq)
+    dim = 3 if shape == "hex" else 2
+    S = shape.capitalize()
+    for E in (1, 2, 3, 5, 17, 130):
+        print(f"[case] {shape} nm={nm} nq={nq} nElmt={E}", flush=True)
+        verts = np.ascontiguousarray(full[:E])
+        jac, df = geom(verts, nq, b["z"], True)
+        u = rng.uniform(-1, 1, E * nm ** dim)
+        f = rng.uniform(-1, 1, E * nq ** dim)
+        ref_b = O.bwdtrans(shape, b, E, u)
+        ref_i = O.iproduct(shape, b, E, True, jac, f)
+        ref_h = O.helmholtz(shape, b, E, True, jac, df, 1.5, u)
+        for prec, tol, dt in ((H.FP64, TOL64, torch.float64), (H.FP32, TOL32 * 10, torch.float32)):
+            got = _op(f"BwdTrans_{S}_Regular", nm, nq, E, b, prec=prec).apply(_dev(u, dt))
+            assert rel(got.double().cpu().numpy(), ref_b) < tol
+            got = _op(f"IProduct_{S}_Deformed", nm, nq, E, b, jac, prec=prec).apply(_dev(f, dt))
+            assert rel(got.double().cpu().numpy(), ref_i) < tol
+            op = _op(f"Helmholtz_{S}_Deformed", nm, nq, E, b, jac, df, prec=prec)
+            op.set_lambda(1.5)
+            assert rel(op.apply(_dev(u, dt)).double().cpu().numpy(), ref_h) < tol

@@ -117,3 +117,39 @@ def test_full_batch_4096():
         err = _max_rel(bt, x.cpu().numpy(), ref)
         print(f"method {method}: max rel err vs oracle {err:.3e}")
         assert err < TOL
+
+
+@pytest.mark.parametrize("method", [rom.LU_FP64, rom.LU_FP32_IR])
+def test_status_codes_and_degenerate_batches(method):
+    """A singular instance (every A_q zero) reports a zero pivot (status 1) without
+    disturbing the other instances; one-term systems (Q = 1) of every size solve; a
+    refinement budget of zero steps leaves the FP32 solutions unconverged (status 2, no
+    refinement) and the default budget converges them; a batch of one instance solves."""
+    bt = rom.affine_batch(n_inst=40, seed=5, nmin=1, nmax=256, qmin=1, qmax=3)
+    k = 7
+    Aq, _, _ = R.instance(bt, k)
+    a0 = int(bt["a_off"][k])
+    bt["A"][a0:a0 + Aq.size] = 0.0
+    ok = [i for i in range(40) if i != k]
+    print(f"[case] status codes method={method}", flush=True)
+    B = rom.Batch(bt["N"], bt["Q"])
+    A, th, b = _dev(bt["A"]), _dev(bt["theta"]), _dev(bt["b"])
+    iters = torch.zeros(B.n, dtype=torch.int32, device="cuda")
+    status = torch.full((B.n,), -1, dtype=torch.int32, device="cuda")
+    x = B.solve(A, th, b, method=method, iters=iters, status=status).cpu().numpy()
+    st = status.cpu().numpy()
+    assert st[k] == 1 and np.all(st[ok] == 0)
+    ref = R.solve(bt, idx=ok)
+    for i in ok:
+        _, _, sl = R.instance(bt, i)
+        assert np.linalg.norm(x[sl] - ref[sl]) <= TOL * np.linalg.norm(ref[sl])
+    if method == rom.LU_FP32_IR:
+        status.fill_(-1)
+        B.solve(A, th, b, method=method, max_iter=0, iters=iters, status=status)
+        st0 = status.cpu().numpy()
+        assert np.all(st0[ok] == 2) and int(iters.max()) == 0 and st0[k] == 1
+    one = rom.affine_batch(n_inst=1, seed=6, nmin=256, nmax=256, qmin=1, qmax=1)
+    B1 = rom.Batch(one["N"], one["Q"])
+    x1 = B1.solve(_dev(one["A"]), _dev(one["theta"]), _dev(one["b"]), method=method)
+    r1 = R.solve(one)
+    assert np.linalg.norm(x1.cpu().numpy() - r1) <= TOL * np.linalg.norm(r1)

@@ -129,6 +129,25 @@ class Mesh:
                 dtype=np.int64).reshape(-1, 4)
 
     # -- local vertex order --------------------------------------------------------
+    def element_orders(self, element_ids=None, field=None):
+        """NUMMODES of each element (file order, or the given ids) from the EXPANSIONS
+        entries and the composites they name (the reference's per-composite expansions of
+        a variable-order session); field restricts to entries listing that field."""
+        ids = self.elements if element_ids is None else np.asarray(element_ids, dtype=np.int64)
+        order = {}
+        for e in self.expansions:
+            if field is not None and e["fields"] != [""] and field not in e["fields"]:
+                continue
+            for c in _composite_ids(e["composite"]):
+                if int(c) not in self.composites:
+                    raise NektarIOError(f"expansion names composite {int(c)}, not in the mesh")
+                for el in _composite_ids(self.composites[int(c)]):
+                    order[int(el)] = e["nummodes"]
+        try:
+            return np.array([order[int(i)] for i in ids], dtype=np.int32)
+        except KeyError as exc:
+            raise NektarIOError(f"element {exc.args[0]} has no expansion") from None
+
     def _quad_vertices(self, eids):
         verts = []
         for j in range(4):
@@ -196,6 +215,13 @@ class Mesh:
             return np.array([pos[int(e)] for e in element_ids])
         except KeyError as exc:
             raise NektarIOError(f"element {exc.args[0]} is not in the mesh") from None
+
+
+def _composite_ids(text):
+    """Ids of a composite or of an expansion's COMPOSITE attribute: every "X[...]" group of
+    the text in the range syntax ("H[0-7]", "Q[1,3-5] Q[9]", "C[0,2]")."""
+    return np.concatenate([_id_list(g) for g in re.findall(r"\[([^\]]*)\]", text or "")]
+                          or [np.zeros(0, dtype=np.int64)])
 
 
 def read_mesh(path):

@@ -274,3 +274,23 @@ def test_mixorder_round_trip_into_variable_order_arrays(tmp_path):
     from ithaca_hip import assembly
     l2g, sign, bnd, ng = assembly.assembly_map(ids, nm, "hex")
     assert len(l2g) == len(c)
+
+
+def test_variable_order_session(tmp_path):
+    """A session with two expansions on two composites (NUMMODES 4 on elements 0-3, 6 on
+    4-7): the per-element orders feed the variable-order numbering, whose random global
+    fields are continuous across the element faces in the session's own local frames."""
+    text = open(os.path.join(HERE, "Hex_channel_C0helmsmoothing.xml")).read()
+    text = text.replace('<C ID="0"> H[0-7] </C>', '<C ID="0"> H[0-3] </C>\n<C ID="7"> H[4-7] </C>')
+    text = text.replace('<E COMPOSITE="C[0]" NUMMODES="4" FIELDS="u,v,w,p" TYPE="MODIFIED" />',
+                        '<E COMPOSITE="C[0]" NUMMODES="4" FIELDS="u,v,w,p" TYPE="MODIFIED" />\n'
+                        '<E COMPOSITE="C[7]" NUMMODES="6" FIELDS="u,v,w,p" TYPE="MODIFIED" />')
+    p = tmp_path / "varp.xml"
+    p.write_text(text)
+    m = N.read_mesh(str(p))
+    nmv = m.element_orders(field="u")
+    assert list(nmv) == [4, 4, 4, 4, 6, 6, 6, 6]
+    from test_assembly_varp import _continuity
+    assert _continuity("hex", m.element_coords(), m.vertex_ids(), nmv, 8) < 1e-12
+    with pytest.raises(N.NektarIOError, match="no expansion"):
+        N.read_mesh(os.path.join(HERE, "Hex_channel_C0helmsmoothing.xml")).element_orders([99])

@@ -21,6 +21,7 @@
 #include <cmath>
 #include <cstring>
 #include <string>
+#include <tuple>
 #include <unordered_map>
 #include <vector>
 
@@ -608,8 +609,8 @@ struct imf_asm_s {
     // solver workspace (allocated on the first solve)
     double *d_ws = nullptr, *d_loc = nullptr, *d_part = nullptr, *d_dots = nullptr;
     int nparts = 0;
-    // operators and lambdas the cached diagonal belongs to
-    std::vector<std::pair<const void *, double>> diag_key;
+    // operators, their geometry versions and lambdas the cached diagonal belongs to
+    std::vector<std::tuple<const void *, unsigned long, double>> diag_key;
 };
 
 namespace {
@@ -856,9 +857,10 @@ int imf_helmsolve_cg_groups_device(imf_asm a, int nops, const imf_op *ops, const
         a->diag_key.clear();
     }
     double *r = a->d_ws, *p = r + ng, *q = p + ng, *w = q + ng, *sv = w + ng, *dinv = sv + ng;
-    // assembled diagonal (Jacobi preconditioner), cached per operators and lambdas
-    std::vector<std::pair<const void *, double>> key;
-    for (int g = 0; g < nops; ++g) key.emplace_back((const void *)ops[g], ops[g]->lambda);
+    // assembled diagonal (Jacobi preconditioner), cached per operators, geometry and lambdas
+    std::vector<std::tuple<const void *, unsigned long, double>> key;
+    for (int g = 0; g < nops; ++g)
+        key.emplace_back((const void *)ops[g], ops[g]->geom_version, ops[g]->lambda);
     if (a->diag_key != key) {
         e0 = 0;
         for (int g = 0; g < nops; ++g) {

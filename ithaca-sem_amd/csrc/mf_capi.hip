@@ -71,6 +71,7 @@ int maybe_pack(imf_op op)
     if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
     if (e != hipSuccess) return hip_fail(e, "pack_geometry");
     op->geom_ready = true;
+    ++op->geom_version;
     return IMF_OK;
 }
 

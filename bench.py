@@ -414,6 +414,59 @@ def pod_cpu_baseline(cpu_slice, G64, ev, r):
             "eig_rel_err_top_r_gpu_vs_cpu": float(np.max(np.abs(ev[:r] - ev_cpu[:r])
                                                          / ev_cpu[:r]))}
 
+def regular_bench(args):
+    """Helmholtz_Hex_Regular on 48^3 undistorted hexes of [0,1]^3 (P=5, 7 GLL points,
+    lambda = 10, FP64, 50 applies): the per-element constant geometry (J and the 6 metric
+    terms, 7 doubles per element) replaces the 7 doubles per quadrature point of the
+    deformed path, so the apply streams 8*(2M + 7) bytes per element; CPU oracle on an
+    8192-element sample."""
+    import numpy as np
+    import torch
+
+    import ithaca_hip as H
+    from ithaca_hip import mesh
+
+    w = mesh.workload_hex_helmholtz(args.n)
+    nm, nq, E, lam = w["nm"], w["nq"], w["nElmt"], w["lam"]
+    b = H.basis_modified_a(nm, nq)
+    verts = mesh.hex_box(args.n, 0.0)
+    gj, gd = H.geom_device("hex", torch.from_numpy(verts).cuda(), nq, b["z"], False)
+    op = H.Operator("Helmholtz_Hex_Regular", nm, nq, E, basis=b)
+    op.set_jac(gj)
+    op.set_df(gd)
+    op.set_lambda(lam)
+    x = torch.from_numpy(w["u"]).cuda()
+    y = torch.empty(op.out_size, dtype=torch.float64, device="cuda")
+    ms = _event_time(lambda: op.apply(x, y), 50, 5)
+    ab = 8 * (2 * nm ** 3 + 7) * E
+    res = {"key": "Helmholtz_Hex_Regular", "workload": f"{E} regular hexes, P=5, 7 GLL pts, "
+                                                       "lambda=10",
+           "us_per_apply": ms * 1e3, "gdof_per_s": E * nm ** 3 / (ms * 1e-3) / 1e9,
+           "algorithmic_bytes": ab, "achieved_GBs": ab / (ms * 1e-3) / 1e9,
+           "speedup_over_deformed_path": None}
+    del op, x, y, gj, gd
+    if not args.no_cpu:
+        import oracle as O
+        threads = min(16, os.cpu_count() or 1)
+        O.set_threads(threads)
+        Es = 8192
+        bo = O.basis(nm, nq)
+        jac, df = O.geom_hex(np.ascontiguousarray(verts[:Es]), nq, bo["z"], False)
+        u = np.ascontiguousarray(w["u"][:Es * nm ** 3])
+        O.helmholtz("hex", bo, Es, False, jac, df, lam, u)
+        ts = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            O.helmholtz("hex", bo, Es, False, jac, df, lam, u)
+            ts.append(time.perf_counter() - t0)
+        res["cpu_baseline"] = {"gdof_per_s": Es * nm ** 3 / float(np.median(ts)) / 1e9,
+                               "cores": threads, "kind": "port",
+                               "sample": f"{Es} elements, 3 applies, median; oracle "
+                                         "restatement of HelmholtzHexImpl (regular)"}
+    torch.cuda.empty_cache()
+    return res
+
+
 def deriv_bench(args):
     """PhysDeriv_Hex_Deformed and IProductWRTDerivBase_Hex_Deformed on the 48^3 mesh
     (FP64, 50 applies each), with their algorithmic bytes per apply and the CPU oracle
@@ -813,6 +866,9 @@ def main():
         section("quad_helmholtz", quad_bench)
         section("quad_helmsolve", quad_helmsolve_bench)
         section("deriv_operators", deriv_bench)
+        section("regular_hex", regular_bench)
+        out["regular_hex"]["speedup_over_deformed_path"] = (
+            out["regular_hex"]["gdof_per_s"] / out["value"])
         section("helmsolve", helmsolve_bench)
         section("helmsolve_varp", varp_helmsolve_bench)
         if not args.no_pod:

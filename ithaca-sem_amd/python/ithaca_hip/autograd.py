@@ -10,6 +10,10 @@ No reference counterpart: the reference has no derivative capability on this pat
 * ``L2Misfit``      0.5 * ||B u_hat - f||^2 in the quadrature-weighted L2 norm
   (sum_q w_q J_q r_q^2); its gradient is IProductWRTBase(B u_hat - f), evaluated by the
   IProduct kernel itself.
+* ``HelmSolveFn``   u = A(lambda)^{-1} rhs, the assembled Helmholtz solve with homogeneous
+  Dirichlet modes (A = K + lambda M on the free modes, symmetric).  Backward is one more
+  solve, w = A^{-1} dL/du, and returns dL/drhs = w and dL/dlambda = -w^T M u, with M u
+  assembled from the element mass operator.
 """
 import torch
 
@@ -60,3 +64,35 @@ class L2Misfit(torch.autograd.Function):
     def backward(ctx, g):
         (r,) = ctx.saved_tensors
         return g * ctx.ipr.apply(r), None, None, None, None
+
+
+class HelmSolveFn(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, rhs, lam, asm, op, mass, tol):
+        op.set_lambda(float(lam))
+        x, _, _ = asm.helmsolve(op, rhs.detach().contiguous(), tol=tol)
+        ctx.asm, ctx.op, ctx.mass, ctx.tol, ctx.lam = asm, op, mass, tol, float(lam)
+        ctx.lam_like = lam if torch.is_tensor(lam) else None
+        ctx.save_for_backward(x)
+        return x
+
+    @staticmethod
+    def backward(ctx, g):
+        (x,) = ctx.saved_tensors
+        ctx.op.set_lambda(ctx.lam)
+        w, _, _ = ctx.asm.helmsolve(ctx.op, g.contiguous(), tol=ctx.tol)
+        glam = None
+        if ctx.needs_input_grad[1]:
+            if ctx.mass is None:
+                raise ValueError("HelmSolveFn: a mass operator is needed for d/dlambda")
+            mu = ctx.asm.assemble(ctx.mass.apply(ctx.asm.global_to_local(x)))
+            ref = ctx.lam_like
+            glam = (-torch.dot(w, mu)).to(device=ref.device, dtype=ref.dtype).reshape(ref.shape)
+        return w, glam, None, None, None, None
+
+
+def helm_solve(asm, op, rhs, lam, mass=None, tol=1e-12):
+    """u = A(lambda)^{-1} rhs through the assembly `asm` and Helmholtz operator `op`
+    (differentiable in rhs and, given the element mass operator `mass`, in lambda, a
+    0-d tensor)."""
+    return HelmSolveFn.apply(rhs, lam, asm, op, mass, tol)

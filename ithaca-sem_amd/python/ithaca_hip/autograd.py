@@ -66,10 +66,15 @@ class L2Misfit(torch.autograd.Function):
         return g * ctx.ipr.apply(r), None, None, None, None
 
 
+def _set_lambda(op, lam):
+    for o in (op if isinstance(op, (list, tuple)) else [op]):
+        o.set_lambda(lam)
+
+
 class HelmSolveFn(torch.autograd.Function):
     @staticmethod
     def forward(ctx, rhs, lam, asm, op, mass, tol):
-        op.set_lambda(float(lam))
+        _set_lambda(op, float(lam))
         x, _, _ = asm.helmsolve(op, rhs.detach().contiguous(), tol=tol)
         ctx.asm, ctx.op, ctx.mass, ctx.tol, ctx.lam = asm, op, mass, tol, float(lam)
         ctx.lam_like = lam if torch.is_tensor(lam) else None
@@ -79,13 +84,13 @@ class HelmSolveFn(torch.autograd.Function):
     @staticmethod
     def backward(ctx, g):
         (x,) = ctx.saved_tensors
-        ctx.op.set_lambda(ctx.lam)
+        _set_lambda(ctx.op, ctx.lam)
         w, _, _ = ctx.asm.helmsolve(ctx.op, g.contiguous(), tol=ctx.tol)
         glam = None
         if ctx.needs_input_grad[1]:
             if ctx.mass is None:
                 raise ValueError("HelmSolveFn: a mass operator is needed for d/dlambda")
-            mu = ctx.asm.assemble(ctx.mass.apply(ctx.asm.global_to_local(x)))
+            mu = ctx.asm.assemble(ctx.asm.apply_local(ctx.mass, ctx.asm.global_to_local(x)))
             ref = ctx.lam_like
             glam = (-torch.dot(w, mu)).to(device=ref.device, dtype=ref.dtype).reshape(ref.shape)
         return w, glam, None, None, None, None
@@ -94,5 +99,6 @@ class HelmSolveFn(torch.autograd.Function):
 def helm_solve(asm, op, rhs, lam, mass=None, tol=1e-12):
     """u = A(lambda)^{-1} rhs through the assembly `asm` and Helmholtz operator `op`
     (differentiable in rhs and, given the element mass operator `mass`, in lambda, a
-    0-d tensor)."""
+    0-d tensor).  For a variable-order assembly, `op` and `mass` are lists with one
+    operator per run of equal-order elements, as Assembly.helmsolve takes them."""
     return HelmSolveFn.apply(rhs, lam, asm, op, mass, tol)

@@ -158,3 +158,47 @@ def test_helmsolve_adjoint():
     fd = (Jof(rhs.detach() + H_FD * d, 3.0) - Jof(rhs.detach() - H_FD * d, 3.0)) / (2 * H_FD)
     an = torch.dot(rhs.grad, d).item()
     assert abs(fd - an) <= 1e-6 * abs(an)
+
+
+def test_helmsolve_adjoint_variable_order():
+    """The same adjoint over a variable-order assembly (one Helmholtz and one mass operator
+    per run of equal-order elements): gradients equal sparse direct solves of the
+    variable-order system."""
+    import scipy.sparse.linalg as spla
+    from test_assembly_cpu import _mesh
+    from test_assembly_varp import varp_system
+    from ithaca_hip import assembly
+    from ithaca_hip.autograd import helm_solve
+    verts, ids = _mesh(2, True, seed=3, jitter=0.05)
+    nmv = np.repeat([4, 6], 4)  # elements are z-major: lower and upper layers
+    print("[case] helmsolve adjoint, variable order", flush=True)
+    ops, masses = [], []
+    for e0, e1, nm in ((0, 4, 4), (4, 8, 6)):
+        nq = nm + 1
+        b = H.basis_modified_a(nm, nq)
+        jac, df = H.geom_device("hex", _dev(verts[e0:e1]), nq, b["z"], True)
+        op = H.Operator("Helmholtz_Hex_Deformed", nm, nq, e1 - e0, basis=b)
+        op.set_jac(jac)
+        op.set_df(df)
+        ms = H.Operator("Mass_Hex_Deformed", nm, nq, e1 - e0, basis=b)
+        ms.set_jac(jac)
+        ops.append(op)
+        masses.append(ms)
+    asm = assembly.Assembly(ids, nmv)
+    ng = asm.nglobal
+    rng = np.random.default_rng(8)
+    rhs = _dev(rng.standard_normal(ng)).requires_grad_(True)
+    lam = torch.tensor(2.0, dtype=torch.float64, requires_grad=True)
+    l = _dev(rng.standard_normal(ng))
+    torch.dot(l, helm_solve(asm, ops, rhs, lam, masses, tol=1e-13)).backward()
+    zero = lambda X: np.zeros(len(X))  # noqa: E731
+    A, _, dm, _, _ = varp_system(verts, ids, nmv, 2.0, zero)
+    A3 = varp_system(verts, ids, nmv, 3.0, zero)[0]
+    free = ~dm
+    Af = A[free][:, free].tocsc()
+    w = spla.spsolve(Af, l.cpu().numpy()[free])
+    u = spla.spsolve(Af, rhs.detach().cpu().numpy()[free])
+    g = rhs.grad.cpu().numpy()
+    assert np.linalg.norm(g[free] - w) <= 1e-9 * np.linalg.norm(w)
+    gl = -w @ ((A3 - A)[free][:, free] @ u)
+    assert abs(lam.grad.item() - gl) <= 1e-8 * abs(gl)

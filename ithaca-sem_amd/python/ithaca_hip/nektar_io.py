@@ -289,6 +289,8 @@ class FieldBlock:
         kind, _, vals = spec.partition(":")
         vals = [int(x) for x in vals.split(",") if x.strip()]
         dim, E = len(self.basis), len(self.ids)
+        if E == 0:
+            raise NektarIOError("ELEMENTS block without element ids")
         if kind == "UNIORDER" and len(vals) == dim:
             self.nummodes = vals
             self.modes = np.tile(np.array(vals, dtype=np.int64), (E, 1))

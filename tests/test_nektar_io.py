@@ -134,6 +134,7 @@ def test_text_format_mesh_and_field(tmp_path):
     ('FIELDS="u" SHAPE="Triangle" BASIS="Modified_A,Modified_B" NUMMODESPERDIR="UNIORDER:2,2" ID="0"', "shape"),
     ('FIELDS="u" SHAPE="Quadrilateral" BASIS="Modified_A,Modified_A" NUMMODESPERDIR="MIXORDER:2,2,3" ID="0"', "MIXORDER"),
     ('FIELDS="u" SHAPE="Quadrilateral" BASIS="Modified_A,Modified_A" NUMMODESPERDIR="MIXORDER:2,2" ID="0-1"', "MIXORDER"),
+    ('FIELDS="u" SHAPE="Quadrilateral" BASIS="Modified_A,Modified_A" NUMMODESPERDIR="UNIORDER:2,2" ID=""', "element ids"),
     ('FIELDS="u" SHAPE="Quadrilateral" BASIS="Modified_A,Modified_A" NUMMODESPERDIR="UNIORDER:2,2" ID="0-1"', "length"),
 ])
 def test_fld_rejects_what_it_cannot_read(tmp_path, attrs, msg):

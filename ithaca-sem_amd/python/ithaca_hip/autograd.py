@@ -100,5 +100,6 @@ def helm_solve(asm, op, rhs, lam, mass=None, tol=1e-12):
     """u = A(lambda)^{-1} rhs through the assembly `asm` and Helmholtz operator `op`
     (differentiable in rhs and, given the element mass operator `mass`, in lambda, a
     0-d tensor).  For a variable-order assembly, `op` and `mass` are lists with one
-    operator per run of equal-order elements, as Assembly.helmsolve takes them."""
+    operator per run of equal-order elements, as Assembly.helmsolve takes them.  The
+    operators' lambda is set to `lam` (and left there)."""
     return HelmSolveFn.apply(rhs, lam, asm, op, mass, tol)

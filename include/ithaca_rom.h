@@ -18,6 +18,8 @@
 
 #include <stddef.h>
 
+#include "ithaca_mf.h"
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -26,7 +28,10 @@ typedef struct irom_batch_s *irom_batch;
 
 enum irom_method {
     IROM_LU_FP64 = 0,    /* assemble + LU (partial pivoting) + solve, all FP64 */
-    IROM_LU_FP32_IR = 1  /* FP32 LU + FP64 residuals / iterative refinement */
+    IROM_LU_FP32_IR = 1, /* FP32 LU + FP64 residuals / iterative refinement */
+    IROM_LU_FP32_IR_FALLBACK = 2  /* as IROM_LU_FP32_IR; every instance whose refinement
+                                     does not converge is solved again by FP64 LU in a
+                                     second persistent launch (status 3) */
 };
 
 /* ---- batched online solves: A(mu_i) x_i = b_i, i < nInst ----------------------------
@@ -46,9 +51,10 @@ int irom_batch_layout(irom_batch b, long *a_off, long *t_off, long *v_off, size_
 
 /* Solve every instance (transpose != 0 solves A(mu)^T x = b, the adjoint system).
  * tol: relative backward-error target of the refinement (IROM_LU_FP32_IR); max_iter caps it.
- * One persistent launch (plus a counter reset).  Per-instance refinement iterations and
- * status (0 ok, 1 zero pivot, 2 refinement not converged) go to d_iters / d_status when
- * non-NULL (int arrays of nInst). */
+ * One persistent launch (plus a counter reset; two for IROM_LU_FP32_IR_FALLBACK).
+ * Per-instance refinement iterations and status (0 ok, 1 zero pivot, 2 refinement not
+ * converged, 3 refinement not converged and solved by the FP64 fallback) go to d_iters /
+ * d_status when non-NULL (int arrays of nInst). */
 int irom_batch_solve(irom_batch b, int method, int transpose, const double *d_A,
                      const double *d_theta, const double *d_b, double *d_x, double tol,
                      int max_iter, int *d_iters, int *d_status, void *stream);
@@ -95,14 +101,38 @@ int irom_gram(int mode, int m, int n, long K, const double *d_X, long ldx, const
               void *stream);
 const char *irom_gram_last_error(void);
 
+/* ---- Galerkin projection of one affine operator term ---------------------------------
+ * C (row-major r x r, ldc) = V^T (A V) for the r basis vectors of V (column-major N x r:
+ * vector j at d_V + j*ldv, N = imf_in_size(op)) and the FP64 matrix-free operator op, a
+ * term that maps coefficients to coefficients ("Helmholtz_Hex_*" with its lambda, e.g.
+ * lambda = 0 for the stiffness term, or "Mass_Hex_*" for the mass term).  d_W (N x r,
+ * ldw) receives A V: one launch applies op to all r vectors, staging each element's
+ * geometry once; then C = V^T W by irom_gram in `mode` (IROM_GRAM_SYMMETRIC_RESULT may be
+ * OR-ed in: both terms are symmetric).  d_work holds irom_gram_workspace_size(r, r, N)
+ * bytes.  Replaces the per-term dense products RB^T A RB of the reference's offline phase
+ * (CoupledLinearNS_ROM.cpp:2788-2802); the reduced operator of a parameter is then
+ * sum_q theta_q C_q (gen_affine_mat_proj, :4694-4704).  Asynchronous; allocates nothing.
+ * Message of a failure via irom_project_last_error(). */
+int irom_galerkin_project(imf_op op, int r, const double *d_V, long ldv, double *d_W, long ldw,
+                          double *d_C, long ldc, int mode, void *d_work, size_t work_bytes,
+                          void *stream);
+const char *irom_project_last_error(void);
+
 /* ---- POD offline: FP64 symmetric eigen-decomposition --------------------------------
- * Eigenpairs of the symmetric positive semi-definite n x n matrix d_G (row-major, device):
- * h_evals (host, n) in descending order, d_evecs (device, column-major n x n, column j the
- * eigenvector of h_evals[j]).  One-sided Jacobi, rotations while |w_p.w_q| > tol ||w_p|| ||w_q||.
- * Replaces the reference's BDCSVD of the snapshot matrix (CoupledLinearNS_ROM.cpp:3925)
- * through the method of snapshots.  Synchronous; allocates its own workspace. */
+ * Eigenpairs of the symmetric positive semi-definite n x n matrix d_G (row-major, device,
+ * n <= 4096): h_evals (host, n) in descending order, d_evecs (device, column-major n x n,
+ * column j the eigenvector of h_evals[j]).  Blocked one-sided Jacobi in one persistent
+ * launch; pairs rotate while |w_p.w_q| > tol ||w_p|| ||w_q||; at most max_sweeps (<= 64)
+ * sweeps, the count in *sweeps.  Replaces the reference's BDCSVD of the snapshot matrix
+ * (CoupledLinearNS_ROM.cpp:3925) through the method of snapshots.  Synchronous.
+ * irom_eig_sym keeps a workspace per device (allocated on the first call that needs a
+ * larger one, kept until the process ends); irom_eig_sym_ws takes the caller's workspace
+ * of irom_eig_workspace_size(n) bytes and allocates nothing. */
+size_t irom_eig_workspace_size(int n);
 int irom_eig_sym(int n, const double *d_G, double *h_evals, double *d_evecs, double tol,
                  int max_sweeps, int *sweeps, void *stream);
+int irom_eig_sym_ws(int n, const double *d_G, double *h_evals, double *d_evecs, double tol,
+                    int max_sweeps, int *sweeps, void *d_work, size_t work_bytes, void *stream);
 const char *irom_eig_last_error(void);
 
 const char *irom_last_error(void);

@@ -857,7 +857,8 @@ int imf_helmsolve_cg_groups_device(imf_asm a, int nops, const imf_op *ops, const
         a->diag_key.clear();
     }
     double *r = a->d_ws, *p = r + ng, *q = p + ng, *w = q + ng, *sv = w + ng, *dinv = sv + ng;
-    // assembled diagonal (Jacobi preconditioner), cached per operators, geometry and lambdas
+    // assembled diagonal (Jacobi preconditioner), cached per geometry pack (a process-wide
+    // id, never reused) and lambda of every operator
     std::vector<std::tuple<const void *, unsigned long, double>> key;
     for (int g = 0; g < nops; ++g)
         key.emplace_back((const void *)ops[g], ops[g]->geom_version, ops[g]->lambda);

@@ -4,6 +4,7 @@
 // GetDerivFactors arrays, Collections/CoalescedGeomData.cpp:53-112, :251-313) and the
 // packed streamed form derived from them, plus the 1D basis tables as a kernel-argument
 // image.  Applies on device arrays allocate nothing.
+#include <atomic>
 #include <cstdio>
 #include <cstring>
 #include <string>
@@ -71,7 +72,10 @@ int maybe_pack(imf_op op)
     if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
     if (e != hipSuccess) return hip_fail(e, "pack_geometry");
     op->geom_ready = true;
-    ++op->geom_version;
+    // process-wide id of this geometry pack: a cache keyed on it cannot mistake a new
+    // operator allocated at a freed operator's address for the old one
+    static std::atomic<unsigned long> next_geom_id{1};
+    op->geom_version = next_geom_id.fetch_add(1);
     return IMF_OK;
 }
 
@@ -331,7 +335,8 @@ int imf_apply_many_device(imf_op op, int nvec, const void *d_in, long ld_in, voi
     if (ld_in < (long)imf_in_size(op) || ld_out < (long)imf_out_size(op))
         return fail(IMF_ERR_ARG, "imf_apply_many_device: leading dimension smaller than the "
                                  "vector size");
-    if (op->kind == imf::kMass) return run(op, op->fwd, d_in, d_out, stream, nvec, ld_in, ld_out);
+    if (op->kind == imf::kMass || (op->kind == imf::kHelmholtz && op->shape == imf::kHex))
+        return run(op, op->fwd, d_in, d_out, stream, nvec, ld_in, ld_out);
     const size_t es = elt_size(op);
     for (int v = 0; v < nvec; ++v)
         if (int s = run(op, op->fwd, (const char *)d_in + (size_t)v * ld_in * es,

@@ -355,17 +355,32 @@ __global__ __launch_bounds__(HexCfg<NQ>::BLOCK) void hex_mass_kernel(
 // (J, J G00, ...), weights applied in the kernel.
 // GATHER: `in` is a global vector read through (gmap, gsign), as GlobalToLocal would
 // expand it, so the solver's operator needs no local copy of its input.
-template <int NM, int NQ, typename T, int GEO, bool GATHER = false>
+// MANY: nvec vectors per launch (vector v at in + v*ldin, out + v*ldout), the operator
+// applied to each in turn by the same workgroup; deformed geometry is staged in LDS once
+// (when the element's 7Q values fit beside the work buffers) and read from there for
+// every vector, so each further vector streams only its coefficients in and out.
+template <int NM, int NQ, typename T, int GEO>
+constexpr bool helm_geo_in_lds()
+{
+    using Lb = HexLds<NM, NQ>;
+    return GEO == kDeformed &&
+           HexCfg<NQ>::EPB * (4 * Lb::QB + 7 * Lb::Q) * (int)sizeof(T) <= 64 * 1024;
+}
+
+template <int NM, int NQ, typename T, int GEO, bool GATHER = false, bool MANY = false>
 __global__ __launch_bounds__(HexCfg<NQ>::BLOCK) void hex_helmholtz_kernel(
     const Tab<NM, NQ, T> tab, int nElmt, T lambda, const T *__restrict__ in,
     const T *__restrict__ geom, T *__restrict__ out, const int *__restrict__ gmap = nullptr,
-    const signed char *__restrict__ gsign = nullptr)
+    const signed char *__restrict__ gsign = nullptr, int nvec = 1, long ldin = 0,
+    long ldout = 0)
 {
     using C = HexCfg<NQ>;
     using Lb = HexLds<NM, NQ>;
     constexpr int M = Lb::M, Q = Lb::Q, TPE = C::TPE;
     constexpr long GS = geo_stride(7 * Q, sizeof(T));
+    constexpr bool GLDS = MANY && helm_geo_in_lds<NM, NQ, T, GEO>();
     __shared__ T lds[C::EPB][4][Lb::QB];
+    __shared__ T lgeo[GLDS ? C::EPB : 1][GLDS ? 7 * Q : 1];
     const int eb = threadIdx.x / TPE;
     const int t = threadIdx.x - eb * TPE;
     const bool active = eb < C::EPB;
@@ -375,6 +390,21 @@ __global__ __launch_bounds__(HexCfg<NQ>::BLOCK) void hex_helmholtz_kernel(
     const bool valid = active && eb < nv;
     T *buf0 = lds[es][0], *buf1 = lds[es][1], *buf2 = lds[es][2], *buf3 = lds[es][3];
     const int ci = t % NQ, cj = t / NQ;  // this thread's column (i, j)
+    if constexpr (GLDS) {
+        // the block's elements' geometry, coalesced, once (made visible by the first
+        // barrier below)
+        for (int f = threadIdx.x; f < nv * 7 * Q; f += blockDim.x) {
+            const int sl = f / (7 * Q), o = f - sl * (7 * Q);
+            lgeo[sl][o] = ld_stream(geom + (e0 + sl) * GS + o);
+        }
+    }
+    const int nrep = MANY ? nvec : 1;
+  for (int vv = 0; vv < nrep; ++vv) {
+    if (MANY && vv > 0) {
+        in += ldin;
+        out += ldout;
+        __syncthreads();  // the previous vector's store has read buf2 of every slot
+    }
 
     // 1. BwdTrans into k-columns; keep a copy of u in LDS for the cross derivatives.
     if constexpr (GATHER)
@@ -436,7 +466,16 @@ __global__ __launch_bounds__(HexCfg<NQ>::BLOCK) void hex_helmholtz_kernel(
             const T d0 = buf0[k * TPE + t];
             const T d1 = buf1[k * TPE + t];
             T jw, m00, m01, m02, m11, m12, m22;
-            if (GEO == kDeformed) {
+            if (GLDS) {
+                const T *g = &lgeo[GLDS ? es : 0][k * TPE + t];
+                jw = g[0];
+                m00 = g[Q];
+                m01 = g[2 * Q];
+                m02 = g[3 * Q];
+                m11 = g[4 * Q];
+                m12 = g[5 * Q];
+                m22 = g[6 * Q];
+            } else if (GEO == kDeformed) {
                 const T *g = geom + e * GS + k * TPE + t;
                 jw = ld_stream(g);
                 m00 = ld_stream(g + Q);
@@ -521,6 +560,7 @@ __global__ __launch_bounds__(HexCfg<NQ>::BLOCK) void hex_helmholtz_kernel(
                                     [&](int p, T v) { buf2[t * (NM + 1) + p] = v; });
     __syncthreads();
     store_modes_block<NM, NQ, T>(out + e0 * M, nv, [&](int sl) { return lds[sl][2]; });
+  }
 }
 
 // Reference-space derivatives of the element's quadrature values u (held as this thread's

@@ -34,6 +34,12 @@ hipError_t launch_hex(const LaunchArgs &a)
             }
         }
         if (a.gmap) return hipErrorInvalidValue;  // gathered input: FP64 only
+        if (a.nvec > 1) {
+            hex_helmholtz_kernel<NM, NQ, T, GEO, false, true><<<grid, block, 0, a.stream>>>(
+                tab, a.nElmt, T(a.lambda), in, geom, out, nullptr, nullptr, a.nvec, a.ldin,
+                a.ldout);
+            return hipGetLastError();
+        }
         hex_helmholtz_kernel<NM, NQ, T, GEO>
             <<<grid, block, 0, a.stream>>>(tab, a.nElmt, T(a.lambda), in, geom, out);
     } else if constexpr (KIND == kMass) {

@@ -18,7 +18,7 @@ struct imf_op_s {
     size_t n_jac = 0, n_df = 0;
     void *d_geom = nullptr;
     bool geom_ready = false;
-    unsigned long geom_version = 0;        // bumped at every geometry (re)pack
+    unsigned long geom_version = 0;        // process-wide unique id of the current geometry pack
     void *d_in = nullptr, *d_out = nullptr;  // staging for host applies
     imf::LaunchFn fwd = nullptr, tr = nullptr;
 };

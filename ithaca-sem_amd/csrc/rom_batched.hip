@@ -51,6 +51,12 @@ struct Args {
     int *counter;
     double *ws64;   // per workgroup NMAX*NMAX (assembled FP64 A / FP64 factors)
     float *ws32;    // per workgroup NMAX*NMAX (FP32 factors, MIXED only)
+    // FP64 fallback of the mixed method.  MIXED kernel: instances whose refinement does
+    // not converge are appended to fail_list (count in *fail_count) when fail_list is set.
+    // FP64 kernel: with fail_count set, it solves fail_list[0 .. *fail_count) instead of
+    // the batch order and reports status 3 (solved by the fallback) or 1 (zero pivot).
+    int *fail_list;
+    int *fail_count;
 };
 
 // Value of v in lane l of this wave (l uniform).
@@ -517,6 +523,8 @@ batched_solve_kernel(Args a)
     double *A64 = a.ws64 + (long)blockIdx.x * NMAX * NMAX;
     T *LU = MIXED ? (T *)(a.ws32 + (long)blockIdx.x * NMAX * NMAX) : (T *)A64;
 
+    const bool fallback = !MIXED && a.fail_count != nullptr;
+    const int total = fallback ? *(volatile int *)a.fail_count : a.bd.nInst;
     for (;;) {
         if (tid == 0) {
             s_inst = atomicAdd(a.counter, 1);
@@ -525,8 +533,8 @@ batched_solve_kernel(Args a)
         __syncthreads();
         const int k = s_inst;
         __syncthreads();
-        if (k >= a.bd.nInst) break;
-        const int inst = a.bd.order[k];
+        if (k >= total) break;
+        const int inst = fallback ? a.fail_list[k] : a.bd.order[k];
         const int n = a.bd.N[inst], nq = a.bd.Q[inst];
         const double *Ai = a.A + a.bd.aoff[inst];
         const double *th = a.theta + a.bd.toff[inst];
@@ -606,8 +614,15 @@ batched_solve_kernel(Args a)
                 for (int i = tid; i < n; i += NT) x64[i] += (double)yv[i];
                 __syncthreads();
             }
-            if (!conv && !st) st = 2;
+            if (!conv && !st) {
+                st = 2;
+                if (a.fail_list) {
+                    st = 3;
+                    if (tid == 0) a.fail_list[atomicAdd(a.fail_count, 1)] = inst;
+                }
+            }
         }
+        if (fallback && !st) st = 3;
         for (int i = tid; i < n; i += NT) a.x[vo + i] = x64[i];
         if (tid == 0) {
             if (a.iters) a.iters[inst] = it;
@@ -618,6 +633,11 @@ batched_solve_kernel(Args a)
 }
 
 __global__ void reset_counter(int *c) { *c = 0; }
+__global__ void reset_counters(int *c, int *d)
+{
+    *c = 0;
+    *d = 0;
+}
 
 // dJ/dmu_q = -lambda^T A_q x for every instance and term (one workgroup per instance).
 __global__ __launch_bounds__(NT) void param_grad_kernel(BatchDev bd, const double *A,
@@ -682,6 +702,7 @@ struct irom_batch_s {
     std::vector<long> aoff, toff, voff;
     size_t a_size = 0, t_size = 0, v_size = 0;
     int *d_order = nullptr, *d_N = nullptr, *d_Q = nullptr, *d_counter = nullptr;
+    int *d_fail = nullptr;  // FP64-fallback list (nInst) followed by its count
     long *d_aoff = nullptr, *d_toff = nullptr, *d_voff = nullptr;
     double *d_ws64 = nullptr;
     float *d_ws32 = nullptr;
@@ -718,17 +739,8 @@ irom::BatchDev dev_view(const irom_batch b)
     return irom::BatchDev{b->d_order, b->d_N, b->d_Q, b->d_aoff, b->d_toff, b->d_voff, b->nInst};
 }
 
-int launch_solve(irom_batch b, int method, int variant, int transpose, const double *d_A,
-                 const double *d_theta, const double *d_b, double *d_x, double tol, int max_iter,
-                 int *d_iters, int *d_status, hipStream_t s)
+int launch_kernel(irom_batch b, irom::KernelFn fn, const irom::Args &a, hipStream_t s)
 {
-    if (method != IROM_LU_FP64 && method != IROM_LU_FP32_IR) return fail(1, "unknown method");
-    if (!d_A || !d_theta || !d_b || !d_x) return fail(1, "null device array");
-    irom::Args a{dev_view(b), transpose, d_A, d_theta, d_b, d_x, tol, max_iter, d_iters,
-                 d_status, b->d_counter, b->d_ws64, b->d_ws32};
-    irom::reset_counter<<<1, 1, 0, s>>>(b->d_counter);
-    const irom::Variant &v = irom::kVariants[variant];
-    irom::KernelFn fn = method == IROM_LU_FP64 ? v.fp64 : v.mixed;
     // persistent grid: as many workgroups as are co-resident, capped by the workspace
     int per_cu = 1;
     IROM_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, (const void *)fn, irom::NT, 0),
@@ -737,6 +749,35 @@ int launch_solve(irom_batch b, int method, int variant, int transpose, const dou
     fn<<<grid, irom::NT, 0, s>>>(a);
     IROM_CHECK(hipGetLastError(), "batched solve launch");
     return 0;
+}
+
+int launch_solve(irom_batch b, int method, int variant, int transpose, const double *d_A,
+                 const double *d_theta, const double *d_b, double *d_x, double tol, int max_iter,
+                 int *d_iters, int *d_status, hipStream_t s)
+{
+    if (method != IROM_LU_FP64 && method != IROM_LU_FP32_IR && method != IROM_LU_FP32_IR_FALLBACK)
+        return fail(1, "unknown method");
+    if (!d_A || !d_theta || !d_b || !d_x) return fail(1, "null device array");
+    irom::Args a{dev_view(b), transpose, d_A, d_theta, d_b, d_x, tol, max_iter, d_iters,
+                 d_status, b->d_counter, b->d_ws64, b->d_ws32, nullptr, nullptr};
+    const irom::Variant &v = irom::kVariants[variant];
+    if (method == IROM_LU_FP64) {
+        irom::reset_counter<<<1, 1, 0, s>>>(b->d_counter);
+        return launch_kernel(b, v.fp64, a, s);
+    }
+    if (method == IROM_LU_FP32_IR) {
+        irom::reset_counter<<<1, 1, 0, s>>>(b->d_counter);
+        return launch_kernel(b, v.mixed, a, s);
+    }
+    // mixed, then FP64 LU for the instances whose refinement did not converge (the
+    // second launch reads the list length from device memory; an empty list exits at once)
+    a.fail_list = b->d_fail;
+    a.fail_count = b->d_fail + b->nInst;
+    irom::reset_counters<<<1, 1, 0, s>>>(b->d_counter, a.fail_count);
+    if (int st = launch_kernel(b, v.mixed, a, s)) return st;
+    irom::reset_counter<<<1, 1, 0, s>>>(b->d_counter);
+    const irom::Variant &v64 = irom::kVariants[b->variant[0]];
+    return launch_kernel(b, v64.fp64, a, s);
 }
 
 }  // namespace
@@ -781,6 +822,7 @@ int irom_batch_create(int nInst, const int *N, const int *Q, irom_batch *out)
     IROM_CHECK(hipMalloc(&b->d_N, nInst * sizeof(int)), "hipMalloc");
     IROM_CHECK(hipMalloc(&b->d_Q, nInst * sizeof(int)), "hipMalloc");
     IROM_CHECK(hipMalloc(&b->d_counter, sizeof(int)), "hipMalloc");
+    IROM_CHECK(hipMalloc(&b->d_fail, (nInst + 1) * sizeof(int)), "hipMalloc");
     IROM_CHECK(hipMalloc(&b->d_aoff, nInst * sizeof(long)), "hipMalloc");
     IROM_CHECK(hipMalloc(&b->d_toff, nInst * sizeof(long)), "hipMalloc");
     IROM_CHECK(hipMalloc(&b->d_voff, nInst * sizeof(long)), "hipMalloc");
@@ -805,7 +847,7 @@ int irom_batch_destroy(irom_batch b)
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
     if (b->cap_stream) (void)hipStreamDestroy(b->cap_stream);
-    for (void *p : {(void *)b->d_order, (void *)b->d_N, (void *)b->d_Q, (void *)b->d_counter,
+    for (void *p : {(void *)b->d_order, (void *)b->d_N, (void *)b->d_Q, (void *)b->d_counter, (void *)b->d_fail,
                     (void *)b->d_aoff, (void *)b->d_toff, (void *)b->d_voff, (void *)b->d_ws64,
                     (void *)b->d_ws32})
         (void)hipFree(p);
@@ -831,7 +873,7 @@ int irom_batch_solve(irom_batch b, int method, int transpose, const double *d_A,
                      int max_iter, int *d_iters, int *d_status, void *stream)
 {
     if (!b) return fail(1, "null batch");
-    return launch_solve(b, method, b->variant[method == IROM_LU_FP32_IR], transpose, d_A,
+    return launch_solve(b, method, b->variant[method != IROM_LU_FP64], transpose, d_A,
                         d_theta, d_b, d_x, tol, max_iter, d_iters, d_status,
                         (hipStream_t)stream);
 }
@@ -845,22 +887,23 @@ const char *irom_batch_variant_name(int v)
 
 int irom_batch_set_variant(irom_batch b, int method, int v)
 {
-    if (!b || v < 0 || v >= irom::kNumVariants || (method != 0 && method != 1))
+    if (!b || v < 0 || v >= irom::kNumVariants || method < 0 || method > 2)
         return fail(1, "irom_batch_set_variant: bad argument");
-    b->variant[method] = v;
+    b->variant[method != IROM_LU_FP64] = v;
     return 0;
 }
 
 int irom_batch_autotune(irom_batch b, int method, const double *d_A, const double *d_theta,
                         const double *d_b, double *d_x, int reps, int *chosen, void *stream)
 {
-    if (!b || (method != 0 && method != 1)) return fail(1, "irom_batch_autotune: bad argument");
+    if (!b || method < 0 || method > 2) return fail(1, "irom_batch_autotune: bad argument");
+    const int slot = method != IROM_LU_FP64;
     hipStream_t s = (hipStream_t)stream;
     hipEvent_t e0, e1;
     IROM_CHECK(hipEventCreate(&e0), "event");
     IROM_CHECK(hipEventCreate(&e1), "event");
     float best = 1e30f;
-    int bestv = b->variant[method];
+    int bestv = b->variant[slot];
     reps = std::max(1, reps);
     for (int v = 0; v < irom::kNumVariants; ++v) {
         if (int st = launch_solve(b, method, v, 0, d_A, d_theta, d_b, d_x, 1e-14, 10, nullptr,
@@ -882,7 +925,7 @@ int irom_batch_autotune(irom_batch b, int method, const double *d_A, const doubl
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    b->variant[method] = bestv;
+    b->variant[slot] = bestv;
     if (chosen) *chosen = bestv;
     return 0;
 }
@@ -902,7 +945,7 @@ int irom_batch_capture(irom_batch b, int method, int transpose, const double *d_
         b->graph = nullptr;
     }
     IROM_CHECK(hipStreamBeginCapture(b->cap_stream, hipStreamCaptureModeThreadLocal), "capture");
-    int st = launch_solve(b, method, b->variant[method == IROM_LU_FP32_IR], transpose, d_A,
+    int st = launch_solve(b, method, b->variant[method != IROM_LU_FP64], transpose, d_A,
                           d_theta, d_b, d_x, tol, max_iter, d_iters, d_status, b->cap_stream);
     hipError_t e = hipStreamEndCapture(b->cap_stream, &b->graph);
     if (st) return st;

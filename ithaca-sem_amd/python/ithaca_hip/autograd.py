@@ -8,8 +8,8 @@ No reference counterpart: the reference has no derivative capability on this pat
 * ``BwdTransFn``    u = B u_hat; backward is the Euclidean transpose B^T (an IProduct
   with unit weights).
 * ``L2Misfit``      0.5 * ||B u_hat - f||^2 in the quadrature-weighted L2 norm
-  (sum_q w_q J_q r_q^2); its gradient is IProductWRTBase(B u_hat - f), evaluated by the
-  IProduct kernel itself.
+  (sum_q w_q J_q r_q^2); its gradient in u_hat is IProductWRTBase(B u_hat - f), evaluated
+  by the IProduct kernel itself, and its gradient in f is -diag(w J)(B u_hat - f).
 * ``HelmSolveFn``   u = A(lambda)^{-1} rhs, the assembled Helmholtz solve with homogeneous
   Dirichlet modes (A = K + lambda M on the free modes, symmetric).  Backward is one more
   solve, w = A^{-1} dL/du, and returns dL/drhs = w and dL/dlambda = -w^T M u, with M u
@@ -55,15 +55,17 @@ class BwdTransFn(torch.autograd.Function):
 class L2Misfit(torch.autograd.Function):
     @staticmethod
     def forward(ctx, u_hat, f, bwd, ipr, jw):
-        r = bwd.apply(u_hat.detach().contiguous()) - f
+        r = bwd.apply(u_hat.detach().contiguous()) - f.detach()
         ctx.ipr = ipr
-        ctx.save_for_backward(r)
+        ctx.save_for_backward(r, jw)
         return 0.5 * torch.dot(jw * r, r)
 
     @staticmethod
     def backward(ctx, g):
-        (r,) = ctx.saved_tensors
-        return g * ctx.ipr.apply(r), None, None, None, None
+        r, jw = ctx.saved_tensors
+        gu = g * ctx.ipr.apply(r) if ctx.needs_input_grad[0] else None
+        gf = -g * jw * r if ctx.needs_input_grad[1] else None
+        return gu, gf, None, None, None
 
 
 def _set_lambda(op, lam):

@@ -36,6 +36,8 @@ def _lib():
         L.irom_gram_last_error.restype = ctypes.c_char_p
         L.irom_eig_sym.argtypes = [i, vp, vp, vp, d, i, ctypes.POINTER(i), vp]
         L.irom_eig_last_error.restype = ctypes.c_char_p
+        L.irom_galerkin_project.argtypes = [vp, i, vp, l_, vp, l_, vp, l_, i, vp, sz, vp]
+        L.irom_project_last_error.restype = ctypes.c_char_p
         _SET = True
     return L
 
@@ -109,6 +111,54 @@ def reference_rbsize(sigma, pod_tolerance):
         if c[i] < pod_tolerance:
             rb = i + 2
     return rb
+
+
+def galerkin_project(op, V, mode=GRAM_FP64, W=None, out=None, work=None, stream=None,
+                     symmetric_result=True):
+    """C = V A V^T (r x r) for the basis V (r, N) and the matrix-free operator term A
+    (Helmholtz or Mass, FP64): A applied to all r rows in one launch (into W, returned
+    too), then the product on MFMA in `mode`.  The reference's per-term RB^T A RB
+    (CoupledLinearNS_ROM.cpp:2788-2802)."""
+    L = _lib()
+    r, N = V.shape
+    assert V.dtype == torch.float64 and V.stride(1) == 1
+    if W is None:
+        W = torch.empty((r, N), dtype=torch.float64, device=V.device)
+    if out is None:
+        out = torch.empty((r, r), dtype=torch.float64, device=V.device)
+    if symmetric_result:
+        mode |= GRAM_SYMMETRIC_RESULT
+    need = L.irom_gram_workspace_size(r, r, N)
+    if work is None or work.numel() * 8 < need:
+        work = torch.empty((need + 7) // 8, dtype=torch.float64, device=V.device)
+    st = L.irom_galerkin_project(op._h, r, ctypes.c_void_p(V.data_ptr()), V.stride(0),
+                                 ctypes.c_void_p(W.data_ptr()), W.stride(0),
+                                 ctypes.c_void_p(out.data_ptr()), out.stride(0), mode,
+                                 ctypes.c_void_p(work.data_ptr()), work.numel() * 8, _s(stream))
+    if st:
+        raise MFError(L.irom_project_last_error().decode())
+    return out, W
+
+
+def project_affine(terms, V, mode=GRAM_FP64):
+    """Reduced matrices of the affine terms {name: operator} on the basis V (r, N), one
+    Galerkin projection per term; the reduced operator of a parameter is
+    sum_q theta_q C_q (gen_affine_mat_proj, CoupledLinearNS_ROM.cpp:4694-4704)."""
+    W = torch.empty_like(V)
+    return {name: galerkin_project(op, V, mode, W=W)[0].clone() for name, op in terms.items()}
+
+
+def helmholtz_affine_terms(nm, nq, nElmt, basis, jac, df, deformed=True):
+    """The two affine terms of the hex Helmholtz operator A(lambda) = K + lambda M as
+    matrix-free operators: the stiffness K (Helmholtz with lambda = 0) and the mass M."""
+    geo = "Deformed" if deformed else "Regular"
+    K = Operator(f"Helmholtz_Hex_{geo}", nm, nq, nElmt, FP64, basis)
+    K.set_jac(jac)
+    K.set_df(df)
+    K.set_lambda(0.0)
+    Mo = Operator(f"Mass_Hex_{geo}", nm, nq, nElmt, FP64, basis)
+    Mo.set_jac(jac)
+    return {"stiffness": K, "mass": Mo}
 
 
 def mass_apply(S, bwd, ipr, out=None, phys=None):

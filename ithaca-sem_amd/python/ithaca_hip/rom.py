@@ -11,7 +11,8 @@ import torch
 
 from . import LIB_PATH, MFError
 
-LU_FP64, LU_FP32_IR = 0, 1
+LU_FP64, LU_FP32_IR, LU_FP32_IR_FALLBACK = 0, 1, 2
+KINDS = ("normalised", "literal", "nonsymmetric")
 _LIB = None
 
 
@@ -124,19 +125,7 @@ class Batch:
         return grad
 
 
-def affine_batch(n_inst=4096, seed=3, nmin=16, nmax=256, qmin=3, qmax=8):
-    """Synthetic affine reduced systems (host numpy, float64).
-
-    Per instance: Q ~ U{qmin..qmax} terms and size N ~ U{nmin..nmax};
-    A_0 = U diag(10^{U[0,4]}) U^T with U Haar-orthogonal (QR of a Gaussian matrix, signs
-    fixed by diag(R)); A_q (q >= 1) = 0.1 sym(G) / sqrt(2N) with G Gaussian and
-    sym(G) = (G + G^T)/2, i.e. random symmetric matrices scaled to spectral norm ~0.1
-    (sqrt(2N) is the semicircle edge of sym(G)); theta_q = mu_q ~ U[0.1, 1]; b ~ N(0, 1).
-    The normalisation keeps every A(mu) symmetric positive definite with condition number
-    ~1e4 (unnormalised 0.1 sym(G) would make a third of the instances indefinite and
-    push condition numbers to ~1e7).  Returns the packed arrays of include/ithaca_rom.h
-    in instance order."""
-    rng = np.random.default_rng(seed)
+def _layout(rng, n_inst, nmin, nmax, qmin, qmax):
     Q = rng.integers(qmin, qmax + 1, size=n_inst)
     N = rng.integers(nmin, nmax + 1, size=n_inst)
     a_off = np.zeros(n_inst, dtype=np.int64)
@@ -145,6 +134,31 @@ def affine_batch(n_inst=4096, seed=3, nmin=16, nmax=256, qmin=3, qmax=8):
     a_off[1:] = np.cumsum(Q * N * N)[:-1]
     t_off[1:] = np.cumsum(Q)[:-1]
     v_off[1:] = np.cumsum(N)[:-1]
+    return Q, N, a_off, t_off, v_off
+
+
+def affine_batch(n_inst=4096, seed=3, nmin=16, nmax=256, qmin=3, qmax=8, kind="normalised"):
+    """Synthetic affine reduced systems (host numpy, float64).
+
+    Per instance: Q ~ U{qmin..qmax} terms and size N ~ U{nmin..nmax}; theta_q = mu_q ~
+    U[0.1, 1]; b ~ N(0, 1); A_0 built from Haar-orthogonal factors (QR of a Gaussian
+    matrix, signs fixed by diag(R)) and singular values 10^{U[0,4]}.  ``kind``:
+
+    * ``"normalised"``   A_0 = U diag U^T, A_q (q >= 1) = 0.1 sym(G) / sqrt(2N) with
+      sym(G) = (G + G^T)/2: symmetric terms scaled to spectral norm ~0.1 (sqrt(2N) is the
+      semicircle edge of sym(G)), so every A(mu) is symmetric positive definite with
+      condition number ~1e4;
+    * ``"literal"``      A_0 = U diag U^T, A_q = 0.1 sym(G) unnormalised, the workload as
+      stated: about 38 % of the A(mu) are indefinite and condition numbers reach ~1e7;
+    * ``"nonsymmetric"`` A_0 = U diag V^T with independent U, V, and A_q = 0.05 G / sqrt(N)
+      (spectral norm ~0.1): non-symmetric systems, as the reference's coupled
+      Navier-Stokes reduced matrices are.
+
+    Returns the packed arrays of include/ithaca_rom.h in instance order."""
+    if kind not in KINDS:
+        raise ValueError(f"kind must be one of {KINDS}")
+    rng = np.random.default_rng(seed)
+    Q, N, a_off, t_off, v_off = _layout(rng, n_inst, nmin, nmax, qmin, qmax)
     A = np.empty(int(np.sum(Q * N * N)))
     theta = rng.uniform(0.1, 1.0, size=int(Q.sum()))
     b = rng.standard_normal(int(N.sum()))
@@ -154,10 +168,20 @@ def affine_batch(n_inst=4096, seed=3, nmin=16, nmax=256, qmin=3, qmax=8):
         U, R = np.linalg.qr(G)
         U *= np.sign(np.diagonal(R, axis1=1, axis2=2))[:, None, :]
         lam = 10.0 ** rng.uniform(0.0, 4.0, size=(len(idx), n))
-        A0 = (U * lam[:, None, :]) @ np.swapaxes(U, 1, 2)
+        if kind == "nonsymmetric":
+            V, R = np.linalg.qr(rng.standard_normal((len(idx), n, n)))
+            V *= np.sign(np.diagonal(R, axis1=1, axis2=2))[:, None, :]
+        else:
+            V = U
+        A0 = (U * lam[:, None, :]) @ np.swapaxes(V, 1, 2)
         npert = int(np.sum(Q[idx] - 1))
         P = rng.standard_normal((npert, n, n))
-        P = (0.05 / np.sqrt(2.0 * n)) * (P + np.swapaxes(P, 1, 2))
+        if kind == "normalised":
+            P = (0.05 / np.sqrt(2.0 * n)) * (P + np.swapaxes(P, 1, 2))
+        elif kind == "literal":
+            P = 0.05 * (P + np.swapaxes(P, 1, 2))
+        else:
+            P *= 0.05 / np.sqrt(n)
         c = 0
         for r, i in enumerate(idx):
             base = a_off[i]
@@ -170,38 +194,43 @@ def affine_batch(n_inst=4096, seed=3, nmin=16, nmax=256, qmin=3, qmax=8):
 
 
 def affine_batch_device(n_inst=4096, seed=3, nmin=16, nmax=256, qmin=3, qmax=8,
-                        device="cuda"):
-    """The distribution of ``affine_batch`` generated on the GPU (for large batches).
+                        device="cuda", kind="normalised"):
+    """The distributions of ``affine_batch`` generated on the GPU (for large batches).
 
     N, Q, theta and b are drawn exactly as in ``affine_batch`` (numpy PCG64, same seed);
     the matrices come from torch's Philox generator seeded with ``seed`` on the device, so
     their values differ from the host generator's.  Returns the same dict with ``A`` a cuda
     float64 tensor and ``theta``, ``b`` as host arrays."""
+    if kind not in KINDS:
+        raise ValueError(f"kind must be one of {KINDS}")
     rng = np.random.default_rng(seed)
-    Q = rng.integers(qmin, qmax + 1, size=n_inst)
-    N = rng.integers(nmin, nmax + 1, size=n_inst)
-    a_off = np.zeros(n_inst, dtype=np.int64)
-    t_off = np.zeros(n_inst, dtype=np.int64)
-    v_off = np.zeros(n_inst, dtype=np.int64)
-    a_off[1:] = np.cumsum(Q * N * N)[:-1]
-    t_off[1:] = np.cumsum(Q)[:-1]
-    v_off[1:] = np.cumsum(N)[:-1]
+    Q, N, a_off, t_off, v_off = _layout(rng, n_inst, nmin, nmax, qmin, qmax)
     theta = rng.uniform(0.1, 1.0, size=int(Q.sum()))
     b = rng.standard_normal(int(N.sum()))
     gen = torch.Generator(device=device).manual_seed(seed)
     A = torch.empty(int(np.sum(Q * N * N)), dtype=torch.float64, device=device)
+
+    def haar(m, n):
+        G = torch.randn((m, n, n), generator=gen, dtype=torch.float64, device=device)
+        U, R = torch.linalg.qr(G)
+        return U * torch.sign(torch.diagonal(R, dim1=1, dim2=2))[:, None, :]
+
     for n in np.unique(N):
         n = int(n)
         idx = np.nonzero(N == n)[0]
-        G = torch.randn((len(idx), n, n), generator=gen, dtype=torch.float64, device=device)
-        U, R = torch.linalg.qr(G)
-        U = U * torch.sign(torch.diagonal(R, dim1=1, dim2=2))[:, None, :]
+        U = haar(len(idx), n)
         lam = 10.0 ** (4.0 * torch.rand((len(idx), n), generator=gen, dtype=torch.float64,
                                         device=device))
-        A0 = (U * lam[:, None, :]) @ U.transpose(1, 2).contiguous()
+        V = haar(len(idx), n) if kind == "nonsymmetric" else U
+        A0 = (U * lam[:, None, :]) @ V.transpose(1, 2).contiguous()
         npert = int(np.sum(Q[idx] - 1))
         P = torch.randn((npert, n, n), generator=gen, dtype=torch.float64, device=device)
-        P = (0.05 / np.sqrt(2.0 * n)) * (P + P.transpose(1, 2))
+        if kind == "normalised":
+            P = (0.05 / np.sqrt(2.0 * n)) * (P + P.transpose(1, 2))
+        elif kind == "literal":
+            P = 0.05 * (P + P.transpose(1, 2))
+        else:
+            P = (0.05 / np.sqrt(n)) * P
         c = 0
         for r, i in enumerate(idx):
             base = int(a_off[i])
@@ -211,4 +240,3 @@ def affine_batch_device(n_inst=4096, seed=3, nmin=16, nmax=256, qmin=3, qmax=8,
             c += k
     return dict(N=N.astype(np.int32), Q=Q.astype(np.int32), A=A, theta=theta, b=b,
                 a_off=a_off, t_off=t_off, v_off=v_off)
-

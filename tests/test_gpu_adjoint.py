@@ -18,10 +18,13 @@ def _dev(a):
     return torch.from_numpy(np.ascontiguousarray(a)).to("cuda", torch.float64)
 
 
+@pytest.mark.parametrize("kind", ["normalised", "nonsymmetric"])
 @pytest.mark.parametrize("method", [rom.LU_FP64, rom.LU_FP32_IR])
-def test_reduced_solve_parameter_gradient(method):
-    bt = rom.affine_batch(n_inst=512, seed=3)
-    print(f"[case] reduced-solve adjoint method={method}", flush=True)
+def test_reduced_solve_parameter_gradient(method, kind):
+    """With non-symmetric A_q the adjoint must solve with A^T and form -lambda^T A_q x
+    (not x^T A_q lambda); the oracle does both with numpy on the transposed matrices."""
+    bt = rom.affine_batch(n_inst=512, seed=3, kind=kind)
+    print(f"[case] reduced-solve adjoint method={method} kind={kind}", flush=True)
     B = rom.Batch(bt["N"], bt["Q"])
     A = _dev(bt["A"])
     theta = _dev(bt["theta"]).requires_grad_(True)
@@ -71,9 +74,21 @@ def test_bwdtrans_misfit_gradient_full_hex_mesh():
         f += a * torch.sin(np.pi * k[0] * xyz[0]) * torch.cos(np.pi * k[1] * xyz[1]) * \
             torch.sin(np.pi * k[2] * xyz[2] + 0.3)
     u = _dev(w["u"]).requires_grad_(True)
+    f.requires_grad_(True)
     J = L2Misfit.apply(u, f, bwd, ipr, jw)
     J.backward()
     g1 = u.grad.detach().clone()
+    gf = f.grad.detach().clone()
+    f = f.detach()
+    # gradient in the target f: -diag(wJ) r, checked by central differences
+    with torch.no_grad():
+        d = torch.rand(f.numel(), dtype=torch.float64, device="cuda",
+                       generator=torch.Generator("cuda").manual_seed(11)) - 0.5
+        Jp = L2Misfit.apply(u.detach(), f + H_FD * d, bwd, ipr, jw)
+        Jm = L2Misfit.apply(u.detach(), f - H_FD * d, bwd, ipr, jw)
+        fd = ((Jp - Jm) / (2 * H_FD)).item()
+        an = torch.dot(gf, d).item()
+        assert abs(fd - an) <= 1e-6 * abs(an), (fd, an)
     # autograd through the Euclidean transpose of BwdTrans
     u2 = _dev(w["u"]).requires_grad_(True)
     r = BwdTransFn.apply(u2, bwd) - f

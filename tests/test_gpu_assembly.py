@@ -339,3 +339,29 @@ def test_preconditioner_follows_new_geometry():
     print(f"iterations {it1} / {it2}")
     assert it1 == it2
     assert torch.equal(x1, x2)
+
+
+def test_preconditioner_follows_a_recreated_operator():
+    """An operator destroyed and created again (its handle may land at the freed address)
+    on new geometry with the same lambda: the cached assembled diagonal is keyed on a
+    process-wide geometry id, so the reused assembly solves exactly as a fresh one."""
+    import gc
+    verts, ids = _mesh(2, True, seed=3, jitter=0.05)
+    nm, nq = 5, 6
+    A, Hop, rhs, b = _gpu_problem(verts, ids, nm, nq, 10.0)
+    A.helmsolve(Hop, rhs, tol=1e-12)  # caches the diagonal of the first geometry
+    for s in (0.5, 2.0):
+        del Hop
+        gc.collect()
+        verts2 = np.ascontiguousarray(verts * np.array([1.0, s, 1.0]))
+        jac2, df2 = H.geom_device("hex", _dev(verts2), nq, b["z"], True)
+        Hop = H.Operator("Helmholtz_Hex_Deformed", nm, nq, len(ids), basis=b)
+        Hop.set_jac(jac2)
+        Hop.set_df(df2)
+        Hop.set_lambda(10.0)
+        x1, it1, _ = A.helmsolve(Hop, rhs, tol=1e-12)
+        A2, Hop2, _, _ = _gpu_problem(verts2, ids, nm, nq, 10.0)
+        x2, it2, _ = A2.helmsolve(Hop2, rhs, tol=1e-12)
+        print(f"scale {s}: iterations {it1} / {it2}")
+        assert it1 == it2
+        assert torch.equal(x1, x2)

@@ -132,46 +132,62 @@ def _event_time(fn, steps, warmup):
 
 def rom_bench(args):
     """Batched online reduced solves: 4096 affine systems A(mu) = sum_q theta_q A_q,
-    N ~ U{16..256}, Q ~ U{3..8}, A_0 SPD with eigenvalues log-uniform in [1, 1e4], A_q
-    random symmetric of spectral norm ~0.1, theta ~ U[0.1, 1], b ~ N(0,1) (seed 3).
-    One step = assemble + factor + solve every instance in one persistent launch."""
+    N ~ U{16..256}, Q ~ U{3..8}, theta ~ U[0.1, 1], b ~ N(0,1) (seed 3), A_0 with
+    eigenvalues log-uniform in [1, 1e4], in two distributions: "normalised" (A_q random
+    symmetric of spectral norm ~0.1: SPD, condition ~1e4) and "literal" (A_q = 0.1 sym(randn)
+    unnormalised, as the workload states: ~38 % indefinite, condition up to ~1e7).
+    One step = assemble + factor + solve every instance in one persistent launch (two for
+    the FP32 path with the FP64 fallback)."""
     import numpy as np
     import torch
 
     from ithaca_hip import rom
 
-    bt = rom.affine_batch_device(args.rom_instances, seed=3)
-    B = rom.Batch(bt["N"], bt["Q"])
-    A = bt["A"]
-    th = torch.from_numpy(bt["theta"]).cuda()
-    b = torch.from_numpy(bt["b"]).cuda()
-    n = B.n
-    res = {"workload": f"{n} affine reduced systems, N~U{{16..256}}, Q~U{{3..8}}, seed 3",
-           "flops_lu_total": float(np.sum((2.0 / 3.0) * bt["N"].astype(float) ** 3)),
-           "bytes_A_q_total": float(bt["A"].numel() * 8)}
-    xs = {}
-    times = {}
-    for method, name in ((rom.LU_FP64, "fp64_lu"), (rom.LU_FP32_IR, "fp32_lu_fp64_ir")):
-        x = torch.empty_like(b)
-        v = B.autotune(A, th, b, x, method, reps=2)
-        ms = _event_time(lambda: B.solve(A, th, b, x, method=method), 10, 2)
-        iters = torch.zeros(n, dtype=torch.int32, device="cuda")
-        status = torch.full((n,), -1, dtype=torch.int32, device="cuda")
-        B.solve(A, th, b, x, method=method, iters=iters, status=status)
-        torch.cuda.synchronize()
-        xs[name] = x.cpu().numpy()
-        times[name] = ms
-        res[name] = {"solves_per_s": n / (ms * 1e-3), "ms_per_batch": ms,
-                     "variant": B.variants()[v],
-                     "refinement_iters_max": int(iters.max()),
-                     "refinement_iters_mean": float(iters.float().mean()),
-                     "status_nonzero": int((status != 0).sum())}
-    xg = torch.empty_like(b)
-    B.capture(A, th, b, xg, method=rom.LU_FP32_IR)
-    res["fp32_lu_fp64_ir"]["graph_replay_ms"] = _event_time(lambda: B.replay(), 10, 2)
-    res["speedup_fp32ir_over_fp64"] = times["fp64_lu"] / times["fp32_lu_fp64_ir"]
-    if not args.no_cpu:
-        res["cpu_baseline"] = rom_cpu_baseline(bt, xs)
+    res = {}
+    for kind in ("normalised", "literal"):
+        bt = rom.affine_batch_device(args.rom_instances, seed=3, kind=kind)
+        B = rom.Batch(bt["N"], bt["Q"])
+        A = bt["A"]
+        th = torch.from_numpy(bt["theta"]).cuda()
+        b = torch.from_numpy(bt["b"]).cuda()
+        n = B.n
+        a_bytes = float(bt["A"].numel() * 8)
+        r = {"workload": f"{n} affine reduced systems ({kind}), N~U{{16..256}}, Q~U{{3..8}}, "
+                         "seed 3",
+             "flops_lu_total": float(np.sum((2.0 / 3.0) * bt["N"].astype(float) ** 3)),
+             "bytes_A_q_total": a_bytes,
+             "hbm_floor_ms": a_bytes / (HBM_PEAK_GBS * 1e9) * 1e3}
+        mixed = rom.LU_FP32_IR if kind == "normalised" else rom.LU_FP32_IR_FALLBACK
+        xs, times = {}, {}
+        for method, name in ((rom.LU_FP64, "fp64_lu"), (mixed, "fp32_lu_fp64_ir")):
+            x = torch.empty_like(b)
+            v = B.autotune(A, th, b, x, method, reps=2)
+            ms = _event_time(lambda: B.solve(A, th, b, x, method=method), 10, 2)
+            iters = torch.zeros(n, dtype=torch.int32, device="cuda")
+            status = torch.full((n,), -1, dtype=torch.int32, device="cuda")
+            B.solve(A, th, b, x, method=method, iters=iters, status=status)
+            torch.cuda.synchronize()
+            xs[name] = x.cpu().numpy()
+            times[name] = ms
+            st = status.cpu().numpy()
+            r[name] = {"solves_per_s": n / (ms * 1e-3), "ms_per_batch": ms,
+                       "frac_of_hbm_floor": r["hbm_floor_ms"] / ms,
+                       "variant": B.variants()[v],
+                       "refinement_iters_max": int(iters.max()),
+                       "refinement_iters_mean": float(iters.float().mean()),
+                       "status_counts": np.bincount(st, minlength=4).tolist()}
+            if method == rom.LU_FP32_IR_FALLBACK:
+                r[name]["method"] = "FP32 LU + FP64 refinement, FP64 LU fallback (status 3)"
+        if kind == "normalised":
+            xg = torch.empty_like(b)
+            B.capture(A, th, b, xg, method=rom.LU_FP32_IR)
+            r["fp32_lu_fp64_ir"]["graph_replay_ms"] = _event_time(lambda: B.replay(), 10, 2)
+        r["speedup_fp32ir_over_fp64"] = times["fp64_lu"] / times["fp32_lu_fp64_ir"]
+        if not args.no_cpu:
+            r["cpu_baseline"] = rom_cpu_baseline(bt, xs)
+        res[kind] = r
+        del bt, B, A, th, b
+        torch.cuda.empty_cache()
     return res
 
 
@@ -192,6 +208,7 @@ def rom_cpu_baseline(bt, xs):
         A_host[i] = bt["A"][a0:a0 + q * n * n].cpu().numpy().reshape(q, n, n)
     use_eigen = R.eigen_lib() is not None
     errs = {k: 0.0 for k in xs}
+    bwd = {k: 0.0 for k in xs}
     t_tot = 0.0
     for i in idx:
         n = int(bt["N"][i])
@@ -206,14 +223,18 @@ def rom_cpu_baseline(bt, xs):
             xr = np.linalg.solve(Am, rhs)
         t_tot += time.perf_counter() - t0
         for k, x in xs.items():
-            errs[k] = max(errs[k], float(np.linalg.norm(x[v0:v0 + n] - xr) / np.linalg.norm(xr)))
+            xi = x[v0:v0 + n]
+            errs[k] = max(errs[k], float(np.linalg.norm(xi - xr) / np.linalg.norm(xr)))
+            rr = rhs - Am @ xi
+            bwd[k] = max(bwd[k], float(np.abs(rr).max() / (np.abs(Am).sum(1).max()
+                                                         * np.abs(xi).max() + np.abs(rhs).max())))
     del host
     return {"value": len(idx) / t_tot, "unit": "solves/s", "cores": 1,
             "kind": "reference" if use_eigen else "port",
             "sample": f"every 16th instance ({len(idx)} of {len(bt['N'])}): affine assembly + "
                       + ("Eigen 3.3.7 ColPivHouseholderQR (reference solver, oracle/_ref)"
                          if use_eigen else "LAPACK LU (numpy)"),
-            "max_rel_err_vs_reference": errs}
+            "max_rel_err_vs_reference": errs, "max_backward_err": bwd}
 
 
 def pod_bench(args):
@@ -302,7 +323,8 @@ def pod_bench(args):
     work = torch.empty((pod._lib().irom_gram_workspace_size(k_snap, k_snap, N) + 7) // 8,
                        dtype=torch.float64, device=dev)
     G = {}
-    flops = 2.0 * k_snap * k_snap * N
+    flops = 2.0 * k_snap * k_snap * N          # every block of C = S (M S)^T
+    flops_sym = 1.0 * k_snap * (k_snap + 1) * N  # upper triangle only (symmetric result)
     for mode, name in ((pod.GRAM_FP64, "fp64"), (pod.GRAM_FP32X3, "fp32x3"),
                        (pod.GRAM_BF16X3, "bf16x3")):
         out = torch.empty((k_snap, k_snap), dtype=torch.float64, device=dev)
@@ -310,26 +332,58 @@ def pod_bench(args):
         ms = _event_time(lambda: pod.gram(S, MS, mode=mode, out=out, work=work,
                                           symmetric_result=True), 2, 1)
         G[name] = out
-        res[f"gram_{name}"] = {"ms": ms, "TFLOPs_algorithmic": flops / (ms * 1e-3) / 1e12,
+        res[f"gram_{name}"] = {"ms_symmetric": ms,
+                               "TFLOPs_symmetric": flops_sym / (ms * 1e-3) / 1e12,
+                               "flops_symmetric": flops_sym,
                                "ms_all_blocks": ms_full,
                                "TFLOPs_all_blocks": flops / (ms_full * 1e-3) / 1e12,
-                               "HBM_GBs_reading_S_and_MS_once": 2 * k_snap * N * 8
-                               / (ms * 1e-3) / 1e9}
+                               "flops_all_blocks": flops,
+                               "HBM_floor_ms_reading_S_and_MS_once": 2 * k_snap * N * 8
+                               / (HBM_PEAK_GBS * 1e9) * 1e3}
+        res[f"gram_{name}"]["frac_of_hbm_floor_symmetric"] = (
+            res[f"gram_{name}"]["HBM_floor_ms_reading_S_and_MS_once"] / ms)
     gn = torch.linalg.matrix_norm(G["fp64"]).item()
     for name in ("fp32x3", "bf16x3"):
-        res[f"gram_{name}"]["rel_err_vs_fp64"] = (
+        res[f"gram_{name}"]["rel_err_vs_gpu_fp64"] = (
             torch.linalg.matrix_norm(G[name] - G["fp64"]).item() / gn)
     res["gram_fp64"]["mfma_frac_of_78.6TF_all_blocks"] = (res["gram_fp64"]["TFLOPs_all_blocks"]
                                                           / 78.6)
+    res["gram_fp64"]["mfma_frac_of_78.6TF_symmetric"] = (res["gram_fp64"]["TFLOPs_symmetric"]
+                                                         / 78.6)
+    if not args.no_cpu:
+        # FP64 CPU oracle over the full K = N: the 32 x 32 principal submatrix of the Gram
+        # on every 16th snapshot, numpy dot products of the full-length rows
+        sub = torch.arange(0, k_snap, 16, device=dev)
+        Sh = S[sub].cpu().numpy()
+        MSh = MS[sub].cpu().numpy()
+        from threadpoolctl import threadpool_limits
+        with threadpool_limits(limits=min(16, os.cpu_count() or 1)):
+            t0 = time.perf_counter()
+            Gsub = Sh @ MSh.T
+            t_sub = time.perf_counter() - t0
+        gsn = np.linalg.norm(Gsub)
+        for name in ("fp64", "fp32x3", "bf16x3"):
+            Gg = G[name][sub][:, sub].cpu().numpy()
+            res[f"gram_{name}"]["rel_err_vs_cpu_fp64_submatrix"] = float(
+                np.linalg.norm(Gg - Gsub) / gsn)
+        res["gram_cpu_fp64_oracle"] = {
+            "sample": f"{len(sub)} x {len(sub)} principal submatrix (every 16th snapshot), "
+                      f"full K = {N}, numpy FP64", "s": t_sub}
+        del Sh, MSh
     res["gram_fp64"]["rocblas_dgemm_ms"] = _event_time(lambda: torch.matmul(S, MS.T), 1, 1)
 
     # 5. eigendecomposition (FP64 Jacobi on the GPU), truncation, basis
-    t0 = time.perf_counter()
     Gs = 0.5 * (G["fp64"] + G["fp64"].T)
-    ev, Psi, sweeps = pod.eig_sym(Gs)
-    torch.cuda.synchronize()
-    res["eig_ms"] = (time.perf_counter() - t0) * 1e3
+    ev, Psi, sweeps = pod.eig_sym(Gs)  # first call: allocates the kept workspace
+    te = []
+    for _ in range(3):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ev, Psi, sweeps = pod.eig_sym(Gs)
+        te.append((time.perf_counter() - t0) * 1e3)
+    res["eig_ms"] = float(np.median(te))
     res["eig_sweeps"] = sweeps
+    res["eig_method"] = "blocked one-sided Jacobi, one persistent launch (median of 3)"
     r = pod.energy_truncation(ev, 1.0 - 1e-8)
     res["rank_energy_1e-8"] = r
     C = Psi[:r] / torch.sqrt(torch.as_tensor(ev[:r], device=dev))[:, None]
@@ -355,31 +409,51 @@ def pod_bench(args):
     O = pod.gram(V, MV)
     res["basis_M_orthonormality_err"] = (O - torch.eye(r, dtype=torch.float64,
                                                        device=dev)).abs().max().item()
-    # 6. Galerkin projection of the Helmholtz operator: A_r = V^T H V
+    # 6. Galerkin projection of the affine terms of the Helmholtz operator, one at a
+    # time (stiffness K = Helmholtz with lambda = 0, mass M): C_q = V A_q V^T, A_q applied
+    # to all r basis vectors in one launch, then the product on MFMA; A(10) = K + 10 M
+    terms = pod.helmholtz_affine_terms(nm, nq, E, b, jac, df)
+    AV = MV
+    Cq, msq = {}, {}
+    for name, opq in terms.items():
+        msq[name] = _event_time(lambda: pod.galerkin_project(opq, V, W=AV), 2, 1)
+        Cq[name] = {m: pod.galerkin_project(opq, V, mode=mode, W=AV)[0]
+                    for m, mode in (("fp64", pod.GRAM_FP64), ("fp32x3", pod.GRAM_FP32X3),
+                                    ("bf16x3", pod.GRAM_BF16X3))}
+        if not args.no_cpu:
+            # FP64 CPU oracle of the product: V and A_q V (FP64 GPU apply, itself pinned to
+            # the oracle to 1e-12) to the host, numpy V (A_q V)^T over the full N
+            Vh, AVh = V.cpu().numpy(), AV.cpu().numpy()
+            Cq[name]["cpu"] = Vh @ AVh.T
+            del Vh, AVh
     Hop = H.Operator("Helmholtz_Hex_Deformed", nm, nq, E, basis=b)
     Hop.set_jac(jac)
     Hop.set_df(df)
     Hop.set_lambda(w["lam"])
-    HV = MV
-
-    def project():
-        for i in range(r):
-            Hop.apply(V[i], HV[i])
-        return pod.gram(V, HV)
-    ms_proj = _event_time(project, 2, 1)
-    Ar = project()
-    res["galerkin_projection"] = {
-        "ms": ms_proj, "r": r,
-        "symmetry_rel_err": (torch.linalg.matrix_norm(Ar - Ar.T)
-                             / torch.linalg.matrix_norm(Ar)).item()}
-    for name, mode in (("fp32x3", pod.GRAM_FP32X3), ("bf16x3", pod.GRAM_BF16X3)):
-        Am = pod.gram(V, HV, mode=mode)
-        res["galerkin_projection"][f"{name}_rel_err_vs_fp64"] = (
-            torch.linalg.matrix_norm(Am - Ar) / torch.linalg.matrix_norm(Ar)).item()
+    ms_proj = _event_time(lambda: pod.galerkin_project(Hop, V, W=AV), 2, 1)
+    Ar, _ = pod.galerkin_project(Hop, V, W=AV)
+    comb = w["lam"] * Cq["mass"]["fp64"] + Cq["stiffness"]["fp64"]
+    gp = {"r": r, "ms_helmholtz_lambda10": ms_proj,
+          "ms_per_term": msq,
+          "term_recombination_rel_err": (torch.linalg.matrix_norm(comb - Ar)
+                                         / torch.linalg.matrix_norm(Ar)).item(),
+          "symmetry_rel_err": (torch.linalg.matrix_norm(Ar - Ar.T)
+                               / torch.linalg.matrix_norm(Ar)).item(),
+          "bytes_per_launch_apply": E * 8 * (2 * M * r + 7 * Q),
+          "entry": "irom_galerkin_project (multi-vector apply + MFMA product)"}
+    for name in terms:
+        ref = Cq[name].get("cpu")
+        if ref is None:
+            continue
+        for m in ("fp64", "fp32x3", "bf16x3"):
+            gp[f"{name}_{m}_rel_err_vs_cpu_fp64"] = float(
+                np.linalg.norm(Cq[name][m].cpu().numpy() - ref) / np.linalg.norm(ref))
+    res["galerkin_projection"] = gp
+    HV = AV
 
     if not args.no_cpu:
         res["cpu_baseline"] = pod_cpu_baseline(cpu_slice, G["fp64"], ev, r)
-    del S, V, MV, HV, Hop, proj, work, G
+    del S, V, MV, HV, Hop, proj, work, G, terms, Cq
     torch.cuda.empty_cache()
     return res
 
@@ -748,6 +822,83 @@ def copy_calibration(args):
                       "ms_per_step": ms, "steps": args.steps, "frac_of_8TBs": gbs / HBM_PEAK_GBS}))
 
 
+OP_KEYS = {"helmholtz": "Helmholtz_Hex_Deformed", "bwdtrans": "BwdTrans_Hex_Regular",
+           "iproduct": "IProduct_Hex_Deformed", "physderiv": "PhysDeriv_Hex_Deformed",
+           "iprodderiv": "IProductWRTDerivBase_Hex_Deformed"}
+
+
+def time_operator(w, opname, precision, steps, warmup, keep=False):
+    """Mean time of one apply of the hex operator `opname` over the workload w, timed with
+    HIP events recorded on the stream the kernel is launched on; algorithmic bytes and the
+    fraction of the 8 TB/s HBM peak."""
+    import torch
+
+    import ithaca_hip as H
+
+    dev = torch.device("cuda:0")
+    nm, nq, E = w["nm"], w["nq"], w["nElmt"]
+    prec = H.FP64 if precision == "fp64" else H.FP32
+    dtype = torch.float64 if prec == H.FP64 else torch.float32
+    b = H.basis_modified_a(nm, nq)
+    verts = torch.from_numpy(w["verts"]).to(dev)
+    key = OP_KEYS[opname]
+    op = H.Operator(key, nm, nq, E, precision=prec, basis=b)
+    if op.needs_jac() or op.needs_df():
+        gj, gd = H.geom_device("hex", verts, nq, b["z"], True)
+        if op.needs_jac():
+            op.set_jac(gj)
+        if op.needs_df():
+            op.set_df(gd)
+        del gj, gd
+    op.set_lambda(w["lam"])
+    if opname in ("iproduct", "physderiv", "iprodderiv"):
+        x = torch.rand(op.in_size, dtype=torch.float64, device=dev).mul_(2).sub_(1).to(dtype)
+    else:
+        x = torch.from_numpy(w["u"]).to(dev, dtype)
+    y = torch.empty(op.out_size, dtype=dtype, device=dev)
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    for _ in range(warmup):
+        op.apply(x, y, stream)
+    torch.cuda.synchronize()
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    ev0.record(stream)
+    for _ in range(steps):
+        op.apply(x, y, stream)
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms = ev0.elapsed_time(ev1) / steps
+    elt = 8 if prec == H.FP64 else 4
+    abytes = algorithmic_bytes(opname, nm, nq, elt) * E
+    achieved = abytes / (ms * 1e-3) / 1e9
+    res = {"key": key, "precision": precision, "ms": ms, "wall": wall,
+           "gdof_per_s": E * nm ** 3 / (ms * 1e-3) / 1e9, "achieved": achieved,
+           "abytes": abytes, "frac_of_8TBs": achieved / HBM_PEAK_GBS,
+           "algorithmic_bytes_per_element": algorithmic_bytes(opname, nm, nq, elt)}
+    if keep:
+        res.update(_op=op, _x=x, _y=y)
+    return res
+
+
+def hex_operators_bench(args):
+    """The other timings of the 48^3 hex workload: Helmholtz in FP32, and the standalone
+    BwdTrans and IProductWRTBase (f = BwdTrans(u_hat)-sized random input) in FP64 and FP32,
+    20 applies each."""
+    from ithaca_hip import mesh
+
+    w = mesh.workload_hex_helmholtz(args.n)
+    out = {}
+    for opname, prec in (("helmholtz", "fp32"), ("bwdtrans", "fp64"), ("iproduct", "fp64"),
+                         ("bwdtrans", "fp32"), ("iproduct", "fp32")):
+        r = time_operator(w, opname, prec, 20, 3)
+        out[f"{opname}_{prec}"] = {k: r[k] for k in ("key", "ms", "gdof_per_s", "frac_of_8TBs",
+                                                      "algorithmic_bytes_per_element")}
+    return out
+
+
 def main():
     args = parse()
     if args.op == "copy":
@@ -762,50 +913,13 @@ def main():
     import ithaca_hip as H
     from ithaca_hip import mesh
 
-    dev = torch.device("cuda:0")
     w = mesh.workload_hex_helmholtz(args.n)
     nm, nq, E = w["nm"], w["nq"], w["nElmt"]
+    t = time_operator(w, args.op, args.precision, args.steps, args.warmup, keep=True)
+    key, ms, wall, achieved, abytes = t["key"], t["ms"], t["wall"], t["achieved"], t["abytes"]
+    op, x, y = t.pop("_op"), t.pop("_x"), t.pop("_y")
     prec = H.FP64 if args.precision == "fp64" else H.FP32
-    dtype = torch.float64 if prec == H.FP64 else torch.float32
-    b = H.basis_modified_a(nm, nq)
-    verts = torch.from_numpy(w["verts"]).to(dev)
-    key = {"helmholtz": "Helmholtz_Hex_Deformed", "bwdtrans": "BwdTrans_Hex_Regular",
-           "iproduct": "IProduct_Hex_Deformed", "physderiv": "PhysDeriv_Hex_Deformed",
-           "iprodderiv": "IProductWRTDerivBase_Hex_Deformed"}[args.op]
-    op = H.Operator(key, nm, nq, E, precision=prec, basis=b)
-    if op.needs_jac() or op.needs_df():
-        gj, gd = H.geom_device("hex", verts, nq, b["z"], True)
-        if op.needs_jac():
-            op.set_jac(gj)
-        if op.needs_df():
-            op.set_df(gd)
-        del gj, gd
-    op.set_lambda(w["lam"])
-    if args.op in ("iproduct", "physderiv", "iprodderiv"):
-        x = torch.rand(op.in_size, dtype=torch.float64, device=dev).mul_(2).sub_(1).to(dtype)
-    else:
-        x = torch.from_numpy(w["u"]).to(dev, dtype)
-    y = torch.empty(op.out_size, dtype=dtype, device=dev)
-    torch.cuda.synchronize()
-
-    stream = torch.cuda.current_stream()
-    for _ in range(args.warmup):
-        op.apply(x, y, stream)
-    torch.cuda.synchronize()
-    ev0 = torch.cuda.Event(enable_timing=True)
-    ev1 = torch.cuda.Event(enable_timing=True)
-    t0 = time.perf_counter()
-    ev0.record(stream)
-    for _ in range(args.steps):
-        op.apply(x, y, stream)
-    ev1.record(stream)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    ms = ev0.elapsed_time(ev1) / args.steps
     ndof = E * nm ** 3
-    elt = 8 if prec == H.FP64 else 4
-    abytes = algorithmic_bytes(args.op, nm, nq, elt) * E
-    achieved = abytes / (ms * 1e-3) / 1e9
 
     traffic = None
     tpath = os.path.join(REPO, "profiles", f"traffic_{args.op}_hex_{args.precision}.json")
@@ -863,6 +977,7 @@ def main():
         print(f"[bench] headline: {out['value']:.2f} GDOF/s", file=sys.stderr, flush=True)
         if not args.no_rom:
             section("batched_reduced_solves", rom_bench)
+        section("hex_operators", hex_operators_bench)
         section("quad_helmholtz", quad_bench)
         section("quad_helmsolve", quad_helmsolve_bench)
         section("deriv_operators", deriv_bench)

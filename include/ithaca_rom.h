@@ -50,7 +50,10 @@ int irom_batch_layout(irom_batch b, long *a_off, long *t_off, long *v_off, size_
                       size_t *t_size, size_t *v_size);
 
 /* Solve every instance (transpose != 0 solves A(mu)^T x = b, the adjoint system).
- * tol: relative backward-error target of the refinement (IROM_LU_FP32_IR); max_iter caps it.
+ * tol: refinement target (IROM_LU_FP32_IR*): an instance has converged when the normwise
+ * backward error ||b - A x||inf / (||A||inf ||x||inf + ||b||inf) is at most tol and the last
+ * correction is at most 100 tol ||x||inf or above 0.9 of the one before (the accuracy of
+ * the FP64 residual is reached); max_iter caps the number of corrections.
  * One persistent launch (plus a counter reset; two for IROM_LU_FP32_IR_FALLBACK).
  * Per-instance refinement iterations and status (0 ok, 1 zero pivot, 2 refinement not
  * converged, 3 refinement not converged and solved by the FP64 fallback) go to d_iters /

@@ -591,6 +591,13 @@ batched_solve_kernel(Args a)
             const double bnorm = block_max(bmax, sv);
             double anorm = 0.0;
             bool conv = false;
+            // Stopping rule: the normwise backward error is at most tol, and the last
+            // correction is either at most 100 tol relative to x or no longer shrinking
+            // (above 0.9 of the one before: the iterate has reached the accuracy of the FP64
+            // residual).  A backward-error test alone would stop while the forward error is
+            // still about cond(A) * tol; refinement that converges slowly (cond(A) near
+            // 1 / eps_FP32) reaches max_iter instead and reports status 2.
+            double dn = 0.0, dprev = INFINITY;
             for (; it <= a.max_iter; ++it) {
                 const double am = it == 0 ? residual<true>(A64, n, x64, b64, r64)
                                           : residual<false>(A64, n, x64, b64, r64);
@@ -603,7 +610,8 @@ batched_solve_kernel(Args a)
                 }
                 const double rn = block_max(rm, sv);
                 const double xn = block_max(xm, sv);
-                if (rn <= a.tol * (anorm * xn + bnorm)) {
+                if (it > 0 && rn <= a.tol * (anorm * xn + bnorm) &&
+                    (dn <= 100.0 * a.tol * xn || dn > 0.9 * dprev)) {
                     conv = true;
                     break;
                 }
@@ -611,8 +619,14 @@ batched_solve_kernel(Args a)
                 for (int i = tid; i < n; i += NT) yv[i] = (T)r64[pv.posAll[i]];
                 __syncthreads();
                 lu_solve_rows<T>(LU, n, yv, ys);
-                for (int i = tid; i < n; i += NT) x64[i] += (double)yv[i];
-                __syncthreads();
+                double dm = 0.0;
+                for (int i = tid; i < n; i += NT) {
+                    const double d = (double)yv[i];
+                    x64[i] += d;
+                    dm = fmax(dm, fabs(d));
+                }
+                dprev = it == 0 ? INFINITY : dn;
+                dn = block_max(dm, sv);  // ends with a barrier
             }
             if (!conv && !st) {
                 st = 2;

@@ -249,14 +249,18 @@ __global__ __launch_bounds__(1024) void sort_desc(int n, int m, const double *__
     }
 }
 
-// out column j (ld n) <- V column perm[j] restricted to the first n rows
+// out column j (ld n) <- V column perm[j] restricted to the first n rows, divided by its
+// norm vn[perm[j]]: the accumulated products of rotations leave each column's length off
+// one by O(rotations * eps); the normalisation removes that drift (directions unchanged)
 __global__ void gather_columns(int n, int np, const int *__restrict__ perm,
-                               const double *__restrict__ V, double *__restrict__ out)
+                               const double *__restrict__ V, const double *__restrict__ vn,
+                               double *__restrict__ out)
 {
     const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= (long)n * n) return;
     const int j = (int)(g / n), i = (int)(g - (long)j * n);
-    out[g] = V[(long)perm[j] * np + i];
+    const int c = perm[j];
+    out[g] = V[(long)c * np + i] / vn[c];
 }
 
 // Block width for order n: the widest B with 2B staged columns of np rows in EIG_LDS.
@@ -281,7 +285,7 @@ inline EigPlan eig_plan(int n)
 }
 
 // Workspace layout (bytes, each part 256-byte aligned): W, V (np^2 doubles each), norms
-// (np), sorted values (n), perm (n ints), rot flags, barrier words, sweeps, err.
+// (2 np: of W's and of V's columns), sorted values (n), perm (n ints), rot flags, barrier words, sweeps, err.
 struct EigWs {
     double *W, *V, *nrm, *ev;
     int *perm, *ints;  // ints: rot[EIG_MAX_SWEEPS], bar[2], sweeps, err
@@ -292,7 +296,7 @@ inline size_t al256(size_t b) { return (b + 255) / 256 * 256; }
 inline size_t eig_ws_bytes(int n)
 {
     const EigPlan p = eig_plan(n);
-    return 2 * al256(sizeof(double) * (size_t)p.np * p.np) + al256(sizeof(double) * p.np) +
+    return 2 * al256(sizeof(double) * (size_t)p.np * p.np) + al256(sizeof(double) * 2 * p.np) +
            al256(sizeof(double) * n) + al256(sizeof(int) * n) +
            al256(sizeof(int) * (EIG_MAX_SWEEPS + 4));
 }
@@ -306,8 +310,8 @@ inline EigWs eig_ws(int n, void *base)
     c += al256(sizeof(double) * (size_t)p.np * p.np);
     w.V = (double *)c;
     c += al256(sizeof(double) * (size_t)p.np * p.np);
-    w.nrm = (double *)c;
-    c += al256(sizeof(double) * p.np);
+    w.nrm = (double *)c;  // W column norms, then V column norms
+    c += al256(sizeof(double) * 2 * p.np);
     w.ev = (double *)c;
     c += al256(sizeof(double) * n);
     w.perm = (int *)c;
@@ -373,10 +377,11 @@ int eig_run(int n, const double *d_G, double *h_evals, double *d_evecs, double t
     }
     IROM_ECHECK(e, "block_jacobi launch");
     irom::column_norms<<<p.np, irom::ET, 0, s>>>(p.np, w.W, w.nrm);
+    irom::column_norms<<<p.np, irom::ET, 0, s>>>(p.np, w.V, w.nrm + p.np);
     const size_t sort_lds = (sizeof(double) + sizeof(int)) * (size_t)p.m;
     irom::sort_desc<<<1, 1024, sort_lds, s>>>(n, p.m, w.nrm, w.ev, w.perm);
-    irom::gather_columns<<<(unsigned)(((long)n * n + 255) / 256), 256, 0, s>>>(n, p.np, w.perm,
-                                                                              w.V, d_evecs);
+    irom::gather_columns<<<(unsigned)(((long)n * n + 255) / 256), 256, 0, s>>>(
+        n, p.np, w.perm, w.V, w.nrm + p.np, d_evecs);
     IROM_ECHECK(hipGetLastError(), "eig finish");
     int info[2] = {0, 0};
     IROM_ECHECK(hipMemcpyAsync(h_evals, w.ev, sizeof(double) * n, hipMemcpyDeviceToHost, s), "copy");

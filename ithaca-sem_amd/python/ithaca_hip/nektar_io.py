@@ -348,12 +348,21 @@ def field_coefficients(blocks, field, element_ids):
         raise NektarIOError(f"field {field!r} has no data for element {exc.args[0]}") from None
 
 
-def expansion_arrays(mesh, blocks, field):
+def expansion_arrays(mesh, blocks, field, points="session"):
     """Inputs for the operators and the assembly of one field, in the element order of
     the file: (coords (E, 2^dim, dim) in local vertex order, vertex_ids (E, 2^dim),
     coefficients (element-major), nm, nq), with nm and nq ints when every element has the
-    same order and per-element arrays otherwise (MIXORDER files of variable-order runs).  The basis is Modified_A with
-    nm + 1 Gauss-Lobatto-Legendre points per direction (the reference's TYPE="MODIFIED")."""
+    same order and per-element arrays otherwise (MIXORDER files of variable-order runs).
+    The basis is Modified_A with Gauss-Lobatto-Legendre points, their number per direction
+    set by `points`:
+      "session": nm + 1, the reference's TYPE="MODIFIED" expansion of a session file;
+      "field":   the expansion the reference builds from a field file's own definition
+                 when it names no points (MeshGraph::SetExpansionInfo(fielddef),
+                 SpatialDomains/MeshGraph.cpp:1179-1183): nm points for hexahedra and
+                 nm + 1 for quadrilaterals.  FieldConvert uses it whenever a field file is
+                 given without --useSessionExpansion (FieldUtils/Field.hpp:141)."""
+    if points not in ("session", "field"):
+        raise ValueError("points must be 'session' or 'field'")
     blk = [b for b in blocks if field in b.data]
     if not blk:
         raise NektarIOError(f"no data for field {field!r}")
@@ -368,8 +377,9 @@ def expansion_arrays(mesh, blocks, field):
     nmv = modes[:, 0]
     # one order everywhere: ints; variable order: per-element arrays (Assembly takes either)
     nm = int(nmv[0]) if np.all(nmv == nmv[0]) else nmv.astype(np.int32)
+    nq = nm if (points == "field" and mesh.dim == 3) else nm + 1
     return (mesh.element_coords(ids), mesh.vertex_ids(ids),
-            field_coefficients(blocks, field, ids), nm, nm + 1)
+            field_coefficients(blocks, field, ids), nm, nq)
 
 
 # ---------------------------------------------------------------- writing

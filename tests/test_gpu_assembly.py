@@ -162,44 +162,75 @@ def test_quad_helmsolve_rejects_a_hex_operator():
 
 def test_smoothing_a_nektar_field():
     """The reference's FieldConvert Helmholtz smoother (ProcessC0Projection.cpp:196-224:
-    lambda = (2 pi / L)^2, forcing -lambda u, natural boundary conditions) on the mesh and
-    field of its test Hex_channel_C0helmsmoothing (read with nektar_io): GPU BwdTrans,
-    IProduct, assembly and CG against a dense-element sparse direct solve."""
+    lambda = (2 pi / L)^2 with L = 0.6, forcing -lambda u, natural boundary conditions) on
+    the mesh and field of its test Hex_channel_C0helmsmoothing, through the GPU BwdTrans,
+    IProduct, assembly and CG: the solution equals a dense-element sparse direct solve, and
+    the seven printed L2 / "L inf" lines of Hex_channel_C0helmsmoothing.tst are reproduced
+    at their tolerances.  As in the reference, the expansion comes from the field file
+    (hexahedra: nm GLL points per direction, MeshGraph.cpp:1179-1183), the norms are taken
+    after re-expanding on nm evenly spaced points per direction (OutputFileBase.cpp:83-86)
+    and "L inf" is the largest signed value (ExpList.cpp:3034-3037)."""
+    from test_nektar_io import SMOOTHING_GOLDEN, equispaced_rule, printed
     gold = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "nektar")
-    mesh = N.read_mesh(os.path.join(gold, "Hex_channel_C0helmsmoothing.xml"))
+    nmesh = N.read_mesh(os.path.join(gold, "Hex_channel_C0helmsmoothing.xml"))
     blocks = N.read_fld(os.path.join(gold, "Hex_channel_C0helmsmoothing.fld"))
-    X, ids, c, nm, nq = N.expansion_arrays(mesh, blocks, "u")
     lam = (2 * np.pi / 0.6) ** 2
-    E, M = len(ids), nm ** 3
-    # CPU: dense element matrices assembled with the same map, every mode free
-    l2g, sign, _, ng = assembly.assembly_map_hex(ids, nm)
-    rows, cols, vals, rhs = [], [], [], np.zeros(ng)
-    for e in range(E):
-        m = DR.element_mats("hex", nm, nq, X[e])
-        g, s_ = l2g[e * M:(e + 1) * M], sign[e * M:(e + 1) * M].astype(float)
-        rows.append(np.repeat(g, M))
-        cols.append(np.tile(g, M))
-        vals.append((DR.helmholtz_matrix(m, lam) * np.outer(s_, s_)).ravel())
-        phys = m["B"] @ c[e * M:(e + 1) * M]
-        np.add.at(rhs, g, s_ * (m["B"].T @ (m["W"] * m["J"] * lam * phys)))
-    Ag = sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))),
-                       shape=(ng, ng))
-    x_cpu = spla.spsolve(Ag.tocsc(), rhs)
-    # GPU
-    b = H.basis_modified_a(nm, nq)
-    jac, df = H.geom_device("hex", _dev(X), nq, b["z"], True)
-    Hop = H.Operator("Helmholtz_Hex_Deformed", nm, nq, E, basis=b)
-    Hop.set_jac(jac)
-    Hop.set_df(df)
-    Hop.set_lambda(lam)
-    ipr = H.Operator("IProduct_Hex_Deformed", nm, nq, E, basis=b)
-    ipr.set_jac(jac)
-    bwd = H.Operator("BwdTrans_Hex_Regular", nm, nq, E, basis=b)
-    A = assembly.Assembly(ids, nm, dirichlet=False)
-    phys = bwd.apply(_dev(c))
-    x, it, res = A.helmsolve(Hop, A.assemble(ipr.apply(lam * phys)), tol=1e-13)
-    print(f"iterations {it}, residual {res:.2e}")
-    assert np.linalg.norm(x.cpu().numpy() - x_cpu) <= 1e-10 * np.linalg.norm(x_cpu)
+    got = {}
+    for f in ("u", "v", "w", "p"):
+        print(f"[case] C0 Helmholtz smoothing of field {f}", flush=True)
+        X, ids, c, nm, nq = N.expansion_arrays(nmesh, blocks, f, points="field")
+        E, M = len(ids), nm ** 3
+        # CPU: dense element matrices assembled with the same map, every mode free
+        l2g, sign, _, ng = assembly.assembly_map_hex(ids, nm)
+        rows, cols, vals, rhs = [], [], [], np.zeros(ng)
+        for e in range(E):
+            m = DR.element_mats("hex", nm, nq, X[e])
+            g, s_ = l2g[e * M:(e + 1) * M], sign[e * M:(e + 1) * M].astype(float)
+            rows.append(np.repeat(g, M))
+            cols.append(np.tile(g, M))
+            vals.append((DR.helmholtz_matrix(m, lam) * np.outer(s_, s_)).ravel())
+            phys = m["B"] @ c[e * M:(e + 1) * M]
+            np.add.at(rhs, g, s_ * (m["B"].T @ (m["W"] * m["J"] * lam * phys)))
+        Ag = sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))),
+                           shape=(ng, ng))
+        x_cpu = spla.spsolve(Ag.tocsc(), rhs)
+        # GPU
+        b = H.basis_modified_a(nm, nq)
+        vd = _dev(X)
+        jac, df = H.geom_device("hex", vd, nq, b["z"], True)
+        Hop = H.Operator("Helmholtz_Hex_Deformed", nm, nq, E, basis=b)
+        Hop.set_jac(jac)
+        Hop.set_df(df)
+        Hop.set_lambda(lam)
+        ipr = H.Operator("IProduct_Hex_Deformed", nm, nq, E, basis=b)
+        ipr.set_jac(jac)
+        bwd = H.Operator("BwdTrans_Hex_Regular", nm, nq, E, basis=b)
+        A = assembly.Assembly(ids, nm, dirichlet=False)
+        phys = bwd.apply(_dev(c))
+        x, it, res = A.helmsolve(Hop, A.assemble(ipr.apply(lam * phys)), tol=1e-13)
+        print(f"iterations {it}, residual {res:.2e}")
+        assert np.linalg.norm(x.cpu().numpy() - x_cpu) <= 1e-10 * np.linalg.norm(x_cpu)
+        # the output expansion: nm evenly spaced points per direction, Newton-Cotes weights
+        ze, we = equispaced_rule(nm)
+        B1, _ = DR.modified_a(nm, ze)
+        beq = dict(bdata=np.ascontiguousarray(B1.T.ravel()), dbdata=np.zeros(nm * nm),
+                   D=np.zeros(nm * nm), w=we, nm=nm, nq=nm)
+        out = H.Operator("BwdTrans_Hex_Regular", nm, nm, E, basis=beq)
+        ue = out.apply(A.global_to_local(x))
+        jeq, _ = H.geom_device("hex", vd, nm, ze, True)
+        W = torch.from_numpy(np.kron(np.kron(we, we), we)).cuda()
+        wj = (jeq.view(E, -1) * W).reshape(-1)
+        got[f] = (torch.sqrt(torch.dot(wj * ue, ue)).item(), ue.max().item())
+        if f == "u":
+            Xe = mesh.hex_quad_points(vd, ze)
+            for a, k in zip("xyz", range(3)):
+                col = Xe[..., k].reshape(-1)
+                got[a] = (torch.sqrt(torch.dot(wj * col, col)).item(), col.max().item())
+    for name, l2, tl2, linf, tlinf in SMOOTHING_GOLDEN:
+        print(f"{name}: L2 {got[name][0]:.7g} (golden {l2}), L inf {got[name][1]:.7g} "
+              f"(golden {linf})")
+        assert abs(printed(got[name][0]) - l2) <= tl2, name
+        assert abs(printed(got[name][1]) - linf) <= tlinf, name
 
 
 def test_helmholtz3d_hex_golden_on_the_gpu():

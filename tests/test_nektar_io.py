@@ -295,3 +295,97 @@ def test_variable_order_session(tmp_path):
     assert _continuity("hex", m.element_coords(), m.vertex_ids(), nmv, 8) < 1e-12
     with pytest.raises(N.NektarIOError, match="no expansion"):
         N.read_mesh(os.path.join(HERE, "Hex_channel_C0helmsmoothing.xml")).element_orders([99])
+
+
+# The printed norms of utilities/FieldConvert/Tests/Hex_channel_C0helmsmoothing.tst:
+# (field, L2, tolerance, "L inf", tolerance).
+SMOOTHING_GOLDEN = [("x", 0.57735, 1e-4, 1.0, 1e-4), ("y", 0.57735, 1e-4, 1.0, 1e-4),
+                    ("z", 0.57735, 1e-4, 1.0, 1e-4), ("u", 0.635792, 1e-6, 0.739606, 1e-6),
+                    ("v", 0.635792, 1e-6, 0.739606, 1e-6), ("w", 0.635792, 1e-6, 0.739606, 1e-6),
+                    ("p", 1.10303, 1e-6, 2.21882, 1e-6)]
+
+
+def printed(x):
+    """The value as the reference's output stream prints it (6 significant digits), which
+    is what its tester compares with a .tst value."""
+    return float(f"{x:.6g}")
+
+
+def equispaced_rule(n):
+    """n evenly spaced points on [-1, 1] and their weights (the exact integral of each
+    Lagrange polynomial, PolyEPoints::CalculateWeights, Foundations/PolyEPoints.cpp:69-96)."""
+    z = np.linspace(-1.0, 1.0, n)
+    V = np.vander(z, n, increasing=True)
+    mom = np.array([(1.0 - (-1.0) ** (k + 1)) / (k + 1) for k in range(n)])
+    return z, np.linalg.solve(V.T, mom)
+
+
+def test_helmholtz_smoothing_golden_on_the_cpu():
+    """FieldConvert -m C0Projection:helmsmoothing=0.6 on the reference's Hex_channel fixture,
+    restated with dense element matrices (tests/dense_ref.py) and a sparse direct solve,
+    reproduces all seven L2 / "L inf" lines of Hex_channel_C0helmsmoothing.tst.  Three facts
+    of the reference pipeline decide the numbers:
+    * the expansion comes from the field file, whose hexahedra get nm (not nm + 1) GLL
+      points per direction (MeshGraph.cpp:1179-1183; Field.hpp:141), so the smoothing runs
+      on 4 points for the 4-mode field;
+    * the .plt output re-expands every field on nm evenly spaced points per direction
+      (OutputFileBase.cpp:83-86, MeshGraph.cpp:1463-1497) and the L2 norm integrates with
+      their Newton-Cotes weights;
+    * "L inf error" without an exact solution is Vmath::Vmax, the largest signed value
+      (ExpList.cpp:3034-3037), not the largest magnitude.
+    The smoother is ProcessC0Projection.cpp:196-224: lambda = (2 pi / 0.6)^2, forcing
+    -lambda u, natural boundary conditions (a ContField of "DefaultVar" with no boundary
+    expansion)."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+    from ithaca_hip import assembly
+
+    mesh = N.read_mesh(os.path.join(HERE, "Hex_channel_C0helmsmoothing.xml"))
+    blocks = N.read_fld(os.path.join(HERE, "Hex_channel_C0helmsmoothing.fld"))
+    lam = (2 * np.pi / 0.6) ** 2
+    got = {}
+    for f in ("u", "v", "w", "p"):
+        X, ids, c, nm, nq = N.expansion_arrays(mesh, blocks, f, points="field")
+        assert (nm, nq) == (4, 4)
+        E, M = len(ids), nm ** 3
+        l2g, sign, _, ng = assembly.assembly_map_hex(ids, nm)
+        rows, cols, vals, rhs = [], [], [], np.zeros(ng)
+        for e in range(E):
+            m = DR.element_mats("hex", nm, nq, X[e])
+            g, s_ = l2g[e * M:(e + 1) * M], sign[e * M:(e + 1) * M].astype(float)
+            rows.append(np.repeat(g, M))
+            cols.append(np.tile(g, M))
+            vals.append((DR.helmholtz_matrix(m, lam) * np.outer(s_, s_)).ravel())
+            phys = m["B"] @ c[e * M:(e + 1) * M]
+            np.add.at(rhs, g, s_ * (m["B"].T @ (m["W"] * m["J"] * lam * phys)))
+        A = sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))),
+                          shape=(ng, ng))
+        x = spla.spsolve(A.tocsc(), rhs)
+        loc = sign * x[l2g]
+        ze, we = equispaced_rule(nm)
+        B1, _ = DR.modified_a(nm, ze)
+        Be = np.kron(np.kron(B1, B1), B1)
+        We = np.kron(np.kron(we, we), we)
+        pts = [np.array((ze[i], ze[j], ze[k]))
+               for k in range(nm) for j in range(nm) for i in range(nm)]
+        l2 = 0.0
+        vmax = -np.inf
+        cmax = {a: -np.inf for a in "xyz"}
+        cl2 = {a: 0.0 for a in "xyz"}
+        for e in range(E):
+            J = np.array([abs(np.linalg.det(DR.hex_map(X[e], p)[1])) for p in pts])
+            xyz = np.array([DR.hex_map(X[e], p)[0] for p in pts])
+            ue = Be @ loc[e * M:(e + 1) * M]
+            l2 += np.sum(We * J * ue * ue)
+            vmax = max(vmax, ue.max())
+            for a, col in zip("xyz", xyz.T):
+                cl2[a] += np.sum(We * J * col * col)
+                cmax[a] = max(cmax[a], col.max())
+        got[f] = (np.sqrt(l2), vmax)
+        for a in "xyz":
+            got[a] = (np.sqrt(cl2[a]), cmax[a])
+    for name, l2, tl2, linf, tlinf in SMOOTHING_GOLDEN:
+        print(f"{name}: L2 {got[name][0]:.7g} (golden {l2}), L inf {got[name][1]:.7g} "
+              f"(golden {linf})")
+        assert abs(printed(got[name][0]) - l2) <= tl2, name
+        assert abs(printed(got[name][1]) - linf) <= tlinf, name

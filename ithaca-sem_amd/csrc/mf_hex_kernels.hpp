@@ -364,7 +364,7 @@ constexpr bool helm_geo_in_lds()
 {
     using Lb = HexLds<NM, NQ>;
     return GEO == kDeformed &&
-           HexCfg<NQ>::EPB * (4 * Lb::QB + 7 * Lb::Q) * (int)sizeof(T) <= 64 * 1024;
+           HexCfg<NQ>::EPB * (3 * Lb::QB + 7 * Lb::Q) * (int)sizeof(T) <= 64 * 1024;
 }
 
 template <int NM, int NQ, typename T, int GEO, bool GATHER = false, bool MANY = false>
@@ -379,7 +379,7 @@ __global__ __launch_bounds__(HexCfg<NQ>::BLOCK) void hex_helmholtz_kernel(
     constexpr int M = Lb::M, Q = Lb::Q, TPE = C::TPE;
     constexpr long GS = geo_stride(7 * Q, sizeof(T));
     constexpr bool GLDS = MANY && helm_geo_in_lds<NM, NQ, T, GEO>();
-    __shared__ T lds[C::EPB][4][Lb::QB];
+    __shared__ T lds[C::EPB][3][Lb::QB];
     __shared__ T lgeo[GLDS ? C::EPB : 1][GLDS ? 7 * Q : 1];
     const int eb = threadIdx.x / TPE;
     const int t = threadIdx.x - eb * TPE;
@@ -388,7 +388,7 @@ __global__ __launch_bounds__(HexCfg<NQ>::BLOCK) void hex_helmholtz_kernel(
     const long e0 = (long)blockIdx.x * C::EPB, e = e0 + eb;
     const int nv = (int)min((long)C::EPB, (long)nElmt - e0);
     const bool valid = active && eb < nv;
-    T *buf0 = lds[es][0], *buf1 = lds[es][1], *buf2 = lds[es][2], *buf3 = lds[es][3];
+    T *buf0 = lds[es][0], *buf1 = lds[es][1], *buf2 = lds[es][2];
     const int ci = t % NQ, cj = t / NQ;  // this thread's column (i, j)
     if constexpr (GLDS) {
         // the block's elements' geometry, coalesced, once (made visible by the first
@@ -504,11 +504,12 @@ __global__ __launch_bounds__(HexCfg<NQ>::BLOCK) void hex_helmholtz_kernel(
 #pragma unroll
         for (int k = 0; k < NQ; ++k) g0[k] = g1[k] = g2[k] = acc[k] = T(0);
     }
+    // g0, g1 replace d0, d1 at this thread's own positions (no other thread reads them)
     if (active) {
 #pragma unroll
         for (int k = 0; k < NQ; ++k) {
-            buf2[k * TPE + t] = g0[k];
-            buf3[k * TPE + t] = g1[k];
+            buf0[k * TPE + t] = g0[k];
+            buf1[k * TPE + t] = g1[k];
         }
     }
     // D^T along k in registers.
@@ -521,42 +522,43 @@ __global__ __launch_bounds__(HexCfg<NQ>::BLOCK) void hex_helmholtz_kernel(
     }
     __syncthreads();
 
-    // 4. D^T along i (pencils of thread (j,k)) and along j (pencils of thread (i,k)).
+    // 4. D^T along i (pencils of thread (j,k)) into buf2, then D^T along j (pencils of
+    // thread (i,k)) added to it: three buffers instead of four per element.
     if (active) {
         T v[NQ];
-        {
-            const int pj = t % NQ, pk = t / NQ;
+        const int pj = t % NQ, pk = t / NQ;
 #pragma unroll
-            for (int l = 0; l < NQ; ++l) v[l] = buf2[(pk * NQ + pj) * NQ + l];
+        for (int l = 0; l < NQ; ++l) v[l] = buf0[(pk * NQ + pj) * NQ + l];
 #pragma unroll
-            for (int i = 0; i < NQ; ++i) {
-                T s = v[0] * tab.D[i * NQ];
+        for (int i = 0; i < NQ; ++i) {
+            T s = v[0] * tab.D[i * NQ];
 #pragma unroll
-                for (int l = 1; l < NQ; ++l) s = fma(v[l], tab.D[i * NQ + l], s);
-                buf0[(pk * NQ + pj) * NQ + i] = s;
-            }
+            for (int l = 1; l < NQ; ++l) s = fma(v[l], tab.D[i * NQ + l], s);
+            buf2[(pk * NQ + pj) * NQ + i] = s;
         }
-        {
-            const int pi = t % NQ, pk = t / NQ;
+    }
+    __syncthreads();
+    if (active) {
+        T v[NQ];
+        const int pi = t % NQ, pk = t / NQ;
 #pragma unroll
-            for (int l = 0; l < NQ; ++l) v[l] = buf3[(pk * NQ + l) * NQ + pi];
+        for (int l = 0; l < NQ; ++l) v[l] = buf1[(pk * NQ + l) * NQ + pi];
 #pragma unroll
-            for (int j = 0; j < NQ; ++j) {
-                T s = v[0] * tab.D[j * NQ];
+        for (int j = 0; j < NQ; ++j) {
+            T s = v[0] * tab.D[j * NQ];
 #pragma unroll
-                for (int l = 1; l < NQ; ++l) s = fma(v[l], tab.D[j * NQ + l], s);
-                buf1[(pk * NQ + j) * NQ + pi] = s;
-            }
+            for (int l = 1; l < NQ; ++l) s = fma(v[l], tab.D[j * NQ + l], s);
+            buf2[(pk * NQ + j) * NQ + pi] += s;
         }
     }
     __syncthreads();
     if (active) {
 #pragma unroll
-        for (int k = 0; k < NQ; ++k) acc[k] += buf0[k * TPE + t] + buf1[k * TPE + t];
+        for (int k = 0; k < NQ; ++k) acc[k] += buf2[k * TPE + t];
     }
 
     // 5. (B x B x B)^T of the combined quadrature-space field, then one coalesced store.
-    hex_columns_to_modes<NM, NQ, T>(tab, acc, active, t, buf2, buf3,
+    hex_columns_to_modes<NM, NQ, T>(tab, acc, active, t, buf0, buf1,
                                     [&](int p, T v) { buf2[t * (NM + 1) + p] = v; });
     __syncthreads();
     store_modes_block<NM, NQ, T>(out + e0 * M, nv, [&](int sl) { return lds[sl][2]; });

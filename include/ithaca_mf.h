@@ -42,9 +42,9 @@ enum imf_precision { IMF_FP64 = 0, IMF_FP32 = 1 };
  * with the Helper<DIM,DEFORMED> constructor that copies the basis tables
  * (Operator.hpp:205-275).  Per direction d < dim: bdata[d] (nm*nq, bdata[p*nq+i]),
  * dbdata[d] (nm*nq), D[d] (nq*nq, D[a*nq+b] = l_a'(z_b)), w[d] (nq).  Like the
- * reference kernels the operator requires homogeneous modes/points and supports
- * nm in 2..8 with nq in nm..2*nm (Helmholtz.h:669-760).  No padding restriction on
- * nElmt (the reference's vec_t::width padding is handled internally). */
+ * reference kernels the isotropic operators support nm in 2..8 with nq in nm..2*nm
+ * (Helmholtz.h:669-760).  No padding restriction on nElmt (the reference's vec_t::width
+ * padding is handled internally). */
 /* Also accepted, as in the reference's MatrixFree factory:
  *   "PhysDeriv_{Hex,Quad}_{Regular,Deformed}" (MatrixFreeOps/PhysDeriv.cpp:14-36;
  *     PhysDeriv.h:260-642): in = nElmt*Q point values, out = dim consecutive arrays of
@@ -55,6 +55,12 @@ enum imf_precision { IMF_FP64 = 0, IMF_FP32 = 1 };
  *     consecutive arrays of nElmt*Q (the reference's in[0..dim-1]), out = nElmt*M;
  *     needs SetJac and SetDF.  Quads take 2D coordinates (dim*dim = 4 factors).
  * Neither has a transpose (imf_apply_transpose_device reports IMF_ERR_UNSUPPORTED).
+ * Different modes, points or tables per direction (hexahedra, FP64; BwdTrans, IProduct,
+ * Helmholtz and Mass): the reference's MatrixFree kernels are isotropic and use direction
+ * 0's tables for all three (Helmholtz.h:844-856); such expansions run there through the
+ * StdRegions matrix path.  Here they get runtime-size kernels (2 <= nm[d] <= nq[d] <= 16),
+ * reading SetJac / SetDF arrays over nq[0] nq[1] nq[2] points per element; the coefficient
+ * layout is (r*nm[1] + q)*nm[0] + p and the point layout (k*nq[1] + j)*nq[0] + i.
  * Also accepted, with no MatrixFreeOps counterpart: "Mass_Hex_Deformed" and
  * "Mass_Hex_Regular", the elemental mass action out = B^T diag(w J) B in of
  * StdRegions/StdExpansion.cpp:647-661 (MassMatrixOp_MatFree: BwdTrans, then
@@ -162,6 +168,18 @@ int imf_assembly_map_varp(int dim, int nElmt, const int *nm, const long long *ve
                           int *l2g, signed char *sign, unsigned char *on_boundary, long *nglobal);
 int imf_assembly_create_varp(int dim, int nElmt, const int *nm, const long long *vertex_ids,
                              int dirichlet, imf_asm *assembly);
+
+/* Hexahedra with different modes per direction (expansions such as NUMMODES="4,6,8",
+ * Demos/MultiRegions/Tests/Helmholtz3D_CG_Hex_Heterogeneous.tst): element e has
+ * nm3[3e + d] modes along its local direction d and nm3[3e] nm3[3e+1] nm3[3e+2] local
+ * coefficients (p fastest).  An edge along direction d carries nm3[3e + d] - 2 interior
+ * modes, a face the interior modes of its two in-plane directions, each counted in the
+ * face's frame fixed by the vertex ids; where elements disagree the fewest are kept, as in
+ * the variable-order maps above (sign 0 beyond them). */
+int imf_assembly_map_aniso(int nElmt, const int *nm3, const long long *vertex_ids, int *l2g,
+                           signed char *sign, unsigned char *on_boundary, long *nglobal);
+int imf_assembly_create_aniso(int nElmt, const int *nm3, const long long *vertex_ids,
+                              int dirichlet, imf_asm *assembly);
 long imf_assembly_nglobal(imf_asm assembly);
 int imf_assembly_destroy(imf_asm assembly);
 
@@ -217,6 +235,11 @@ int imf_geom_hex_device(const double *d_verts, int nElmt, int nq, const double *
                         int deformed, double *d_jac, double *d_df, void *stream);
 int imf_geom_quad_device(const double *d_verts, int nElmt, int nq, const double *z,
                          int deformed, double *d_jac, double *d_df, void *stream);
+/* The hex factors on nq[d] points z[d] (host) along direction d; points ordered
+ * (k*nq[1] + j)*nq[0] + i.  For the operators with per-direction points. */
+int imf_geom_hex_aniso_device(const double *d_verts, int nElmt, const int *nq,
+                              const double *const *z, int deformed, double *d_jac, double *d_df,
+                              void *stream);
 
 #ifdef __cplusplus
 }

@@ -62,28 +62,65 @@ struct H4 {
 };
 
 // l2g[loff[e] + m] = global index, sgn = +-1, bnd = 1 if the local mode lies on a face owned
-// by one element only.  Element e has nmv[e] modes per direction and its coefficients start
-// at loff[e] = sum_{f<e} nmv[f]^3 (ExpList's coefficient offsets).  Variable order follows
-// AssemblyMapCG.cpp:1380-1440 and :1771-1791: an edge carries the fewest interior modes of
-// the elements that share it, a face the fewest of its two elements, and an element's modes
-// beyond those map to global 0 with sign 0 (they take no part in the global system and are
-// not marked as boundary modes).
-int build_map_hex(int E, const int *nmv, const long long *vid, std::vector<int> &l2g,
+// by one element only.  Element e has nm3[3e + d] modes along its local direction d and its
+// coefficients start at loff[e] = sum_{f<e} nm3[3f] nm3[3f+1] nm3[3f+2] (ExpList's
+// coefficient offsets), p fastest.  Variable order follows AssemblyMapCG.cpp:1380-1440 and
+// :1771-1791: an edge carries the fewest interior modes of the elements that share it, a
+// face the fewest of its two elements along each of its two axes (counted in the face's own
+// frame, fixed by the vertex ids), and an element's modes beyond those map to global 0 with
+// sign 0 (they take no part in the global system and are not marked as boundary modes).
+int build_map_hex(int E, const int *nm3, const long long *vid, std::vector<int> &l2g,
                   std::vector<signed char> &sgn, std::vector<unsigned char> &bnd, long &nglobal)
 {
-    if (E <= 0 || !nmv || !vid) return afail(IMF_ERR_ARG, "assembly: bad argument");
+    if (E <= 0 || !nm3 || !vid) return afail(IMF_ERR_ARG, "assembly: bad argument");
     std::vector<long> loff(E + 1, 0);
     for (long e = 0; e < E; ++e) {
-        if (nmv[e] < 2) return afail(IMF_ERR_ARG, "assembly: fewer than 2 modes on an element");
-        loff[e + 1] = loff[e] + (long)nmv[e] * nmv[e] * nmv[e];
+        for (int d = 0; d < 3; ++d)
+            if (nm3[3 * e + d] < 2) return afail(IMF_ERR_ARG, "assembly: fewer than 2 modes on an element");
+        loff[e + 1] = loff[e] + (long)nm3[3 * e] * nm3[3 * e + 1] * nm3[3 * e + 2];
     }
     std::unordered_map<long long, int> vmap;
     std::unordered_map<std::array<long long, 2>, int, H2> emap;
     std::unordered_map<std::array<long long, 4>, int, H4> fmap;
-    std::vector<int> ecnt, fcnt, fuse;  // per id: interior modes (per direction), users
+    std::vector<int> ecnt, fuse;               // per id: interior modes, users
+    std::vector<std::array<int, 2>> fcnt;      // per face: interior modes along its axes A, B
+    // the face of element e normal to local direction df on side s in the face's own frame:
+    // corner ids g[c1][c2] over the in-plane directions d1 < d2, the origin (o1, o2) at the
+    // smallest id, tr = the first axis runs along d2
+    struct FaceFrame {
+        long long g[2][2];
+        std::array<long long, 4> key;
+        int d1, d2, o1, o2;
+        bool tr;
+    };
+    auto face_frame = [&](const long long *v, int df, int side) {
+        FaceFrame f;
+        f.d1 = df == 0 ? 1 : 0;
+        f.d2 = df == 2 ? 1 : 2;
+        for (int c1 = 0; c1 < 2; ++c1)
+            for (int c2 = 0; c2 < 2; ++c2) {
+                int bits[3];
+                bits[df] = side;
+                bits[f.d1] = c1;
+                bits[f.d2] = c2;
+                f.g[c1][c2] = v[vert_of(bits[0], bits[1], bits[2])];
+                f.key[c1 * 2 + c2] = f.g[c1][c2];
+            }
+        std::sort(f.key.begin(), f.key.end());
+        f.o1 = 0;
+        f.o2 = 0;
+        for (int c1 = 0; c1 < 2; ++c1)
+            for (int c2 = 0; c2 < 2; ++c2)
+                if (f.g[c1][c2] < f.g[f.o1][f.o2]) {
+                    f.o1 = c1;
+                    f.o2 = c2;
+                }
+        f.tr = f.g[f.o1][1 - f.o2] < f.g[1 - f.o1][f.o2];
+        return f;
+    };
     for (long e = 0; e < E; ++e) {
         const long long *v = vid + e * 8;
-        const int ni = nmv[e] - 2;
+        const int *nme = nm3 + 3 * e;
         for (int a = 0; a < 8; ++a) vmap.emplace(v[a], (int)vmap.size());
         for (int d = 0; d < 3; ++d)
             for (int c = 0; c < 4; ++c) {  // the 4 edges along direction d
@@ -97,6 +134,7 @@ int build_map_hex(int E, const int *nmv, const long long *vid, std::vector<int> 
                 long long a0 = v[vert_of(bits[0], bits[1], bits[2])];
                 long long a1 = v[vert_of(bb[0], bb[1], bb[2])];
                 if (a0 == a1) return afail(IMF_ERR_ARG, "assembly: degenerate edge");
+                const int ni = nme[d] - 2;
                 auto ins = emap.emplace(std::array<long long, 2>{std::min(a0, a1), std::max(a0, a1)},
                                         (int)emap.size());
                 if (ins.second) ecnt.push_back(ni);
@@ -104,23 +142,18 @@ int build_map_hex(int E, const int *nmv, const long long *vid, std::vector<int> 
             }
         for (int df = 0; df < 3; ++df)
             for (int s = 0; s < 2; ++s) {
-                std::array<long long, 4> k;
-                int n = 0;
-                for (int c = 0; c < 4; ++c) {
-                    int bits[3];
-                    bits[df] = s;
-                    bits[(df + 1) % 3] = c & 1;
-                    bits[(df + 2) % 3] = c >> 1;
-                    k[n++] = v[vert_of(bits[0], bits[1], bits[2])];
-                }
-                std::sort(k.begin(), k.end());
-                auto ins = fmap.emplace(k, (int)fmap.size());
+                const FaceFrame f = face_frame(v, df, s);
+                const int n1 = nme[f.d1] - 2, n2 = nme[f.d2] - 2;
+                const std::array<int, 2> nab = f.tr ? std::array<int, 2>{n2, n1}
+                                                    : std::array<int, 2>{n1, n2};
+                auto ins = fmap.emplace(f.key, (int)fmap.size());
                 if (ins.second) {
-                    fcnt.push_back(ni);
+                    fcnt.push_back(nab);
                     fuse.push_back(1);
                 } else {
                     const int id = ins.first->second;
-                    fcnt[id] = std::min(fcnt[id], ni);
+                    fcnt[id][0] = std::min(fcnt[id][0], nab[0]);
+                    fcnt[id][1] = std::min(fcnt[id][1], nab[1]);
                     if (++fuse[id] > 2)
                         return afail(IMF_ERR_ARG, "assembly: a face is shared by more than two elements");
                 }
@@ -131,12 +164,10 @@ int build_map_hex(int E, const int *nmv, const long long *vid, std::vector<int> 
     eoff[0] = nV;
     for (long i = 0; i < nE; ++i) eoff[i + 1] = eoff[i] + ecnt[i];
     foff[0] = eoff[nE];
-    for (long i = 0; i < nF; ++i) foff[i + 1] = foff[i] + (long)fcnt[i] * fcnt[i];
+    for (long i = 0; i < nF; ++i) foff[i + 1] = foff[i] + (long)fcnt[i][0] * fcnt[i][1];
     ioff[0] = foff[nF];
-    for (long e = 0; e < E; ++e) {
-        const long ni = nmv[e] - 2;
-        ioff[e + 1] = ioff[e] + ni * ni * ni;
-    }
+    for (long e = 0; e < E; ++e)
+        ioff[e + 1] = ioff[e] + (long)(nm3[3 * e] - 2) * (nm3[3 * e + 1] - 2) * (nm3[3 * e + 2] - 2);
     nglobal = ioff[E];
     if (nglobal > 2147483647L) return afail(IMF_ERR_ARG, "assembly: more than 2^31 global modes");
     l2g.assign((size_t)loff[E], 0);
@@ -144,12 +175,12 @@ int build_map_hex(int E, const int *nmv, const long long *vid, std::vector<int> 
     bnd.assign((size_t)loff[E], 0);
     for (long e = 0; e < E; ++e) {
         const long long *v = vid + e * 8;
-        const int nm = nmv[e], ni = nm - 2;
-        for (int r = 0; r < nm; ++r)
-            for (int q = 0; q < nm; ++q)
-                for (int p = 0; p < nm; ++p) {
+        const int n0 = nm3[3 * e], n1 = nm3[3 * e + 1], n2 = nm3[3 * e + 2];
+        for (int r = 0; r < n2; ++r)
+            for (int q = 0; q < n1; ++q)
+                for (int p = 0; p < n0; ++p) {
                     const int m[3] = {p, q, r};
-                    const size_t li = (size_t)loff[e] + (r * nm + q) * nm + p;
+                    const size_t li = (size_t)loff[e] + ((long)r * n1 + q) * n0 + p;
                     int nv = 0;
                     for (int d = 0; d < 3; ++d) nv += m[d] < 2;
                     long gi = 0;
@@ -174,39 +205,19 @@ int build_map_hex(int E, const int *nmv, const long long *vid, std::vector<int> 
                     } else if (nv == 1) {
                         int df = 0;
                         while (m[df] >= 2) ++df;
-                        const int d1 = df == 0 ? 1 : 0, d2 = df == 2 ? 1 : 2;  // d1 < d2
-                        long long g[2][2];
-                        std::array<long long, 4> k;
-                        for (int c1 = 0; c1 < 2; ++c1)
-                            for (int c2 = 0; c2 < 2; ++c2) {
-                                int bits[3];
-                                bits[df] = m[df];
-                                bits[d1] = c1;
-                                bits[d2] = c2;
-                                g[c1][c2] = v[vert_of(bits[0], bits[1], bits[2])];
-                                k[c1 * 2 + c2] = g[c1][c2];
-                            }
-                        std::sort(k.begin(), k.end());
-                        const int idx = fmap[k], nf = fcnt[idx];
-                        int o1 = 0, o2 = 0;
-                        for (int c1 = 0; c1 < 2; ++c1)
-                            for (int c2 = 0; c2 < 2; ++c2)
-                                if (g[c1][c2] < g[o1][o2]) {
-                                    o1 = c1;
-                                    o2 = c2;
-                                }
-                        const bool tr = g[o1][1 - o2] < g[1 - o1][o2];  // first axis along d2
-                        const int mA = tr ? m[d2] : m[d1], mB = tr ? m[d1] : m[d2];
-                        const bool fA = tr ? o2 == 1 : o1 == 1, fB = tr ? o1 == 1 : o2 == 1;
-                        if (mA - 2 < nf && mB - 2 < nf) {
-                            gi = foff[idx] + (long)(mA - 2) * nf + (mB - 2);
+                        const FaceFrame f = face_frame(v, df, m[df]);
+                        const int idx = fmap[f.key];
+                        const int mA = f.tr ? m[f.d2] : m[f.d1], mB = f.tr ? m[f.d1] : m[f.d2];
+                        const bool fA = f.tr ? f.o2 == 1 : f.o1 == 1, fB = f.tr ? f.o1 == 1 : f.o2 == 1;
+                        if (mA - 2 < fcnt[idx][0] && mB - 2 < fcnt[idx][1]) {
+                            gi = foff[idx] + (long)(mA - 2) * fcnt[idx][1] + (mB - 2);
                             if (fA && (mA & 1)) sg = -sg;
                             if (fB && (mB & 1)) sg = -sg;
                         } else {
                             sg = 0;
                         }
                     } else {
-                        gi = ioff[e] + ((long)(r - 2) * ni + (q - 2)) * ni + (p - 2);
+                        gi = ioff[e] + ((long)(r - 2) * (n1 - 2) + (q - 2)) * (n0 - 2) + (p - 2);
                     }
                     l2g[li] = (int)gi;
                     sgn[li] = (signed char)sg;
@@ -215,21 +226,13 @@ int build_map_hex(int E, const int *nmv, const long long *vid, std::vector<int> 
         // face's normal direction equals the face side
         for (int df = 0; df < 3; ++df)
             for (int s = 0; s < 2; ++s) {
-                std::array<long long, 4> k;
-                for (int c = 0; c < 4; ++c) {
-                    int bits[3];
-                    bits[df] = s;
-                    bits[(df + 1) % 3] = c & 1;
-                    bits[(df + 2) % 3] = c >> 1;
-                    k[c] = v[vert_of(bits[0], bits[1], bits[2])];
-                }
-                std::sort(k.begin(), k.end());
-                if (fuse[fmap[k]] != 1) continue;
-                for (int r = 0; r < nm; ++r)
-                    for (int q = 0; q < nm; ++q)
-                        for (int p = 0; p < nm; ++p) {
+                const FaceFrame f = face_frame(v, df, s);
+                if (fuse[fmap[f.key]] != 1) continue;
+                for (int r = 0; r < n2; ++r)
+                    for (int q = 0; q < n1; ++q)
+                        for (int p = 0; p < n0; ++p) {
                             const int m[3] = {p, q, r};
-                            const size_t li = (size_t)loff[e] + (r * nm + q) * nm + p;
+                            const size_t li = (size_t)loff[e] + ((long)r * n1 + q) * n0 + p;
                             if (m[df] == s && sgn[li] != 0) bnd[li] = 1;
                         }
             }
@@ -342,11 +345,20 @@ int build_map_quad(int E, const int *nmv, const long long *vid, std::vector<int>
     return IMF_OK;
 }
 
-int build_map(int shape, int E, const int *nmv, const long long *vid, std::vector<int> &l2g,
-              std::vector<signed char> &sgn, std::vector<unsigned char> &bnd, long &nglobal)
+// nm: one order per element, or (hexes, per_dir) three per element (directions 0, 1, 2)
+int build_map(int shape, int E, const int *nm, bool per_dir, const long long *vid,
+              std::vector<int> &l2g, std::vector<signed char> &sgn,
+              std::vector<unsigned char> &bnd, long &nglobal)
 {
-    return shape == imf::kHex ? build_map_hex(E, nmv, vid, l2g, sgn, bnd, nglobal)
-                              : build_map_quad(E, nmv, vid, l2g, sgn, bnd, nglobal);
+    if (shape != imf::kHex) {
+        if (per_dir) return afail(IMF_ERR_ARG, "assembly: per-direction orders need hexahedra");
+        return build_map_quad(E, nm, vid, l2g, sgn, bnd, nglobal);
+    }
+    if (per_dir) return build_map_hex(E, nm, vid, l2g, sgn, bnd, nglobal);
+    if (E <= 0 || !nm) return afail(IMF_ERR_ARG, "assembly: bad argument");
+    std::vector<int> nm3(3 * (size_t)E);
+    for (long e = 0; e < E; ++e) nm3[3 * e] = nm3[3 * e + 1] = nm3[3 * e + 2] = nm[e];
+    return build_map_hex(E, nm3.data(), vid, l2g, sgn, bnd, nglobal);
 }
 
 // --------------------------------------------------------------------------- kernels
@@ -600,7 +612,8 @@ constexpr long kAsmParts = 4096;
 struct imf_asm_s {
     int shape = 0, nElmt = 0, nm = 0, dirichlet = 0;  // nm = 0: orders vary by element
     long nloc = 0, ng = 0;
-    std::vector<int> nmv;    // modes per direction of each element
+    std::vector<int> nmv;    // modes per direction of each element (isotropic elements)
+    std::vector<int> nm3;    // hexahedra: modes along directions 0, 1, 2 of each element
     std::vector<long> loff;  // first local coefficient of each element (nElmt + 1)
     int *d_l2g = nullptr;
     signed char *d_sg = nullptr;
@@ -629,7 +642,8 @@ int apply_global_dots(imf_asm a, int nops, const imf_op *ops, const double *w, c
                       double *out, hipStream_t s)
 {
     double *l0 = a->d_loc, *l1 = a->d_loc + a->nloc;
-    if (a->shape == imf::kHex) {  // GlobalToLocal fused into each operator's coefficient load
+    const bool any_aniso = std::any_of(ops, ops + nops, [](const imf_op o) { return o->aniso; });
+    if (a->shape == imf::kHex && !any_aniso) {  // GlobalToLocal fused into the coefficient load
         long e0 = 0;
         for (int g = 0; g < nops; ++g) {
             const long o = a->loff[e0];
@@ -661,13 +675,14 @@ extern "C" {
 const char *imf_assembly_last_error(void) { return g_aerr.c_str(); }
 
 static int map_any(int shape, int nElmt, const int *nm, const long long *vertex_ids, int *l2g,
-                   signed char *sign, unsigned char *on_boundary, long *nglobal)
+                   signed char *sign, unsigned char *on_boundary, long *nglobal,
+                   bool per_dir = false)
 {
     std::vector<int> m;
     std::vector<signed char> sg;
     std::vector<unsigned char> bd;
     long ng = 0;
-    if (int st = build_map(shape, nElmt, nm, vertex_ids, m, sg, bd, ng)) return st;
+    if (int st = build_map(shape, nElmt, nm, per_dir, vertex_ids, m, sg, bd, ng)) return st;
     if (l2g) std::memcpy(l2g, m.data(), m.size() * sizeof(int));
     if (sign) std::memcpy(sign, sg.data(), sg.size());
     if (on_boundary) std::memcpy(on_boundary, bd.data(), bd.size());
@@ -676,25 +691,39 @@ static int map_any(int shape, int nElmt, const int *nm, const long long *vertex_
 }
 
 static int create_any(int shape, int nElmt, const int *nm, const long long *vertex_ids,
-                      int dirichlet, imf_asm *out)
+                      int dirichlet, imf_asm *out, bool per_dir = false)
 {
     if (!out) return afail(IMF_ERR_ARG, "imf_assembly_create: null output");
     std::vector<int> m;
     std::vector<signed char> sg;
     std::vector<unsigned char> bd;
     long ng = 0;
-    if (int st = build_map(shape, nElmt, nm, vertex_ids, m, sg, bd, ng)) return st;
+    if (int st = build_map(shape, nElmt, nm, per_dir, vertex_ids, m, sg, bd, ng)) return st;
     if (m.size() > 2147483647UL)
         return afail(IMF_ERR_ARG, "imf_assembly_create: more than 2^31 local modes");
     auto *a = new imf_asm_s();
     a->shape = shape;
     a->nElmt = nElmt;
-    a->nmv.assign(nm, nm + nElmt);
-    a->nm = std::all_of(nm, nm + nElmt, [&](int v) { return v == nm[0]; }) ? nm[0] : 0;
+    if (per_dir) {
+        a->nm3.assign(nm, nm + 3 * (size_t)nElmt);
+        // an element with equal orders in its three directions also counts as isotropic
+        a->nmv.resize(nElmt);
+        for (int e = 0; e < nElmt; ++e)
+            a->nmv[e] = (nm[3 * e] == nm[3 * e + 1] && nm[3 * e] == nm[3 * e + 2]) ? nm[3 * e] : 0;
+    } else {
+        a->nmv.assign(nm, nm + nElmt);
+        if (shape == imf::kHex) {
+            a->nm3.resize(3 * (size_t)nElmt);
+            for (int e = 0; e < nElmt; ++e) a->nm3[3 * e] = a->nm3[3 * e + 1] = a->nm3[3 * e + 2] = nm[e];
+        }
+    }
+    a->nm = std::all_of(a->nmv.begin(), a->nmv.end(), [&](int v) { return v == a->nmv[0]; })
+                ? a->nmv[0] : 0;
     a->loff.assign(nElmt + 1, 0);
-    const int dim = shape == imf::kHex ? 3 : 2;
     for (int e = 0; e < nElmt; ++e)
-        a->loff[e + 1] = a->loff[e] + (long)(dim == 3 ? nm[e] * nm[e] * nm[e] : nm[e] * nm[e]);
+        a->loff[e + 1] = a->loff[e] + (shape == imf::kHex
+                                           ? (long)a->nm3[3 * e] * a->nm3[3 * e + 1] * a->nm3[3 * e + 2]
+                                           : (long)nm[e] * nm[e]);
     a->dirichlet = dirichlet ? 1 : 0;
     a->nloc = (long)m.size();
     a->ng = ng;
@@ -792,6 +821,18 @@ int imf_assembly_create_varp(int dim, int nElmt, const int *nm, const long long 
 
 long imf_assembly_nglobal(imf_asm a) { return a ? a->ng : 0; }
 
+int imf_assembly_map_aniso(int nElmt, const int *nm3, const long long *vertex_ids, int *l2g,
+                           signed char *sign, unsigned char *on_boundary, long *nglobal)
+{
+    return map_any(imf::kHex, nElmt, nm3, vertex_ids, l2g, sign, on_boundary, nglobal, true);
+}
+
+int imf_assembly_create_aniso(int nElmt, const int *nm3, const long long *vertex_ids,
+                              int dirichlet, imf_asm *out)
+{
+    return create_any(imf::kHex, nElmt, nm3, vertex_ids, dirichlet, out, true);
+}
+
 int imf_assembly_destroy(imf_asm a)
 {
     if (!a) return IMF_OK;
@@ -830,14 +871,22 @@ int imf_helmsolve_cg_groups_device(imf_asm a, int nops, const imf_op *ops, const
     for (int g = 0; g < nops; ++g) {
         const imf_op op = ops[g];
         if (!op) return afail(IMF_ERR_ARG, "imf_helmsolve_cg_device: null operator");
+        bool orders_match = e0 + op->nElmt <= a->nElmt;
+        for (long e = e0; orders_match && e < e0 + op->nElmt; ++e) {
+            if (a->shape == imf::kHex) {
+                for (int d = 0; d < 3; ++d)
+                    orders_match = orders_match &&
+                                   a->nm3[3 * e + d] == (op->aniso ? op->nmd[d] : op->nm);
+            } else {
+                orders_match = a->nmv[e] == op->nm;
+            }
+        }
         if (op->kind != imf::kHelmholtz || op->shape != a->shape || op->prec != IMF_FP64 ||
-            e0 + op->nElmt > a->nElmt ||
-            !std::all_of(a->nmv.begin() + e0, a->nmv.begin() + e0 + op->nElmt,
-                         [&](int v) { return v == op->nm; }))
+            !orders_match)
             return afail(IMF_ERR_ARG, "imf_helmsolve_cg_device: needs FP64 Helmholtz operators "
                                       "of the assembly's shape on its elements and modes");
         if (!op->geom_ready) return afail(IMF_ERR_STATE, "Jacobian and derivative factors not set");
-        if (op->dbdata.size() != (size_t)op->nm * op->nq || op->nq > 16)
+        if (!op->aniso && (op->dbdata.size() != (size_t)op->nm * op->nq || op->nq > 16))
             return afail(IMF_ERR_ARG, "imf_helmsolve_cg_device: operator created without dbdata");
         e0 += op->nElmt;
     }
@@ -873,7 +922,10 @@ int imf_helmsolve_cg_groups_device(imf_asm a, int nops, const imf_op *ops, const
             std::copy(op->w.begin(), op->w.end(), t.w);
             double *dl = a->d_loc + a->loff[e0];
             const long n = a->loff[e0 + op->nElmt] - a->loff[e0];
-            if (a->shape == imf::kHex)
+            if (op->aniso)
+                ACHECK(imf::launch_aniso_diag(aniso_args(op, nullptr, nullptr, 1, 0, 0), dl, s),
+                       "anisotropic diagonal");
+            else if (a->shape == imf::kHex)
                 helm_diag_hex<<<nblk(n), RB, 0, s>>>(t, nm, nq, op->nElmt, op->deformed, op->lambda,
                                                     (const double *)op->d_geom, dl);
             else

@@ -62,6 +62,12 @@ int maybe_pack(imf_op op)
     if (!needs_jac(op) && !needs_df(op)) return IMF_OK;
     if (needs_jac(op) && !op->d_jac) return IMF_OK;
     if (needs_df(op) && !op->d_df) return IMF_OK;
+    if (op->aniso) {  // the per-direction kernels read GetJac / GetDerivFactors directly
+        static std::atomic<unsigned long> next_id{1ul << 62};
+        op->geom_ready = true;
+        op->geom_version = next_id.fetch_add(1);
+        return IMF_OK;
+    }
     if (!op->d_geom) {
         hipError_t e = hipMalloc(&op->d_geom, geom_len(op) * elt_size(op));
         if (e != hipSuccess) return hip_fail(e, "hipMalloc(geometry)");
@@ -130,6 +136,13 @@ int run(imf_op op, imf::LaunchFn fn, const void *in, void *out, void *stream, in
                                    : needs_df(op) ? "Jacobian and derivative factors not set"
                                                   : "Jacobian not set");
     if (!in || !out) return fail(IMF_ERR_ARG, "null input/output array");
+    if (op->aniso) {
+        if (gmap) return fail(IMF_ERR_UNSUPPORTED, "gathered input: isotropic operators only");
+        hipError_t e = imf::launch_aniso(aniso_args(op, in, out, nvec, ldin, ldout),
+                                         (hipStream_t)stream);
+        if (e != hipSuccess) return hip_fail(e, "kernel launch");
+        return IMF_OK;
+    }
     imf::LaunchArgs a{op->tab.data(), op->nElmt, op->lambda, op->d_geom, in, out,
                       (hipStream_t)stream, nvec, ldin, ldout, gmap, gsign};
     hipError_t e = fn(a);
@@ -137,14 +150,90 @@ int run(imf_op op, imf::LaunchFn fn, const void *in, void *out, void *stream, in
     return IMF_OK;
 }
 
+// Operators whose directions differ in modes, points or tables (FP64 hexahedra): the
+// runtime-size kernels of mf_hex_aniso.hip.  Each direction needs 2 <= nm <= nq <= 16.
+int create_aniso(const std::string &k, int kind, int shape, int deformed, const int *nm,
+                 const int *nq, const double *const *bdata, const double *const *dbdata,
+                 const double *const *D, const double *const *w, int nElmt, int precision,
+                 imf_op *out)
+{
+    if (shape != imf::kHex || precision != IMF_FP64 ||
+        (kind != imf::kBwdTrans && kind != imf::kIProduct && kind != imf::kHelmholtz &&
+         kind != imf::kMass))
+        return fail(IMF_ERR_UNSUPPORTED, k + ": different modes / points per direction are "
+                                             "supported for FP64 hexahedral BwdTrans, IProduct, "
+                                             "Helmholtz and Mass operators");
+    for (int d = 0; d < 3; ++d)
+        if (nm[d] < 2 || nq[d] < nm[d] || nq[d] > 16 || !bdata[d] || !D[d] || !w[d])
+            return fail(IMF_ERR_UNSUPPORTED, k + ": # of modes / points combo not initialised.");
+    long buf = 1;
+    for (int d = 0; d < 3; ++d) buf *= std::max(nm[d], nq[d]);
+    const size_t lds = sizeof(double) * (imf::aniso_table_len(nm, nq) + 4 * (size_t)buf);
+    if (lds > 150 * 1024)
+        return fail(IMF_ERR_UNSUPPORTED, k + ": element too large for the per-direction kernels");
+    std::vector<double> tab;
+    for (int d = 0; d < 3; ++d) tab.insert(tab.end(), bdata[d], bdata[d] + (size_t)nm[d] * nq[d]);
+    for (int d = 0; d < 3; ++d) tab.insert(tab.end(), D[d], D[d] + (size_t)nq[d] * nq[d]);
+    for (int d = 0; d < 3; ++d) tab.insert(tab.end(), w[d], w[d] + nq[d]);
+    imf_op op = new imf_op_s();
+    op->kind = kind; op->shape = shape; op->deformed = deformed; op->prec = precision;
+    op->dim = 3; op->nm = nm[0]; op->nq = nq[0]; op->nElmt = nElmt;
+    op->aniso = true;
+    op->M = 1; op->Q = 1;
+    for (int d = 0; d < 3; ++d) {
+        op->nmd[d] = nm[d];
+        op->nqd[d] = nq[d];
+        op->M *= nm[d];
+        op->Q *= nq[d];
+    }
+    op->bufLen = (int)buf;
+    op->w.assign(w[0], w[0] + nq[0]);
+    op->bdata.assign(bdata[0], bdata[0] + (size_t)nm[0] * nq[0]);
+    if (dbdata && dbdata[0]) op->dbdata.assign(dbdata[0], dbdata[0] + (size_t)nm[0] * nq[0]);
+    hipError_t e = hipMalloc(&op->d_atab, tab.size() * sizeof(double));
+    if (e == hipSuccess)
+        e = hipMemcpy(op->d_atab, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        (void)hipFree(op->d_atab);
+        delete op;
+        return hip_fail(e, "anisotropic basis tables");
+    }
+    *out = op;
+    return IMF_OK;
+}
+
 }  // namespace
+
+imf::AnisoArgs aniso_args(const imf_op_s *op, const void *in, void *out, int nvec, long ldin,
+                          long ldout)
+{
+    imf::AnisoArgs a{};
+    a.kind = op->kind;
+    a.geo = op->kind == imf::kBwdTrans ? imf::kUnit : (op->deformed ? imf::kDeformed : imf::kRegular);
+    for (int d = 0; d < 3; ++d) {
+        a.nm[d] = op->nmd[d];
+        a.nq[d] = op->nqd[d];
+    }
+    a.nElmt = op->nElmt;
+    a.bufLen = op->bufLen;
+    a.lambda = op->lambda;
+    a.tab = op->d_atab;
+    a.in = (const double *)in;
+    a.out = (double *)out;
+    a.jac = op->d_jac;
+    a.df = op->d_df;
+    a.nvec = nvec;
+    a.ldin = ldin;
+    a.ldout = ldout;
+    return a;
+}
 
 int imf_apply_gather_device(imf_op_s *op, const double *glob, const int *gmap,
                             const signed char *gsign, double *out, hipStream_t stream)
 {
     if (int s = check_op(op)) return s;
     if (op->kind != imf::kHelmholtz || op->shape != imf::kHex || op->prec != IMF_FP64 ||
-        !gmap || !gsign)
+        op->aniso || !gmap || !gsign)
         return fail(IMF_ERR_UNSUPPORTED, "gathered input: FP64 Helmholtz_Hex only");
     return run(op, op->fwd, glob, out, stream, 1, 0, 0, gmap, gsign);
 }
@@ -189,15 +278,20 @@ int imf_create(const char *key, int dim, const int *nm, const int *nq,
     if (precision != IMF_FP64 && precision != IMF_FP32)
         return fail(IMF_ERR_ARG, "imf_create: unknown precision");
     if (nElmt <= 0) return fail(IMF_ERR_ARG, "imf_create: nElmt must be positive");
+    bool same = true;
     for (int d = 1; d < dim; ++d) {
-        if (nm[d] != nm[0] || nq[d] != nq[0])
-            return fail(IMF_ERR_UNSUPPORTED, "MatrixFree requires homogenous modes/points");
+        if (nm[d] != nm[0] || nq[d] != nq[0]) {
+            same = false;
+            break;
+        }
         const size_t nb = (size_t)nm[0] * nq[0];
         if (std::memcmp(bdata[d], bdata[0], nb * sizeof(double)) ||
             std::memcmp(D[d], D[0], (size_t)nq[0] * nq[0] * sizeof(double)) ||
             std::memcmp(w[d], w[0], (size_t)nq[0] * sizeof(double)))
-            return fail(IMF_ERR_UNSUPPORTED, "MatrixFree requires identical bases in all directions");
+            same = false;
     }
+    if (!same) return create_aniso(k, kind, shape, deformed, nm, nq, bdata, dbdata, D, w,
+                                   nElmt, precision, out);
     const int m = nm[0], q = nq[0];
     const char *oname = kind == imf::kHelmholtz ? "Helmholtz" : kind == imf::kIProduct ? "IProduct"
                       : kind == imf::kMass ? "Mass" : kind == imf::kPhysDeriv ? "PhysDeriv"
@@ -247,6 +341,7 @@ int imf_destroy(imf_op op)
     (void)hipFree(op->d_geom);
     (void)hipFree(op->d_in);
     (void)hipFree(op->d_out);
+    (void)hipFree(op->d_atab);
     delete op;
     return IMF_OK;
 }
@@ -320,6 +415,8 @@ int imf_apply_device(imf_op op, const void *d_in, void *d_out, void *stream)
 int imf_apply_transpose_device(imf_op op, const void *d_in, void *d_out, void *stream)
 {
     if (int s = check_op(op)) return s;
+    if (op->aniso && (op->kind == imf::kHelmholtz || op->kind == imf::kMass))
+        return run(op, op->fwd, d_in, d_out, stream);
     if (!op->tr) return fail(IMF_ERR_UNSUPPORTED, "transpose not available");
     if (op->kind == imf::kHelmholtz || op->kind == imf::kMass)
         return run(op, op->fwd, d_in, d_out, stream);
@@ -335,7 +432,8 @@ int imf_apply_many_device(imf_op op, int nvec, const void *d_in, long ld_in, voi
     if (ld_in < (long)imf_in_size(op) || ld_out < (long)imf_out_size(op))
         return fail(IMF_ERR_ARG, "imf_apply_many_device: leading dimension smaller than the "
                                  "vector size");
-    if (op->kind == imf::kMass || (op->kind == imf::kHelmholtz && op->shape == imf::kHex))
+    if (op->aniso || op->kind == imf::kMass ||
+        (op->kind == imf::kHelmholtz && op->shape == imf::kHex))
         return run(op, op->fwd, d_in, d_out, stream, nvec, ld_in, ld_out);
     const size_t es = elt_size(op);
     for (int v = 0; v < nvec; ++v)
@@ -364,6 +462,20 @@ int imf_apply(imf_op op, const double *in, double *out)
     if ((e = hipMemcpy(out, op->d_out, nout * sizeof(double), hipMemcpyDeviceToHost)) != hipSuccess)
         return hip_fail(e, "hipMemcpy(out)");
     return IMF_OK;
+}
+
+int imf_geom_hex_aniso_device(const double *d_verts, int nElmt, const int *nq,
+                              const double *const *z, int deformed, double *d_jac, double *d_df,
+                              void *stream)
+{
+    if (!d_verts || !nq || !z || !d_jac || !d_df || nElmt <= 0)
+        return fail(IMF_ERR_ARG, "imf_geom_hex_aniso_device: bad argument");
+    for (int d = 0; d < 3; ++d)
+        if (nq[d] < 2 || nq[d] > 16 || !z[d])
+            return fail(IMF_ERR_ARG, "imf_geom_hex_aniso_device: bad points");
+    hipError_t e = imf::geom_hex_aniso(d_verts, nElmt, nq, z, deformed, d_jac, d_df,
+                                       (hipStream_t)stream);
+    return e == hipSuccess ? IMF_OK : hip_fail(e, "geom_hex_aniso");
 }
 
 int imf_geom_hex_device(const double *d_verts, int nElmt, int nq, const double *z, int deformed,

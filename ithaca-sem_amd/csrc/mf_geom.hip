@@ -248,17 +248,20 @@ __device__ void factors_from_deriv(int dim, const double *deriv, double *jac, do
         }
 }
 
-__global__ void geom_hex_kernel(const double *__restrict__ verts, long nElmt, int nq, Pts z,
-                                int deformed, double *__restrict__ jac, double *__restrict__ df)
+// Points (z0, z1, z2) per direction, nq0 x nq1 x nq2 per element (all equal for the
+// isotropic operators).
+__global__ void geom_hex_kernel(const double *__restrict__ verts, long nElmt, int nq0, int nq1,
+                                int nq2, Pts z0, Pts z1, Pts z2, int deformed,
+                                double *__restrict__ jac, double *__restrict__ df)
 {
-    const int Q = deformed ? nq * nq * nq : 1;
+    const int Q = deformed ? nq0 * nq1 * nq2 : 1;
     const long n = nElmt * Q;
     const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= n) return;
     const long e = g / Q;
     const int q = (int)(g - e * Q);
-    const int i = q % nq, j = (q / nq) % nq, k = q / (nq * nq);
-    const double xi[3] = {z.v[i], z.v[j], z.v[k]};
+    const int i = q % nq0, j = (q / nq0) % nq1, k = q / (nq0 * nq1);
+    const double xi[3] = {z0.v[i], z1.v[j], z2.v[k]};
     const double *v = verts + e * 24;
     double deriv[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     for (int nv = 0; nv < 8; ++nv) {
@@ -318,9 +321,26 @@ hipError_t geom_from_vertices(int shape, const double *d_verts, long nElmt, int 
     const int bs = 256;
     const dim3 g((unsigned)((n + bs - 1) / bs));
     if (shape == kHex)
-        geom_hex_kernel<<<g, bs, 0, s>>>(d_verts, nElmt, nq, pz, deformed, d_jac, d_df);
+        geom_hex_kernel<<<g, bs, 0, s>>>(d_verts, nElmt, nq, nq, nq, pz, pz, pz, deformed, d_jac,
+                                         d_df);
     else
         geom_quad_kernel<<<g, bs, 0, s>>>(d_verts, nElmt, nq, pz, deformed, d_jac, d_df);
+    return hipGetLastError();
+}
+
+hipError_t geom_hex_aniso(const double *d_verts, long nElmt, const int *nq,
+                          const double *const *z, int deformed, double *d_jac, double *d_df,
+                          hipStream_t s)
+{
+    Pts pz[3] = {};
+    for (int d = 0; d < 3; ++d)
+        for (int i = 0; i < nq[d] && i < 16; ++i) pz[d].v[i] = z[d][i];
+    const long Q = deformed ? (long)nq[0] * nq[1] * nq[2] : 1;
+    const long n = nElmt * Q;
+    const int bs = 256;
+    const dim3 g((unsigned)((n + bs - 1) / bs));
+    geom_hex_kernel<<<g, bs, 0, s>>>(d_verts, nElmt, nq[0], nq[1], nq[2], pz[0], pz[1], pz[2],
+                                     deformed, d_jac, d_df);
     return hipGetLastError();
 }
 

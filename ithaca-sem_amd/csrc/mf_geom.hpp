@@ -16,4 +16,10 @@ hipError_t geom_from_vertices(int shape, const double *d_verts, long nElmt, int 
                               const double *z, int deformed, double *d_jac, double *d_df,
                               hipStream_t s);
 
+// The same for hexes with nq[d] points z[d] along direction d (point (i, j, k) at index
+// (k*nq1 + j)*nq0 + i).
+hipError_t geom_hex_aniso(const double *d_verts, long nElmt, const int *nq,
+                          const double *const *z, int deformed, double *d_jac, double *d_df,
+                          hipStream_t s);
+
 }  // namespace imf

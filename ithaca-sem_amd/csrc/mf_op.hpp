@@ -5,6 +5,7 @@
 #include <vector>
 
 #include "imf_common.hpp"
+#include "mf_aniso.hpp"
 
 struct imf_op_s {
     int kind = 0, shape = 0, deformed = 0, prec = 0, dim = 0, nm = 0, nq = 0;
@@ -21,7 +22,18 @@ struct imf_op_s {
     unsigned long geom_version = 0;        // process-wide unique id of the current geometry pack
     void *d_in = nullptr, *d_out = nullptr;  // staging for host applies
     imf::LaunchFn fwd = nullptr, tr = nullptr;
+    // hexahedra with different modes / points per direction (mf_hex_aniso.hip, FP64):
+    // per-direction sizes, the device table image (B, D, w of the three directions) and
+    // the LDS work-buffer length; the geometry is read unpacked from d_jac / d_df
+    bool aniso = false;
+    int nmd[3] = {0, 0, 0}, nqd[3] = {0, 0, 0};
+    double *d_atab = nullptr;
+    int bufLen = 0;
 };
+
+// Kernel arguments of an anisotropic operator's apply (in, out, nvec vectors at ld strides).
+imf::AnisoArgs aniso_args(const imf_op_s *op, const void *in, void *out, int nvec, long ldin,
+                          long ldout);
 
 // FP64 hex Helmholtz with its input gathered from a global vector: out = H(local), where
 // local[i] = gsign[i] * glob[gmap[i]] (GlobalToLocal fused into the kernel's load).

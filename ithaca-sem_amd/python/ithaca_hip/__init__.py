@@ -57,6 +57,7 @@ def lib():
         L.imf_basis_modified_a.argtypes = [i, i, vp, vp, vp, vp, vp]
         L.imf_geom_hex_device.argtypes = [vp, i, i, vp, i, vp, vp, vp]
         L.imf_geom_quad_device.argtypes = [vp, i, i, vp, i, vp, vp, vp]
+        L.imf_geom_hex_aniso_device.argtypes = [vp, i, vp, vp, i, vp, vp, vp]
         _LIB = L
     return _LIB
 
@@ -94,6 +95,20 @@ def geom_device(shape, verts, nq, z, deformed=True, stream=None):
     verts = verts.contiguous()
     E = verts.shape[0]
     dim = 3 if shape == "hex" else 2
+    if isinstance(nq, (tuple, list)):  # hexes with nq[d] points z[d] along direction d
+        if dim != 3 or len(nq) != 3 or len(z) != 3:
+            raise MFError("geom_device: per-direction points need hexes, 3 counts, 3 arrays")
+        npts = E * (int(np.prod(nq)) if deformed else 1)
+        jac = torch.empty(npts, dtype=torch.float64, device=verts.device)
+        df = torch.empty(9 * npts, dtype=torch.float64, device=verts.device)
+        zs = [np.ascontiguousarray(x, dtype=np.float64) for x in z]
+        nq_a = (ctypes.c_int * 3)(*[int(n) for n in nq])
+        z_a = (_dp * 3)(*[x.ctypes.data_as(_dp) for x in zs])
+        _check(lib().imf_geom_hex_aniso_device(ctypes.c_void_p(verts.data_ptr()), E, nq_a, z_a,
+                                               int(deformed), ctypes.c_void_p(jac.data_ptr()),
+                                               ctypes.c_void_p(df.data_ptr()),
+                                               _stream_ptr(stream)))
+        return jac, df
     npts = E * (nq ** dim if deformed else 1)
     jac = torch.empty(npts, dtype=torch.float64, device=verts.device)
     df = torch.empty(dim * dim * npts, dtype=torch.float64, device=verts.device)
@@ -110,17 +125,29 @@ class Operator:
     nElmt)`` of the reference, implemented on the GPU."""
 
     def __init__(self, key, nm, nq, nElmt, precision=FP64, basis=None):
+        """nm, nq: the modes and points of every direction, or (hexes) tuples of the three
+        directions' counts with basis a list of the three directions' tables."""
         self.key = key
         shape = key.split("_")[1]
         self.dim = 3 if shape == "Hex" else 2
         self.nm, self.nq, self.nElmt, self.precision = nm, nq, nElmt, precision
-        self.basis = basis if basis is not None else basis_modified_a(nm, nq)
-        b = self.basis
-        self._keep = [np.ascontiguousarray(b[k], dtype=np.float64)
+        per_dir = isinstance(nm, (tuple, list))
+        if per_dir:
+            nms, nqs = [int(x) for x in nm], [int(x) for x in nq]
+            if basis is None:
+                basis = [basis_modified_a(a, b) for a, b in zip(nms, nqs)]
+            bases = list(basis)
+        else:
+            nms, nqs = [nm] * self.dim, [nq] * self.dim
+            if basis is None:
+                basis = basis_modified_a(nm, nq)
+            bases = [basis] * self.dim
+        self.basis = basis
+        self._keep = [[np.ascontiguousarray(b[k], dtype=np.float64) for b in bases]
                       for k in ("bdata", "dbdata", "D", "w")]
-        arr = lambda a: (_dp * self.dim)(*([a.ctypes.data_as(_dp)] * self.dim))
-        nm_a = (ctypes.c_int * self.dim)(*([nm] * self.dim))
-        nq_a = (ctypes.c_int * self.dim)(*([nq] * self.dim))
+        arr = lambda col: (_dp * self.dim)(*[a.ctypes.data_as(_dp) for a in col])
+        nm_a = (ctypes.c_int * self.dim)(*nms)
+        nq_a = (ctypes.c_int * self.dim)(*nqs)
         h = ctypes.c_void_p()
         _check(lib().imf_create(key.encode(), self.dim, nm_a, nq_a, arr(self._keep[0]),
                                 arr(self._keep[1]), arr(self._keep[2]), arr(self._keep[3]),

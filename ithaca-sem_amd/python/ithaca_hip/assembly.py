@@ -27,6 +27,8 @@ def _lib():
                                                                     ctypes.POINTER(vp)]
         L.imf_assembly_map_varp.argtypes = [i, i, vp, vp, vp, vp, vp, ctypes.POINTER(l_)]
         L.imf_assembly_create_varp.argtypes = [i, i, vp, vp, i, ctypes.POINTER(vp)]
+        L.imf_assembly_map_aniso.argtypes = [i, vp, vp, vp, vp, vp, ctypes.POINTER(l_)]
+        L.imf_assembly_create_aniso.argtypes = [i, vp, vp, i, ctypes.POINTER(vp)]
         L.imf_helmsolve_cg_groups_device.argtypes = [vp, i, vp, vp, vp, d, i, ctypes.POINTER(i),
                                                      ctypes.POINTER(d), vp]
         L.imf_assembly_nglobal.argtypes = [vp]
@@ -63,6 +65,22 @@ def _orders(nm, E):
     return nmv
 
 
+def _orders3(nm, E):
+    """Per-direction orders of hexahedra as an (E, 3) int32 array, from a tuple of three
+    (every element; a list or array of length E gives per-element orders instead) or an
+    (E, 3) array; None when nm is not per direction."""
+    if isinstance(nm, tuple):
+        if len(nm) != 3:
+            raise MFError("assembly: per-direction orders are a tuple (nm0, nm1, nm2)")
+        return np.ascontiguousarray(np.tile(np.asarray(nm, dtype=np.int32), (E, 1)))
+    a = np.asarray(nm)
+    if a.ndim == 2:
+        if a.shape != (E, 3):
+            raise MFError(f"assembly: per-direction orders must be (nElmt, 3), got {a.shape}")
+        return np.ascontiguousarray(a, dtype=np.int32)
+    return None
+
+
 def assembly_map(vertex_ids, nm, shape):
     """(l2g, sign, on_boundary, nglobal) of the C0 numbering (host only, no device).  nm is
     the order of every element, or an array of per-element orders (variable order: modes
@@ -72,6 +90,19 @@ def assembly_map(vertex_ids, nm, shape):
         raise MFError(f"assembly: vertex ids must be (nElmt, {_nverts(shape)}) for {shape}")
     E = ids.shape[0]
     dim = 3 if shape == "hex" else 2
+    nm3 = _orders3(nm, E) if shape == "hex" else None
+    if nm3 is not None:
+        n = int(np.sum(np.prod(nm3.astype(np.int64), axis=1)))
+        l2g = np.empty(n, dtype=np.int32)
+        sign = np.empty(n, dtype=np.int8)
+        bnd = np.empty(n, dtype=np.uint8)
+        ng = ctypes.c_long()
+        _check(_lib().imf_assembly_map_aniso(E, nm3.ctypes.data_as(ctypes.c_void_p),
+                                             ids.ctypes.data_as(ctypes.c_void_p),
+                                             l2g.ctypes.data_as(ctypes.c_void_p),
+                                             sign.ctypes.data_as(ctypes.c_void_p),
+                                             bnd.ctypes.data_as(ctypes.c_void_p), ctypes.byref(ng)))
+        return l2g, sign, bnd, ng.value
     nmv = _orders(nm, E)
     n = int(np.sum(nmv.astype(np.int64) ** dim)) if nmv is not None else E * nm ** dim
     l2g = np.empty(n, dtype=np.int32)
@@ -118,47 +149,55 @@ def _constrained_1d(vals, B, w):
     return c
 
 
-def _constrained_2d(F, B, w):
+def _constrained_2d(F, B, w, Bs=None, ws=None):
     """StdQuadExp::v_FwdTrans_BndConstrained (StdRegions/StdQuadExp.cpp:260-354) on a face
-    grid F[j, i] (i along the first tangential direction): vertices, then each edge by the
-    1D constrained projection, then the interior by the interior mass block.  Returns
+    grid F[j, i] (i along the first tangential direction, tables B, w; j along the second,
+    tables Bs, ws, the same as the first by default): vertices, then each edge by the 1D
+    constrained projection, then the interior by the interior mass block.  Returns
     c[q, p].  The face Jacobian of a straight-sided (parallelogram) face is constant and
     cancels."""
-    nm = B.shape[1]
-    c = np.zeros((nm, nm))
+    Bs = B if Bs is None else Bs
+    ws = w if ws is None else ws
+    nmi, nmj = B.shape[1], Bs.shape[1]
+    c = np.zeros((nmj, nmi))
     c[0, :] = _constrained_1d(F[0, :], B, w)
     c[1, :] = _constrained_1d(F[-1, :], B, w)
-    c[:, 0] = _constrained_1d(F[:, 0], B, w)
-    c[:, 1] = _constrained_1d(F[:, -1], B, w)
-    if nm > 2:
-        B2 = np.kron(B, B)
-        W2 = np.kron(w, w)
+    c[:, 0] = _constrained_1d(F[:, 0], Bs, ws)
+    c[:, 1] = _constrained_1d(F[:, -1], Bs, ws)
+    if nmi > 2 and nmj > 2:
+        B2 = np.kron(Bs, B)
+        W2 = np.kron(ws, w)
         M2 = B2.T @ (W2[:, None] * B2)
-        it = np.zeros((nm, nm), dtype=bool)
+        it = np.zeros((nmj, nmi), dtype=bool)
         it[2:, 2:] = True
         it = it.ravel()
         cb = c.ravel().copy()
         cb[it] = 0.0
         rhs = B2.T @ (W2 * F.ravel()) - M2 @ cb
         cb[it] = np.linalg.solve(M2[np.ix_(it, it)], rhs[it])
-        c = cb.reshape(nm, nm)
+        c = cb.reshape(nmj, nmi)
     return c
 
 
 def _points(coords, z):
     """Physical GLL points of straight-sided elements: (E, nq, nq[, nq], dim), index
-    [k][j][i] (hex) or [j][i] (quad), from the vertices in local order."""
+    [k][j][i] (hex) or [j][i] (quad), from the vertices in local order.  z: the points of
+    every direction, or (hexes) a list of the three directions' points."""
     coords = np.asarray(coords, dtype=np.float64)
     nv, dim = coords.shape[1], coords.shape[2]
-    lo, hi = 0.5 * (1.0 - z), 0.5 * (1.0 + z)
     if nv == 8:
+        zs = z if isinstance(z, (list, tuple)) else [z, z, z]
+        lo = [0.5 * (1.0 - np.asarray(x)) for x in zs]
+        hi = [0.5 * (1.0 + np.asarray(x)) for x in zs]
         corners = [(0, 0, 0), (1, 0, 0), (1, 1, 0), (0, 1, 0),
                    (0, 0, 1), (1, 0, 1), (1, 1, 1), (0, 1, 1)]
         P = 0.0
         for v, (a, b, c) in enumerate(corners):
-            f = np.einsum("k,j,i->kji", hi if c else lo, hi if b else lo, hi if a else lo)
+            f = np.einsum("k,j,i->kji", hi[2] if c else lo[2], hi[1] if b else lo[1],
+                          hi[0] if a else lo[0])
             P = P + f[None, ..., None] * coords[:, v, None, None, None, :]
         return P
+    lo, hi = 0.5 * (1.0 - z), 0.5 * (1.0 + z)
     corners = [(0, 0), (1, 0), (1, 1), (0, 1)]
     P = 0.0
     for v, (a, b) in enumerate(corners):
@@ -174,6 +213,8 @@ def dirichlet_lift(coords, vertex_ids, nm, fn, nq=None):
     conditions (DisContField.cpp:3630-3663: FwdTrans_BndConstrained of the boundary
     expansion), on the element's own GLL points (nq = nm + 1 by default, the reference's
     TYPE="MODIFIED").  Host side (numpy); pass the result as `lift` to helmsolve."""
+    if isinstance(nm, tuple):
+        return _dirichlet_lift_aniso(coords, vertex_ids, tuple(int(m) for m in nm), fn, nq)
     if np.ndim(nm) != 0:
         raise MFError("dirichlet_lift: one order for every element (variable order is not "
                       "supported here)")
@@ -205,11 +246,44 @@ def dirichlet_lift(coords, vertex_ids, nm, fn, nq=None):
     return g
 
 
+def _dirichlet_lift_aniso(coords, vertex_ids, nm3, fn, nq3=None):
+    """dirichlet_lift for hexahedra with nm3[d] modes on nq3[d] GLL points (nm3[d] + 1 by
+    default) along direction d, the same for every element: each boundary face is
+    projected with the tables of its two in-plane directions."""
+    coords = np.asarray(coords, dtype=np.float64)
+    nq3 = tuple(m + 1 for m in nm3) if nq3 is None else tuple(nq3)
+    tabs = [_tables(nm3[d], nq3[d]) for d in range(3)]
+    l2g, sign, bnd, ng = assembly_map(vertex_ids, nm3, "hex")
+    g = np.zeros(ng)
+    P = _points(coords, [t[0] for t in tabs])
+    E = coords.shape[0]
+    M = nm3[0] * nm3[1] * nm3[2]
+    loc = np.arange(M).reshape(nm3[2], nm3[1], nm3[0])  # [r][q][p]
+    for e in range(E):
+        U = fn(P[e].reshape(-1, 3)).reshape(nq3[2], nq3[1], nq3[0])
+        for d in range(3):                # normal direction of the face
+            fast, slow = [x for x in range(3) if x != d]
+            for side in range(2):
+                sl = [slice(None)] * 3
+                sl[2 - d] = 0 if side == 0 else nq3[d] - 1
+                ml = [slice(None)] * 3
+                ml[2 - d] = side
+                li = e * M + loc[tuple(ml)].ravel()
+                if not np.all(bnd[li]):
+                    continue
+                c = _constrained_2d(U[tuple(sl)], tabs[fast][2], tabs[fast][1],
+                                    tabs[slow][2], tabs[slow][1])
+                g[l2g[li]] = sign[li] * c.ravel()
+    return g
+
+
 class Assembly:
     """Device assembly map of a hex mesh (vertex_ids (E, 8)) or a quad mesh (shape="quad",
     vertex_ids (E, 4)); dirichlet=True fixes every mode on the boundary faces (edges) to
-    zero.  nm: the order of every element, or per-element orders (variable order; local
-    arrays then hold element e's nm[e]^dim coefficients after those of elements < e)."""
+    zero.  nm: the order of every element, per-element orders (variable order; local
+    arrays then hold element e's nm[e]^dim coefficients after those of elements < e), or
+    for hexahedra orders per direction: a tuple (nm0, nm1, nm2) for every element or an
+    (nElmt, 3) array (element e then holds nm0 nm1 nm2 coefficients)."""
 
     def __init__(self, vertex_ids, nm, dirichlet=True, shape="hex"):
         ids = np.ascontiguousarray(vertex_ids, dtype=np.int64)
@@ -217,9 +291,15 @@ class Assembly:
             raise MFError(f"assembly: vertex ids must be (nElmt, {_nverts(shape)}) for {shape}")
         self.shape, self.nElmt, self.nm = shape, ids.shape[0], nm
         dim = 3 if shape == "hex" else 2
-        nmv = _orders(nm, self.nElmt)
+        nm3 = _orders3(nm, self.nElmt) if shape == "hex" else None
+        nmv = _orders(nm, self.nElmt) if nm3 is None else None
         h = ctypes.c_void_p()
-        if nmv is None:
+        if nm3 is not None:
+            _check(_lib().imf_assembly_create_aniso(
+                self.nElmt, nm3.ctypes.data_as(ctypes.c_void_p),
+                ids.ctypes.data_as(ctypes.c_void_p), int(dirichlet), ctypes.byref(h)))
+            self.nlocal = int(np.sum(np.prod(nm3.astype(np.int64), axis=1)))
+        elif nmv is None:
             _check(getattr(_lib(), f"imf_assembly_create_{shape}")(
                 self.nElmt, nm, ids.ctypes.data_as(ctypes.c_void_p), int(dirichlet),
                 ctypes.byref(h)))

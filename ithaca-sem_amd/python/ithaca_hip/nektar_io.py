@@ -108,7 +108,9 @@ class Mesh:
 
     elements: int64 ids in file order; element_vertex_ids[e]: the element's vertex ids in
     the reference's local order (4 or 8); coords[v]: vertex coordinates by id (dim values);
-    expansions: list of (element ids, nummodes, type, fields)."""
+    expansions: list of dicts (composite, nummodes: an int, or a tuple of per-direction
+    counts for NUMMODES="4,6,8", numpoints likewise or None, basistype / pointstype lists or
+    None, type, fields)."""
 
     def __init__(self, dim, vertices, edges, faces, elements, composites, expansions):
         self.dim = dim
@@ -141,6 +143,9 @@ class Mesh:
             for c in _composite_ids(e["composite"]):
                 if int(c) not in self.composites:
                     raise NektarIOError(f"expansion names composite {int(c)}, not in the mesh")
+                if isinstance(e["nummodes"], tuple):
+                    raise NektarIOError("element_orders: the expansion has per-direction "
+                                        "NUMMODES; read them from mesh.expansions")
                 for el in _composite_ids(self.composites[int(c)]):
                     order[int(el)] = e["nummodes"]
         try:
@@ -262,8 +267,15 @@ def read_mesh(path):
     expansions = []
     ex = _section(text, "EXPANSIONS")
     if ex is not None:
+        def counts(v):
+            # "6" -> 6; "4,6,8" (one count per direction) -> (4, 6, 8)
+            vals = [int(x) for x in (v or "0").split(",") if x.strip()]
+            return vals[0] if len(vals) == 1 else tuple(vals)
         for e in ex:
-            expansions.append(dict(composite=e.get("COMPOSITE"), nummodes=int(e.get("NUMMODES", 0)),
+            expansions.append(dict(composite=e.get("COMPOSITE"), nummodes=counts(e.get("NUMMODES")),
+                                   numpoints=counts(e.get("NUMPOINTS")) if e.get("NUMPOINTS") else None,
+                                   basistype=(e.get("BASISTYPE") or "").split(",") if e.get("BASISTYPE") else None,
+                                   pointstype=(e.get("POINTSTYPE") or "").split(",") if e.get("POINTSTYPE") else None,
                                    type=e.get("TYPE"), fields=(e.get("FIELDS") or "").split(",")))
     return Mesh(dim, vertices, edges, faces, elements, composites, expansions)
 

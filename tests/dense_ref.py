@@ -62,7 +62,25 @@ def quad_map(verts, xi):
 
 def element_mats(shape, nm, nq, verts):
     """Dense per-element matrices: B (Q x M), grads G_a (Q x M) in reference space,
-    quadrature weights W (Q), |det A| (Q), A^{-1} (Q x dim x dim), coordinates."""
+    quadrature weights W (Q), |det A| (Q), A^{-1} (Q x dim x dim), coordinates.  For hexes
+    nm and nq may be tuples of the three directions' modes and points (coefficient index
+    (r nm1 + q) nm0 + p, point index (k nq1 + j) nq0 + i)."""
+    if shape == "hex" and isinstance(nm, tuple):
+        zz = [gll(n) for n in nq]
+        Bd = [modified_a(m, z_[0]) for m, z_ in zip(nm, zz)]
+        (B0, dB0), (B1, dB1), (B2, dB2) = Bd
+        B = np.kron(np.kron(B2, B1), B0)
+        G = [np.kron(np.kron(B2, B1), dB0), np.kron(np.kron(B2, dB1), B0),
+             np.kron(np.kron(dB2, B1), B0)]
+        W = np.kron(np.kron(zz[2][1], zz[1][1]), zz[0][1])
+        pts = [(zz[0][0][i], zz[1][0][j], zz[2][0][k]) for k in range(nq[2])
+               for j in range(nq[1]) for i in range(nq[0])]
+        dets, Ainv, X = [], [], []
+        for p in pts:
+            x, A = hex_map(verts, np.array(p))
+            dets.append(abs(np.linalg.det(A))); Ainv.append(np.linalg.inv(A)); X.append(x)
+        return dict(B=B, G=G, W=W, J=np.array(dets), Ainv=np.array(Ainv), X=np.array(X),
+                    z=[z_[0] for z_ in zz], w=[z_[1] for z_ in zz])
     z, w = gll(nq)
     B1, dB1 = modified_a(nm, z)
     if shape == "hex":

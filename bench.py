@@ -80,25 +80,41 @@ def cpu_baseline(w, seconds):
         O.helmholtz("hex", b, E, True, jac, df, w["lam"], u)
         times.append(time.perf_counter() - t0)
     t = float(np.median(times))
+    # the AVX2 SIMD ABI (four elements per vector), same sample and budget
+    O.helmholtz("hex", b, E, True, jac, df, w["lam"], u, simd="avx2")
+    tv = []
+    t_end = time.perf_counter() + seconds
+    while time.perf_counter() < t_end or len(tv) < 3:
+        t0 = time.perf_counter()
+        O.helmholtz("hex", b, E, True, jac, df, w["lam"], u, simd="avx2")
+        tv.append(time.perf_counter() - t0)
     # one core on a smaller contiguous sample (the reference runs one MPI rank per core)
     O.set_threads(1)
     E1 = 1024
     u1 = np.ascontiguousarray(u[:E1 * nm ** 3])
     jac1 = np.ascontiguousarray(jac[:E1 * nq ** 3])
     df1 = np.ascontiguousarray(df.reshape(9, -1)[:, :E1 * nq ** 3])
-    O.helmholtz("hex", b, E1, True, jac1, df1, w["lam"], u1)
-    t1s = []
-    for _ in range(5):
-        t0 = time.perf_counter()
-        O.helmholtz("hex", b, E1, True, jac1, df1, w["lam"], u1)
-        t1s.append(time.perf_counter() - t0)
+    one = {}
+    for simd in ("scalar", "avx2"):
+        O.helmholtz("hex", b, E1, True, jac1, df1, w["lam"], u1, simd=simd)
+        t1s = []
+        for _ in range(5):
+            t0 = time.perf_counter()
+            O.helmholtz("hex", b, E1, True, jac1, df1, w["lam"], u1, simd=simd)
+            t1s.append(time.perf_counter() - t0)
+        one[simd] = E1 * nm ** 3 / float(np.median(t1s)) / 1e9
     O.set_threads(threads)
     return {"value": E * nm ** 3 / t / 1e9, "unit": "GDOF/s", "cores": threads,
             "kind": "port",
             "sample": f"{E} of {w['nElmt']} elements, {len(times)} applies, median; "
                       "oracle restatement of HelmholtzHexImpl (scalar element loop, "
                       f"width 1), OpenMP over contiguous element chunks, {threads} threads",
-            "single_core_value": E1 * nm ** 3 / float(np.median(t1s)) / 1e9,
+            "avx2_value": E * nm ** 3 / float(np.median(tv)) / 1e9,
+            "avx2_sample": f"same elements, {len(tv)} applies, median; the same restatement "
+                           "with four elements interleaved per 256-bit vector (the "
+                           "reference's AVX2 SIMD ABI)",
+            "single_core_value": one["scalar"],
+            "single_core_avx2_value": one["avx2"],
             "single_core_sample": f"{E1} elements, 5 applies, median",
             "cpu_model": _cpu_model(), "host_cpus": os.cpu_count()}
 
@@ -951,6 +967,7 @@ def main():
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(w, args.cpu_seconds)
         out["gpu_over_cpu"] = out["value"] / out["cpu_baseline"]["value"]
+        out["gpu_over_cpu_avx2"] = out["value"] / out["cpu_baseline"]["avx2_value"]
     if args.op == "helmholtz" and args.precision == "fp64" and args.n == 48 and not args.headline_only:
         # the host-buffer entry (imf_apply: pageable host arrays copied in and out around
         # the same kernel, geometry resident); reported beside value, never as it

@@ -141,12 +141,18 @@ def iprod_deriv(shape, b, E, deformed, jac, df, ins):
     return out
 
 
-def helmholtz(shape, b, E, deformed, jac, df, lam, u):
+def helmholtz(shape, b, E, deformed, jac, df, lam, u, simd="scalar"):
+    """simd: "scalar" (width 1, the reference's default build) or "avx2" (hexes: four
+    elements interleaved per vector, the reference's AVX2 SIMD ABI)."""
     nm, nq = b["nm"], b["nq"]
     d = 3 if shape == "hex" else 2
     u = np.ascontiguousarray(u, dtype=np.float64)
     out = np.zeros(E * nm ** d)
     fn = lib().or_helmholtz_hex if d == 3 else lib().or_helmholtz_quad
+    if simd == "avx2":
+        if d != 3:
+            raise ValueError("the AVX2 variant covers hexahedra")
+        fn = lib().or_helmholtz_hex_avx2
     fn(nm, nq, _p(b["bdata"]), _p(b["dbdata"]), _p(b["D"]), _p(b["w"]), E, int(deformed),
        _p(jac), _p(df), ctypes.c_double(lam), _p(u), _p(out))
     return out

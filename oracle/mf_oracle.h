@@ -70,6 +70,11 @@ void or_iprod_deriv_quad(int nm, int nq, const double *bdata, const double *dbda
                          const double *w, int E, int deformed, const double *jac,
                          const double *df, const double *in0, const double *in1,
                          double *out);
+/* The same with the AVX2 SIMD ABI: four elements interleaved per vector (mf_oracle_avx2.c). */
+void or_helmholtz_hex_avx2(int nm, int nq, const double *bdata, const double *dbdata,
+                           const double *D, const double *w, int E, int deformed,
+                           const double *jac, const double *df, double lambda,
+                           const double *in, double *out);
 void or_helmholtz_quad(int nm, int nq, const double *bdata, const double *dbdata,
                        const double *D, const double *w, int E, int deformed,
                        const double *jac, const double *df, double lambda,

@@ -246,3 +246,23 @@ def test_helmholtz_invariants():
     Hv = O.helmholtz("hex", b, E, True, jac, df, 10.0, v)
     assert abs(v @ Hu - u @ Hv) < 1e-12 * abs(v @ Hu)
     assert u @ Hu > 0
+
+
+@pytest.mark.parametrize("E,deformed", [(1, True), (6, True), (7, False)])
+def test_avx2_helmholtz_vs_dense(E, deformed):
+    """The width-4 (AVX2 SIMD ABI) restatement against the dense operator, with element
+    counts that leave a partly filled last vector."""
+    rng = np.random.default_rng(17)
+    verts = _rand_hex(rng, E)
+    if not deformed:
+        verts = np.stack([np.array(REG_HEX).reshape(8, 3) * (1 + 0.1 * e) + e for e in range(E)])
+    b = O.basis(5, 7)
+    jac, df = O.geom_hex(verts, 7, b["z"], deformed)
+    u = rng.uniform(-1, 1, E * 125)
+    h4 = O.helmholtz("hex", b, E, deformed, jac, df, 2.5, u, simd="avx2")
+    h1 = O.helmholtz("hex", b, E, deformed, jac, df, 2.5, u)
+    assert np.abs(h4 - h1).max() <= 1e-13 * np.abs(h1).max()
+    for e in range(E):
+        m = R.element_mats("hex", 5, 7, verts[e])
+        ref = R.helmholtz_matrix(m, 2.5) @ u[e * 125:(e + 1) * 125]
+        assert np.linalg.norm(h4[e * 125:(e + 1) * 125] - ref) < 1e-12 * np.linalg.norm(ref)

@@ -136,6 +136,14 @@ int irom_eig_sym(int n, const double *d_G, double *h_evals, double *d_evecs, dou
                  int max_sweeps, int *sweeps, void *stream);
 int irom_eig_sym_ws(int n, const double *d_G, double *h_evals, double *d_evecs, double tol,
                     int max_sweeps, int *sweeps, void *d_work, size_t work_bytes, void *stream);
+
+/* irom_eig_sym without any allocation, for callers that cannot allocate device memory
+ * (tight serving loops, code that forbids hipMalloc).  No counterpart in the reference.
+ * The workspace is a static device array sized for n <= 512 (4.2 MB per GPU, held until
+ * the process ends); larger n is rejected with an error.  Every call on a GPU shares that
+ * GPU's array: calls that overlap on one GPU (other host threads or streams) race. */
+int irom_eig_sym_noalloc(int n, const double *d_G, double *h_evals, double *d_evecs,
+                         double tol, int max_sweeps, int *sweeps, void *stream);
 const char *irom_eig_last_error(void);
 
 const char *irom_last_error(void);

@@ -20,6 +20,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -268,10 +269,10 @@ struct EigPlan {
     int B, np, nb, m;
 };
 
-inline EigPlan eig_plan(int n)
+constexpr EigPlan eig_plan(int n)
 {
     EigPlan p{1, 0, 0, 1};
-    for (int B : {16, 8, 4, 2, 1}) {
+    for (int B = 16; B >= 1; B /= 2) {
         const int np = (n + 2 * B - 1) / (2 * B) * (2 * B);
         if ((long)2 * B * np * 8 <= EIG_LDS || B == 1) {
             p.B = B;
@@ -291,9 +292,9 @@ struct EigWs {
     int *perm, *ints;  // ints: rot[EIG_MAX_SWEEPS], bar[2], sweeps, err
 };
 
-inline size_t al256(size_t b) { return (b + 255) / 256 * 256; }
+constexpr size_t al256(size_t b) { return (b + 255) / 256 * 256; }
 
-inline size_t eig_ws_bytes(int n)
+constexpr size_t eig_ws_bytes(int n)
 {
     const EigPlan p = eig_plan(n);
     return 2 * al256(sizeof(double) * (size_t)p.np * p.np) + al256(sizeof(double) * 2 * p.np) +
@@ -392,6 +393,30 @@ int eig_run(int n, const double *d_G, double *h_evals, double *d_evecs, double t
     return info[0] < max_sweeps ? 0 : efail(2, "irom_eig_sym: not converged");
 }
 
+// Allocation-free variant: a file-scope device array holds the workspace of the largest
+// order it accepts.  Each GPU has its own copy at its own address, taken once per device
+// with hipGetSymbolAddress (the array's name in host code is not a device address).
+constexpr int EIG_NOALLOC_MAX_N = 512;
+constexpr size_t EIG_SCRATCH_BYTES = irom::eig_ws_bytes(EIG_NOALLOC_MAX_N);
+static __device__ __align__(256) unsigned char scratch[EIG_SCRATCH_BYTES];
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[64];
+
+int scratch_address(int dev, void **out)
+{
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return efail(3, std::string("irom_eig_sym_noalloc: hipGetSymbolAddress: ") +
+                                hipGetErrorString(e));
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return 0;
+}
+
 // Workspace of irom_eig_sym, per device, grown on demand and kept until the process ends.
 struct EigCache {
     void *p = nullptr;
@@ -439,6 +464,21 @@ int irom_eig_sym_ws(int n, const double *d_G, double *h_evals, double *d_evecs, 
     if (work_bytes < irom::eig_ws_bytes(n))
         return efail(1, "irom_eig_sym_ws: workspace too small (see irom_eig_workspace_size)");
     return eig_run(n, d_G, h_evals, d_evecs, tol, max_sweeps, sweeps, d_work, (hipStream_t)stream);
+}
+
+int irom_eig_sym_noalloc(int n, const double *d_G, double *h_evals, double *d_evecs,
+                         double tol, int max_sweeps, int *sweeps, void *stream)
+{
+    if (n <= 0 || !d_G || !h_evals || !d_evecs) return efail(1, "irom_eig_sym_noalloc: bad argument");
+    if (n > EIG_NOALLOC_MAX_N)
+        return efail(1, "irom_eig_sym_noalloc: n = " + std::to_string(n) +
+                            " exceeds the static workspace (n <= 512)");
+    int dev = 0;
+    IROM_ECHECK(hipGetDevice(&dev), "hipGetDevice");
+    if (dev < 0 || dev >= 64) return efail(1, "irom_eig_sym_noalloc: device index out of range");
+    void *ws = nullptr;
+    if (int st = scratch_address(dev, &ws)) return st;
+    return eig_run(n, d_G, h_evals, d_evecs, tol, max_sweeps, sweeps, ws, (hipStream_t)stream);
 }
 
 }  // extern "C"

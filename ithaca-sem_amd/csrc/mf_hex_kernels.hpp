@@ -50,14 +50,27 @@ struct HexLds {
 
 // Block-cooperative, fully coalesced load of the block's contiguous span of nv elements'
 // coefficients (nv*M scalars from src) into the padded pencil buffers buf(slot).
+// All of a thread's loads are issued before its first LDS store, so the span costs one
+// memory round trip rather than one per loop iteration.
 template <int NM, int NQ, typename T, typename BufFn>
 __device__ __forceinline__ void load_modes_block(const T *__restrict__ src, int nv, BufFn buf)
 {
     constexpr int M = NM * NM * NM;
-    for (int f = threadIdx.x; f < nv * M; f += blockDim.x) {
-        const int sl = f / M, o = f - sl * M;
-        const int tp = o / NM, p = o - tp * NM;
-        buf(sl)[tp * (NM + 1) + p] = ld_stream(src + f);
+    constexpr int BLOCK = HexCfg<NQ>::BLOCK, PER = (HexCfg<NQ>::EPB * M + BLOCK - 1) / BLOCK;
+    T v[PER];
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+        const int f = threadIdx.x + u * BLOCK;
+        v[u] = f < nv * M ? ld_stream(src + f) : T(0);
+    }
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+        const int f = threadIdx.x + u * BLOCK;
+        if (f < nv * M) {
+            const int sl = f / M, o = f - sl * M;
+            const int tp = o / NM, p = o - tp * NM;
+            buf(sl)[tp * (NM + 1) + p] = v[u];
+        }
     }
 }
 
@@ -71,10 +84,26 @@ __device__ __forceinline__ void load_modes_block_gather(const T *__restrict__ gl
                                                         int nv, BufFn buf)
 {
     constexpr int M = NM * NM * NM;
-    for (int f = threadIdx.x; f < nv * M; f += blockDim.x) {
-        const int sl = f / M, o = f - sl * M;
-        const int tp = o / NM, p = o - tp * NM;
-        buf(sl)[tp * (NM + 1) + p] = T(gsign[f]) * glob[gmap[f]];
+    constexpr int BLOCK = HexCfg<NQ>::BLOCK, PER = (HexCfg<NQ>::EPB * M + BLOCK - 1) / BLOCK;
+    int gi[PER];
+    signed char sg[PER];
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+        const int f = threadIdx.x + u * BLOCK;
+        gi[u] = f < nv * M ? gmap[f] : 0;
+        sg[u] = f < nv * M ? gsign[f] : 0;
+    }
+    T v[PER];
+#pragma unroll
+    for (int u = 0; u < PER; ++u) v[u] = T(sg[u]) * glob[gi[u]];
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+        const int f = threadIdx.x + u * BLOCK;
+        if (f < nv * M) {
+            const int sl = f / M, o = f - sl * M;
+            const int tp = o / NM, p = o - tp * NM;
+            buf(sl)[tp * (NM + 1) + p] = v[u];
+        }
     }
 }
 
@@ -367,6 +396,18 @@ constexpr bool helm_geo_in_lds()
            HexCfg<NQ>::EPB * (3 * Lb::QB + 7 * Lb::Q) * (int)sizeof(T) <= 64 * 1024;
 }
 
+// Number of the 7 geometry arrays a single-vector deformed launch prefetches into
+// registers: as many as fit in 56 VGPRs per thread (NQ values each).  For nm = 6, nq = 7
+// in FP64 that is 4 (162 VGPRs, 3 waves per SIMD), measured 7 % faster than none (103
+// VGPRs, 4 waves) and than all 7 (182 VGPRs, 2 waves).
+template <int NQ, typename T, int GEO, bool MANY>
+constexpr int helm_geo_prefetch()
+{
+    if (GEO != kDeformed || MANY) return 0;
+    const int per = NQ * (int)sizeof(T) / 4;
+    return per > 56 ? 0 : (56 / per > 7 ? 7 : 56 / per);
+}
+
 template <int NM, int NQ, typename T, int GEO, bool GATHER = false, bool MANY = false>
 __global__ __launch_bounds__(HexCfg<NQ>::BLOCK) void hex_helmholtz_kernel(
     const Tab<NM, NQ, T> tab, int nElmt, T lambda, const T *__restrict__ in,
@@ -379,6 +420,11 @@ __global__ __launch_bounds__(HexCfg<NQ>::BLOCK) void hex_helmholtz_kernel(
     constexpr int M = Lb::M, Q = Lb::Q, TPE = C::TPE;
     constexpr long GS = geo_stride(7 * Q, sizeof(T));
     constexpr bool GLDS = MANY && helm_geo_in_lds<NM, NQ, T, GEO>();
+    // Single-vector deformed launches read the first NP geometry arrays of this thread's
+    // points into registers before anything else, so they arrive with the coefficients, in
+    // one memory round trip, while the BwdTrans runs; the rest are read where they are used.
+    constexpr int NP = helm_geo_prefetch<NQ, T, GEO, MANY>();
+    constexpr bool PRE = NP > 0;
     __shared__ T lds[C::EPB][3][Lb::QB];
     __shared__ T lgeo[GLDS ? C::EPB : 1][GLDS ? 7 * Q : 1];
     const int eb = threadIdx.x / TPE;
@@ -396,6 +442,16 @@ __global__ __launch_bounds__(HexCfg<NQ>::BLOCK) void hex_helmholtz_kernel(
         for (int f = threadIdx.x; f < nv * 7 * Q; f += blockDim.x) {
             const int sl = f / (7 * Q), o = f - sl * (7 * Q);
             lgeo[sl][o] = ld_stream(geom + (e0 + sl) * GS + o);
+        }
+    }
+    T gp[PRE ? NP : 1][PRE ? NQ : 1];
+    if constexpr (PRE) {
+        if (valid) {
+            const T *g = geom + e * GS + t;
+#pragma unroll
+            for (int k = 0; k < NQ; ++k)
+#pragma unroll
+                for (int c = 0; c < NP; ++c) gp[c][k] = ld_stream(g + c * Q + k * TPE);
         }
     }
     const int nrep = MANY ? nvec : 1;
@@ -466,7 +522,17 @@ __global__ __launch_bounds__(HexCfg<NQ>::BLOCK) void hex_helmholtz_kernel(
             const T d0 = buf0[k * TPE + t];
             const T d1 = buf1[k * TPE + t];
             T jw, m00, m01, m02, m11, m12, m22;
-            if (GLDS) {
+            if constexpr (PRE) {
+                const T *g = geom + e * GS + k * TPE + t;
+                auto get = [&](int c) { return c < NP ? gp[c < NP ? c : 0][k] : ld_stream(g + c * Q); };
+                jw = get(0);
+                m00 = get(1);
+                m01 = get(2);
+                m02 = get(3);
+                m11 = get(4);
+                m12 = get(5);
+                m22 = get(6);
+            } else if (GLDS) {
                 const T *g = &lgeo[GLDS ? es : 0][k * TPE + t];
                 jw = g[0];
                 m00 = g[Q];

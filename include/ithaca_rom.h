@@ -125,8 +125,9 @@ const char *irom_project_last_error(void);
  * Eigenpairs of the symmetric positive semi-definite n x n matrix d_G (row-major, device,
  * n <= 4096): h_evals (host, n) in descending order, d_evecs (device, column-major n x n,
  * column j the eigenvector of h_evals[j]).  Blocked one-sided Jacobi in one persistent
- * launch; pairs rotate while |w_p.w_q| > tol ||w_p|| ||w_q||; at most max_sweeps (<= 64)
- * sweeps, the count in *sweeps.  Replaces the reference's BDCSVD of the snapshot matrix
+ * launch (block pairs diagonalised through their Gram matrices); column pairs rotate while
+ * |w_p.w_q| > tol ||w_p|| ||w_q|| and |w_p.w_q| > (tol ||G||_F)^2; at most max_sweeps
+ * (<= 64) sweeps, the count in *sweeps.  Replaces the reference's BDCSVD of the snapshot matrix
  * (CoupledLinearNS_ROM.cpp:3925) through the method of snapshots.  Synchronous.
  * irom_eig_sym keeps a workspace per device (allocated on the first call that needs a
  * larger one, kept until the process ends); irom_eig_sym_ws takes the caller's workspace

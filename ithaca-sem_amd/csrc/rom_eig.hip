@@ -7,19 +7,23 @@
 // Algorithm: blocked one-sided (Hestenes) Jacobi on G.  For a symmetric positive
 // semi-definite G, rotating the columns of W = G V until they are mutually orthogonal
 // gives lambda_j = ||w_j|| and eigenvectors v_j.  The np columns are split into nb blocks
-// of B columns; a sweep is nb-1 rounds of the round-robin ("circle") pairing of blocks.
-// In a round, a workgroup takes a block pair: its 2B columns of W are staged in LDS, one
-// wave per column pair runs the inner round-robin over the 2B local columns (dot products
-// reduced by lane exchanges inside the wave, the rotations applied in LDS and accumulated
-// in a 2B x 2B matrix R), then W is written back and V's 2B columns become V R.  All
-// rounds and sweeps run in one persistent launch; workgroups meet at a grid barrier
-// between rounds (an atomic arrival counter and a generation word, with a bounded wait),
-// and stop together after the first sweep without a rotation.  Eigenvalues are sorted on
-// the device (bitonic sort in one workgroup) and the eigenvectors gathered in that order.
+// of 16; a sweep is nb-1 rounds of the round-robin ("circle") pairing of blocks.  In a
+// round, a workgroup takes a block pair: the 32 x 32 Gram matrix P = W_pair^T W_pair is
+// formed with FP64 MFMA (the four waves sum disjoint row ranges), diagonalised in LDS by a
+// parallel two-sided Jacobi (16 disjoint rotations per step, one thread per 2 x 2 block of
+// P, rotations accumulated in R), and the pair's columns of W and V become W R and V R
+// (MFMA again).  The rotation of a column pair is the one-sided Jacobi rotation of the
+// same two columns, so one inner sweep is exactly the column-by-column method; the columns
+// themselves are touched twice per round instead of once per inner step.  All rounds and
+// sweeps run in one persistent launch; workgroups meet at a grid barrier between rounds
+// (an atomic arrival counter and a generation word, with a bounded wait), and stop
+// together after the first sweep without a rotation.  Eigenvalues are sorted on the
+// device (bitonic sort in one workgroup) and the eigenvectors gathered in that order.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -28,8 +32,7 @@
 
 namespace irom {
 
-constexpr int EIG_MAX_SWEEPS = 64;   // length of the per-sweep rotation flag array
-constexpr int EIG_LDS = 128 * 1024;  // LDS for the staged columns of W
+constexpr int EIG_MAX_SWEEPS = 64;  // length of the per-sweep rotation flag array
 
 __device__ __forceinline__ double wave_sum64(double v)
 {
@@ -72,11 +75,23 @@ __device__ __forceinline__ void grid_sync(unsigned *bar, unsigned nblk, unsigned
     __syncthreads();
 }
 
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+constexpr int JB = 16;       // columns per block
+constexpr int JC = 2 * JB;   // columns of a block pair
+constexpr int JP = JC + 1;   // padded LDS row
+constexpr int NW = 8;        // waves per workgroup
+constexpr int JT = 64 * NW;  // threads per workgroup
+constexpr int GU = 2;        // 16-row blocks per wave and load group in the update
+
 struct EigArgs {
-    int np;          // padded order (multiple of 2B)
-    int nb;          // number of column blocks (np / B, even)
-    double tol;      // rotate while |w_p.w_q| > tol ||w_p|| ||w_q||
+    int n;           // order
+    int np;          // padded order (multiple of 2 JB)
+    int nb;          // number of column blocks (np / JB, even)
+    double tol;      // rotate while |w_p.w_q| > tol ||w_p|| ||w_q|| and > (tol ||G||_F)^2
     int max_sweeps;  // <= EIG_MAX_SWEEPS
+    int max_inner;   // inner sweeps over a pair's Gram matrix per round
+    const double *G; // n x n (for ||G||_F)
     double *W;       // np x np column-major (W = G V)
     double *V;       // np x np column-major
     int *rot;        // [EIG_MAX_SWEEPS] per-sweep "some pair was rotated" flags (zeroed)
@@ -85,85 +100,266 @@ struct EigArgs {
     int *err;        // barrier expiry
 };
 
-// One persistent launch for all sweeps.  B columns per block, one wave per local column
-// pair (64*B threads); W's 2B staged columns in dynamic LDS ([2B][np]), R in static LDS.
-template <int B>
-__global__ __launch_bounds__(64 * B) void block_jacobi_kernel(EigArgs a)
+// 1/x and 1/sqrt(x) from the hardware approximations (about 2^-23 relative) refined by
+// Newton steps; each step squares the relative error.
+template <int STEPS>
+__device__ __forceinline__ double rcp_nr(double x)
 {
-    extern __shared__ double lw[];
-    __shared__ double R[2 * B][2 * B + 1];  // R[row][col]
-    __shared__ int s_any;
-    constexpr int C2 = 2 * B;
-    const int np = a.np, nb = a.nb;
-    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    double y = __builtin_amdgcn_rcp(x);
+#pragma unroll
+    for (int i = 0; i < STEPS; ++i) y = fma(y, fma(-x, y, 1.0), y);
+    return y;
+}
+template <int STEPS>
+__device__ __forceinline__ double rsq_nr(double x)
+{
+    double y = __builtin_amdgcn_rsq(x);
+#pragma unroll
+    for (int i = 0; i < STEPS; ++i) y = fma(0.5 * y, fma(-x * y, y, 1.0), y);
+    return y;
+}
+
+// Rotation (c, s) that makes columns p, q of W orthogonal, from app = |w_p|^2,
+// aqq = |w_q|^2, apq = w_p.w_q: w_p <- c w_p - s w_q, w_q <- s w_p + c w_q.  Pairs already
+// orthogonal to tol (relative; tol2 = tol^2), or whose product is below the absolute
+// floor, keep c = 1, s = 0.  t = tan(theta) is the smaller root of t^2 + 2 zeta t - 1 = 0,
+// zeta = (aqq - app) / (2 apq), computed to about 1e-14: nearly parallel columns (the
+// columns of a rank-deficient Gram matrix) need the angle to about 1/kappa of the pair, so
+// an FP32 angle would stall them; a rotation by a slightly different angle is still a
+// rotation.  c = (1 + t^2)^(-1/2) is refined to full precision, so c^2 + s^2 = 1 to
+// rounding and V stays orthogonal.
+__device__ __forceinline__ bool rot_params(double app, double aqq, double apq, double tol2,
+                                           double floor_, double &c, double &s)
+{
+    if (apq == 0.0 || apq * apq <= tol2 * (app * aqq) || fabs(apq) <= floor_) {
+        c = 1.0;
+        s = 0.0;
+        return false;
+    }
+    const double zeta = (aqq - app) * rcp_nr<1>(2.0 * apq);
+    const double az = fabs(zeta);
+    double t;
+    if (!(az <= 1e60)) {
+        t = az == az ? 0.5 / zeta : 0.0;  // t ~ 1/(2 zeta); no rotation for a NaN
+    } else {
+        const double h = fma(zeta, zeta, 1.0);
+        t = rcp_nr<1>(az + h * rsq_nr<1>(h));
+        if (zeta < 0.0) t = -t;
+    }
+    c = rsq_nr<3>(fma(t, t, 1.0));
+    s = c * t;
+    return true;
+}
+
+// One persistent launch for all sweeps; JT threads, one block pair at a time.
+__global__ __launch_bounds__(JT) void gram_jacobi_kernel(EigArgs a)
+{
+    __shared__ double Pw[NW / 2][JC][JP];  // partial Gram matrices (waves w and w + NW/2)
+    __shared__ double Pb[2][JC][JP];       // the pair's Gram matrix, current and next
+    __shared__ double R[JC][JP];           // accumulated rotation
+    __shared__ double s_red[NW];
+    const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
+    const int i16 = lane & 15, g4 = lane >> 4;
+    const int np = a.np, nb = a.nb, nrb = np / 16;
+
+    // ||G||_F, summed in the same order by every workgroup
+    double fro = 0.0;
+    for (long i = tid; i < (long)a.n * a.n; i += JT) fro = fma(a.G[i], a.G[i], fro);
+    fro = wave_sum64(fro);
+    if (lane == 0) s_red[wv] = fro;
+    __syncthreads();
+    double fsum = 0.0;
+    for (int w = 0; w < NW; ++w) fsum += s_red[w];
+    const double tg = a.tol * sqrt(fsum);
+    const double floor_ = tg * tg, tol2 = a.tol * a.tol;
+
     unsigned gen = 0;
     int sw = 0;
     for (; sw < a.max_sweeps; ++sw) {
         for (int r = 0; r < nb - 1; ++r) {
             for (int k = blockIdx.x; k < nb / 2; k += gridDim.x) {
                 const int bi = player(k, r, nb), bj = player(nb - 1 - k, r, nb);
-                auto gcol = [&](int c) { return c < B ? bi * B + c : bj * B + (c - B); };
-                for (int idx = threadIdx.x; idx < C2 * np; idx += blockDim.x) {
-                    const int c = idx / np, i = idx - c * np;
-                    lw[idx] = a.W[(long)gcol(c) * np + i];
-                }
-                for (int idx = threadIdx.x; idx < C2 * C2; idx += blockDim.x) {
-                    const int rr = idx / C2, cc = idx - rr * C2;
-                    R[rr][cc] = rr == cc ? 1.0 : 0.0;
-                }
-                if (threadIdx.x == 0) s_any = 0;
-                __syncthreads();
-                // inner round-robin over the 2B local columns: wave wv rotates one pair
-                for (int s = 0; s < C2 - 1; ++s) {
-                    const int p = player(wv, s, C2), q = player(C2 - 1 - wv, s, C2);
-                    double *wp = lw + (long)p * np, *wq = lw + (long)q * np;
-                    double x2 = 0.0, y2 = 0.0, xy = 0.0;
-                    for (int i = lane; i < np; i += 64) {
-                        const double x = wp[i], y = wq[i];
-                        x2 = fma(x, x, x2);
-                        y2 = fma(y, y, y2);
-                        xy = fma(x, y, xy);
+                auto gcol = [&](int c) { return c < JB ? bi * JB + c : bj * JB + (c - JB); };
+                // 1. P = W_pair^T W_pair.  Lane (i16, g4) holds rows 4 g4 .. 4 g4 + 3 of a
+                // 16-row block for column i16 of each block; MFMA step t pairs row 4 g4 + t
+                // of both operands (any one-to-one pairing of rows gives the same sum).
+                // Groups of GP row blocks per wave: all loads of a group before its MFMAs.
+                d4 a00 = {0, 0, 0, 0}, a01 = {0, 0, 0, 0}, a11 = {0, 0, 0, 0};
+                {
+                    const double *w0 = a.W + (long)gcol(i16) * np + 4 * g4;
+                    const double *w1 = a.W + (long)gcol(JB + i16) * np + 4 * g4;
+                    constexpr int GP = 4;
+                    for (int rg = wv; rg < nrb; rg += NW * GP) {
+                        double2 xv[GP][2], yv[GP][2];
+#pragma unroll
+                        for (int u = 0; u < GP; ++u) {
+                            // rows past the end: load the last block (in bounds), use zeros
+                            const int rb = min(rg + NW * u, nrb - 1);
+                            xv[u][0] = *reinterpret_cast<const double2 *>(w0 + rb * 16);
+                            xv[u][1] = *reinterpret_cast<const double2 *>(w0 + rb * 16 + 2);
+                            yv[u][0] = *reinterpret_cast<const double2 *>(w1 + rb * 16);
+                            yv[u][1] = *reinterpret_cast<const double2 *>(w1 + rb * 16 + 2);
+                        }
+#pragma unroll
+                        for (int u = 0; u < GP; ++u) {
+                            const double m = rg + NW * u < nrb ? 1.0 : 0.0;
+                            const double x[4] = {m * xv[u][0].x, m * xv[u][0].y, m * xv[u][1].x, m * xv[u][1].y};
+                            const double y[4] = {m * yv[u][0].x, m * yv[u][0].y, m * yv[u][1].x, m * yv[u][1].y};
+#pragma unroll
+                            for (int t = 0; t < 4; ++t) {
+                                a00 = __builtin_amdgcn_mfma_f64_16x16x4f64(x[t], x[t], a00, 0, 0, 0);
+                                a01 = __builtin_amdgcn_mfma_f64_16x16x4f64(x[t], y[t], a01, 0, 0, 0);
+                                a11 = __builtin_amdgcn_mfma_f64_16x16x4f64(y[t], y[t], a11, 0, 0, 0);
+                            }
+                        }
                     }
-                    x2 = wave_sum64(x2);
-                    y2 = wave_sum64(y2);
-                    xy = wave_sum64(xy);
-                    if (xy != 0.0 && fabs(xy) > a.tol * sqrt(x2 * y2)) {
-                        const double zeta = (y2 - x2) / (2.0 * xy);
-                        const double t = (zeta >= 0.0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
-                        const double c = 1.0 / sqrt(1.0 + t * t), sn = c * t;
-                        for (int i = lane; i < np; i += 64) {
-                            const double x = wp[i], y = wq[i];
-                            wp[i] = c * x - sn * y;
-                            wq[i] = sn * x + c * y;
+                }
+                // component q of lane l: D[l/16 + 4q][l%16]; waves w < NW/2 store, then waves
+                // w + NW/2 add theirs
+                const int hw = wv % (NW / 2);
+#pragma unroll
+                for (int half = 0; half < 2; ++half) {
+                    if ((wv >= NW / 2) == (half == 1)) {
+#pragma unroll
+                        for (int q = 0; q < 4; ++q) {
+                            const int ii = g4 + 4 * q;
+                            double *d0 = &Pw[hw][ii][i16], *d1 = &Pw[hw][ii][JB + i16];
+                            double *d2 = &Pw[hw][JB + ii][JB + i16];
+                            *d0 = half ? *d0 + a00[q] : a00[q];
+                            *d1 = half ? *d1 + a01[q] : a01[q];
+                            *d2 = half ? *d2 + a11[q] : a11[q];
                         }
-                        if (lane < C2) {
-                            const double u = R[lane][p], w = R[lane][q];
-                            R[lane][p] = c * u - sn * w;
-                            R[lane][q] = sn * u + c * w;
-                        }
-                        if (lane == 0) s_any = 1;
                     }
                     __syncthreads();
                 }
-                // W back; V's 2B columns <- V R (row i of the block: 2B loads, 2B outputs)
-                for (int idx = threadIdx.x; idx < C2 * np; idx += blockDim.x) {
-                    const int c = idx / np, i = idx - c * np;
-                    a.W[(long)gcol(c) * np + i] = lw[idx];
+                // sum of the partials in a fixed order; the upper triangle, mirrored
+                for (int idx = tid; idx < JC * JC; idx += JT) {
+                    const int ii = idx / JC, jj = idx - ii * JC;
+                    const int u = min(ii, jj), v = max(ii, jj);
+                    double sum = Pw[0][u][v];
+#pragma unroll
+                    for (int w = 1; w < NW / 2; ++w) sum += Pw[w][u][v];
+                    Pb[0][ii][jj] = sum;
+                    R[ii][jj] = ii == jj ? 1.0 : 0.0;
                 }
-                if (s_any) {
-                    for (int i = threadIdx.x; i < np; i += blockDim.x) {
-                        double v[C2];
+
+                // The update's first load group (rows of W and V, lane (i16, g4): row i16
+                // of a 16-row block, column 4 kk + g4), issued now so that it arrives while
+                // the Gram matrix is diagonalised.
+                // column 4 kk + g4 of the pair: kk < 4 in block bi, else in block bj
+                const long kb0 = ((long)bi * JB + g4) * np + i16, kb1 = ((long)bj * JB + g4) * np + i16;
+                auto ka = [&](int kk) { return (kk < 4 ? kb0 : kb1) + (long)(4 * (kk & 3)) * np; };
+                double aw[GU][8], av[GU][8];
+                auto load_group = [&](int g) {
 #pragma unroll
-                        for (int d = 0; d < C2; ++d) v[d] = a.V[(long)gcol(d) * np + i];
-#pragma unroll 1
-                        for (int c = 0; c < C2; ++c) {
-                            double s = 0.0;
+                    for (int u = 0; u < GU; ++u) {
+                        // rows past the end: the last block (in bounds), not used
+                        const int rb = min(wv + NW * (g * GU + u), nrb - 1);
 #pragma unroll
-                            for (int d = 0; d < C2; ++d) s = fma(v[d], R[d][c], s);
-                            a.V[(long)gcol(c) * np + i] = s;
+                        for (int kk = 0; kk < 8; ++kk) {
+                            aw[u][kk] = a.W[ka(kk) + rb * 16];
+                            av[u][kk] = a.V[ka(kk) + rb * 16];
                         }
                     }
-                    if (threadIdx.x == 0) atomicOr(&a.rot[sw], 1);
+                };
+                load_group(0);
+                __syncthreads();
+
+                // 2. Pairs already orthogonal skip the rest.  Otherwise two-sided Jacobi on
+                // P: in step st the 16 pairs of the circle schedule rotate at once; thread
+                // (k1, k2) of the first half forms the 2 x 2 block of J^T P J on rows
+                // {p1, q1} and columns {p2, q2} (P double-buffered), the same thread of the
+                // second half updates that block of R <- R J.  The blocks partition the
+                // matrices.
+                int need = 0;
+                for (int idx = tid; idx < JC * JC; idx += JT) {
+                    const int ii = idx / JC, jj = idx - ii * JC;
+                    if (ii < jj) {
+                        const double apq = Pb[0][ii][jj];
+                        if (apq != 0.0 && apq * apq > tol2 * (Pb[0][ii][ii] * Pb[0][jj][jj]) &&
+                            fabs(apq) > floor_)
+                            need = 1;
+                    }
+                }
+                bool pair_rot = false;
+                if (__syncthreads_or(need)) {
+                    const bool rhalf = tid >= JT / 2;
+                    const int hb = rhalf ? tid - JT / 2 : tid;
+                    const int k1 = hb >> 4, k2 = hb & 15;
+                    int cur = 0;
+#pragma unroll 1
+                    for (int it = 0; it < a.max_inner; ++it) {
+                        int any = 0;
+                        int pl[4] = {player(k1, 0, JC), player(JC - 1 - k1, 0, JC), player(k2, 0, JC),
+                                     player(JC - 1 - k2, 0, JC)};
+#pragma unroll 1
+                        for (int st = 0; st < JC - 1; ++st) {
+                            const int p1 = pl[0], q1 = pl[1], p2 = pl[2], q2 = pl[3];
+                            double(*P)[JP] = Pb[cur];
+                            double c2, s2;
+                            rot_params(P[p2][p2], P[q2][q2], P[p2][q2], tol2, floor_, c2, s2);
+                            if (!rhalf) {
+                                double(*Pn)[JP] = Pb[cur ^ 1];
+                                double c1, s1;
+                                const bool r1 = rot_params(P[p1][p1], P[q1][q1], P[p1][q1], tol2, floor_, c1, s1);
+                                const double mpp = P[p1][p2], mpq = P[p1][q2], mqp = P[q1][p2], mqq = P[q1][q2];
+                                const double xpp = c2 * mpp - s2 * mpq, xpq = s2 * mpp + c2 * mpq;
+                                const double xqp = c2 * mqp - s2 * mqq, xqq = s2 * mqp + c2 * mqq;
+                                Pn[p1][p2] = c1 * xpp - s1 * xqp;
+                                Pn[p1][q2] = c1 * xpq - s1 * xqq;
+                                Pn[q1][p2] = s1 * xpp + c1 * xqp;
+                                Pn[q1][q2] = s1 * xpq + c1 * xqq;
+                                if (k1 == k2 && r1) any = 1;
+                            } else {
+                                const double rpp = R[p1][p2], rpq = R[p1][q2], rqp = R[q1][p2], rqq = R[q1][q2];
+                                R[p1][p2] = c2 * rpp - s2 * rpq;
+                                R[p1][q2] = s2 * rpp + c2 * rpq;
+                                R[q1][p2] = c2 * rqp - s2 * rqq;
+                                R[q1][q2] = s2 * rqp + c2 * rqq;
+                            }
+                            // the circle schedule: every position but the first advances
+                            // by one, 31 wrapping to 1
+#pragma unroll
+                            for (int u = 0; u < 4; ++u) pl[u] = pl[u] == 0 ? 0 : (pl[u] == JC - 1 ? 1 : pl[u] + 1);
+                            cur ^= 1;
+                            __syncthreads();
+                        }
+                        if (!__syncthreads_or(any)) break;
+                        pair_rot = true;
+                    }
+                }
+
+                // 3. The pair's columns of W and V <- (W, V) R: A = the rows loaded above,
+                // B = R; each wave writes only its own rows.
+                if (pair_rot) {
+                    const long kc0 = (long)gcol(i16) * np + g4, kc1 = (long)gcol(JB + i16) * np + g4;
+                    for (int g = 0; wv + NW * g * GU < nrb; ++g) {
+                        if (g > 0) load_group(g);
+#pragma unroll
+                        for (int u = 0; u < GU; ++u) {
+                            const int rb = wv + NW * (g * GU + u);
+                            const int r0 = rb * 16;
+                            if (rb < nrb) {
+                            d4 ow0 = {0, 0, 0, 0}, ow1 = {0, 0, 0, 0}, ov0 = {0, 0, 0, 0}, ov1 = {0, 0, 0, 0};
+#pragma unroll
+                            for (int kk = 0; kk < 8; ++kk) {
+                                const double rb0 = R[4 * kk + g4][i16], rb1 = R[4 * kk + g4][JB + i16];
+                                ow0 = __builtin_amdgcn_mfma_f64_16x16x4f64(aw[u][kk], rb0, ow0, 0, 0, 0);
+                                ow1 = __builtin_amdgcn_mfma_f64_16x16x4f64(aw[u][kk], rb1, ow1, 0, 0, 0);
+                                ov0 = __builtin_amdgcn_mfma_f64_16x16x4f64(av[u][kk], rb0, ov0, 0, 0, 0);
+                                ov1 = __builtin_amdgcn_mfma_f64_16x16x4f64(av[u][kk], rb1, ov1, 0, 0, 0);
+                            }
+#pragma unroll
+                            for (int q = 0; q < 4; ++q) {
+                                a.W[kc0 + r0 + 4 * q] = ow0[q];
+                                a.W[kc1 + r0 + 4 * q] = ow1[q];
+                                a.V[kc0 + r0 + 4 * q] = ov0[q];
+                                a.V[kc1 + r0 + 4 * q] = ov1[q];
+                            }
+                            }
+                        }
+                    }
+                    if (tid == 0) atomicOr(&a.rot[sw], 1);
                 }
                 __syncthreads();  // LDS reused by the next block pair of this workgroup
             }
@@ -264,23 +460,16 @@ __global__ void gather_columns(int n, int np, const int *__restrict__ perm,
     out[g] = V[(long)c * np + i] / vn[c];
 }
 
-// Block width for order n: the widest B with 2B staged columns of np rows in EIG_LDS.
+// Padded order (a multiple of the 32 columns of a block pair), block count, and the
+// power of two m >= n of the sort.
 struct EigPlan {
-    int B, np, nb, m;
+    int np, nb, m;
 };
 
 constexpr EigPlan eig_plan(int n)
 {
-    EigPlan p{1, 0, 0, 1};
-    for (int B = 16; B >= 1; B /= 2) {
-        const int np = (n + 2 * B - 1) / (2 * B) * (2 * B);
-        if ((long)2 * B * np * 8 <= EIG_LDS || B == 1) {
-            p.B = B;
-            p.np = np;
-            p.nb = np / B;
-            break;
-        }
-    }
+    EigPlan p{(n + JC - 1) / JC * JC, 0, 1};
+    p.nb = p.np / JB;
     while (p.m < n) p.m <<= 1;
     return p;
 }
@@ -324,6 +513,10 @@ inline EigWs eig_ws(int n, void *base)
 }  // namespace irom
 
 namespace {
+const int g_inner = [] {
+    const char *v = getenv("IROM_EIG_INNER");
+    return v ? std::max(1, atoi(v)) : 1;
+}();
 thread_local std::string g_eerr;
 int efail(int c, const std::string &m)
 {
@@ -337,18 +530,13 @@ int efail(int c, const std::string &m)
         if (e_ != hipSuccess) return efail(3, std::string(what) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
-template <int B>
-hipError_t launch_block_jacobi(const irom::EigArgs &a, int ncu, hipStream_t s)
+hipError_t launch_gram_jacobi(const irom::EigArgs &a, int ncu, hipStream_t s)
 {
-    const size_t lds = sizeof(double) * 2 * B * (size_t)a.np;
-    hipError_t e = hipFuncSetAttribute((const void *)&irom::block_jacobi_kernel<B>,
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    if (e != hipSuccess) return e;
     // every workgroup must be resident at once (grid barrier): at most one per CU here
     const int grid = std::max(1, std::min(a.nb / 2, ncu));
     void *args[] = {(void *)&a};
-    return hipLaunchCooperativeKernel((const void *)&irom::block_jacobi_kernel<B>, dim3(grid),
-                                      dim3(64 * B), args, lds, s);
+    return hipLaunchCooperativeKernel((const void *)&irom::gram_jacobi_kernel, dim3(grid),
+                                      dim3(irom::JT), args, 0, s);
 }
 
 // The whole decomposition on the workspace `ws` (eig_ws_bytes(n) bytes of device memory).
@@ -367,16 +555,9 @@ int eig_run(int n, const double *d_G, double *h_evals, double *d_evecs, double t
     irom::eig_init<<<(unsigned)((tot + 255) / 256), 256, 0, s>>>(n, p.np, d_G, w.W, w.V, w.ints,
                                                                  irom::EIG_MAX_SWEEPS + 4);
     IROM_ECHECK(hipGetLastError(), "eig_init");
-    irom::EigArgs a{p.np, p.nb, tol, max_sweeps, w.W, w.V, rot, (unsigned *)bar, sw_d, err_d};
-    hipError_t e;
-    switch (p.B) {
-    case 16: e = launch_block_jacobi<16>(a, ncu, s); break;
-    case 8: e = launch_block_jacobi<8>(a, ncu, s); break;
-    case 4: e = launch_block_jacobi<4>(a, ncu, s); break;
-    case 2: e = launch_block_jacobi<2>(a, ncu, s); break;
-    default: e = launch_block_jacobi<1>(a, ncu, s); break;
-    }
-    IROM_ECHECK(e, "block_jacobi launch");
+    irom::EigArgs a{n, p.np, p.nb, tol, max_sweeps, g_inner, d_G, w.W, w.V, rot,
+                    (unsigned *)bar, sw_d, err_d};
+    IROM_ECHECK(launch_gram_jacobi(a, ncu, s), "gram_jacobi launch");
     irom::column_norms<<<p.np, irom::ET, 0, s>>>(p.np, w.W, w.nrm);
     irom::column_norms<<<p.np, irom::ET, 0, s>>>(p.np, w.V, w.nrm + p.np);
     const size_t sort_lds = (sizeof(double) + sizeof(int)) * (size_t)p.m;

@@ -156,3 +156,73 @@ def helmholtz(shape, b, E, deformed, jac, df, lam, u, simd="scalar"):
     fn(nm, nq, _p(b["bdata"]), _p(b["dbdata"]), _p(b["D"]), _p(b["w"]), E, int(deformed),
        _p(jac), _p(df), ctypes.c_double(lam), _p(u), _p(out))
     return out
+
+
+# ---- triangles (mf_oracle_tri.c) ----
+
+def basis_tri(nm, nq0, nq1=None):
+    """Modified_A on nq0 GLL points (eta0) and Modified_B on nq1 Gauss-Radau (alpha = 1)
+    points (eta1, default nq0 - 1 as MatrixFree requires); w1 is the Helper's 0.5-scaled
+    weight, w1_raw the Gauss-Radau one."""
+    nq1 = nq0 - 1 if nq1 is None else nq1
+    a = basis(nm, nq0)
+    M = nm * (nm + 1) // 2
+    z1 = np.zeros(nq1); w1 = np.zeros(nq1); D1 = np.zeros(nq1 * nq1)
+    b1 = np.zeros(M * nq1); db1 = np.zeros(M * nq1)
+    lib().or_basis_modified_b(nm, nq1, _p(z1), _p(w1), _p(D1), _p(b1), _p(db1))
+    return dict(nm=nm, nq0=nq0, nq1=nq1, M=M, Q=nq0 * nq1,
+                z0=a["z"], w0=a["w"], D0=a["D"], bdata0=a["bdata"], dbdata0=a["dbdata"],
+                z1=z1, w1_raw=w1, w1=0.5 * w1, D1=D1, bdata1=b1, dbdata1=db1)
+
+
+def geom_tri(verts, b, deformed=False):
+    """Straight-sided triangles (E x 3 x 2 vertices): jac and GetDerivFactors layouts."""
+    verts = np.ascontiguousarray(verts, dtype=np.float64)
+    E = verts.shape[0]
+    n = E * b["Q"] if deformed else E
+    jac = np.zeros(n); df = np.zeros(4 * n)
+    lib().or_geom_tri(_p(verts), E, b["nq0"], b["nq1"], int(deformed), _p(jac), _p(df))
+    return jac, df
+
+
+def coords_tri(verts, b):
+    verts = np.ascontiguousarray(verts, dtype=np.float64)
+    E = verts.shape[0]
+    xy = np.zeros(2 * E * b["Q"])
+    lib().or_coords_tri(_p(verts), E, b["nq0"], b["nq1"], _p(b["z0"]), _p(b["z1"]), _p(xy))
+    return xy.reshape(2, -1)
+
+
+def bwdtrans_tri(b, E, u):
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    out = np.zeros(E * b["Q"])
+    lib().or_bwdtrans_tri(b["nm"], b["nq0"], b["nq1"], _p(b["bdata0"]), _p(b["bdata1"]), E,
+                          _p(u), _p(out))
+    return out
+
+
+def iproduct_tri(b, E, deformed, jac, f):
+    f = np.ascontiguousarray(f, dtype=np.float64)
+    out = np.zeros(E * b["M"])
+    lib().or_iproduct_tri(b["nm"], b["nq0"], b["nq1"], _p(b["bdata0"]), _p(b["bdata1"]),
+                          _p(b["w0"]), _p(b["w1"]), E, int(deformed), _p(jac), _p(f), _p(out))
+    return out
+
+
+def physderiv_tri(b, E, deformed, df, f):
+    f = np.ascontiguousarray(f, dtype=np.float64)
+    outs = [np.zeros(E * b["Q"]) for _ in range(2)]
+    lib().or_physderiv_tri(b["nq0"], b["nq1"], _p(b["D0"]), _p(b["D1"]), _p(b["z0"]),
+                           _p(b["z1"]), E, int(deformed), _p(df), _p(f), *[_p(o) for o in outs])
+    return outs
+
+
+def helmholtz_tri(b, E, deformed, jac, df, lam, u):
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    out = np.zeros(E * b["M"])
+    lib().or_helmholtz_tri(b["nm"], b["nq0"], b["nq1"], _p(b["bdata0"]), _p(b["dbdata0"]),
+                           _p(b["bdata1"]), _p(b["dbdata1"]), _p(b["D0"]), _p(b["D1"]),
+                           _p(b["z0"]), _p(b["z1"]), _p(b["w0"]), _p(b["w1"]), E,
+                           int(deformed), _p(jac), _p(df), ctypes.c_double(lam), _p(u),
+                           _p(out))
+    return out

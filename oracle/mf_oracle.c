@@ -85,7 +85,7 @@ void or_jacobfd(int np, const double *z, double *poly_in, double *polyd, int n,
 /* Zeros of P_n^{(alpha,beta)} by Newton iteration with deflation, Chebyshev
  * starting guesses: Polylib.cpp Jacobz (selected because POLYNOMIAL_DEFLATION is
  * defined, Polylib.cpp:55-63). */
-static void or_jacobz(int n, double *z, double alpha, double beta)
+void or_jacobz(int n, double *z, double alpha, double beta)
 {
     const double EPS = 100 * 2.220446049250313e-16;
     const int STOP = 30;
@@ -197,7 +197,7 @@ static void adj3(const double *a, double *b)
 /* From the map derivatives deriv[i][k] = dx_k/dxi_i at one point compute
  * jac = sqrt(det g) and factors[k*dim+j] = sum_i deriv[i][k] g^{-1}[i][j]:
  * GeomFactors.cpp:420-474 (ComputeDerivFactors).  dim = coordim. */
-static void deriv_factors(int dim, const double *deriv, double *jac, double *fac)
+void or_deriv_factors(int dim, const double *deriv, double *jac, double *fac)
 {
     double g[9] = {0}, gi[9];
     for (int i = 0, l = 0; i < dim; ++i)
@@ -282,7 +282,7 @@ void or_geom_hex(const double *verts, int E, int nq, const double *z, int deform
             int i = q % nq, j = (q / nq) % nq, k = q / (nq * nq);
             double xi[3] = {z[i], z[j], z[k]}, deriv[9], J, fac[9];
             hex_map_deriv(v, xi, deriv, NULL);
-            deriv_factors(3, deriv, &J, fac);
+            or_deriv_factors(3, deriv, &J, fac);
             size_t idx = deformed ? (size_t)e * Q + q : (size_t)e;
             jac[idx] = J;
             for (int d = 0; d < 9; ++d) df[d * np + idx] = fac[d];
@@ -302,7 +302,7 @@ void or_geom_quad(const double *verts, int E, int nq, const double *z, int defor
             int i = q % nq, j = q / nq;
             double xi[2] = {z[i], z[j]}, deriv[4], J, fac[4];
             quad_map_deriv(v, xi, deriv, NULL);
-            deriv_factors(2, deriv, &J, fac);
+            or_deriv_factors(2, deriv, &J, fac);
             size_t idx = deformed ? (size_t)e * Q + q : (size_t)e;
             jac[idx] = J;
             for (int d = 0; d < 4; ++d) df[d * np + idx] = fac[d];

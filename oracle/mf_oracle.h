@@ -80,6 +80,32 @@ void or_helmholtz_quad(int nm, int nq, const double *bdata, const double *dbdata
                        const double *jac, const double *df, double lambda,
                        const double *in, double *out);
 
+/* ---- Triangles (mf_oracle_tri.c): Modified_A x Modified_B on GLL x Gauss-Radau ---- */
+void or_jacobz(int n, double *z, double alpha, double beta);
+void or_deriv_factors(int dim, const double *deriv, double *jac, double *fac);
+void or_zwgrjm(double *z, double *w, int np, double alpha, double beta);
+void or_Dnodes(double *D, const double *z, int np);
+void or_basis_modified_b(int nm, int nq, double *z, double *w, double *D, double *bdata,
+                         double *dbdata);
+void or_geom_tri(const double *verts, int E, int nq0, int nq1, int deformed, double *jac,
+                 double *df);
+void or_coords_tri(const double *verts, int E, int nq0, int nq1, const double *z0,
+                   const double *z1, double *xy);
+void or_bwdtrans_tri(int nm, int nq0, int nq1, const double *b0, const double *b1, int E,
+                     const double *in, double *out);
+void or_iproduct_tri(int nm, int nq0, int nq1, const double *b0, const double *b1,
+                     const double *w0, const double *w1, int E, int deformed,
+                     const double *jac, const double *in, double *out);
+void or_physderiv_tri(int nq0, int nq1, const double *D0, const double *D1,
+                      const double *z0, const double *z1, int E, int deformed,
+                      const double *df, const double *in, double *out0, double *out1);
+void or_helmholtz_tri(int nm, int nq0, int nq1, const double *b0, const double *db0,
+                      const double *b1, const double *db1, const double *D0,
+                      const double *D1, const double *z0, const double *z1,
+                      const double *w0, const double *w1, int E, int deformed,
+                      const double *jac, const double *df, double lambda,
+                      const double *in, double *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -112,3 +112,84 @@ def helmholtz_matrix(m, lam):
         dc = sum(m["Ainv"][:, a, c][:, None] * m["G"][a] for a in range(dim))
         H += dc.T @ (WJ[:, None] * dc)
     return H
+
+
+# ---- triangles: Modified_A (eta0, GLL) x Modified_B (eta1, Gauss-Radau alpha = 1) ----
+
+def gauss_radau_a1(nq):
+    """Nodes -1 and the roots of P_{nq-1}^{(1,1)} (scipy), weights w_i = int l_i (1-z) dz
+    by a 40-point Gauss-Legendre rule (exact for these polynomial degrees)."""
+    from scipy.special import roots_jacobi
+    z = np.concatenate([[-1.0], np.sort(roots_jacobi(nq - 1, 1.0, 1.0)[0])]) if nq > 1 else np.zeros(1)
+    xg, wg = L.leggauss(40)
+    w = np.zeros(nq)
+    for i in range(nq):
+        li = np.ones_like(xg)
+        for k in range(nq):
+            if k != i:
+                li *= (xg - z[k]) / (z[i] - z[k])
+        w[i] = np.sum(wg * li * (1 - xg))
+    return z, w
+
+
+def modified_b(nm, z):
+    """B[i, mode] in the (p outer, q inner) packing, closed forms with scipy Jacobi."""
+    cols = []
+    a, b = 0.5 * (1 - z), 0.5 * (1 + z)
+    for p in range(nm):
+        for q in range(nm - p):
+            if p == 0:
+                v = a if q == 0 else b if q == 1 else a * b * eval_jacobi(q - 2, 1, 1, z)
+            elif p == 1:
+                v = a if q == 0 else a * b * eval_jacobi(q - 1, 1, 1, z)
+            else:
+                v = a ** p if q == 0 else a ** p * b * eval_jacobi(q - 1, 2 * p - 1, 1, z)
+            cols.append(v)
+    return np.array(cols).T
+
+
+def tri_element_mats(nm, nq0, nq1, verts):
+    """Dense triangle operators: B (Q x M, the collapsed vertex mode 1 includes phi_1(eta0)),
+    physical gradients G_c (Q x M) from an exact monomial fit in (xi0, xi1), quadrature
+    weights W (with the collapse factor 1/2), the constant |det A| and coordinates."""
+    z0, w0 = gll(nq0)
+    z1, w1 = gauss_radau_a1(nq1)
+    Ba, _ = modified_a(nm, z0)
+    Bb = modified_b(nm, z1)
+    E0 = np.repeat(z0[None, :], nq1, 0).ravel()        # point (j, i): i fastest
+    E1 = np.repeat(z1[:, None], nq0, 1).ravel()
+    I0 = np.tile(np.arange(nq0), nq1)
+    I1 = np.repeat(np.arange(nq1), nq0)
+    M = nm * (nm + 1) // 2
+    B = np.zeros((nq0 * nq1, M))
+    mode = 0
+    for p in range(nm):
+        for q in range(nm - p):
+            B[:, mode] = Ba[I0, p] * Bb[I1, mode]
+            mode += 1
+    B[:, 1] += Ba[I0, 1] * Bb[I1, 1]
+    xi0 = 0.5 * (1 + E0) * (1 - E1) - 1.0
+    xi1 = E1
+    pw = [(a, c) for a in range(nm) for c in range(nm - a)]
+    V = np.stack([xi0 ** a * xi1 ** c for a, c in pw], 1)
+    coef = np.linalg.lstsq(V, B, rcond=None)[0]
+    dV0 = np.stack([a * xi0 ** max(a - 1, 0) * xi1 ** c for a, c in pw], 1)
+    dV1 = np.stack([c * xi0 ** a * xi1 ** max(c - 1, 0) for a, c in pw], 1)
+    G = [dV0 @ coef, dV1 @ coef]
+    v = np.asarray(verts, dtype=float).reshape(3, 2)
+    A = np.stack([0.5 * (v[1] - v[0]), 0.5 * (v[2] - v[0])], 1)   # A[c, a] = dx_c / dxi_a
+    Ainv = np.linalg.inv(A)
+    Gp = [Ainv[0, c] * G[0] + Ainv[1, c] * G[1] for c in range(2)]
+    X = np.stack([-(xi0 + xi1) / 2 * v[0][c] + (1 + xi0) / 2 * v[1][c] + (1 + xi1) / 2 * v[2][c]
+                  for c in range(2)])
+    W = 0.5 * w0[I0] * w1[I1]
+    return dict(B=B, Gphys=Gp, W=W, J=abs(np.linalg.det(A)), X=X, z0=z0, z1=z1, w0=w0,
+                w1_raw=w1, fit_residual=np.abs(V @ coef - B).max())
+
+
+def tri_helmholtz_matrix(m, lam):
+    WJ = m["W"] * m["J"]
+    H = lam * m["B"].T @ (WJ[:, None] * m["B"])
+    for g in m["Gphys"]:
+        H += g.T @ (WJ[:, None] * g)
+    return H

@@ -61,6 +61,18 @@ enum imf_precision { IMF_FP64 = 0, IMF_FP32 = 1 };
  * StdRegions matrix path.  Here they get runtime-size kernels (2 <= nm[d] <= nq[d] <= 16),
  * reading SetJac / SetDF arrays over nq[0] nq[1] nq[2] points per element; the coefficient
  * layout is (r*nm[1] + q)*nm[0] + p and the point layout (k*nq[1] + j)*nq[0] + i.
+ * Triangles, "{BwdTrans,IProduct,Helmholtz,PhysDeriv}_Tri_{Regular,Deformed}" (FP64;
+ * BwdTrans.h:333-470, IProductTri.cpp, Helmholtz.h:288-620, PhysDeriv.h; kernels
+ * BwdTransKernels.hpp:77-126, IProductKernels.hpp:134-236, PhysDerivKernels.hpp:150-216):
+ * dim = 2, nm[0] = nm[1] modes (nm(nm+1)/2 coefficients, p-major with q fastest), nq[0]
+ * GLL points along eta0 and nq[1] Gauss-Radau (alpha = 1) points along eta1 (the
+ * reference requires nq[1] = nq[0] - 1; here 2 <= nm <= 8, nm <= nq[0] <= 16,
+ * nm - 1 <= nq[1] <= 16).  Direction 0 takes the Modified_A tables of
+ * imf_basis_modified_a, direction 1 the Modified_B tables of imf_basis_modified_b (w[1]
+ * the Gauss-Radau weights; the factor 1/2 of the collapse is applied here, as the
+ * reference's Helper does).  Points are ordered j*nq[0] + i (eta0 fastest); SetJac /
+ * SetDF take one value (4 factors) per element for _Regular, per point for _Deformed.
+ * Transposes: BwdTrans and IProduct (Euclidean), Helmholtz (symmetric).
  * Also accepted, with no MatrixFreeOps counterpart: "Mass_Hex_Deformed" and
  * "Mass_Hex_Regular", the elemental mass action out = B^T diag(w J) B in of
  * StdRegions/StdExpansion.cpp:647-661 (MassMatrixOp_MatFree: BwdTrans, then
@@ -226,6 +238,13 @@ const char *imf_assembly_last_error(void);
  * D[nq*nq], bdata[nm*nq], dbdata[nm*nq]. */
 int imf_basis_modified_a(int nm, int nq, double *z, double *w, double *D,
                          double *bdata, double *dbdata);
+/* LibUtilities::Basis::GenBasis for eModified_B on nq Gauss-Radau-Jacobi points with
+ * z = -1 and weight (1 - z) (eGaussRadauMAlpha1Beta0; Basis.cpp:423-509, Polylib.cpp
+ * zwgrjm): the collapsed direction of a triangle.  Outputs z[nq], w[nq] (the quadrature
+ * weights of the points, without the factor 1/2 of the collapse), D[nq*nq],
+ * bdata[nm(nm+1)/2 * nq] and dbdata (modes packed p-major, q fastest). */
+int imf_basis_modified_b(int nm, int nq, double *z, double *w, double *D,
+                         double *bdata, double *dbdata);
 
 /* GeomFactors::ComputeDerivFactors (SpatialDomains/GeomFactors.cpp:399-478) for
  * straight-sided (trilinear hex / bilinear quad) elements, evaluated on the GPU.
@@ -235,6 +254,12 @@ int imf_geom_hex_device(const double *d_verts, int nElmt, int nq, const double *
                         int deformed, double *d_jac, double *d_df, void *stream);
 int imf_geom_quad_device(const double *d_verts, int nElmt, int nq, const double *z,
                          int deformed, double *d_jac, double *d_df, void *stream);
+/* Straight-sided triangles (TriGeom, SpatialDomains/TriGeom.cpp; GeomFactors.cpp:420-474):
+ * d_verts nElmt x 3 x 2 in the element's vertex order; the map is linear, so the factors
+ * are constant: one value per element (the reference's eRegular factors), or nq0*nq1
+ * copies when deformed != 0.  Layouts as above (4 factors, d = c*2 + a). */
+int imf_geom_tri_device(const double *d_verts, int nElmt, int nq0, int nq1, int deformed,
+                        double *d_jac, double *d_df, void *stream);
 /* The hex factors on nq[d] points z[d] (host) along direction d; points ordered
  * (k*nq[1] + j)*nq[0] + i.  For the operators with per-direction points. */
 int imf_geom_hex_aniso_device(const double *d_verts, int nElmt, const int *nq,

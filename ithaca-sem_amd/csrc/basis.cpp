@@ -1,9 +1,10 @@
-// Host-side 1D basis tables: Gauss-Lobatto-Legendre points/weights, the collocation
-// derivative matrix and the Modified_A (C0 hierarchical) basis.
+// Host-side 1D basis tables: Gauss-Lobatto-Legendre and Gauss-Radau-Jacobi points and
+// weights, collocation derivative matrices, and the Modified_A / Modified_B (C0
+// hierarchical, collapsed triangle direction) bases.
 //
-// Restates LibUtilities/Polylib/Polylib.cpp (jacobfd, Jacobz, zwglj :195-232,
-// Dglj :690-730) and LibUtilities/Foundations/Basis.cpp:392-421 (eModified_A) for
-// alpha = beta = 0, which is what the operators need at set-up time.
+// Restates LibUtilities/Polylib/Polylib.cpp (jacobfd, Jacobz, zwglj :195-232, zwgrjm
+// :119-144, Dglj :690-730) and LibUtilities/Foundations/Basis.cpp:392-509 (eModified_A,
+// eModified_B), which is what the operators need at set-up time.
 #include <cmath>
 #include <vector>
 
@@ -126,6 +127,82 @@ extern "C" int imf_basis_modified_a(int nm, int nq, double *z, double *w, double
             double s = 0.0;
             for (int k = 0; k < nq; ++k) s += D[k * nq + i] * bdata[p * nq + k];
             dbdata[p * nq + i] = s;
+        }
+    return IMF_OK;
+}
+
+extern "C" int imf_basis_modified_b(int nm, int nq, double *z, double *w, double *D,
+                                    double *bdata, double *dbdata)
+{
+    if (nm < 2 || nq < 1 || nq > 64 || !z || !w || !D || !bdata || !dbdata) return IMF_ERR_ARG;
+    // Gauss-Radau-Jacobi with z = -1, weight (1 - z): -1 and the zeros of P_{nq-1}^{(1,1)};
+    // w_i = 2 / (nq (nq + 1)) (1 - z_i) / P_{nq-1}^{(1,0)}(z_i)^2 (beta = 0: no end
+    // factor)
+    if (nq == 1) {
+        z[0] = 0.0;
+        w[0] = 2.0;
+    } else {
+        z[0] = -1.0;
+        jacobz(nq - 1, z + 1, 1.0, 1.0);
+        jacobfd(nq, z, w, nullptr, nq - 1, 1.0, 0.0);
+        const double fac = 2.0 / (double(nq) * double(nq + 1));
+        for (int i = 0; i < nq; ++i) w[i] = fac * (1.0 - z[i]) / (w[i] * w[i]);
+    }
+    // derivative matrix of the Lagrange basis through z, D[a*nq + b] = l_a'(z_b), from the
+    // barycentric weights; each l' row sums to zero
+    std::vector<double> bw(nq);
+    for (int a = 0; a < nq; ++a) {
+        double p = 1.0;
+        for (int c = 0; c < nq; ++c)
+            if (c != a) p *= z[a] - z[c];
+        bw[a] = 1.0 / p;
+    }
+    for (int b = 0; b < nq; ++b) {
+        double diag = 0.0;
+        for (int a = 0; a < nq; ++a) {
+            if (a == b) continue;
+            const double v = (bw[a] / bw[b]) / (z[b] - z[a]);
+            D[a * nq + b] = v;
+            diag -= v;
+        }
+        D[b * nq + b] = diag;
+    }
+    // Modified_B, modes packed p-major (q fastest):
+    //   p = 0: (1-z)/2, (1+z)/2, (1-z)/2 (1+z)/2 P_{q-2}^{(1,1)}
+    //   p = 1: (1-z)/2, (1-z)/2 (1+z)/2 P_{q-1}^{(1,1)}
+    //   p >= 2: ((1-z)/2)^p, ((1-z)/2)^p (1+z)/2 P_{q-1}^{(2p-1,1)}
+    std::vector<double> a(nq), bp(nq), pw(nq), P(nq);
+    for (int i = 0; i < nq; ++i) {
+        a[i] = 0.5 * (1.0 - z[i]);
+        bp[i] = 0.5 * (1.0 + z[i]);
+    }
+    int mode = 0;
+    for (int p = 0; p < nm; ++p) {
+        if (p >= 2)
+            for (int i = 0; i < nq; ++i) pw[i] = p == 2 ? a[i] * a[i] : pw[i] * a[i];
+        for (int q = 0; q < nm - p; ++q, ++mode) {
+            double *out = bdata + (size_t)mode * nq;
+            if (p == 0 && q < 2) {
+                for (int i = 0; i < nq; ++i) out[i] = q == 0 ? a[i] : bp[i];
+            } else if (p == 1 && q == 0) {
+                for (int i = 0; i < nq; ++i) out[i] = a[i];
+            } else if (p >= 2 && q == 0) {
+                for (int i = 0; i < nq; ++i) out[i] = pw[i];
+            } else {
+                const int n = p == 0 ? q - 2 : q - 1;
+                const double alpha = p <= 1 ? 1.0 : 2.0 * p - 1.0;
+                jacobfd(nq, z, P.data(), nullptr, n, alpha, 1.0);
+                for (int i = 0; i < nq; ++i)
+                    out[i] = P[i] * (p <= 1 ? a[i] : pw[i]) * bp[i];
+            }
+        }
+    }
+    const int M = nm * (nm + 1) / 2;
+    for (int m = 0; m < M; ++m)
+        for (int i = 0; i < nq; ++i) {
+            double sum = 0.0;
+            for (int k = 0; k < nq; ++k) sum += D[k * nq + i] * bdata[m * nq + k];
+            dbdata[m * nq + i] = sum;
         }
     return IMF_OK;
 }

@@ -62,7 +62,7 @@ int maybe_pack(imf_op op)
     if (!needs_jac(op) && !needs_df(op)) return IMF_OK;
     if (needs_jac(op) && !op->d_jac) return IMF_OK;
     if (needs_df(op) && !op->d_df) return IMF_OK;
-    if (op->aniso) {  // the per-direction kernels read GetJac / GetDerivFactors directly
+    if (op->aniso || op->tri) {  // these kernels read GetJac / GetDerivFactors directly
         static std::atomic<unsigned long> next_id{1ul << 62};
         op->geom_ready = true;
         op->geom_version = next_id.fetch_add(1);
@@ -127,6 +127,11 @@ int check_op(const imf_op op)
     return IMF_OK;
 }
 
+// Stands for "the transpose" in run() for triangles, whose kernels take the direction as
+// an argument instead of being separate launch functions; never called.
+hipError_t tri_transpose_fn(const imf::LaunchArgs &) { return hipErrorInvalidValue; }
+imf::LaunchFn tri_transpose_marker() { return &tri_transpose_fn; }
+
 int run(imf_op op, imf::LaunchFn fn, const void *in, void *out, void *stream, int nvec = 1,
         long ldin = 0, long ldout = 0, const int *gmap = nullptr,
         const signed char *gsign = nullptr)
@@ -136,6 +141,26 @@ int run(imf_op op, imf::LaunchFn fn, const void *in, void *out, void *stream, in
                                    : needs_df(op) ? "Jacobian and derivative factors not set"
                                                   : "Jacobian not set");
     if (!in || !out) return fail(IMF_ERR_ARG, "null input/output array");
+    if (op->tri) {
+        if (nvec != 1 || gmap) return fail(IMF_ERR_UNSUPPORTED, "triangles: one vector per apply");
+        imf::TriArgs t{};
+        t.kind = op->kind;
+        t.transpose = fn == tri_transpose_marker() ? 1 : 0;
+        t.geo = op->kind == imf::kBwdTrans ? imf::kUnit : (op->deformed ? imf::kDeformed : imf::kRegular);
+        t.nm = op->nmd[0];
+        t.nq0 = op->nqd[0];
+        t.nq1 = op->nqd[1];
+        t.nElmt = op->nElmt;
+        t.lambda = op->lambda;
+        t.tab = op->d_atab;
+        t.in = (const double *)in;
+        t.out = (double *)out;
+        t.jac = op->d_jac;
+        t.df = op->d_df;
+        hipError_t e = imf::launch_tri(t, (hipStream_t)stream);
+        if (e != hipSuccess) return hip_fail(e, "kernel launch");
+        return IMF_OK;
+    }
     if (op->aniso) {
         if (gmap) return fail(IMF_ERR_UNSUPPORTED, "gathered input: isotropic operators only");
         hipError_t e = imf::launch_aniso(aniso_args(op, in, out, nvec, ldin, ldout),
@@ -197,6 +222,77 @@ int create_aniso(const std::string &k, int kind, int shape, int deformed, const 
         (void)hipFree(op->d_atab);
         delete op;
         return hip_fail(e, "anisotropic basis tables");
+    }
+    *out = op;
+    return IMF_OK;
+}
+
+// Triangles: Modified_A x Modified_B in collapsed coordinates (mf_tri.hip), FP64.
+// Direction 0: bdata/dbdata nm*nq0, D nq0^2, w nq0 (GLL); direction 1: bdata/dbdata
+// nm(nm+1)/2 * nq1, D nq1^2, w nq1 (the Gauss-Radau alpha = 1 weights of the basis; the
+// factor 1/2 of the collapse is applied here, as the reference's Helper does).  The points
+// themselves (needed by the metric and PhysDeriv) are regenerated for GLL / Gauss-Radau.
+int create_tri(const std::string &k, int kind, int deformed, const int *nm, const int *nq,
+               const double *const *bdata, const double *const *dbdata,
+               const double *const *D, const double *const *w, int nElmt, int precision,
+               imf_op *out)
+{
+    if (precision != IMF_FP64 ||
+        (kind != imf::kBwdTrans && kind != imf::kIProduct && kind != imf::kHelmholtz &&
+         kind != imf::kPhysDeriv))
+        return fail(IMF_ERR_UNSUPPORTED, k + ": triangles are supported for FP64 BwdTrans, "
+                                             "IProduct, Helmholtz and PhysDeriv");
+    const int m = nm[0], q0 = nq[0], q1 = nq[1];
+    if (m < 2 || m > 8 || q0 < 2 || q1 < 1 || q0 > 16 || q1 > 16 ||
+        (kind != imf::kPhysDeriv && (nm[1] != m || q0 < m || q1 < m - 1)))
+        return fail(IMF_ERR_UNSUPPORTED, k + ": # of modes / points combo not implemented.");
+    const bool helm = kind == imf::kHelmholtz;
+    if (!D[0] || !D[1] || !w[0] || !w[1] ||
+        (kind != imf::kPhysDeriv && (!bdata[0] || !bdata[1])) ||
+        (helm && (!dbdata || !dbdata[0] || !dbdata[1])))
+        return fail(IMF_ERR_ARG, k + ": missing basis tables");
+    const int M = imf::tri_modes(m);
+    std::vector<double> z0(q0), z1(q1), scratch;
+    {
+        std::vector<double> ww(16), dd(256), bb(16 * 16 * 16), db(16 * 16 * 16);
+        if (imf_basis_modified_a(2, q0, z0.data(), ww.data(), dd.data(), bb.data(), db.data()) ||
+            imf_basis_modified_b(2, q1, z1.data(), ww.data(), dd.data(), bb.data(), db.data()))
+            return fail(IMF_ERR_ARG, k + ": points");
+    }
+    std::vector<double> tab;
+    tab.reserve(imf::tri_table_len(m, q0, q1));
+    auto put = [&](const double *p, size_t n) {
+        if (p) tab.insert(tab.end(), p, p + n);
+        else tab.insert(tab.end(), n, 0.0);
+    };
+    const bool tabs = kind != imf::kPhysDeriv;
+    put(tabs ? bdata[0] : nullptr, (size_t)m * q0);
+    put(helm ? dbdata[0] : nullptr, (size_t)m * q0);
+    put(D[0], (size_t)q0 * q0);
+    put(w[0], q0);
+    put(z0.data(), q0);
+    put(tabs ? bdata[1] : nullptr, (size_t)M * q1);
+    put(helm ? dbdata[1] : nullptr, (size_t)M * q1);
+    put(D[1], (size_t)q1 * q1);
+    for (int j = 0; j < q1; ++j) tab.push_back(0.5 * w[1][j]);
+    put(z1.data(), q1);
+    imf_op op = new imf_op_s();
+    op->kind = kind; op->shape = imf::kTri; op->deformed = deformed; op->prec = precision;
+    op->dim = 2; op->nm = m; op->nq = q0; op->nElmt = nElmt;
+    op->tri = true;
+    op->nmd[0] = op->nmd[1] = m;
+    op->nqd[0] = q0;
+    op->nqd[1] = q1;
+    op->M = (size_t)M;
+    op->Q = (size_t)q0 * q1;
+    op->w.assign(w[0], w[0] + q0);
+    hipError_t e = hipMalloc(&op->d_atab, tab.size() * sizeof(double));
+    if (e == hipSuccess)
+        e = hipMemcpy(op->d_atab, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        (void)hipFree(op->d_atab);
+        delete op;
+        return hip_fail(e, "triangle basis tables");
     }
     *out = op;
     return IMF_OK;
@@ -267,12 +363,23 @@ int imf_create(const char *key, int dim, const int *nm, const int *nq,
     else if (rest == "Quad_Deformed") { shape = imf::kQuad; deformed = 1; }
     else if (rest == "Hex_Regular") { shape = imf::kHex; deformed = 0; }
     else if (rest == "Hex_Deformed") { shape = imf::kHex; deformed = 1; }
+    else if (rest == "Tri_Regular") { shape = imf::kTri; deformed = 0; }
+    else if (rest == "Tri_Deformed") { shape = imf::kTri; deformed = 1; }
     else return fail(IMF_ERR_UNSUPPORTED, "No such operator key: " + k);
     // The reference registers BwdTrans only as _Regular (MatrixFreeOps/BwdTrans.cpp:7-33).
     if (kind == imf::kBwdTrans && deformed)
         return fail(IMF_ERR_UNSUPPORTED, "No such operator key: " + k);
     if (kind == imf::kMass && shape != imf::kHex)
         return fail(IMF_ERR_UNSUPPORTED, "No such operator key: " + k);
+    if (shape == imf::kTri) {
+        if (dim != 2) return fail(IMF_ERR_ARG, "imf_create: dim does not match the shape of " + k);
+        if (kind == imf::kIProdDeriv)
+            return fail(IMF_ERR_UNSUPPORTED, "No such operator key: " + k);
+        if (precision != IMF_FP64 && precision != IMF_FP32)
+            return fail(IMF_ERR_ARG, "imf_create: unknown precision");
+        if (nElmt <= 0) return fail(IMF_ERR_ARG, "imf_create: nElmt must be positive");
+        return create_tri(k, kind, deformed, nm, nq, bdata, dbdata, D, w, nElmt, precision, out);
+    }
     if (dim != shape)
         return fail(IMF_ERR_ARG, "imf_create: dim does not match the shape of " + k);
     if (precision != IMF_FP64 && precision != IMF_FP32)
@@ -417,6 +524,11 @@ int imf_apply_transpose_device(imf_op op, const void *d_in, void *d_out, void *s
     if (int s = check_op(op)) return s;
     if (op->aniso && (op->kind == imf::kHelmholtz || op->kind == imf::kMass))
         return run(op, op->fwd, d_in, d_out, stream);
+    if (op->tri) {
+        if (op->kind == imf::kHelmholtz) return run(op, op->fwd, d_in, d_out, stream);
+        if (op->kind == imf::kPhysDeriv) return fail(IMF_ERR_UNSUPPORTED, "transpose not available");
+        return run(op, tri_transpose_marker(), d_in, d_out, stream);
+    }
     if (!op->tr) return fail(IMF_ERR_UNSUPPORTED, "transpose not available");
     if (op->kind == imf::kHelmholtz || op->kind == imf::kMass)
         return run(op, op->fwd, d_in, d_out, stream);
@@ -496,6 +608,16 @@ int imf_geom_quad_device(const double *d_verts, int nElmt, int nq, const double 
     hipError_t e = imf::geom_from_vertices(imf::kQuad, d_verts, nElmt, nq, z, deformed, d_jac,
                                            d_df, (hipStream_t)stream);
     return e == hipSuccess ? IMF_OK : hip_fail(e, "geom_quad");
+}
+
+int imf_geom_tri_device(const double *d_verts, int nElmt, int nq0, int nq1, int deformed,
+                        double *d_jac, double *d_df, void *stream)
+{
+    if (!d_verts || !d_jac || !d_df || nElmt <= 0 || nq0 < 1 || nq1 < 1 || nq0 > 64 || nq1 > 64)
+        return fail(IMF_ERR_ARG, "imf_geom_tri_device: bad argument");
+    hipError_t e = imf::geom_tri(d_verts, nElmt, nq0 * nq1, deformed, d_jac, d_df,
+                                 (hipStream_t)stream);
+    return e == hipSuccess ? IMF_OK : hip_fail(e, "geom_tri");
 }
 
 }  // extern "C"

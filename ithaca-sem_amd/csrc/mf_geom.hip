@@ -310,6 +310,38 @@ __global__ void geom_quad_kernel(const double *__restrict__ verts, long nElmt, i
     for (int d = 0; d < 4; ++d) df[(long)d * n + g] = fac[d];
 }
 
+// Straight-sided triangles (TriGeom: x(xi) = -(xi0 + xi1)/2 v0 + (1 + xi0)/2 v1 +
+// (1 + xi1)/2 v2): a constant map derivative, so Q copies per element when deformed.
+__global__ void geom_tri_kernel(const double *__restrict__ verts, long nElmt, int Q, int deformed,
+                                double *__restrict__ jac, double *__restrict__ df)
+{
+    const int per = deformed ? Q : 1;
+    const long n = nElmt * per;
+    const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n) return;
+    const long e = g / per;
+    const double *v = verts + e * 6;
+    double deriv[4];
+    for (int c = 0; c < 2; ++c) {
+        deriv[0 * 2 + c] = 0.5 * (v[2 + c] - v[c]);
+        deriv[1 * 2 + c] = 0.5 * (v[4 + c] - v[c]);
+    }
+    double J, fac[4];
+    factors_from_deriv(2, deriv, &J, fac);
+    jac[g] = J;
+    for (int d = 0; d < 4; ++d) df[(long)d * n + g] = fac[d];
+}
+
+hipError_t geom_tri(const double *d_verts, long nElmt, int Q, int deformed, double *d_jac,
+                    double *d_df, hipStream_t s)
+{
+    const long n = nElmt * (deformed ? Q : 1);
+    const int bs = 256;
+    geom_tri_kernel<<<(unsigned)((n + bs - 1) / bs), bs, 0, s>>>(d_verts, nElmt, Q, deformed,
+                                                                 d_jac, d_df);
+    return hipGetLastError();
+}
+
 hipError_t geom_from_vertices(int shape, const double *d_verts, long nElmt, int nq,
                               const double *z, int deformed, double *d_jac, double *d_df,
                               hipStream_t s)

@@ -22,4 +22,8 @@ hipError_t geom_hex_aniso(const double *d_verts, long nElmt, const int *nq,
                           const double *const *z, int deformed, double *d_jac, double *d_df,
                           hipStream_t s);
 
+// Straight-sided triangles (vertices nElmt x 3 x 2), Q points per element when deformed.
+hipError_t geom_tri(const double *d_verts, long nElmt, int Q, int deformed, double *d_jac,
+                    double *d_df, hipStream_t s);
+
 }  // namespace imf

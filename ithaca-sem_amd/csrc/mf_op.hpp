@@ -6,6 +6,7 @@
 
 #include "imf_common.hpp"
 #include "mf_aniso.hpp"
+#include "mf_tri.hpp"
 
 struct imf_op_s {
     int kind = 0, shape = 0, deformed = 0, prec = 0, dim = 0, nm = 0, nq = 0;
@@ -29,6 +30,9 @@ struct imf_op_s {
     int nmd[3] = {0, 0, 0}, nqd[3] = {0, 0, 0};
     double *d_atab = nullptr;
     int bufLen = 0;
+    // triangles (mf_tri.hip, FP64): nmd[0] modes, nqd[0] / nqd[1] points along eta0 / eta1,
+    // the table image in d_atab; geometry read unpacked from d_jac / d_df
+    bool tri = false;
 };
 
 // Kernel arguments of an anisotropic operator's apply (in, out, nvec vectors at ld strides).

@@ -58,6 +58,8 @@ def lib():
         L.imf_geom_hex_device.argtypes = [vp, i, i, vp, i, vp, vp, vp]
         L.imf_geom_quad_device.argtypes = [vp, i, i, vp, i, vp, vp, vp]
         L.imf_geom_hex_aniso_device.argtypes = [vp, i, vp, vp, i, vp, vp, vp]
+        L.imf_basis_modified_b.argtypes = [i, i, vp, vp, vp, vp, vp]
+        L.imf_geom_tri_device.argtypes = [vp, i, i, i, i, vp, vp, vp]
         _LIB = L
     return _LIB
 
@@ -87,6 +89,18 @@ def basis_modified_a(nm, nq):
     return dict(z=z, w=w, D=D, bdata=b, dbdata=db, nm=nm, nq=nq)
 
 
+def basis_modified_b(nm, nq):
+    """Modified_B basis on nq Gauss-Radau (alpha = 1, beta = 0) points (Basis.cpp:423-509),
+    computed by the library: the collapsed direction of a triangle.  w are the Gauss-Radau
+    weights (without the collapse factor 1/2); bdata has nm(nm+1)/2 modes."""
+    M = nm * (nm + 1) // 2
+    z, w = np.zeros(nq), np.zeros(nq)
+    D, b, db = np.zeros(nq * nq), np.zeros(M * nq), np.zeros(M * nq)
+    _check(lib().imf_basis_modified_b(nm, nq, _np_ptr(z), _np_ptr(w), _np_ptr(D), _np_ptr(b),
+                                      _np_ptr(db)))
+    return dict(z=z, w=w, D=D, bdata=b, dbdata=db, nm=nm, nq=nq)
+
+
 def geom_device(shape, verts, nq, z, deformed=True, stream=None):
     """Geometric factors of straight-sided elements on the GPU (GeomFactors.cpp:399-478).
 
@@ -94,6 +108,15 @@ def geom_device(shape, verts, nq, z, deformed=True, stream=None):
     Returns (jac, df) cuda tensors in the GetJac / GetDerivFactors layouts."""
     verts = verts.contiguous()
     E = verts.shape[0]
+    if shape == "tri":  # (nElmt, 3, 2) vertices; nq = (nq0, nq1); z unused (linear map)
+        nq0, nq1 = (int(nq[0]), int(nq[1])) if isinstance(nq, (tuple, list)) else (nq, nq - 1)
+        npts = E * (nq0 * nq1 if deformed else 1)
+        jac = torch.empty(npts, dtype=torch.float64, device=verts.device)
+        df = torch.empty(4 * npts, dtype=torch.float64, device=verts.device)
+        _check(lib().imf_geom_tri_device(ctypes.c_void_p(verts.data_ptr()), E, nq0, nq1,
+                                         int(deformed), ctypes.c_void_p(jac.data_ptr()),
+                                         ctypes.c_void_p(df.data_ptr()), _stream_ptr(stream)))
+        return jac, df
     dim = 3 if shape == "hex" else 2
     if isinstance(nq, (tuple, list)):  # hexes with nq[d] points z[d] along direction d
         if dim != 3 or len(nq) != 3 or len(z) != 3:
@@ -126,13 +149,21 @@ class Operator:
 
     def __init__(self, key, nm, nq, nElmt, precision=FP64, basis=None):
         """nm, nq: the modes and points of every direction, or (hexes) tuples of the three
-        directions' counts with basis a list of the three directions' tables."""
+        directions' counts with basis a list of the three directions' tables.  Triangles
+        ("*_Tri_*"): nm modes, nq = nq0 (with nq1 = nq0 - 1) or (nq0, nq1); basis the
+        Modified_A and Modified_B tables of the two directions."""
         self.key = key
         shape = key.split("_")[1]
         self.dim = 3 if shape == "Hex" else 2
         self.nm, self.nq, self.nElmt, self.precision = nm, nq, nElmt, precision
         per_dir = isinstance(nm, (tuple, list))
-        if per_dir:
+        if shape == "Tri":
+            nq0, nq1 = (int(nq[0]), int(nq[1])) if isinstance(nq, (tuple, list)) else (nq, nq - 1)
+            nms, nqs = [nm, nm], [nq0, nq1]
+            if basis is None:
+                basis = [basis_modified_a(nm, nq0), basis_modified_b(nm, nq1)]
+            bases = list(basis)
+        elif per_dir:
             nms, nqs = [int(x) for x in nm], [int(x) for x in nq]
             if basis is None:
                 basis = [basis_modified_a(a, b) for a, b in zip(nms, nqs)]

@@ -47,6 +47,19 @@ def test_basis_helper_matches_oracle():
             assert np.allclose(a, b[k], atol=1e-14 * scale, rtol=1e-13), k
 
 
+def test_modified_b_helper_matches_oracle():
+    lib = ctypes.CDLL(LIB)
+    for nm, nq in [(2, 1), (2, 2), (6, 6), (4, 7), (8, 15)]:
+        M = nm * (nm + 1) // 2
+        arrs = [np.zeros(nq), np.zeros(nq), np.zeros(nq * nq), np.zeros(M * nq), np.zeros(M * nq)]
+        st = lib.imf_basis_modified_b(nm, nq, *[a.ctypes.data_as(ctypes.c_void_p) for a in arrs])
+        assert st == 0
+        b = O.basis_tri(nm, nq + 1, nq)
+        for a, k in zip(arrs, ["z1", "w1_raw", "D1", "bdata1", "dbdata1"]):
+            scale = max(1.0, np.abs(b["D1"]).max())
+            assert np.allclose(a, b[k], atol=1e-14 * scale, rtol=1e-13), (nm, nq, k)
+
+
 def test_create_rejects_bad_requests_without_gpu():
     lib = ctypes.CDLL(LIB)
     lib.imf_last_error.restype = ctypes.c_char_p

@@ -23,7 +23,6 @@
 
 #include <algorithm>
 #include <cmath>
-#include <cstdlib>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -513,10 +512,6 @@ inline EigWs eig_ws(int n, void *base)
 }  // namespace irom
 
 namespace {
-const int g_inner = [] {
-    const char *v = getenv("IROM_EIG_INNER");
-    return v ? std::max(1, atoi(v)) : 1;
-}();
 thread_local std::string g_eerr;
 int efail(int c, const std::string &m)
 {
@@ -555,7 +550,9 @@ int eig_run(int n, const double *d_G, double *h_evals, double *d_evecs, double t
     irom::eig_init<<<(unsigned)((tot + 255) / 256), 256, 0, s>>>(n, p.np, d_G, w.W, w.V, w.ints,
                                                                  irom::EIG_MAX_SWEEPS + 4);
     IROM_ECHECK(hipGetLastError(), "eig_init");
-    irom::EigArgs a{n, p.np, p.nb, tol, max_sweeps, g_inner, d_G, w.W, w.V, rot,
+    // one inner sweep per round: more inner sweeps saved 1-2 outer sweeps of 13 but cost
+    // more than they saved
+    irom::EigArgs a{n, p.np, p.nb, tol, max_sweeps, 1, d_G, w.W, w.V, rot,
                     (unsigned *)bar, sw_d, err_d};
     IROM_ECHECK(launch_gram_jacobi(a, ncu, s), "gram_jacobi launch");
     irom::column_norms<<<p.np, irom::ET, 0, s>>>(p.np, w.W, w.nrm);

@@ -110,7 +110,11 @@ int imf_set_df_device(imf_op op, const double *d_df, size_t n);
 int imf_set_lambda(imf_op op, double lambda);
 
 /* operator()(in, out) (Operator.hpp:70-72, 99-101, 176-178) on host arrays: copies in,
- * applies, copies out; synchronous.  out is overwritten.  FP64 operators only. */
+ * applies, copies out; synchronous.  out is overwritten.  FP64 operators only.
+ * BwdTrans, IProduct, Helmholtz and Mass on quads and hexes run in sixteen element chunks
+ * through pinned staging buffers (allocated on the first call, kept with the operator),
+ * so the host copies, the transfers each way and the kernels of neighbouring chunks
+ * overlap; the result is bitwise that of imf_apply_device. */
 int imf_apply(imf_op op, const double *in, double *out);
 
 /* operator()(in, out) on device arrays of the operator's precision (double for

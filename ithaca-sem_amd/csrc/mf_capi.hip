@@ -4,10 +4,15 @@
 // GetDerivFactors arrays, Collections/CoalescedGeomData.cpp:53-112, :251-313) and the
 // packed streamed form derived from them, plus the 1D basis tables as a kernel-argument
 // image.  Applies on device arrays allocate nothing.
+#include <algorithm>
 #include <atomic>
+#include <condition_variable>
 #include <cstdio>
 #include <cstring>
+#include <functional>
+#include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/ithaca_mf.h"
@@ -449,6 +454,14 @@ int imf_destroy(imf_op op)
     (void)hipFree(op->d_in);
     (void)hipFree(op->d_out);
     (void)hipFree(op->d_atab);
+    for (int b = 0; b < 2; ++b) {
+        (void)hipHostFree(op->pin_in[b]);
+        (void)hipHostFree(op->pin_out[b]);
+        if (op->ev_h2d[b]) (void)hipEventDestroy(op->ev_h2d[b]);
+        if (op->ev_kern[b]) (void)hipEventDestroy(op->ev_kern[b]);
+        if (op->ev_d2h[b]) (void)hipEventDestroy(op->ev_d2h[b]);
+        if (op->hs[b]) (void)hipStreamDestroy(op->hs[b]);
+    }
     delete op;
     return IMF_OK;
 }
@@ -555,6 +568,174 @@ int imf_apply_many_device(imf_op op, int nvec, const void *d_in, long ld_in, voi
     return IMF_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+// Host memcpy on a small persistent pool of threads (pageable user arrays to and from the
+// pinned staging buffers): one thread copies at about a third of what the PCIe link
+// moves, so the copies are split over the pool.
+class CopyPool {
+public:
+    static CopyPool &get()
+    {
+        static CopyPool p;
+        return p;
+    }
+    void copy(void *dst, const void *src, size_t bytes)
+    {
+        const int n = (int)workers_.size() + 1;
+        if (bytes < (1u << 20) || n == 1) {
+            std::memcpy(dst, src, bytes);
+            return;
+        }
+        const size_t part = (bytes / n + 4095) / 4096 * 4096;
+        std::unique_lock<std::mutex> lk(mu_);
+        dst_ = (char *)dst;
+        src_ = (const char *)src;
+        bytes_ = bytes;
+        part_ = part;
+        pending_ = n - 1;
+        ++gen_;
+        cv_.notify_all();
+        lk.unlock();
+        chunk(0);
+        lk.lock();
+        done_.wait(lk, [&] { return pending_ == 0; });
+    }
+
+private:
+    CopyPool()
+    {
+        const unsigned hc = std::thread::hardware_concurrency();
+        const int nw = (int)std::min(15u, hc > 1 ? hc - 1 : 0u);
+        for (int i = 0; i < nw; ++i) workers_.emplace_back([this, i] { run(i + 1); });
+    }
+    ~CopyPool()
+    {
+        {
+            std::lock_guard<std::mutex> lk(mu_);
+            stop_ = true;
+            ++gen_;
+        }
+        cv_.notify_all();
+        for (auto &t : workers_) t.join();
+    }
+    void chunk(int k)
+    {
+        const size_t b = std::min(bytes_, (size_t)k * part_), e = std::min(bytes_, b + part_);
+        if (e > b) std::memcpy(dst_ + b, src_ + b, e - b);
+    }
+    void run(int k)
+    {
+        unsigned long seen = 0;
+        for (;;) {
+            std::unique_lock<std::mutex> lk(mu_);
+            cv_.wait(lk, [&] { return gen_ != seen; });
+            seen = gen_;
+            if (stop_) return;
+            lk.unlock();
+            chunk(k);
+            lk.lock();
+            if (--pending_ == 0) done_.notify_one();
+        }
+    }
+    std::vector<std::thread> workers_;
+    std::mutex mu_;
+    std::condition_variable cv_, done_;
+    char *dst_ = nullptr;
+    const char *src_ = nullptr;
+    size_t bytes_ = 0, part_ = 0;
+    int pending_ = 0;
+    unsigned long gen_ = 0;
+    bool stop_ = false;
+};
+
+// Element-major operators (inputs, outputs and packed geometry laid out element by
+// element) can be applied to a range of elements by offsetting the three pointers.
+bool element_major(const imf_op op)
+{
+    return !op->aniso && !op->tri && op->fwd && op->prec == IMF_FP64 &&
+           (op->kind == imf::kBwdTrans || op->kind == imf::kIProduct ||
+            op->kind == imf::kHelmholtz || op->kind == imf::kMass);
+}
+
+// imf_apply for element-major operators: the elements in NC chunks through two pinned
+// buffers each way; chunk c's host copy into pinned memory, its H2D copy and kernel (one
+// stream) and its D2H copy (a second stream) overlap the neighbouring chunks' steps.
+int apply_pipelined(imf_op op, const double *in, double *out)
+{
+    const size_t nin = imf_in_size(op), nout = imf_out_size(op);
+    const size_t ine = nin / op->nElmt, oute = nout / op->nElmt;
+    const size_t gstride = (needs_jac(op) || needs_df(op)) ? geom_len(op) / op->nElmt : 0;
+    constexpr int NC = 16;
+    const long per = (op->nElmt + NC - 1) / NC;
+    const size_t need = (size_t)per * std::max(ine, oute) * sizeof(double);
+    hipError_t e = hipSuccess;
+    if (op->pin_bytes < need) {
+        for (int b = 0; b < 2; ++b) {
+            (void)hipHostFree(op->pin_in[b]);
+            (void)hipHostFree(op->pin_out[b]);
+            op->pin_in[b] = op->pin_out[b] = nullptr;
+        }
+        op->pin_bytes = 0;
+        for (int b = 0; b < 2 && e == hipSuccess; ++b) {
+            e = hipHostMalloc(&op->pin_in[b], need, hipHostMallocDefault);
+            if (e == hipSuccess) e = hipHostMalloc(&op->pin_out[b], need, hipHostMallocDefault);
+        }
+        if (e != hipSuccess) return hip_fail(e, "hipHostMalloc(staging)");
+        op->pin_bytes = need;
+    }
+    if (!op->hs[0]) {
+        for (int i = 0; i < 2 && e == hipSuccess; ++i) e = hipStreamCreateWithFlags(&op->hs[i], hipStreamNonBlocking);
+        for (int b = 0; b < 2 && e == hipSuccess; ++b) {
+            e = hipEventCreateWithFlags(&op->ev_h2d[b], hipEventDisableTiming);
+            if (e == hipSuccess) e = hipEventCreateWithFlags(&op->ev_kern[b], hipEventDisableTiming);
+            if (e == hipSuccess) e = hipEventCreateWithFlags(&op->ev_d2h[b], hipEventDisableTiming);
+        }
+        if (e != hipSuccess) return hip_fail(e, "streams / events");
+    }
+    // the streams are non-blocking: finish what earlier calls queued (e.g. the geometry pack)
+    if ((e = hipDeviceSynchronize()) != hipSuccess) return hip_fail(e, "synchronize");
+    CopyPool &pool = CopyPool::get();
+    const int nc = (int)((op->nElmt + per - 1) / per);
+    for (int c = 0; c <= nc; ++c) {
+        if (c < nc) {
+            const int b = c & 1;
+            const long e0 = (long)c * per, ne = std::min<long>(per, op->nElmt - e0);
+            if (c >= 2 && (e = hipEventSynchronize(op->ev_h2d[b])) != hipSuccess)
+                return hip_fail(e, "staging");
+            pool.copy(op->pin_in[b], in + e0 * ine, ne * ine * sizeof(double));
+            double *din = (double *)op->d_in + e0 * ine, *dout = (double *)op->d_out + e0 * oute;
+            if ((e = hipMemcpyAsync(din, op->pin_in[b], ne * ine * sizeof(double),
+                                    hipMemcpyHostToDevice, op->hs[0])) != hipSuccess ||
+                (e = hipEventRecord(op->ev_h2d[b], op->hs[0])) != hipSuccess)
+                return hip_fail(e, "H2D");
+            const void *geom = gstride ? (const void *)((const double *)op->d_geom + e0 * gstride)
+                                       : (const void *)op->d_geom;
+            imf::LaunchArgs a{op->tab.data(), (int)ne, op->lambda, geom, din, dout, op->hs[0]};
+            if ((e = op->fwd(a)) != hipSuccess) return hip_fail(e, "kernel launch");
+            if ((e = hipEventRecord(op->ev_kern[b], op->hs[0])) != hipSuccess ||
+                (e = hipStreamWaitEvent(op->hs[1], op->ev_kern[b], 0)) != hipSuccess ||
+                (e = hipMemcpyAsync(op->pin_out[b], dout, ne * oute * sizeof(double),
+                                    hipMemcpyDeviceToHost, op->hs[1])) != hipSuccess ||
+                (e = hipEventRecord(op->ev_d2h[b], op->hs[1])) != hipSuccess)
+                return hip_fail(e, "D2H");
+        }
+        if (c >= 1) {  // the previous chunk's result to the caller
+            const int b = (c - 1) & 1;
+            const long e0 = (long)(c - 1) * per, ne = std::min<long>(per, op->nElmt - e0);
+            if ((e = hipEventSynchronize(op->ev_d2h[b])) != hipSuccess) return hip_fail(e, "D2H");
+            pool.copy(out + e0 * oute, op->pin_out[b], ne * oute * sizeof(double));
+        }
+    }
+    return IMF_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
 int imf_apply(imf_op op, const double *in, double *out)
 {
     if (int s = check_op(op)) return s;
@@ -567,6 +748,13 @@ int imf_apply(imf_op op, const double *in, double *out)
             return hip_fail(e, "hipMalloc(staging)");
         if ((e = hipMalloc(&op->d_out, nout * sizeof(double))) != hipSuccess)
             return hip_fail(e, "hipMalloc(staging)");
+    }
+    if (element_major(op)) {
+        if ((needs_jac(op) || needs_df(op)) && !op->geom_ready)
+            return fail(IMF_ERR_STATE, !needs_jac(op) ? "derivative factors not set"
+                                       : needs_df(op) ? "Jacobian and derivative factors not set"
+                                                      : "Jacobian not set");
+        return apply_pipelined(op, in, out);
     }
     if ((e = hipMemcpy(op->d_in, in, nin * sizeof(double), hipMemcpyHostToDevice)) != hipSuccess)
         return hip_fail(e, "hipMemcpy(in)");

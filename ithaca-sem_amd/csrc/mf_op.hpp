@@ -22,6 +22,12 @@ struct imf_op_s {
     bool geom_ready = false;
     unsigned long geom_version = 0;        // process-wide unique id of the current geometry pack
     void *d_in = nullptr, *d_out = nullptr;  // staging for host applies
+    // pipelined host applies: two pinned buffers each way, a copy and a compute stream
+    void *pin_in[2] = {nullptr, nullptr}, *pin_out[2] = {nullptr, nullptr};
+    size_t pin_bytes = 0;
+    hipStream_t hs[2] = {nullptr, nullptr};
+    hipEvent_t ev_h2d[2] = {nullptr, nullptr}, ev_kern[2] = {nullptr, nullptr},
+               ev_d2h[2] = {nullptr, nullptr};
     imf::LaunchFn fwd = nullptr, tr = nullptr;
     // hexahedra with different modes / points per direction (mf_hex_aniso.hip, FP64):
     // per-direction sizes, the device table image (B, D, w of the three directions) and

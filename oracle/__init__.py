@@ -226,3 +226,77 @@ def helmholtz_tri(b, E, deformed, jac, df, lam, u):
                            int(deformed), _p(jac), _p(df), ctypes.c_double(lam), _p(u),
                            _p(out))
     return out
+
+
+# ---- prisms (mf_oracle_prism.c) ----
+
+def basis_prism(nm, nq, nq2=None):
+    """Modified_A on nq GLL points along eta0 and xi1, Modified_B on nq2 Gauss-Radau
+    (alpha = 1) points along eta2 (default nq - 1, as MatrixFree requires); w2 is the
+    Helper's 0.5-scaled weight."""
+    nq2 = nq - 1 if nq2 is None else nq2
+    a = basis(nm, nq)
+    t = basis_tri(nm, nq, nq2)
+    return dict(nm=nm, nq0=nq, nq1=nq, nq2=nq2, M=nm * nm * (nm + 1) // 2, Q=nq * nq * nq2,
+                z0=a["z"], w0=a["w"], D0=a["D"], bdata0=a["bdata"], dbdata0=a["dbdata"],
+                z1=a["z"], w1=a["w"], D1=a["D"], bdata1=a["bdata"], dbdata1=a["dbdata"],
+                z2=t["z1"], w2_raw=t["w1_raw"], w2=t["w1"], D2=t["D1"], bdata2=t["bdata1"],
+                dbdata2=t["dbdata1"])
+
+
+def geom_prism(verts, b, deformed=True):
+    """Straight-sided prisms (E x 6 x 3 vertices in StdPrismExp order)."""
+    verts = np.ascontiguousarray(verts, dtype=np.float64)
+    E = verts.shape[0]
+    n = E * b["Q"] if deformed else E
+    jac = np.zeros(n); df = np.zeros(9 * n)
+    lib().or_geom_prism(_p(verts), E, b["nq0"], b["nq1"], b["nq2"], _p(b["z0"]), _p(b["z1"]),
+                        _p(b["z2"]), int(deformed), _p(jac), _p(df))
+    return jac, df
+
+
+def coords_prism(verts, b):
+    verts = np.ascontiguousarray(verts, dtype=np.float64)
+    E = verts.shape[0]
+    xyz = np.zeros(3 * E * b["Q"])
+    lib().or_coords_prism(_p(verts), E, b["nq0"], b["nq1"], b["nq2"], _p(b["z0"]), _p(b["z1"]),
+                          _p(b["z2"]), _p(xyz))
+    return xyz.reshape(3, -1)
+
+
+def bwdtrans_prism(b, E, u):
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    out = np.zeros(E * b["Q"])
+    lib().or_bwdtrans_prism(b["nm"], b["nq0"], b["nq1"], b["nq2"], _p(b["bdata0"]),
+                            _p(b["bdata1"]), _p(b["bdata2"]), E, _p(u), _p(out))
+    return out
+
+
+def iproduct_prism(b, E, deformed, jac, f):
+    f = np.ascontiguousarray(f, dtype=np.float64)
+    out = np.zeros(E * b["M"])
+    lib().or_iproduct_prism(b["nm"], b["nq0"], b["nq1"], b["nq2"], _p(b["bdata0"]),
+                            _p(b["bdata1"]), _p(b["bdata2"]), _p(b["w0"]), _p(b["w1"]),
+                            _p(b["w2"]), E, int(deformed), _p(jac), _p(f), _p(out))
+    return out
+
+
+def physderiv_prism(b, E, deformed, df, f):
+    f = np.ascontiguousarray(f, dtype=np.float64)
+    outs = [np.zeros(E * b["Q"]) for _ in range(3)]
+    lib().or_physderiv_prism(b["nq0"], b["nq1"], b["nq2"], _p(b["D0"]), _p(b["D1"]), _p(b["D2"]),
+                             _p(b["z0"]), _p(b["z2"]), E, int(deformed), _p(df), _p(f),
+                             *[_p(o) for o in outs])
+    return outs
+
+
+def helmholtz_prism(b, E, deformed, jac, df, lam, u):
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    out = np.zeros(E * b["M"])
+    lib().or_helmholtz_prism(b["nm"], b["nq0"], b["nq1"], b["nq2"], _p(b["bdata0"]),
+                             _p(b["dbdata0"]), _p(b["bdata1"]), _p(b["dbdata1"]),
+                             _p(b["bdata2"]), _p(b["dbdata2"]), _p(b["D0"]), _p(b["D1"]),
+                             _p(b["D2"]), _p(b["z0"]), _p(b["z2"]), _p(b["w0"]), _p(b["w1"]),
+                             _p(b["w2"]), E, int(deformed), _p(jac), _p(df),
+                             ctypes.c_double(lam), _p(u), _p(out))
+    return out

@@ -106,6 +106,28 @@ void or_helmholtz_tri(int nm, int nq0, int nq1, const double *b0, const double *
                       const double *jac, const double *df, double lambda,
                       const double *in, double *out);
 
+/* ---- Prisms (mf_oracle_prism.c): Modified_A x Modified_A x Modified_B ---- */
+void or_geom_prism(const double *verts, int E, int nq0, int nq1, int nq2, const double *z0,
+                   const double *z1, const double *z2, int deformed, double *jac, double *df);
+void or_coords_prism(const double *verts, int E, int nq0, int nq1, int nq2, const double *z0,
+                     const double *z1, const double *z2, double *xyz);
+void or_bwdtrans_prism(int nm, int nq0, int nq1, int nq2, const double *b0, const double *b1,
+                       const double *b2, int E, const double *in, double *out);
+void or_iproduct_prism(int nm, int nq0, int nq1, int nq2, const double *b0, const double *b1,
+                       const double *b2, const double *w0, const double *w1, const double *w2,
+                       int E, int deformed, const double *jac, const double *in, double *out);
+void or_physderiv_prism(int nq0, int nq1, int nq2, const double *D0, const double *D1,
+                        const double *D2, const double *z0, const double *z2, int E,
+                        int deformed, const double *df, const double *in, double *out0,
+                        double *out1, double *out2);
+void or_helmholtz_prism(int nm, int nq0, int nq1, int nq2, const double *b0, const double *db0,
+                        const double *b1, const double *db1, const double *b2,
+                        const double *db2, const double *D0, const double *D1,
+                        const double *D2, const double *z0, const double *z2,
+                        const double *w0, const double *w1, const double *w2, int E,
+                        int deformed, const double *jac, const double *df, double lambda,
+                        const double *in, double *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -193,3 +193,65 @@ def tri_helmholtz_matrix(m, lam):
     for g in m["Gphys"]:
         H += g.T @ (WJ[:, None] * g)
     return H
+
+
+# ---- prisms: Modified_A (eta0) x Modified_A (xi1) x Modified_B (eta2) ----
+
+def prism_element_mats(nm, nq, nq2, verts):
+    """Dense prism operators (points (k nq + j) nq + i): B with the singular-edge modes
+    (0, q, 1) also carrying phi_1(eta0), physical gradients from an exact monomial fit in
+    (xi0, xi1, xi2) (degree <= nm-1 in (xi0, xi2) jointly and in xi1), weights with the
+    collapse factor 1/2, and |det| of the map Jacobian per point."""
+    z0, w0 = gll(nq)
+    z2, w2 = gauss_radau_a1(nq2)
+    Ba, _ = modified_a(nm, z0)
+    Bb = modified_b(nm, z2)
+    I0 = np.tile(np.arange(nq), nq * nq2)
+    I1 = np.tile(np.repeat(np.arange(nq), nq), nq2)
+    I2 = np.repeat(np.arange(nq2), nq * nq)
+    M = nm * nm * (nm + 1) // 2
+    B = np.zeros((nq * nq * nq2, M))
+    mode, mpr = 0, 0
+    for p in range(nm):
+        for q in range(nm):
+            for r in range(nm - p):
+                B[:, mode] = Ba[I0, p] * Ba[I1, q] * Bb[I2, mpr + r]
+                mode += 1
+        mpr += nm - p
+    for q in range(nm):
+        B[:, q * nm + 1] += Ba[I0, 1] * Ba[I1, q] * Bb[I2, 1]
+    e0, xi1, xi2 = z0[I0], z0[I1], z2[I2]
+    xi0 = 0.5 * (1 + e0) * (1 - xi2) - 1.0
+    pw = [(a, b, c) for a in range(nm) for c in range(nm - a) for b in range(nm)]
+    V = np.stack([xi0 ** a * xi1 ** b * xi2 ** c for a, b, c in pw], 1)
+    coef = np.linalg.lstsq(V, B, rcond=None)[0]
+    dV = [np.stack([a * xi0 ** max(a - 1, 0) * xi1 ** b * xi2 ** c for a, b, c in pw], 1),
+          np.stack([b * xi0 ** a * xi1 ** max(b - 1, 0) * xi2 ** c for a, b, c in pw], 1),
+          np.stack([c * xi0 ** a * xi1 ** b * xi2 ** max(c - 1, 0) for a, b, c in pw], 1)]
+    G = [d @ coef for d in dV]
+    v = np.asarray(verts, dtype=float).reshape(6, 3)
+    la, lb, lc = -(xi0 + xi2) / 2, (1 + xi0) / 2, (1 + xi2) / 2
+    sl, sh = (1 - xi1) / 2, (1 + xi1) / 2
+    N = [la * sl, lb * sl, lb * sh, la * sh, lc * sl, lc * sh]
+    dN = [(-0.5 * sl, -0.5 * la, -0.5 * sl), (0.5 * sl, -0.5 * lb, 0 * sl),
+          (0.5 * sh, 0.5 * lb, 0 * sl), (-0.5 * sh, 0.5 * la, -0.5 * sh),
+          (0 * sl, -0.5 * lc, 0.5 * sl), (0 * sl, 0.5 * lc, 0.5 * sh)]
+    X = np.stack([sum(N[n] * v[n][c] for n in range(6)) for c in range(3)])
+    J, Gp = [], [np.zeros_like(B) for _ in range(3)]
+    for t in range(len(xi0)):
+        A = np.array([[sum(dN[n][a][t] * v[n][c] for n in range(6)) for a in range(3)]
+                      for c in range(3)])          # A[c, a] = dx_c / dxi_a
+        J.append(abs(np.linalg.det(A)))
+        Ai = np.linalg.inv(A)
+        for c in range(3):
+            Gp[c][t] = sum(Ai[a, c] * G[a][t] for a in range(3))
+    W = w0[I0] * w0[I1] * 0.5 * w2[I2]
+    return dict(B=B, Gphys=Gp, W=W, J=np.array(J), X=X, fit_residual=np.abs(V @ coef - B).max())
+
+
+def prism_helmholtz_matrix(m, lam):
+    WJ = m["W"] * m["J"]
+    H = lam * m["B"].T @ (WJ[:, None] * m["B"])
+    for g in m["Gphys"]:
+        H += g.T @ (WJ[:, None] * g)
+    return H

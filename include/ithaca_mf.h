@@ -73,6 +73,18 @@ enum imf_precision { IMF_FP64 = 0, IMF_FP32 = 1 };
  * reference's Helper does).  Points are ordered j*nq[0] + i (eta0 fastest); SetJac /
  * SetDF take one value (4 factors) per element for _Regular, per point for _Deformed.
  * Transposes: BwdTrans and IProduct (Euclidean), Helmholtz (symmetric).
+ * Prisms, "{BwdTrans,IProduct,Helmholtz,PhysDeriv}_Prism_{Regular,Deformed}" (FP64;
+ * BwdTrans.cpp:31-33, IProductPrism.cpp, Helmholtz.cpp:25-29, PhysDeriv.cpp:38-42; kernels
+ * BwdTransKernels.hpp:224-300, IProductKernels.hpp:315-449, PhysDerivKernels.hpp:219-294,
+ * Helmholtz.h:1291-1460): dim = 3, nm[0] = nm[1] = nm[2] modes (nm*nm(nm+1)/2 coefficients,
+ * mode (p, q, r) with r < nm - p at nm*sum_{p'<p}(nm - p') + q(nm - p) + r), nq[0], nq[1]
+ * GLL points along eta0, xi1 and nq[2] Gauss-Radau (alpha = 1) points along eta2 (the
+ * reference requires nq[2] = nq[0] - 1; here 2 <= nm <= 8, nm <= nq[0], nq[1] <= 16,
+ * nm - 1 <= nq[2] <= 16, and one element's arrays must fit the LDS of a workgroup).
+ * Directions 0 and 1 take Modified_A tables, direction 2 the Modified_B tables of
+ * imf_basis_modified_b (w[2] without the collapse factor 1/2, applied here).  Points are
+ * ordered (k*nq[1] + j)*nq[0] + i; SetJac / SetDF take one value (9 factors) per element
+ * for _Regular, per point for _Deformed; transposes as for triangles.
  * Also accepted, with no MatrixFreeOps counterpart: "Mass_Hex_Deformed" and
  * "Mass_Hex_Regular", the elemental mass action out = B^T diag(w J) B in of
  * StdRegions/StdExpansion.cpp:647-661 (MassMatrixOp_MatFree: BwdTrans, then
@@ -264,6 +276,14 @@ int imf_geom_quad_device(const double *d_verts, int nElmt, int nq, const double 
  * copies when deformed != 0.  Layouts as above (4 factors, d = c*2 + a). */
 int imf_geom_tri_device(const double *d_verts, int nElmt, int nq0, int nq1, int deformed,
                         double *d_jac, double *d_df, void *stream);
+/* Straight-sided prisms (PrismGeom, SpatialDomains/PrismGeom.cpp; GeomFactors.cpp:420-474):
+ * d_verts nElmt x 6 x 3 in StdPrismExp vertex order: 0..3 the xi2 = -1 quad at (xi0, xi1)
+ * = (-1,-1), (1,-1), (1,1), (-1,1); 4 and 5 the collapsed edge at xi2 = 1, xi1 = -1 and 1.
+ * The map is not affine, so deformed != 0 gives factors at each of the nq0*nq1*nq2 points
+ * (nq0, nq1 GLL, nq2 Gauss-Radau, as the prism operators use); deformed = 0 gives one
+ * value per element, at point 0.  Layouts as above (9 factors, d = c*3 + a). */
+int imf_geom_prism_device(const double *d_verts, int nElmt, int nq0, int nq1, int nq2,
+                          int deformed, double *d_jac, double *d_df, void *stream);
 /* The hex factors on nq[d] points z[d] (host) along direction d; points ordered
  * (k*nq[1] + j)*nq[0] + i.  For the operators with per-direction points. */
 int imf_geom_hex_aniso_device(const double *d_verts, int nElmt, const int *nq,

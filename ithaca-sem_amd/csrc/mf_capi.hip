@@ -67,7 +67,7 @@ int maybe_pack(imf_op op)
     if (!needs_jac(op) && !needs_df(op)) return IMF_OK;
     if (needs_jac(op) && !op->d_jac) return IMF_OK;
     if (needs_df(op) && !op->d_df) return IMF_OK;
-    if (op->aniso || op->tri) {  // these kernels read GetJac / GetDerivFactors directly
+    if (op->aniso || op->collapsed) {  // these kernels read GetJac / GetDerivFactors directly
         static std::atomic<unsigned long> next_id{1ul << 62};
         op->geom_ready = true;
         op->geom_version = next_id.fetch_add(1);
@@ -132,8 +132,8 @@ int check_op(const imf_op op)
     return IMF_OK;
 }
 
-// Stands for "the transpose" in run() for triangles, whose kernels take the direction as
-// an argument instead of being separate launch functions; never called.
+// Stands for "the transpose" in run() for triangles and prisms, whose kernels take the
+// direction as an argument instead of being separate launch functions; never called.
 hipError_t tri_transpose_fn(const imf::LaunchArgs &) { return hipErrorInvalidValue; }
 imf::LaunchFn tri_transpose_marker() { return &tri_transpose_fn; }
 
@@ -146,7 +146,28 @@ int run(imf_op op, imf::LaunchFn fn, const void *in, void *out, void *stream, in
                                    : needs_df(op) ? "Jacobian and derivative factors not set"
                                                   : "Jacobian not set");
     if (!in || !out) return fail(IMF_ERR_ARG, "null input/output array");
-    if (op->tri) {
+    if (op->collapsed && op->shape == imf::kPrism) {
+        if (nvec != 1 || gmap) return fail(IMF_ERR_UNSUPPORTED, "prisms: one vector per apply");
+        imf::PrismArgs t{};
+        t.kind = op->kind;
+        t.transpose = fn == tri_transpose_marker() ? 1 : 0;
+        t.geo = op->kind == imf::kBwdTrans ? imf::kUnit : (op->deformed ? imf::kDeformed : imf::kRegular);
+        t.nm = op->nmd[0];
+        t.nq0 = op->nqd[0];
+        t.nq1 = op->nqd[1];
+        t.nq2 = op->nqd[2];
+        t.nElmt = op->nElmt;
+        t.lambda = op->lambda;
+        t.tab = op->d_atab;
+        t.in = (const double *)in;
+        t.out = (double *)out;
+        t.jac = op->d_jac;
+        t.df = op->d_df;
+        hipError_t e = imf::launch_prism(t, (hipStream_t)stream);
+        if (e != hipSuccess) return hip_fail(e, "kernel launch");
+        return IMF_OK;
+    }
+    if (op->collapsed) {
         if (nvec != 1 || gmap) return fail(IMF_ERR_UNSUPPORTED, "triangles: one vector per apply");
         imf::TriArgs t{};
         t.kind = op->kind;
@@ -284,7 +305,7 @@ int create_tri(const std::string &k, int kind, int deformed, const int *nm, cons
     imf_op op = new imf_op_s();
     op->kind = kind; op->shape = imf::kTri; op->deformed = deformed; op->prec = precision;
     op->dim = 2; op->nm = m; op->nq = q0; op->nElmt = nElmt;
-    op->tri = true;
+    op->collapsed = true;
     op->nmd[0] = op->nmd[1] = m;
     op->nqd[0] = q0;
     op->nqd[1] = q1;
@@ -298,6 +319,92 @@ int create_tri(const std::string &k, int kind, int deformed, const int *nm, cons
         (void)hipFree(op->d_atab);
         delete op;
         return hip_fail(e, "triangle basis tables");
+    }
+    *out = op;
+    return IMF_OK;
+}
+
+// Prisms: Modified_A x Modified_A x Modified_B in collapsed coordinates (mf_prism.hip),
+// FP64.  Directions 0 and 1: bdata/dbdata nm*nq[d], D nq[d]^2, w nq[d] (GLL); direction 2:
+// bdata/dbdata nm(nm+1)/2 * nq[2], D nq[2]^2, w nq[2] (the Gauss-Radau alpha = 1 weights;
+// the factor 1/2 of the collapse is applied here).  The points are regenerated for GLL /
+// Gauss-Radau.  One workgroup holds an element's arrays in LDS, so the sizes are also
+// bounded by the device's LDS per workgroup.
+int create_prism(const std::string &k, int kind, int deformed, const int *nm, const int *nq,
+                 const double *const *bdata, const double *const *dbdata,
+                 const double *const *D, const double *const *w, int nElmt, int precision,
+                 imf_op *out)
+{
+    if (precision != IMF_FP64 ||
+        (kind != imf::kBwdTrans && kind != imf::kIProduct && kind != imf::kHelmholtz &&
+         kind != imf::kPhysDeriv))
+        return fail(IMF_ERR_UNSUPPORTED, k + ": prisms are supported for FP64 BwdTrans, "
+                                             "IProduct, Helmholtz and PhysDeriv");
+    const int m = nm[0], q0 = nq[0], q1 = nq[1], q2 = nq[2];
+    if (m < 2 || m > 8 || q0 < 2 || q1 < 2 || q2 < 1 || q0 > 16 || q1 > 16 || q2 > 16 ||
+        (kind != imf::kPhysDeriv &&
+         (nm[1] != m || nm[2] != m || q0 < m || q1 < m || q2 < m - 1)))
+        return fail(IMF_ERR_UNSUPPORTED, k + ": # of modes / points combo not implemented.");
+    int dev = 0, lds_max = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e == hipSuccess)
+        e = hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
+    if (e != hipSuccess) return hip_fail(e, "LDS size query");
+    if (sizeof(double) * (size_t)imf::prism_lds_len(m, q0, q1, q2) > (size_t)lds_max)
+        return fail(IMF_ERR_UNSUPPORTED, k + ": an element of this size does not fit the LDS "
+                                             "of one workgroup");
+    const bool helm = kind == imf::kHelmholtz, tabs = kind != imf::kPhysDeriv;
+    if (!D[0] || !D[1] || !D[2] || !w[0] || !w[1] || !w[2] ||
+        (tabs && (!bdata[0] || !bdata[1] || !bdata[2])) ||
+        (helm && (!dbdata || !dbdata[0] || !dbdata[1] || !dbdata[2])))
+        return fail(IMF_ERR_ARG, k + ": missing basis tables");
+    const int T = imf::tri_modes(m);
+    std::vector<double> z0(q0), z1(q1), z2(q2);
+    {
+        std::vector<double> ww(16), dd(256), bb(16 * 16 * 16), db(16 * 16 * 16);
+        if (imf_basis_modified_a(2, q0, z0.data(), ww.data(), dd.data(), bb.data(), db.data()) ||
+            imf_basis_modified_a(2, q1, z1.data(), ww.data(), dd.data(), bb.data(), db.data()) ||
+            imf_basis_modified_b(2, q2, z2.data(), ww.data(), dd.data(), bb.data(), db.data()))
+            return fail(IMF_ERR_ARG, k + ": points");
+    }
+    std::vector<double> tab;
+    tab.reserve(imf::prism_table_len(m, q0, q1, q2));
+    auto put = [&](const double *p, size_t n) {
+        if (p) tab.insert(tab.end(), p, p + n);
+        else tab.insert(tab.end(), n, 0.0);
+    };
+    const int qd[2] = {q0, q1};
+    const double *zd[2] = {z0.data(), z1.data()};
+    for (int d = 0; d < 2; ++d) {
+        put(tabs ? bdata[d] : nullptr, (size_t)m * qd[d]);
+        put(helm ? dbdata[d] : nullptr, (size_t)m * qd[d]);
+        put(D[d], (size_t)qd[d] * qd[d]);
+        put(w[d], qd[d]);
+        put(zd[d], qd[d]);
+    }
+    put(tabs ? bdata[2] : nullptr, (size_t)T * q2);
+    put(helm ? dbdata[2] : nullptr, (size_t)T * q2);
+    put(D[2], (size_t)q2 * q2);
+    for (int j = 0; j < q2; ++j) tab.push_back(0.5 * w[2][j]);
+    put(z2.data(), q2);
+    imf_op op = new imf_op_s();
+    op->kind = kind; op->shape = imf::kPrism; op->deformed = deformed; op->prec = precision;
+    op->dim = 3; op->nm = m; op->nq = q0; op->nElmt = nElmt;
+    op->collapsed = true;
+    op->nmd[0] = op->nmd[1] = op->nmd[2] = m;
+    op->nqd[0] = q0;
+    op->nqd[1] = q1;
+    op->nqd[2] = q2;
+    op->M = (size_t)imf::prism_modes(m);
+    op->Q = (size_t)q0 * q1 * q2;
+    op->w.assign(w[0], w[0] + q0);
+    e = hipMalloc(&op->d_atab, tab.size() * sizeof(double));
+    if (e == hipSuccess)
+        e = hipMemcpy(op->d_atab, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        (void)hipFree(op->d_atab);
+        delete op;
+        return hip_fail(e, "prism basis tables");
     }
     *out = op;
     return IMF_OK;
@@ -370,19 +477,25 @@ int imf_create(const char *key, int dim, const int *nm, const int *nq,
     else if (rest == "Hex_Deformed") { shape = imf::kHex; deformed = 1; }
     else if (rest == "Tri_Regular") { shape = imf::kTri; deformed = 0; }
     else if (rest == "Tri_Deformed") { shape = imf::kTri; deformed = 1; }
+    else if (rest == "Prism_Regular") { shape = imf::kPrism; deformed = 0; }
+    else if (rest == "Prism_Deformed") { shape = imf::kPrism; deformed = 1; }
     else return fail(IMF_ERR_UNSUPPORTED, "No such operator key: " + k);
     // The reference registers BwdTrans only as _Regular (MatrixFreeOps/BwdTrans.cpp:7-33).
     if (kind == imf::kBwdTrans && deformed)
         return fail(IMF_ERR_UNSUPPORTED, "No such operator key: " + k);
     if (kind == imf::kMass && shape != imf::kHex)
         return fail(IMF_ERR_UNSUPPORTED, "No such operator key: " + k);
-    if (shape == imf::kTri) {
-        if (dim != 2) return fail(IMF_ERR_ARG, "imf_create: dim does not match the shape of " + k);
+    if (shape == imf::kTri || shape == imf::kPrism) {
+        if (dim != (shape == imf::kTri ? 2 : 3))
+            return fail(IMF_ERR_ARG, "imf_create: dim does not match the shape of " + k);
         if (kind == imf::kIProdDeriv)
             return fail(IMF_ERR_UNSUPPORTED, "No such operator key: " + k);
         if (precision != IMF_FP64 && precision != IMF_FP32)
             return fail(IMF_ERR_ARG, "imf_create: unknown precision");
         if (nElmt <= 0) return fail(IMF_ERR_ARG, "imf_create: nElmt must be positive");
+        if (shape == imf::kPrism)
+            return create_prism(k, kind, deformed, nm, nq, bdata, dbdata, D, w, nElmt, precision,
+                                out);
         return create_tri(k, kind, deformed, nm, nq, bdata, dbdata, D, w, nElmt, precision, out);
     }
     if (dim != shape)
@@ -537,7 +650,7 @@ int imf_apply_transpose_device(imf_op op, const void *d_in, void *d_out, void *s
     if (int s = check_op(op)) return s;
     if (op->aniso && (op->kind == imf::kHelmholtz || op->kind == imf::kMass))
         return run(op, op->fwd, d_in, d_out, stream);
-    if (op->tri) {
+    if (op->collapsed) {
         if (op->kind == imf::kHelmholtz) return run(op, op->fwd, d_in, d_out, stream);
         if (op->kind == imf::kPhysDeriv) return fail(IMF_ERR_UNSUPPORTED, "transpose not available");
         return run(op, tri_transpose_marker(), d_in, d_out, stream);
@@ -655,7 +768,7 @@ private:
 // element) can be applied to a range of elements by offsetting the three pointers.
 bool element_major(const imf_op op)
 {
-    return !op->aniso && !op->tri && op->fwd && op->prec == IMF_FP64 &&
+    return !op->aniso && !op->collapsed && op->fwd && op->prec == IMF_FP64 &&
            (op->kind == imf::kBwdTrans || op->kind == imf::kIProduct ||
             op->kind == imf::kHelmholtz || op->kind == imf::kMass);
 }
@@ -806,6 +919,25 @@ int imf_geom_tri_device(const double *d_verts, int nElmt, int nq0, int nq1, int 
     hipError_t e = imf::geom_tri(d_verts, nElmt, nq0 * nq1, deformed, d_jac, d_df,
                                  (hipStream_t)stream);
     return e == hipSuccess ? IMF_OK : hip_fail(e, "geom_tri");
+}
+
+int imf_geom_prism_device(const double *d_verts, int nElmt, int nq0, int nq1, int nq2,
+                          int deformed, double *d_jac, double *d_df, void *stream)
+{
+    if (!d_verts || !d_jac || !d_df || nElmt <= 0 || nq0 < 2 || nq1 < 2 || nq2 < 1 ||
+        nq0 > 16 || nq1 > 16 || nq2 > 16)
+        return fail(IMF_ERR_ARG, "imf_geom_prism_device: bad argument");
+    // nm = 2: at most 3 * 16 table values per direction
+    std::vector<double> z0(nq0), z1(nq1), z2(nq2), ww(16), dd(256), bb(48), db(48);
+    if (imf_basis_modified_a(2, nq0, z0.data(), ww.data(), dd.data(), bb.data(), db.data()) ||
+        imf_basis_modified_a(2, nq1, z1.data(), ww.data(), dd.data(), bb.data(), db.data()) ||
+        imf_basis_modified_b(2, nq2, z2.data(), ww.data(), dd.data(), bb.data(), db.data()))
+        return fail(IMF_ERR_ARG, "imf_geom_prism_device: points");
+    const double *z[3] = {z0.data(), z1.data(), z2.data()};
+    const int nq[3] = {nq0, nq1, nq2};
+    hipError_t e = imf::geom_prism(d_verts, nElmt, nq, z, deformed, d_jac, d_df,
+                                   (hipStream_t)stream);
+    return e == hipSuccess ? IMF_OK : hip_fail(e, "geom_prism");
 }
 
 }  // extern "C"

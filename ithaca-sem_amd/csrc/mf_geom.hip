@@ -342,6 +342,53 @@ hipError_t geom_tri(const double *d_verts, long nElmt, int Q, int deformed, doub
     return hipGetLastError();
 }
 
+// Straight-sided prisms, vertices in StdPrismExp order: 0..3 the xi2 = -1 quad (xi0, xi1
+// = (-1,-1), (1,-1), (1,1), (-1,1)), 4 and 5 the collapsed edge at xi2 = 1 (xi1 = -1, 1).
+// The map is linear in each of (xi0, xi2) and xi1 but not affine overall, so deformed
+// factors vary over the points; regular ones are taken at point 0.  Point (i, j, k) is at
+// xi = ((1 + z0_i)(1 - z2_k)/2 - 1, z1_j, z2_k).
+__global__ void geom_prism_kernel(const double *__restrict__ verts, long nElmt, int nq0,
+                                  int nq1, int nq2, Pts z0, Pts z1, Pts z2, int deformed,
+                                  double *__restrict__ jac, double *__restrict__ df)
+{
+    const int Q = deformed ? nq0 * nq1 * nq2 : 1;
+    const long n = nElmt * Q;
+    const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n) return;
+    const long e = g / Q;
+    const int q = (int)(g - e * Q);
+    const int i = q % nq0, j = (q / nq0) % nq1, k = q / (nq0 * nq1);
+    const double x0 = 0.5 * (1.0 + z0.v[i]) * (1.0 - z2.v[k]) - 1.0, x1 = z1.v[j], x2 = z2.v[k];
+    // triangle (xi0, xi2) shape functions times the segment ones of xi1
+    const double la = -0.5 * (x0 + x2), lb = 0.5 * (1.0 + x0), lc = 0.5 * (1.0 + x2);
+    const double sl = 0.5 * (1.0 - x1), sh = 0.5 * (1.0 + x1);
+    const double dN[6][3] = {{-0.5 * sl, -0.5 * la, -0.5 * sl}, {0.5 * sl, -0.5 * lb, 0.0},
+                             {0.5 * sh, 0.5 * lb, 0.0},         {-0.5 * sh, 0.5 * la, -0.5 * sh},
+                             {0.0, -0.5 * lc, 0.5 * sl},        {0.0, 0.5 * lc, 0.5 * sh}};
+    const double *v = verts + e * 18;
+    double deriv[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int nv = 0; nv < 6; ++nv)
+        for (int a = 0; a < 3; ++a)
+            for (int c = 0; c < 3; ++c) deriv[a * 3 + c] += dN[nv][a] * v[nv * 3 + c];
+    double J, fac[9];
+    factors_from_deriv(3, deriv, &J, fac);
+    jac[g] = J;
+    for (int d = 0; d < 9; ++d) df[(long)d * n + g] = fac[d];
+}
+
+hipError_t geom_prism(const double *d_verts, long nElmt, const int *nq, const double *const *z,
+                      int deformed, double *d_jac, double *d_df, hipStream_t s)
+{
+    Pts pz[3] = {};
+    for (int d = 0; d < 3; ++d)
+        for (int i = 0; i < nq[d] && i < 16; ++i) pz[d].v[i] = z[d][i];
+    const long n = nElmt * (deformed ? (long)nq[0] * nq[1] * nq[2] : 1);
+    const int bs = 256;
+    geom_prism_kernel<<<(unsigned)((n + bs - 1) / bs), bs, 0, s>>>(
+        d_verts, nElmt, nq[0], nq[1], nq[2], pz[0], pz[1], pz[2], deformed, d_jac, d_df);
+    return hipGetLastError();
+}
+
 hipError_t geom_from_vertices(int shape, const double *d_verts, long nElmt, int nq,
                               const double *z, int deformed, double *d_jac, double *d_df,
                               hipStream_t s)

@@ -6,6 +6,7 @@
 
 #include "imf_common.hpp"
 #include "mf_aniso.hpp"
+#include "mf_prism.hpp"
 #include "mf_tri.hpp"
 
 struct imf_op_s {
@@ -36,9 +37,10 @@ struct imf_op_s {
     int nmd[3] = {0, 0, 0}, nqd[3] = {0, 0, 0};
     double *d_atab = nullptr;
     int bufLen = 0;
-    // triangles (mf_tri.hip, FP64): nmd[0] modes, nqd[0] / nqd[1] points along eta0 / eta1,
-    // the table image in d_atab; geometry read unpacked from d_jac / d_df
-    bool tri = false;
+    // collapsed-coordinate shapes (FP64): triangles (mf_tri.hip; nqd[0] / nqd[1] points along
+    // eta0 / eta1) and prisms (mf_prism.hip; nqd[0..2] along eta0 / xi1 / eta2), nmd[0]
+    // modes per direction, the table image in d_atab; geometry read unpacked from d_jac / d_df
+    bool collapsed = false;
 };
 
 // Kernel arguments of an anisotropic operator's apply (in, out, nvec vectors at ld strides).

@@ -60,6 +60,7 @@ def lib():
         L.imf_geom_hex_aniso_device.argtypes = [vp, i, vp, vp, i, vp, vp, vp]
         L.imf_basis_modified_b.argtypes = [i, i, vp, vp, vp, vp, vp]
         L.imf_geom_tri_device.argtypes = [vp, i, i, i, i, vp, vp, vp]
+        L.imf_geom_prism_device.argtypes = [vp, i, i, i, i, i, vp, vp, vp]
         _LIB = L
     return _LIB
 
@@ -104,7 +105,8 @@ def basis_modified_b(nm, nq):
 def geom_device(shape, verts, nq, z, deformed=True, stream=None):
     """Geometric factors of straight-sided elements on the GPU (GeomFactors.cpp:399-478).
 
-    verts: cuda float64 tensor (nElmt, 8, 3) for hexes or (nElmt, 4, 2) for quads.
+    verts: cuda float64 tensor (nElmt, 8, 3) for hexes, (nElmt, 4, 2) for quads, (nElmt, 3, 2)
+    for triangles or (nElmt, 6, 3) for prisms (collapsed shapes: nq the per-direction counts).
     Returns (jac, df) cuda tensors in the GetJac / GetDerivFactors layouts."""
     verts = verts.contiguous()
     E = verts.shape[0]
@@ -116,6 +118,16 @@ def geom_device(shape, verts, nq, z, deformed=True, stream=None):
         _check(lib().imf_geom_tri_device(ctypes.c_void_p(verts.data_ptr()), E, nq0, nq1,
                                          int(deformed), ctypes.c_void_p(jac.data_ptr()),
                                          ctypes.c_void_p(df.data_ptr()), _stream_ptr(stream)))
+        return jac, df
+    if shape == "prism":  # (nElmt, 6, 3) vertices; nq = (nq0, nq1, nq2); z unused
+        nq0, nq1, nq2 = ([int(x) for x in nq] if isinstance(nq, (tuple, list))
+                         else (nq, nq, nq - 1))
+        npts = E * (nq0 * nq1 * nq2 if deformed else 1)
+        jac = torch.empty(npts, dtype=torch.float64, device=verts.device)
+        df = torch.empty(9 * npts, dtype=torch.float64, device=verts.device)
+        _check(lib().imf_geom_prism_device(ctypes.c_void_p(verts.data_ptr()), E, nq0, nq1, nq2,
+                                           int(deformed), ctypes.c_void_p(jac.data_ptr()),
+                                           ctypes.c_void_p(df.data_ptr()), _stream_ptr(stream)))
         return jac, df
     dim = 3 if shape == "hex" else 2
     if isinstance(nq, (tuple, list)):  # hexes with nq[d] points z[d] along direction d
@@ -151,10 +163,12 @@ class Operator:
         """nm, nq: the modes and points of every direction, or (hexes) tuples of the three
         directions' counts with basis a list of the three directions' tables.  Triangles
         ("*_Tri_*"): nm modes, nq = nq0 (with nq1 = nq0 - 1) or (nq0, nq1); basis the
-        Modified_A and Modified_B tables of the two directions."""
+        Modified_A and Modified_B tables of the two directions.  Prisms ("*_Prism_*"): nm
+        modes, nq = nq0 (with nq1 = nq0, nq2 = nq0 - 1) or (nq0, nq1, nq2); basis the
+        Modified_A, Modified_A and Modified_B tables of the three directions."""
         self.key = key
         shape = key.split("_")[1]
-        self.dim = 3 if shape == "Hex" else 2
+        self.dim = 3 if shape in ("Hex", "Prism") else 2
         self.nm, self.nq, self.nElmt, self.precision = nm, nq, nElmt, precision
         per_dir = isinstance(nm, (tuple, list))
         if shape == "Tri":
@@ -162,6 +176,13 @@ class Operator:
             nms, nqs = [nm, nm], [nq0, nq1]
             if basis is None:
                 basis = [basis_modified_a(nm, nq0), basis_modified_b(nm, nq1)]
+            bases = list(basis)
+        elif shape == "Prism":
+            nqs = ([int(x) for x in nq] if isinstance(nq, (tuple, list)) else [nq, nq, nq - 1])
+            nms = [nm] * 3
+            if basis is None:
+                basis = [basis_modified_a(nm, nqs[0]), basis_modified_a(nm, nqs[1]),
+                         basis_modified_b(nm, nqs[2])]
             bases = list(basis)
         elif per_dir:
             nms, nqs = [int(x) for x in nm], [int(x) for x in nq]

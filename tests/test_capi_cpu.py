@@ -69,9 +69,16 @@ def test_create_rejects_bad_requests_without_gpu():
     dp = ctypes.POINTER(ctypes.c_double)
     arr = (dp * 3)(*([b.ctypes.data_as(dp)] * 3))
     h = ctypes.c_void_p()
-    st = lib.imf_create(b"Helmholtz_Prism_Deformed", 3, nm, nq, arr, arr, arr, arr, 8, 0,
+    st = lib.imf_create(b"Helmholtz_Tet_Deformed", 3, nm, nq, arr, arr, arr, arr, 8, 0,
                         ctypes.byref(h))
     assert st == 2 and b"No such operator key" in lib.imf_last_error()
+    nm9 = (ctypes.c_int * 3)(9, 9, 9)
+    st = lib.imf_create(b"Helmholtz_Prism_Deformed", 3, nm9, nq, arr, arr, arr, arr, 8, 0,
+                        ctypes.byref(h))
+    assert st == 2 and b"combo not implemented" in lib.imf_last_error()
+    st = lib.imf_create(b"Helmholtz_Prism_Deformed", 2, nm, nq, arr, arr, arr, arr, 8, 0,
+                        ctypes.byref(h))
+    assert st == 1 and b"dim does not match" in lib.imf_last_error()
     nq13 = (ctypes.c_int * 3)(13, 13, 13)
     st = lib.imf_create(b"Helmholtz_Hex_Deformed", 3, nm, nq13, arr, arr, arr, arr, 8, 0,
                         ctypes.byref(h))

@@ -300,3 +300,82 @@ def helmholtz_prism(b, E, deformed, jac, df, lam, u):
                              _p(b["w2"]), E, int(deformed), _p(jac), _p(df),
                              ctypes.c_double(lam), _p(u), _p(out))
     return out
+
+
+# ---- tetrahedra (mf_oracle_tet.c) ----
+
+def basis_tet(nm, nq0, nq1=None, nq2=None):
+    """Modified_A on nq0 GLL points (eta0), Modified_B on nq1 Gauss-Radau alpha = 1 points
+    (eta1) and Modified_C on nq2 Gauss-Radau alpha = 2 points (eta2), the last two by
+    default nq0 - 1 as MatrixFree requires; w1 / w2 are the Helper's 0.5- / 0.25-scaled
+    weights, w1_raw / w2_raw the Gauss-Radau ones."""
+    nq1 = nq0 - 1 if nq1 is None else nq1
+    nq2 = nq1 if nq2 is None else nq2
+    t = basis_tri(nm, nq0, nq1)
+    M = nm * (nm + 1) * (nm + 2) // 6
+    z2 = np.zeros(nq2); w2 = np.zeros(nq2); D2 = np.zeros(nq2 * nq2)
+    b2 = np.zeros(M * nq2); db2 = np.zeros(M * nq2)
+    lib().or_basis_modified_c(nm, nq2, _p(z2), _p(w2), _p(D2), _p(b2), _p(db2))
+    return dict(nm=nm, nq0=nq0, nq1=nq1, nq2=nq2, M=M, Q=nq0 * nq1 * nq2,
+                z0=t["z0"], w0=t["w0"], D0=t["D0"], bdata0=t["bdata0"], dbdata0=t["dbdata0"],
+                z1=t["z1"], w1_raw=t["w1_raw"], w1=t["w1"], D1=t["D1"], bdata1=t["bdata1"],
+                dbdata1=t["dbdata1"], z2=z2, w2_raw=w2, w2=0.25 * w2, D2=D2, bdata2=b2,
+                dbdata2=db2)
+
+
+def geom_tet(verts, b, deformed=False):
+    """Straight-sided tetrahedra (E x 4 x 3 vertices in StdTetExp order)."""
+    verts = np.ascontiguousarray(verts, dtype=np.float64)
+    E = verts.shape[0]
+    n = E * b["Q"] if deformed else E
+    jac = np.zeros(n); df = np.zeros(9 * n)
+    lib().or_geom_tet(_p(verts), E, b["nq0"], b["nq1"], b["nq2"], int(deformed), _p(jac),
+                      _p(df))
+    return jac, df
+
+
+def coords_tet(verts, b):
+    verts = np.ascontiguousarray(verts, dtype=np.float64)
+    E = verts.shape[0]
+    xyz = np.zeros(3 * E * b["Q"])
+    lib().or_coords_tet(_p(verts), E, b["nq0"], b["nq1"], b["nq2"], _p(b["z0"]), _p(b["z1"]),
+                        _p(b["z2"]), _p(xyz))
+    return xyz.reshape(3, -1)
+
+
+def bwdtrans_tet(b, E, u):
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    out = np.zeros(E * b["Q"])
+    lib().or_bwdtrans_tet(b["nm"], b["nq0"], b["nq1"], b["nq2"], _p(b["bdata0"]),
+                          _p(b["bdata1"]), _p(b["bdata2"]), E, _p(u), _p(out))
+    return out
+
+
+def iproduct_tet(b, E, deformed, jac, f):
+    f = np.ascontiguousarray(f, dtype=np.float64)
+    out = np.zeros(E * b["M"])
+    lib().or_iproduct_tet(b["nm"], b["nq0"], b["nq1"], b["nq2"], _p(b["bdata0"]),
+                          _p(b["bdata1"]), _p(b["bdata2"]), _p(b["w0"]), _p(b["w1"]),
+                          _p(b["w2"]), E, int(deformed), _p(jac), _p(f), _p(out))
+    return out
+
+
+def physderiv_tet(b, E, deformed, df, f):
+    f = np.ascontiguousarray(f, dtype=np.float64)
+    outs = [np.zeros(E * b["Q"]) for _ in range(3)]
+    lib().or_physderiv_tet(b["nq0"], b["nq1"], b["nq2"], _p(b["D0"]), _p(b["D1"]), _p(b["D2"]),
+                           _p(b["z0"]), _p(b["z1"]), _p(b["z2"]), E, int(deformed), _p(df),
+                           _p(f), *[_p(o) for o in outs])
+    return outs
+
+
+def helmholtz_tet(b, E, deformed, jac, df, lam, u):
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    out = np.zeros(E * b["M"])
+    lib().or_helmholtz_tet(b["nm"], b["nq0"], b["nq1"], b["nq2"], _p(b["bdata0"]),
+                           _p(b["dbdata0"]), _p(b["bdata1"]), _p(b["dbdata1"]),
+                           _p(b["bdata2"]), _p(b["dbdata2"]), _p(b["D0"]), _p(b["D1"]),
+                           _p(b["D2"]), _p(b["z0"]), _p(b["z1"]), _p(b["z2"]), _p(b["w0"]),
+                           _p(b["w1"]), _p(b["w2"]), E, int(deformed), _p(jac), _p(df),
+                           ctypes.c_double(lam), _p(u), _p(out))
+    return out

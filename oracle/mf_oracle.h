@@ -87,6 +87,8 @@ void or_zwgrjm(double *z, double *w, int np, double alpha, double beta);
 void or_Dnodes(double *D, const double *z, int np);
 void or_basis_modified_b(int nm, int nq, double *z, double *w, double *D, double *bdata,
                          double *dbdata);
+/* the Modified_B values on the nq points z (nm(nm+1)/2 rows of nq) */
+void or_modified_b_tables(int nm, int nq, const double *z, double *bdata);
 void or_geom_tri(const double *verts, int E, int nq0, int nq1, int deformed, double *jac,
                  double *df);
 void or_coords_tri(const double *verts, int E, int nq0, int nq1, const double *z0,
@@ -127,6 +129,34 @@ void or_helmholtz_prism(int nm, int nq0, int nq1, int nq2, const double *b0, con
                         const double *w0, const double *w1, const double *w2, int E,
                         int deformed, const double *jac, const double *df, double lambda,
                         const double *in, double *out);
+
+/* reference-space derivatives of one element's nq0 x nq1 x nq2 point values along the three
+ * directions (PhysDerivTensor3DKernel order) */
+void or_deriv3_elmt(int nq0, int nq1, int nq2, const double *D0, const double *D1,
+                    const double *D2, const double *in, double *d0, double *d1, double *d2);
+
+/* ---- Tetrahedra (mf_oracle_tet.c): Modified_A x Modified_B x Modified_C ---- */
+void or_basis_modified_c(int nm, int nq, double *z, double *w, double *D, double *bdata,
+                         double *dbdata);
+void or_geom_tet(const double *verts, int E, int nq0, int nq1, int nq2, int deformed,
+                 double *jac, double *df);
+void or_coords_tet(const double *verts, int E, int nq0, int nq1, int nq2, const double *z0,
+                   const double *z1, const double *z2, double *xyz);
+void or_bwdtrans_tet(int nm, int nq0, int nq1, int nq2, const double *b0, const double *b1,
+                     const double *b2, int E, const double *in, double *out);
+void or_iproduct_tet(int nm, int nq0, int nq1, int nq2, const double *b0, const double *b1,
+                     const double *b2, const double *w0, const double *w1, const double *w2,
+                     int E, int deformed, const double *jac, const double *in, double *out);
+void or_physderiv_tet(int nq0, int nq1, int nq2, const double *D0, const double *D1,
+                      const double *D2, const double *z0, const double *z1, const double *z2,
+                      int E, int deformed, const double *df, const double *in, double *out0,
+                      double *out1, double *out2);
+void or_helmholtz_tet(int nm, int nq0, int nq1, int nq2, const double *b0, const double *db0,
+                      const double *b1, const double *db1, const double *b2, const double *db2,
+                      const double *D0, const double *D1, const double *D2, const double *z0,
+                      const double *z1, const double *z2, const double *w0, const double *w1,
+                      const double *w2, int E, int deformed, const double *jac,
+                      const double *df, double lambda, const double *in, double *out);
 
 #ifdef __cplusplus
 }

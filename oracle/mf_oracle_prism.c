@@ -180,8 +180,8 @@ static void iprod_prism_elmt(int nm, int nq0, int nq1, int nq2, const double *b0
 }
 
 /* PhysDerivTensor3DKernel for nq0 x nq1 x nq2 points */
-static void deriv3_elmt(int nq0, int nq1, int nq2, const double *D0, const double *D1,
-                        const double *D2, const double *in, double *d0, double *d1, double *d2)
+void or_deriv3_elmt(int nq0, int nq1, int nq2, const double *D0, const double *D1,
+                    const double *D2, const double *in, double *d0, double *d1, double *d2)
 {
     for (int i = 0; i < nq0; ++i)
         for (int jk = 0; jk < nq1 * nq2; ++jk) {
@@ -249,7 +249,7 @@ void or_physderiv_prism(int nq0, int nq1, int nq2, const double *D0, const doubl
 #pragma omp parallel for schedule(static)
     for (int e = 0; e < E; ++e) {
         double *d0 = out0 + (size_t)e * Q, *d1 = out1 + (size_t)e * Q, *d2 = out2 + (size_t)e * Q;
-        deriv3_elmt(nq0, nq1, nq2, D0, D1, D2, in + (size_t)e * Q, d0, d1, d2);
+        or_deriv3_elmt(nq0, nq1, nq2, D0, D1, D2, in + (size_t)e * Q, d0, d1, d2);
         double f[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
         if (!deformed)
             for (int a = 0; a < 9; ++a) f[a] = df[a * np + e];
@@ -297,7 +297,7 @@ void or_helmholtz_prism(int nm, int nq0, int nq1, int nq2, const double *b0, con
             bwd_prism_elmt(nm, nq0, nq1, nq2, b0, b1, b2, in + (size_t)e * M, ws, ws + nm * nm, bwd);
             iprod_prism_elmt(nm, nq0, nq1, nq2, b0, b1, b2, w0, w1, w2, deformed, jp, bwd, ws,
                              ws + nq1 * nq2, ws + nq1 * nq2 + nq2, o, 1, 0, lambda);
-            deriv3_elmt(nq0, nq1, nq2, D0, D1, D2, bwd, dv0, dv1, dv2);
+            or_deriv3_elmt(nq0, nq1, nq2, D0, D1, D2, bwd, dv0, dv1, dv2);
             for (int k = 0, c = 0; k < nq2; ++k) {
                 const double h1 = 2.0 / (1.0 - z2[k]);
                 for (int j = 0; j < nq1; ++j)

@@ -62,11 +62,8 @@ void or_Dnodes(double *D, const double *z, int np)
     free(bw);
 }
 
-void or_basis_modified_b(int nm, int nq, double *z, double *w, double *D, double *bdata,
-                         double *dbdata)
+void or_modified_b_tables(int nm, int nq, const double *z, double *bdata)
 {
-    or_zwgrjm(z, w, nq, 1.0, 0.0);
-    or_Dnodes(D, z, nq);
     /* p = 0: (1-z)/2, (1+z)/2, then P_{q-2}^{(1,1)} (1-z)/2 (1+z)/2 */
     for (int i = 0; i < nq; ++i) {
         bdata[i] = 0.5 * (1.0 - z[i]);
@@ -95,6 +92,14 @@ void or_basis_modified_b(int nm, int nq, double *z, double *w, double *D, double
             for (int i = 0; i < nq; ++i) mode[i] *= one_m_z_pow[i] * one_p_z[i];
         }
     }
+}
+
+void or_basis_modified_b(int nm, int nq, double *z, double *w, double *D, double *bdata,
+                         double *dbdata)
+{
+    or_zwgrjm(z, w, nq, 1.0, 0.0);
+    or_Dnodes(D, z, nq);
+    or_modified_b_tables(nm, nq, z, bdata);
     const int M = nm * (nm + 1) / 2;
     for (int m = 0; m < M; ++m)
         for (int i = 0; i < nq; ++i) {

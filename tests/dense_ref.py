@@ -255,3 +255,94 @@ def prism_helmholtz_matrix(m, lam):
     for g in m["Gphys"]:
         H += g.T @ (WJ[:, None] * g)
     return H
+
+
+# ---- tetrahedra: Modified_A (eta0) x Modified_B (eta1) x Modified_C (eta2) ----
+
+def gauss_radau(nq, alpha):
+    """Nodes -1 and the roots of P_{nq-1}^{(alpha,1)} (scipy), weights w_i = int l_i
+    (1-z)^alpha dz by a 40-point Gauss-Legendre rule (exact for these degrees)."""
+    from scipy.special import roots_jacobi
+    z = (np.concatenate([[-1.0], np.sort(roots_jacobi(nq - 1, float(alpha), 1.0)[0])])
+         if nq > 1 else np.zeros(1))
+    xg, wg = L.leggauss(40)
+    w = np.zeros(nq)
+    for i in range(nq):
+        li = np.ones_like(xg)
+        for k in range(nq):
+            if k != i:
+                li *= (xg - z[k]) / (z[i] - z[k])
+        w[i] = np.sum(wg * li * (1 - xg) ** alpha)
+    return z, w
+
+
+def tri_index(nm, p, q):
+    return sum(nm - pp for pp in range(p)) + q
+
+
+def modified_c(nm, z):
+    """C[i, mode] for modes (p, q, r), r fastest: phi^c_pqr = phi^b_{p+q, r} (closed form)."""
+    Bb = modified_b(nm, z)
+    cols = [Bb[:, tri_index(nm, p + q, r)]
+            for p in range(nm) for q in range(nm - p) for r in range(nm - p - q)]
+    return np.array(cols).T
+
+
+def tet_element_mats(nm, nq0, nq1, nq2, verts):
+    """Dense tetrahedron operators (points (k nq1 + j) nq0 + i): B with the top vertex
+    (mode 1), the bottom vertex of the collapsed edge (mode nm) and the singular edge
+    (modes nm + r) carrying the extra lower-direction products, physical gradients from an
+    exact monomial fit of total degree <= nm - 1 in (xi0, xi1, xi2), weights with the
+    collapse factors 1/2 and 1/4, and |det| of the affine map."""
+    z0, w0 = gll(nq0)
+    z1, w1 = gauss_radau(nq1, 1)
+    z2, w2 = gauss_radau(nq2, 2)
+    Ba, _ = modified_a(nm, z0)
+    Bb = modified_b(nm, z1)
+    Bc = modified_b(nm, z2)  # phi^c_{pqr} = column tri_index(p + q, r) of this
+    I0 = np.tile(np.arange(nq0), nq1 * nq2)
+    I1 = np.tile(np.repeat(np.arange(nq1), nq0), nq2)
+    I2 = np.repeat(np.arange(nq2), nq0 * nq1)
+    M = nm * (nm + 1) * (nm + 2) // 6
+    B = np.zeros((nq0 * nq1 * nq2, M))
+    mode = 0
+    for p in range(nm):
+        for q in range(nm - p):
+            for r in range(nm - p - q):
+                B[:, mode] = (Ba[I0, p] * Bb[I1, tri_index(nm, p, q)]
+                              * Bc[I2, tri_index(nm, p + q, r)])
+                mode += 1
+    a0, a1 = Ba[I0, 0], Ba[I0, 1]
+    b00, b01 = Bb[I1, tri_index(nm, 0, 0)], Bb[I1, tri_index(nm, 0, 1)]
+    C = lambda r: Bc[I2, tri_index(nm, 0, r)]
+    B[:, 1] += C(1) * (a0 * b01 + a1 * b00 + a1 * b01)
+    B[:, nm] += C(0) * a1 * b01
+    for r in range(1, nm - 1):
+        B[:, nm + r] += C(r + 1) * a1 * b01
+    xi2 = z2[I2]
+    xi1 = 0.5 * (1 + z1[I1]) * (1 - xi2) - 1
+    xi0 = 0.5 * (1 + z0[I0]) * (-xi1 - xi2) - 1
+    pw = [(a, b, c) for a in range(nm) for b in range(nm - a) for c in range(nm - a - b)]
+    V = np.stack([xi0 ** a * xi1 ** b * xi2 ** c for a, b, c in pw], 1)
+    coef = np.linalg.lstsq(V, B, rcond=None)[0]
+    dV = [np.stack([a * xi0 ** max(a - 1, 0) * xi1 ** b * xi2 ** c for a, b, c in pw], 1),
+          np.stack([b * xi0 ** a * xi1 ** max(b - 1, 0) * xi2 ** c for a, b, c in pw], 1),
+          np.stack([c * xi0 ** a * xi1 ** b * xi2 ** max(c - 1, 0) for a, b, c in pw], 1)]
+    G = [d @ coef for d in dV]
+    v = np.asarray(verts, dtype=float).reshape(4, 3)
+    A = np.array([[0.5 * (v[a + 1, c] - v[0, c]) for a in range(3)] for c in range(3)])
+    Ai = np.linalg.inv(A)
+    Gp = [sum(Ai[a, c] * G[a] for a in range(3)) for c in range(3)]
+    W = w0[I0] * w1[I1] * 0.5 * w2[I2] * 0.25
+    Nv = [-0.5 * (1 + xi0 + xi1 + xi2), 0.5 * (1 + xi0), 0.5 * (1 + xi1), 0.5 * (1 + xi2)]
+    X = np.stack([sum(Nv[n] * v[n, c] for n in range(4)) for c in range(3)])
+    return dict(B=B, Gphys=Gp, W=W, J=abs(np.linalg.det(A)) * np.ones(len(W)), X=X,
+                fit_residual=np.abs(V @ coef - B).max(), z2=z2, w2=w2)
+
+
+def tet_helmholtz_matrix(m, lam):
+    WJ = m["W"] * m["J"]
+    H = lam * m["B"].T @ (WJ[:, None] * m["B"])
+    for g in m["Gphys"]:
+        H += g.T @ (WJ[:, None] * g)
+    return H

@@ -85,6 +85,18 @@ enum imf_precision { IMF_FP64 = 0, IMF_FP32 = 1 };
  * imf_basis_modified_b (w[2] without the collapse factor 1/2, applied here).  Points are
  * ordered (k*nq[1] + j)*nq[0] + i; SetJac / SetDF take one value (9 factors) per element
  * for _Regular, per point for _Deformed; transposes as for triangles.
+ * Tetrahedra, "{BwdTrans,IProduct,Helmholtz,PhysDeriv}_Tet_{Regular,Deformed}" (FP64;
+ * BwdTrans.cpp, IProductTet.cpp, Helmholtz.cpp, PhysDeriv.cpp; kernels
+ * BwdTransKernels.hpp:377-484, IProductKernels.hpp:604-760, PhysDerivKernels.hpp:556-694,
+ * Helmholtz.h:2266-2455): dim = 3, nm[0] = nm[1] = nm[2] modes (nm(nm+1)(nm+2)/6
+ * coefficients, mode (p, q, r) with p + q + r < nm, r fastest, then q, then p), nq[0] GLL
+ * points along eta0, nq[1] Gauss-Radau alpha = 1 along eta1 and nq[2] Gauss-Radau
+ * alpha = 2 along eta2 (the reference requires nq[1] = nq[2] = nq[0] - 1; here
+ * 2 <= nm <= 8, nm <= nq[0] <= 16, nm - 1 <= nq[1], nq[2] <= 16, within one workgroup's
+ * LDS).  Tables: Modified_A (imf_basis_modified_a), Modified_B (imf_basis_modified_b)
+ * and Modified_C (imf_basis_modified_c), the raw Gauss-Radau weights as w[1] and w[2]
+ * (the factors 1/2 and 1/4 are applied here).  Points (k*nq[1] + j)*nq[0] + i; SetJac /
+ * SetDF one value (9 factors) per element for _Regular, per point for _Deformed.
  * Also accepted, with no MatrixFreeOps counterpart: "Mass_Hex_Deformed" and
  * "Mass_Hex_Regular", the elemental mass action out = B^T diag(w J) B in of
  * StdRegions/StdExpansion.cpp:647-661 (MassMatrixOp_MatFree: BwdTrans, then
@@ -261,6 +273,13 @@ int imf_basis_modified_a(int nm, int nq, double *z, double *w, double *D,
  * bdata[nm(nm+1)/2 * nq] and dbdata (modes packed p-major, q fastest). */
 int imf_basis_modified_b(int nm, int nq, double *z, double *w, double *D,
                          double *bdata, double *dbdata);
+/* Modified_C on nq Gauss-Radau (alpha = 2, beta = 0) points (Basis.cpp:513-563,
+ * GaussPoints.cpp:123-125): phi^c_{pqr} = phi^b_{p+q,r}, the collapsed eta2 direction of a
+ * tetrahedron.  z, w (the Gauss-Radau weights, without the collapse factor 1/4), D (nq*nq),
+ * bdata and dbdata (nm(nm+1)(nm+2)/6 * nq, mode (p, q, r) with r fastest).  Returns
+ * IMF_ERR_ARG unless 2 <= nm and 1 <= nq <= 64. */
+int imf_basis_modified_c(int nm, int nq, double *z, double *w, double *D,
+                         double *bdata, double *dbdata);
 
 /* GeomFactors::ComputeDerivFactors (SpatialDomains/GeomFactors.cpp:399-478) for
  * straight-sided (trilinear hex / bilinear quad) elements, evaluated on the GPU.
@@ -276,6 +295,12 @@ int imf_geom_quad_device(const double *d_verts, int nElmt, int nq, const double 
  * copies when deformed != 0.  Layouts as above (4 factors, d = c*2 + a). */
 int imf_geom_tri_device(const double *d_verts, int nElmt, int nq0, int nq1, int deformed,
                         double *d_jac, double *d_df, void *stream);
+/* Straight-sided tetrahedra (TetGeom, SpatialDomains/TetGeom.cpp; GeomFactors.cpp:420-474):
+ * d_verts nElmt x 4 x 3 in StdTetExp vertex order (xi = (-1,-1,-1), (1,-1,-1), (-1,1,-1),
+ * (-1,-1,1)); the map is affine, so the factors are constant: one value per element, or
+ * nq0*nq1*nq2 copies when deformed != 0.  Layouts as above (9 factors, d = c*3 + a). */
+int imf_geom_tet_device(const double *d_verts, int nElmt, int nq0, int nq1, int nq2,
+                        int deformed, double *d_jac, double *d_df, void *stream);
 /* Straight-sided prisms (PrismGeom, SpatialDomains/PrismGeom.cpp; GeomFactors.cpp:420-474):
  * d_verts nElmt x 6 x 3 in StdPrismExp vertex order: 0..3 the xi2 = -1 quad at (xi0, xi1)
  * = (-1,-1), (1,-1), (1,1), (-1,1); 4 and 5 the collapsed edge at xi2 = 1, xi1 = -1 and 1.

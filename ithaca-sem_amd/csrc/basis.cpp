@@ -5,6 +5,7 @@
 // Restates LibUtilities/Polylib/Polylib.cpp (jacobfd, Jacobz, zwglj :195-232, zwgrjm
 // :119-144, Dglj :690-730) and LibUtilities/Foundations/Basis.cpp:392-509 (eModified_A,
 // eModified_B), which is what the operators need at set-up time.
+#include <algorithm>
 #include <cmath>
 #include <vector>
 
@@ -131,25 +132,29 @@ extern "C" int imf_basis_modified_a(int nm, int nq, double *z, double *w, double
     return IMF_OK;
 }
 
-extern "C" int imf_basis_modified_b(int nm, int nq, double *z, double *w, double *D,
-                                    double *bdata, double *dbdata)
+namespace {
+
+// Gauss-Radau-Jacobi with z = -1 and weight (1 - z)^alpha (beta = 0): -1 and the zeros of
+// P_{nq-1}^{(alpha,1)}; w_i = 2^alpha / (nq (nq + alpha)) (1 - z_i) / P_{nq-1}^{(alpha,0)}(z_i)^2
+// (beta = 0: no end factor)
+void gauss_radau_m(int nq, double alpha, double *z, double *w)
 {
-    if (nm < 2 || nq < 1 || nq > 64 || !z || !w || !D || !bdata || !dbdata) return IMF_ERR_ARG;
-    // Gauss-Radau-Jacobi with z = -1, weight (1 - z): -1 and the zeros of P_{nq-1}^{(1,1)};
-    // w_i = 2 / (nq (nq + 1)) (1 - z_i) / P_{nq-1}^{(1,0)}(z_i)^2 (beta = 0: no end
-    // factor)
     if (nq == 1) {
         z[0] = 0.0;
         w[0] = 2.0;
-    } else {
-        z[0] = -1.0;
-        jacobz(nq - 1, z + 1, 1.0, 1.0);
-        jacobfd(nq, z, w, nullptr, nq - 1, 1.0, 0.0);
-        const double fac = 2.0 / (double(nq) * double(nq + 1));
-        for (int i = 0; i < nq; ++i) w[i] = fac * (1.0 - z[i]) / (w[i] * w[i]);
+        return;
     }
-    // derivative matrix of the Lagrange basis through z, D[a*nq + b] = l_a'(z_b), from the
-    // barycentric weights; each l' row sums to zero
+    z[0] = -1.0;
+    jacobz(nq - 1, z + 1, alpha, 1.0);
+    jacobfd(nq, z, w, nullptr, nq - 1, alpha, 0.0);
+    const double fac = std::pow(2.0, alpha) / (double(nq) * (double(nq) + alpha));
+    for (int i = 0; i < nq; ++i) w[i] = fac * (1.0 - z[i]) / (w[i] * w[i]);
+}
+
+// derivative matrix of the Lagrange basis through z, D[a*nq + b] = l_a'(z_b), from the
+// barycentric weights; each l' row sums to zero
+void lagrange_deriv(int nq, const double *z, double *D)
+{
     std::vector<double> bw(nq);
     for (int a = 0; a < nq; ++a) {
         double p = 1.0;
@@ -167,10 +172,14 @@ extern "C" int imf_basis_modified_b(int nm, int nq, double *z, double *w, double
         }
         D[b * nq + b] = diag;
     }
-    // Modified_B, modes packed p-major (q fastest):
-    //   p = 0: (1-z)/2, (1+z)/2, (1-z)/2 (1+z)/2 P_{q-2}^{(1,1)}
-    //   p = 1: (1-z)/2, (1-z)/2 (1+z)/2 P_{q-1}^{(1,1)}
-    //   p >= 2: ((1-z)/2)^p, ((1-z)/2)^p (1+z)/2 P_{q-1}^{(2p-1,1)}
+}
+
+// Modified_B on the points z, modes packed p-major (q fastest):
+//   p = 0: (1-z)/2, (1+z)/2, (1-z)/2 (1+z)/2 P_{q-2}^{(1,1)}
+//   p = 1: (1-z)/2, (1-z)/2 (1+z)/2 P_{q-1}^{(1,1)}
+//   p >= 2: ((1-z)/2)^p, ((1-z)/2)^p (1+z)/2 P_{q-1}^{(2p-1,1)}
+void modified_b_values(int nm, int nq, const double *z, double *bdata)
+{
     std::vector<double> a(nq), bp(nq), pw(nq), P(nq);
     for (int i = 0; i < nq; ++i) {
         a[i] = 0.5 * (1.0 - z[i]);
@@ -197,12 +206,47 @@ extern "C" int imf_basis_modified_b(int nm, int nq, double *z, double *w, double
             }
         }
     }
-    const int M = nm * (nm + 1) / 2;
-    for (int m = 0; m < M; ++m)
+}
+
+void apply_deriv(int nrows, int nq, const double *D, const double *bdata, double *dbdata)
+{
+    for (int m = 0; m < nrows; ++m)
         for (int i = 0; i < nq; ++i) {
             double sum = 0.0;
             for (int k = 0; k < nq; ++k) sum += D[k * nq + i] * bdata[m * nq + k];
             dbdata[m * nq + i] = sum;
         }
+}
+
+}  // namespace
+
+extern "C" int imf_basis_modified_b(int nm, int nq, double *z, double *w, double *D,
+                                    double *bdata, double *dbdata)
+{
+    if (nm < 2 || nq < 1 || nq > 64 || !z || !w || !D || !bdata || !dbdata) return IMF_ERR_ARG;
+    gauss_radau_m(nq, 1.0, z, w);
+    lagrange_deriv(nq, z, D);
+    modified_b_values(nm, nq, z, bdata);
+    apply_deriv(nm * (nm + 1) / 2, nq, D, bdata, dbdata);
+    return IMF_OK;
+}
+
+extern "C" int imf_basis_modified_c(int nm, int nq, double *z, double *w, double *D,
+                                    double *bdata, double *dbdata)
+{
+    if (nm < 2 || nq < 1 || nq > 64 || !z || !w || !D || !bdata || !dbdata) return IMF_ERR_ARG;
+    gauss_radau_m(nq, 2.0, z, w);
+    lagrange_deriv(nq, z, D);
+    // phi^c_{pqr} = phi^b_{p+q,r}: block p is the Modified_B groups p .. nm-1 in order
+    std::vector<double> bb((size_t)nm * (nm + 1) / 2 * nq);
+    modified_b_values(nm, nq, z, bb.data());
+    size_t off = 0, boff = 0;
+    for (int p = 0; p < nm; ++p) {
+        const size_t n = (size_t)nq * (nm - p) * (nm - p + 1) / 2;
+        std::copy(bb.begin() + boff, bb.begin() + boff + n, bdata + off);
+        boff += (size_t)nq * (nm - p);
+        off += n;
+    }
+    apply_deriv(nm * (nm + 1) * (nm + 2) / 6, nq, D, bdata, dbdata);
     return IMF_OK;
 }

@@ -342,6 +342,37 @@ hipError_t geom_tri(const double *d_verts, long nElmt, int Q, int deformed, doub
     return hipGetLastError();
 }
 
+// Straight-sided tetrahedra, vertices in StdTetExp order (xi = (-1,-1,-1), (1,-1,-1),
+// (-1,1,-1), (-1,-1,1)): x = -(1 + xi0 + xi1 + xi2)/2 v0 + sum_a (1 + xi_a)/2 v_{a+1}, an
+// affine map, so the factors are constant (repeated at the Q points when deformed).
+__global__ void geom_tet_kernel(const double *__restrict__ verts, long nElmt, int Q, int deformed,
+                                double *__restrict__ jac, double *__restrict__ df)
+{
+    const int per = deformed ? Q : 1;
+    const long n = nElmt * per;
+    const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n) return;
+    const long e = g / per;
+    const double *v = verts + e * 12;
+    double deriv[9];
+    for (int a = 0; a < 3; ++a)
+        for (int c = 0; c < 3; ++c) deriv[a * 3 + c] = 0.5 * (v[(a + 1) * 3 + c] - v[c]);
+    double J, fac[9];
+    factors_from_deriv(3, deriv, &J, fac);
+    jac[g] = J;
+    for (int d = 0; d < 9; ++d) df[(long)d * n + g] = fac[d];
+}
+
+hipError_t geom_tet(const double *d_verts, long nElmt, int Q, int deformed, double *d_jac,
+                    double *d_df, hipStream_t s)
+{
+    const long n = nElmt * (deformed ? Q : 1);
+    const int bs = 256;
+    geom_tet_kernel<<<(unsigned)((n + bs - 1) / bs), bs, 0, s>>>(d_verts, nElmt, Q, deformed,
+                                                                 d_jac, d_df);
+    return hipGetLastError();
+}
+
 // Straight-sided prisms, vertices in StdPrismExp order: 0..3 the xi2 = -1 quad (xi0, xi1
 // = (-1,-1), (1,-1), (1,1), (-1,1)), 4 and 5 the collapsed edge at xi2 = 1 (xi1 = -1, 1).
 // The map is linear in each of (xi0, xi2) and xi1 but not affine overall, so deformed

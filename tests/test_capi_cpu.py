@@ -60,19 +60,35 @@ def test_modified_b_helper_matches_oracle():
             assert np.allclose(a, b[k], atol=1e-14 * scale, rtol=1e-13), (nm, nq, k)
 
 
+def test_modified_c_helper_matches_oracle():
+    lib = ctypes.CDLL(LIB)
+    for nm, nq in [(2, 1), (2, 2), (6, 6), (4, 7), (8, 15)]:
+        M = nm * (nm + 1) * (nm + 2) // 6
+        arrs = [np.zeros(nq), np.zeros(nq), np.zeros(nq * nq), np.zeros(M * nq), np.zeros(M * nq)]
+        st = lib.imf_basis_modified_c(nm, nq, *[a.ctypes.data_as(ctypes.c_void_p) for a in arrs])
+        assert st == 0
+        b = O.basis_tet(nm, nq + 1, nq, nq)
+        for a, k in zip(arrs, ["z2", "w2_raw", "D2", "bdata2", "dbdata2"]):
+            scale = max(1.0, np.abs(b["D2"]).max())
+            assert np.allclose(a, b[k], atol=1e-14 * scale, rtol=1e-13), (nm, nq, k)
+
+
 def test_create_rejects_bad_requests_without_gpu():
     lib = ctypes.CDLL(LIB)
     lib.imf_last_error.restype = ctypes.c_char_p
     nm = (ctypes.c_int * 3)(6, 6, 6)
     nq = (ctypes.c_int * 3)(7, 7, 7)
+    nm9 = (ctypes.c_int * 3)(9, 9, 9)
     b = np.zeros(42)
     dp = ctypes.POINTER(ctypes.c_double)
     arr = (dp * 3)(*([b.ctypes.data_as(dp)] * 3))
     h = ctypes.c_void_p()
-    st = lib.imf_create(b"Helmholtz_Tet_Deformed", 3, nm, nq, arr, arr, arr, arr, 8, 0,
+    st = lib.imf_create(b"Helmholtz_Cube_Deformed", 3, nm, nq, arr, arr, arr, arr, 8, 0,
                         ctypes.byref(h))
     assert st == 2 and b"No such operator key" in lib.imf_last_error()
-    nm9 = (ctypes.c_int * 3)(9, 9, 9)
+    st = lib.imf_create(b"Helmholtz_Tet_Deformed", 3, nm9, nq, arr, arr, arr, arr, 8, 0,
+                        ctypes.byref(h))
+    assert st == 2 and b"combo not implemented" in lib.imf_last_error()
     st = lib.imf_create(b"Helmholtz_Prism_Deformed", 3, nm9, nq, arr, arr, arr, arr, 8, 0,
                         ctypes.byref(h))
     assert st == 2 and b"combo not implemented" in lib.imf_last_error()

@@ -379,3 +379,79 @@ def helmholtz_tet(b, E, deformed, jac, df, lam, u):
                            _p(b["w1"]), _p(b["w2"]), E, int(deformed), _p(jac), _p(df),
                            ctypes.c_double(lam), _p(u), _p(out))
     return out
+
+
+# ---- pyramids (mf_oracle_pyr.c) ----
+
+def basis_pyr(nm, nq, nq2=None):
+    """Modified_A on nq GLL points along eta0 and eta1, ModifiedPyr_C on nq2 Gauss-Radau
+    (alpha = 2) points along eta2 (default nq - 1, as MatrixFree requires); w2 is the
+    Helper's 0.25-scaled weight, w2_raw the Gauss-Radau one."""
+    nq2 = nq - 1 if nq2 is None else nq2
+    a = basis(nm, nq)
+    M = nm * (nm + 1) * (2 * nm + 1) // 6
+    z2 = np.zeros(nq2); w2 = np.zeros(nq2); D2 = np.zeros(nq2 * nq2)
+    b2 = np.zeros(M * nq2); db2 = np.zeros(M * nq2)
+    lib().or_basis_modified_pyr_c(nm, nq2, _p(z2), _p(w2), _p(D2), _p(b2), _p(db2))
+    return dict(nm=nm, nq0=nq, nq1=nq, nq2=nq2, M=M, Q=nq * nq * nq2,
+                z0=a["z"], w0=a["w"], D0=a["D"], bdata0=a["bdata"], dbdata0=a["dbdata"],
+                z1=a["z"], w1=a["w"], D1=a["D"], bdata1=a["bdata"], dbdata1=a["dbdata"],
+                z2=z2, w2_raw=w2, w2=0.25 * w2, D2=D2, bdata2=b2, dbdata2=db2)
+
+
+def geom_pyr(verts, b, deformed=True):
+    """Straight-sided pyramids (E x 5 x 3 vertices in StdPyrExp order)."""
+    verts = np.ascontiguousarray(verts, dtype=np.float64)
+    E = verts.shape[0]
+    n = E * b["Q"] if deformed else E
+    jac = np.zeros(n); df = np.zeros(9 * n)
+    lib().or_geom_pyr(_p(verts), E, b["nq0"], b["nq1"], b["nq2"], _p(b["z0"]), _p(b["z1"]),
+                      _p(b["z2"]), int(deformed), _p(jac), _p(df))
+    return jac, df
+
+
+def coords_pyr(verts, b):
+    verts = np.ascontiguousarray(verts, dtype=np.float64)
+    E = verts.shape[0]
+    xyz = np.zeros(3 * E * b["Q"])
+    lib().or_coords_pyr(_p(verts), E, b["nq0"], b["nq1"], b["nq2"], _p(b["z0"]), _p(b["z1"]),
+                        _p(b["z2"]), _p(xyz))
+    return xyz.reshape(3, -1)
+
+
+def bwdtrans_pyr(b, E, u):
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    out = np.zeros(E * b["Q"])
+    lib().or_bwdtrans_pyr(b["nm"], b["nq0"], b["nq1"], b["nq2"], _p(b["bdata0"]),
+                          _p(b["bdata1"]), _p(b["bdata2"]), E, _p(u), _p(out))
+    return out
+
+
+def iproduct_pyr(b, E, deformed, jac, f):
+    f = np.ascontiguousarray(f, dtype=np.float64)
+    out = np.zeros(E * b["M"])
+    lib().or_iproduct_pyr(b["nm"], b["nq0"], b["nq1"], b["nq2"], _p(b["bdata0"]),
+                          _p(b["bdata1"]), _p(b["bdata2"]), _p(b["w0"]), _p(b["w1"]),
+                          _p(b["w2"]), E, int(deformed), _p(jac), _p(f), _p(out))
+    return out
+
+
+def physderiv_pyr(b, E, deformed, df, f):
+    f = np.ascontiguousarray(f, dtype=np.float64)
+    outs = [np.zeros(E * b["Q"]) for _ in range(3)]
+    lib().or_physderiv_pyr(b["nq0"], b["nq1"], b["nq2"], _p(b["D0"]), _p(b["D1"]), _p(b["D2"]),
+                           _p(b["z0"]), _p(b["z1"]), _p(b["z2"]), E, int(deformed), _p(df),
+                           _p(f), *[_p(o) for o in outs])
+    return outs
+
+
+def helmholtz_pyr(b, E, deformed, jac, df, lam, u):
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    out = np.zeros(E * b["M"])
+    lib().or_helmholtz_pyr(b["nm"], b["nq0"], b["nq1"], b["nq2"], _p(b["bdata0"]),
+                           _p(b["dbdata0"]), _p(b["bdata1"]), _p(b["dbdata1"]),
+                           _p(b["bdata2"]), _p(b["dbdata2"]), _p(b["D0"]), _p(b["D1"]),
+                           _p(b["D2"]), _p(b["z0"]), _p(b["z1"]), _p(b["z2"]), _p(b["w0"]),
+                           _p(b["w1"]), _p(b["w2"]), E, int(deformed), _p(jac), _p(df),
+                           ctypes.c_double(lam), _p(u), _p(out))
+    return out

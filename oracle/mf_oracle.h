@@ -158,6 +158,29 @@ void or_helmholtz_tet(int nm, int nq0, int nq1, int nq2, const double *b0, const
                       const double *w2, int E, int deformed, const double *jac,
                       const double *df, double lambda, const double *in, double *out);
 
+/* ---- Pyramids (mf_oracle_pyr.c): Modified_A x Modified_A x ModifiedPyr_C ---- */
+void or_basis_modified_pyr_c(int nm, int nq, double *z, double *w, double *D, double *bdata,
+                             double *dbdata);
+void or_geom_pyr(const double *verts, int E, int nq0, int nq1, int nq2, const double *z0,
+                 const double *z1, const double *z2, int deformed, double *jac, double *df);
+void or_coords_pyr(const double *verts, int E, int nq0, int nq1, int nq2, const double *z0,
+                   const double *z1, const double *z2, double *xyz);
+void or_bwdtrans_pyr(int nm, int nq0, int nq1, int nq2, const double *b0, const double *b1,
+                     const double *b2, int E, const double *in, double *out);
+void or_iproduct_pyr(int nm, int nq0, int nq1, int nq2, const double *b0, const double *b1,
+                     const double *b2, const double *w0, const double *w1, const double *w2,
+                     int E, int deformed, const double *jac, const double *in, double *out);
+void or_physderiv_pyr(int nq0, int nq1, int nq2, const double *D0, const double *D1,
+                      const double *D2, const double *z0, const double *z1, const double *z2,
+                      int E, int deformed, const double *df, const double *in, double *out0,
+                      double *out1, double *out2);
+void or_helmholtz_pyr(int nm, int nq0, int nq1, int nq2, const double *b0, const double *db0,
+                      const double *b1, const double *db1, const double *b2, const double *db2,
+                      const double *D0, const double *D1, const double *D2, const double *z0,
+                      const double *z1, const double *z2, const double *w0, const double *w1,
+                      const double *w2, int E, int deformed, const double *jac,
+                      const double *df, double lambda, const double *in, double *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -346,3 +346,148 @@ def tet_helmholtz_matrix(m, lam):
     for g in m["Gphys"]:
         H += g.T @ (WJ[:, None] * g)
     return H
+
+
+# ---- pyramids: Modified_A (eta0) x Modified_A (eta1) x ModifiedPyr_C (eta2) ----
+# The pyramid space is rational in xi, so instead of a monomial fit the modes are evaluated
+# as closed-form functions of xi (through eta(xi)) and differentiated by complex step.
+
+def jacobi_c(n, a, b, x):
+    """P_n^{(a,b)}(x) by the three-term recurrence (complex arguments allowed)."""
+    x = np.asarray(x)
+    if n == 0:
+        return np.ones_like(x)
+    p0, p1 = np.ones_like(x), 0.5 * (a - b + (a + b + 2) * x)
+    for k in range(1, n):
+        s = 2 * k + a + b
+        c = 2 * (k + 1) * (k + a + b + 1) * s
+        p0, p1 = p1, ((s + 1) * (s * (s + 2) * x + a * a - b * b) * p1
+                      - 2 * (k + a) * (k + b) * (s + 2) * p0) / c
+    return p1
+
+
+def _mod_a(p, x):
+    if p == 0:
+        return 0.5 * (1 - x)
+    if p == 1:
+        return 0.5 * (1 + x)
+    return 0.25 * (1 - x) * (1 + x) * jacobi_c(p - 2, 1, 1, x)
+
+
+def _mod_b(s, r, z):
+    a, b = 0.5 * (1 - z), 0.5 * (1 + z)
+    if s == 0:
+        return a if r == 0 else b if r == 1 else a * b * jacobi_c(r - 2, 1, 1, z)
+    if s == 1:
+        return a if r == 0 else a * b * jacobi_c(r - 1, 1, 1, z)
+    return a ** s if r == 0 else a ** s * b * jacobi_c(r - 1, 2 * s - 1, 1, z)
+
+
+def pyr_c(p, q, r, z):
+    """phi^{pyr c}_{pqr}: phi^b_{max(p,q), r} if min(p, q) <= 1, else ((1-z)/2)^{p+q-2}
+    times (1+z)/2 P_{r-1}^{(2p+2q-3,1)} for r >= 1."""
+    if min(p, q) <= 1:
+        return _mod_b(max(p, q), r, z)
+    a = (0.5 * (1 - z)) ** (p + q - 2)
+    return a if r == 0 else a * 0.5 * (1 + z) * jacobi_c(r - 1, 2 * p + 2 * q - 3, 1, z)
+
+
+def pyr_modes(nm):
+    return [(p, q, r) for p in range(nm) for q in range(nm) for r in range(nm - max(p, q))]
+
+
+def pyr_mode_fn(p, q, r, xi0, xi1, xi2):
+    """Mode (p, q, r) as a function of xi; the top vertex (0, 0, 1) is the whole (1+eta2)/2
+    (its CORRECT terms)."""
+    e0 = 2 * (1 + xi0) / (1 - xi2) - 1
+    e1 = 2 * (1 + xi1) / (1 - xi2) - 1
+    v = _mod_a(p, e0) * _mod_a(q, e1) * pyr_c(p, q, r, xi2)
+    if (p, q, r) == (0, 0, 1):
+        v = v + pyr_c(0, 0, 1, xi2) * (_mod_a(0, e0) * _mod_a(1, e1) + _mod_a(1, e0) * _mod_a(0, e1)
+                                       + _mod_a(1, e0) * _mod_a(1, e1))
+    return v
+
+
+def pyr_map_fn(v, xi0, xi1, xi2):
+    """x(xi) = sum of the five vertex modes times the vertex coordinates."""
+    e0 = 2 * (1 + xi0) / (1 - xi2) - 1
+    e1 = 2 * (1 + xi1) / (1 - xi2) - 1
+    a0, a1, b0, b1 = 0.5 * (1 - e0), 0.5 * (1 + e0), 0.5 * (1 - e1), 0.5 * (1 + e1)
+    c0, c1 = 0.5 * (1 - xi2), 0.5 * (1 + xi2)
+    N = [a0 * b0 * c0, a1 * b0 * c0, a1 * b1 * c0, a0 * b1 * c0, c1]
+    return [sum(N[n] * v[n, c] for n in range(5)) for c in range(3)]
+
+
+def lagrange_dmat(z):
+    """Dm[i, a] = l_a'(z_i) for the Lagrange basis through z, from the product rule."""
+    n = len(z)
+    Dm = np.zeros((n, n))
+    for a in range(n):
+        den = np.prod([z[a] - z[m] for m in range(n) if m != a])
+        for i in range(n):
+            s = 0.0
+            for m in range(n):
+                if m == a:
+                    continue
+                s += np.prod([z[i] - z[k] for k in range(n) if k not in (a, m)])
+            Dm[i, a] = s / den
+    return Dm
+
+
+def pyr_element_mats(nm, nq0, nq1, nq2, verts, deriv="collocation"):
+    """Dense pyramid operators.  deriv = "collocation": reference-space derivatives are
+    those of the interpolant through the points along each eta direction (what MatrixFree
+    computes; the pyramid's high modes have degree up to 2 nm - 4 in eta2, more than nq2
+    points resolve), turned into xi derivatives with d eta / d xi by complex step;
+    deriv = "exact": the modes differentiated in xi by complex step."""
+    z0, w0 = gll(nq0)
+    z1, w1 = gll(nq1)
+    z2, w2 = gauss_radau(nq2, 2)
+    I0 = np.tile(np.arange(nq0), nq1 * nq2)
+    I1 = np.tile(np.repeat(np.arange(nq1), nq0), nq2)
+    I2 = np.repeat(np.arange(nq2), nq0 * nq1)
+    xi2 = z2[I2]
+    xi0 = 0.5 * (1 + z0[I0]) * (1 - xi2) - 1
+    xi1 = 0.5 * (1 + z1[I1]) * (1 - xi2) - 1
+    modes = pyr_modes(nm)
+    h = 1e-30
+    xi = [xi0.astype(complex), xi1.astype(complex), xi2.astype(complex)]
+    B = np.stack([pyr_mode_fn(p, q, r, xi0, xi1, xi2) for p, q, r in modes], 1)
+    G = []
+    if deriv == "exact":
+        for a in range(3):
+            s = [x.copy() for x in xi]
+            s[a] = s[a] + 1j * h
+            G.append(np.stack([pyr_mode_fn(p, q, r, *s).imag / h for p, q, r in modes], 1))
+    else:
+        U = B.reshape(nq2, nq1, nq0, -1)
+        dU = [np.einsum("ia,kjam->kjim", lagrange_dmat(z0), U).reshape(len(xi0), -1),
+              np.einsum("ja,kaim->kjim", lagrange_dmat(z1), U).reshape(len(xi0), -1),
+              np.einsum("ka,ajim->kjim", lagrange_dmat(z2), U).reshape(len(xi0), -1)]
+        eta = lambda x0, x1, x2: [2 * (1 + x0) / (1 - x2) - 1, 2 * (1 + x1) / (1 - x2) - 1, x2]
+        for a in range(3):
+            s = [x.copy() for x in xi]
+            s[a] = s[a] + 1j * h
+            de = [e.imag / h for e in eta(*s)]  # d eta_b / d xi_a
+            G.append(sum(de[b][:, None] * dU[b] for b in range(3)))
+    v = np.asarray(verts, dtype=float).reshape(5, 3)
+    A = np.zeros((len(xi0), 3, 3))  # A[t, c, a] = dx_c / dxi_a
+    for a in range(3):
+        s = [x.copy() for x in xi]
+        s[a] = s[a] + 1j * h
+        X = pyr_map_fn(v, *s)
+        for c in range(3):
+            A[:, c, a] = X[c].imag / h
+    J = np.abs(np.linalg.det(A))
+    Ai = np.linalg.inv(A)  # Ai[t, a, c] = dxi_a / dx_c
+    Gp = [sum(Ai[:, a, c][:, None] * G[a] for a in range(3)) for c in range(3)]
+    W = w0[I0] * w1[I1] * w2[I2] * 0.25
+    return dict(B=B, Gphys=Gp, W=W, J=J, X=np.stack(pyr_map_fn(v, xi0, xi1, xi2)))
+
+
+def pyr_helmholtz_matrix(m, lam):
+    WJ = m["W"] * m["J"]
+    H = lam * m["B"].T @ (WJ[:, None] * m["B"])
+    for g in m["Gphys"]:
+        H += g.T @ (WJ[:, None] * g)
+    return H

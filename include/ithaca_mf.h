@@ -97,6 +97,16 @@ enum imf_precision { IMF_FP64 = 0, IMF_FP32 = 1 };
  * and Modified_C (imf_basis_modified_c), the raw Gauss-Radau weights as w[1] and w[2]
  * (the factors 1/2 and 1/4 are applied here).  Points (k*nq[1] + j)*nq[0] + i; SetJac /
  * SetDF one value (9 factors) per element for _Regular, per point for _Deformed.
+ * Pyramids, "{BwdTrans,IProduct,Helmholtz,PhysDeriv}_Pyr_{Regular,Deformed}" (FP64;
+ * BwdTrans.cpp, IProductPyr.cpp, Helmholtz.cpp:31-35, PhysDeriv.cpp; kernels
+ * BwdTransKernels.hpp:131-222, IProductKernels.hpp:455-598, PhysDerivKernels.hpp:464-550,
+ * Helmholtz.h:1771-1935): dim = 3, nm[0] = nm[1] = nm[2] modes (nm(nm+1)(2nm+1)/6
+ * coefficients, mode (p, q, r) with r < nm - max(p, q), r fastest, then q, then p), nq[0],
+ * nq[1] GLL points along eta0, eta1 and nq[2] Gauss-Radau alpha = 2 along eta2 (the
+ * reference requires nq[2] = nq[0] - 1; here 2 <= nm <= 8, nm <= nq[0], nq[1] <= 16,
+ * nm - 1 <= nq[2] <= 16, within one workgroup's LDS).  Tables: Modified_A for directions
+ * 0 and 1, ModifiedPyr_C (imf_basis_modified_pyr_c) for direction 2 with its raw
+ * Gauss-Radau weights as w[2] (the factor 1/4 is applied here).  Layouts as for prisms.
  * Also accepted, with no MatrixFreeOps counterpart: "Mass_Hex_Deformed" and
  * "Mass_Hex_Regular", the elemental mass action out = B^T diag(w J) B in of
  * StdRegions/StdExpansion.cpp:647-661 (MassMatrixOp_MatFree: BwdTrans, then
@@ -280,6 +290,13 @@ int imf_basis_modified_b(int nm, int nq, double *z, double *w, double *D,
  * IMF_ERR_ARG unless 2 <= nm and 1 <= nq <= 64. */
 int imf_basis_modified_c(int nm, int nq, double *z, double *w, double *D,
                          double *bdata, double *dbdata);
+/* ModifiedPyr_C on nq Gauss-Radau (alpha = 2, beta = 0) points (Basis.cpp:569-668), the
+ * eta2 direction of a pyramid: mode (p, q, r) with r < nm - max(p, q), r fastest, then q,
+ * then p; phi^b_{max(p,q),r} when min(p, q) <= 1, else ((1-z)/2)^{p+q-2} times
+ * (1+z)/2 P_{r-1}^{(2p+2q-3,1)} for r >= 1.  bdata and dbdata hold nm(nm+1)(2nm+1)/6 * nq
+ * values.  Returns IMF_ERR_ARG unless 2 <= nm and 1 <= nq <= 64. */
+int imf_basis_modified_pyr_c(int nm, int nq, double *z, double *w, double *D,
+                             double *bdata, double *dbdata);
 
 /* GeomFactors::ComputeDerivFactors (SpatialDomains/GeomFactors.cpp:399-478) for
  * straight-sided (trilinear hex / bilinear quad) elements, evaluated on the GPU.
@@ -295,6 +312,13 @@ int imf_geom_quad_device(const double *d_verts, int nElmt, int nq, const double 
  * copies when deformed != 0.  Layouts as above (4 factors, d = c*2 + a). */
 int imf_geom_tri_device(const double *d_verts, int nElmt, int nq0, int nq1, int deformed,
                         double *d_jac, double *d_df, void *stream);
+/* Straight-sided pyramids (PyrGeom, SpatialDomains/PyrGeom.cpp; GeomFactors.cpp):
+ * d_verts nElmt x 5 x 3 in StdPyrExp vertex order (xi = (-1,-1,-1), (1,-1,-1), (1,1,-1),
+ * (-1,1,-1), apex (-1,-1,1)).  A base that is not a parallelogram makes the map
+ * non-affine, so deformed != 0 gives factors at each of the nq0*nq1*nq2 points (nq0, nq1
+ * GLL, nq2 Gauss-Radau alpha = 2); deformed = 0 gives one value per element, at point 0. */
+int imf_geom_pyr_device(const double *d_verts, int nElmt, int nq0, int nq1, int nq2,
+                        int deformed, double *d_jac, double *d_df, void *stream);
 /* Straight-sided tetrahedra (TetGeom, SpatialDomains/TetGeom.cpp; GeomFactors.cpp:420-474):
  * d_verts nElmt x 4 x 3 in StdTetExp vertex order (xi = (-1,-1,-1), (1,-1,-1), (-1,1,-1),
  * (-1,-1,1)); the map is affine, so the factors are constant: one value per element, or

@@ -231,6 +231,39 @@ extern "C" int imf_basis_modified_b(int nm, int nq, double *z, double *w, double
     return IMF_OK;
 }
 
+extern "C" int imf_basis_modified_pyr_c(int nm, int nq, double *z, double *w, double *D,
+                                        double *bdata, double *dbdata)
+{
+    if (nm < 2 || nq < 1 || nq > 64 || !z || !w || !D || !bdata || !dbdata) return IMF_ERR_ARG;
+    gauss_radau_m(nq, 2.0, z, w);
+    lagrange_deriv(nq, z, D);
+    // mode (p, q, r), r < nm - max(p, q): phi^b_{max(p,q), r} when min(p, q) <= 1, else
+    // ((1-z)/2)^{p+q-2}, then times (1+z)/2 P_{r-1}^{(2p+2q-3,1)}
+    std::vector<double> bb((size_t)nm * (nm + 1) / 2 * nq), P(nq);
+    modified_b_values(nm, nq, z, bb.data());
+    double *mode = bdata;
+    for (int p = 0; p < nm; ++p)
+        for (int q = 0; q < nm; ++q) {
+            const int nr = nm - std::max(p, q);
+            if (p <= 1 || q <= 1) {
+                const int s = std::max(p, q), start = s * nm - s * (s - 1) / 2;
+                std::copy(bb.begin() + (size_t)start * nq, bb.begin() + (size_t)(start + nr) * nq,
+                          mode);
+                mode += (size_t)nr * nq;
+            } else {
+                const double *pw = mode;
+                for (int i = 0; i < nq; ++i) mode[i] = std::pow(0.5 * (1.0 - z[i]), p + q - 2);
+                mode += nq;
+                for (int r = 1; r < nr; ++r, mode += nq) {
+                    jacobfd(nq, z, P.data(), nullptr, r - 1, 2.0 * p + 2.0 * q - 3.0, 1.0);
+                    for (int i = 0; i < nq; ++i) mode[i] = P[i] * pw[i] * 0.5 * (1.0 + z[i]);
+                }
+            }
+        }
+    apply_deriv(nm * (nm + 1) * (2 * nm + 1) / 6, nq, D, bdata, dbdata);
+    return IMF_OK;
+}
+
 extern "C" int imf_basis_modified_c(int nm, int nq, double *z, double *w, double *D,
                                     double *bdata, double *dbdata)
 {

@@ -146,6 +146,27 @@ int run(imf_op op, imf::LaunchFn fn, const void *in, void *out, void *stream, in
                                    : needs_df(op) ? "Jacobian and derivative factors not set"
                                                   : "Jacobian not set");
     if (!in || !out) return fail(IMF_ERR_ARG, "null input/output array");
+    if (op->collapsed && op->shape == imf::kPyr) {
+        if (nvec != 1 || gmap) return fail(IMF_ERR_UNSUPPORTED, "pyramids: one vector per apply");
+        imf::PyrArgs t{};
+        t.kind = op->kind;
+        t.transpose = fn == tri_transpose_marker() ? 1 : 0;
+        t.geo = op->kind == imf::kBwdTrans ? imf::kUnit : (op->deformed ? imf::kDeformed : imf::kRegular);
+        t.nm = op->nmd[0];
+        t.nq0 = op->nqd[0];
+        t.nq1 = op->nqd[1];
+        t.nq2 = op->nqd[2];
+        t.nElmt = op->nElmt;
+        t.lambda = op->lambda;
+        t.tab = op->d_atab;
+        t.in = (const double *)in;
+        t.out = (double *)out;
+        t.jac = op->d_jac;
+        t.df = op->d_df;
+        hipError_t e = imf::launch_pyr(t, (hipStream_t)stream);
+        if (e != hipSuccess) return hip_fail(e, "kernel launch");
+        return IMF_OK;
+    }
     if (op->collapsed && op->shape == imf::kTet) {
         if (nvec != 1 || gmap) return fail(IMF_ERR_UNSUPPORTED, "tetrahedra: one vector per apply");
         imf::TetArgs t{};
@@ -512,6 +533,87 @@ int create_tet(const std::string &k, int kind, int deformed, const int *nm, cons
     return IMF_OK;
 }
 
+// Pyramids: Modified_A x Modified_A x ModifiedPyr_C in collapsed coordinates (mf_pyr.hip),
+// FP64.  Directions 0 and 1: bdata/dbdata nm*nq[d], D, w (GLL); direction 2:
+// nm(nm+1)(2nm+1)/6 * nq[2] (ModifiedPyr_C, Gauss-Radau alpha = 2; the factor 1/4 of the
+// weights applied here).  The points are regenerated; one element's arrays must fit the
+// LDS of a workgroup.
+int create_pyr(const std::string &k, int kind, int deformed, const int *nm, const int *nq,
+               const double *const *bdata, const double *const *dbdata,
+               const double *const *D, const double *const *w, int nElmt, int precision,
+               imf_op *out)
+{
+    if (precision != IMF_FP64 ||
+        (kind != imf::kBwdTrans && kind != imf::kIProduct && kind != imf::kHelmholtz &&
+         kind != imf::kPhysDeriv))
+        return fail(IMF_ERR_UNSUPPORTED, k + ": pyramids are supported for FP64 BwdTrans, "
+                                             "IProduct, Helmholtz and PhysDeriv");
+    const int m = nm[0], q0 = nq[0], q1 = nq[1], q2 = nq[2];
+    if (m < 2 || m > 8 || q0 < 2 || q1 < 2 || q2 < 1 || q0 > 16 || q1 > 16 || q2 > 16 ||
+        (kind != imf::kPhysDeriv &&
+         (nm[1] != m || nm[2] != m || q0 < m || q1 < m || q2 < m - 1)))
+        return fail(IMF_ERR_UNSUPPORTED, k + ": # of modes / points combo not implemented.");
+    int dev = 0, lds_max = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e == hipSuccess)
+        e = hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
+    if (e != hipSuccess) return hip_fail(e, "LDS size query");
+    if (sizeof(double) * (size_t)imf::pyr_lds_len(m, q0, q1, q2) > (size_t)lds_max)
+        return fail(IMF_ERR_UNSUPPORTED, k + ": an element of this size does not fit the LDS "
+                                             "of one workgroup");
+    const bool helm = kind == imf::kHelmholtz, tabs = kind != imf::kPhysDeriv;
+    if (!D[0] || !D[1] || !D[2] || !w[0] || !w[1] || !w[2] ||
+        (tabs && (!bdata[0] || !bdata[1] || !bdata[2])) ||
+        (helm && (!dbdata || !dbdata[0] || !dbdata[1] || !dbdata[2])))
+        return fail(IMF_ERR_ARG, k + ": missing basis tables");
+    std::vector<double> z0(q0), z1(q1), z2(q2);
+    {
+        std::vector<double> ww(16), dd(256), bb(5 * 16), db(5 * 16);
+        if (imf_basis_modified_a(2, q0, z0.data(), ww.data(), dd.data(), bb.data(), db.data()) ||
+            imf_basis_modified_a(2, q1, z1.data(), ww.data(), dd.data(), bb.data(), db.data()) ||
+            imf_basis_modified_pyr_c(2, q2, z2.data(), ww.data(), dd.data(), bb.data(), db.data()))
+            return fail(IMF_ERR_ARG, k + ": points");
+    }
+    const size_t rows[3] = {(size_t)m, (size_t)m, (size_t)imf::pyr_modes(m)};
+    const int qd[3] = {q0, q1, q2};
+    const double fac[3] = {1.0, 1.0, 0.25};
+    const double *zd[3] = {z0.data(), z1.data(), z2.data()};
+    std::vector<double> tab;
+    tab.reserve(imf::pyr_table_len(m, q0, q1, q2));
+    auto put = [&](const double *p, size_t n) {
+        if (p) tab.insert(tab.end(), p, p + n);
+        else tab.insert(tab.end(), n, 0.0);
+    };
+    for (int d = 0; d < 3; ++d) {
+        put(tabs ? bdata[d] : nullptr, rows[d] * qd[d]);
+        put(helm ? dbdata[d] : nullptr, rows[d] * qd[d]);
+        put(D[d], (size_t)qd[d] * qd[d]);
+        for (int j = 0; j < qd[d]; ++j) tab.push_back(fac[d] * w[d][j]);
+        put(zd[d], qd[d]);
+    }
+    imf_op op = new imf_op_s();
+    op->kind = kind; op->shape = imf::kPyr; op->deformed = deformed; op->prec = precision;
+    op->dim = 3; op->nm = m; op->nq = q0; op->nElmt = nElmt;
+    op->collapsed = true;
+    op->nmd[0] = op->nmd[1] = op->nmd[2] = m;
+    op->nqd[0] = q0;
+    op->nqd[1] = q1;
+    op->nqd[2] = q2;
+    op->M = (size_t)imf::pyr_modes(m);
+    op->Q = (size_t)q0 * q1 * q2;
+    op->w.assign(w[0], w[0] + q0);
+    e = hipMalloc(&op->d_atab, tab.size() * sizeof(double));
+    if (e == hipSuccess)
+        e = hipMemcpy(op->d_atab, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        (void)hipFree(op->d_atab);
+        delete op;
+        return hip_fail(e, "pyramid basis tables");
+    }
+    *out = op;
+    return IMF_OK;
+}
+
 }  // namespace
 
 imf::AnisoArgs aniso_args(const imf_op_s *op, const void *in, void *out, int nvec, long ldin,
@@ -583,13 +685,15 @@ int imf_create(const char *key, int dim, const int *nm, const int *nq,
     else if (rest == "Prism_Deformed") { shape = imf::kPrism; deformed = 1; }
     else if (rest == "Tet_Regular") { shape = imf::kTet; deformed = 0; }
     else if (rest == "Tet_Deformed") { shape = imf::kTet; deformed = 1; }
+    else if (rest == "Pyr_Regular") { shape = imf::kPyr; deformed = 0; }
+    else if (rest == "Pyr_Deformed") { shape = imf::kPyr; deformed = 1; }
     else return fail(IMF_ERR_UNSUPPORTED, "No such operator key: " + k);
     // The reference registers BwdTrans only as _Regular (MatrixFreeOps/BwdTrans.cpp:7-33).
     if (kind == imf::kBwdTrans && deformed)
         return fail(IMF_ERR_UNSUPPORTED, "No such operator key: " + k);
     if (kind == imf::kMass && shape != imf::kHex)
         return fail(IMF_ERR_UNSUPPORTED, "No such operator key: " + k);
-    if (shape == imf::kTri || shape == imf::kPrism || shape == imf::kTet) {
+    if (shape == imf::kTri || shape == imf::kPrism || shape == imf::kTet || shape == imf::kPyr) {
         if (dim != (shape == imf::kTri ? 2 : 3))
             return fail(IMF_ERR_ARG, "imf_create: dim does not match the shape of " + k);
         if (kind == imf::kIProdDeriv)
@@ -597,6 +701,9 @@ int imf_create(const char *key, int dim, const int *nm, const int *nq,
         if (precision != IMF_FP64 && precision != IMF_FP32)
             return fail(IMF_ERR_ARG, "imf_create: unknown precision");
         if (nElmt <= 0) return fail(IMF_ERR_ARG, "imf_create: nElmt must be positive");
+        if (shape == imf::kPyr)
+            return create_pyr(k, kind, deformed, nm, nq, bdata, dbdata, D, w, nElmt, precision,
+                              out);
         if (shape == imf::kTet)
             return create_tet(k, kind, deformed, nm, nq, bdata, dbdata, D, w, nElmt, precision,
                               out);
@@ -1026,6 +1133,25 @@ int imf_geom_tri_device(const double *d_verts, int nElmt, int nq0, int nq1, int 
     hipError_t e = imf::geom_tri(d_verts, nElmt, nq0 * nq1, deformed, d_jac, d_df,
                                  (hipStream_t)stream);
     return e == hipSuccess ? IMF_OK : hip_fail(e, "geom_tri");
+}
+
+int imf_geom_pyr_device(const double *d_verts, int nElmt, int nq0, int nq1, int nq2,
+                        int deformed, double *d_jac, double *d_df, void *stream)
+{
+    if (!d_verts || !d_jac || !d_df || nElmt <= 0 || nq0 < 2 || nq1 < 2 || nq2 < 1 ||
+        nq0 > 16 || nq1 > 16 || nq2 > 16)
+        return fail(IMF_ERR_ARG, "imf_geom_pyr_device: bad argument");
+    // nm = 2: at most 5 * 16 table values per direction
+    std::vector<double> z0(nq0), z1(nq1), z2(nq2), ww(16), dd(256), bb(80), db(80);
+    if (imf_basis_modified_a(2, nq0, z0.data(), ww.data(), dd.data(), bb.data(), db.data()) ||
+        imf_basis_modified_a(2, nq1, z1.data(), ww.data(), dd.data(), bb.data(), db.data()) ||
+        imf_basis_modified_pyr_c(2, nq2, z2.data(), ww.data(), dd.data(), bb.data(), db.data()))
+        return fail(IMF_ERR_ARG, "imf_geom_pyr_device: points");
+    const double *z[3] = {z0.data(), z1.data(), z2.data()};
+    const int nq[3] = {nq0, nq1, nq2};
+    hipError_t e = imf::geom_pyr(d_verts, nElmt, nq, z, deformed, d_jac, d_df,
+                                 (hipStream_t)stream);
+    return e == hipSuccess ? IMF_OK : hip_fail(e, "geom_pyr");
 }
 
 int imf_geom_tet_device(const double *d_verts, int nElmt, int nq0, int nq1, int nq2,

@@ -373,6 +373,62 @@ hipError_t geom_tet(const double *d_verts, long nElmt, int Q, int deformed, doub
     return hipGetLastError();
 }
 
+// Straight-sided pyramids, vertices in StdPyrExp order (xi = (-1,-1,-1), (1,-1,-1),
+// (1,1,-1), (-1,1,-1), apex (-1,-1,1)): x = sum of the vertex modes a0 b0 c0, a1 b0 c0,
+// a1 b1 c0, a0 b1 c0, c1 (a = (1 -/+ eta0)/2, b the same in eta1, c in eta2) times the
+// vertices, differentiated in eta and turned into xi derivatives by the collapsed chain
+// rule d/dxi0 = h2 d/deta0, d/dxi1 = h2 d/deta1, d/dxi2 = h0 h2 d/deta0 + h1 h2 d/deta1 +
+// d/deta2.  A pyramid whose base is not a parallelogram has a non-affine map, so deformed
+// factors vary over the points; regular ones are taken at point 0.
+__global__ void geom_pyr_kernel(const double *__restrict__ verts, long nElmt, int nq0, int nq1,
+                                int nq2, Pts z0, Pts z1, Pts z2, int deformed,
+                                double *__restrict__ jac, double *__restrict__ df)
+{
+    const int Q = deformed ? nq0 * nq1 * nq2 : 1;
+    const long n = nElmt * Q;
+    const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= n) return;
+    const long e = g / Q;
+    const int q = (int)(g - e * Q);
+    const int i = q % nq0, j = (q / nq0) % nq1, k = q / (nq0 * nq1);
+    const double e0 = z0.v[i], e1 = z1.v[j], e2 = z2.v[k];
+    const double a0 = 0.5 * (1 - e0), a1 = 0.5 * (1 + e0), b0 = 0.5 * (1 - e1),
+                 b1 = 0.5 * (1 + e1), c0 = 0.5 * (1 - e2);
+    const double dN[5][3] = {{-0.5 * b0 * c0, -0.5 * a0 * c0, -0.5 * a0 * b0},
+                             {0.5 * b0 * c0, -0.5 * a1 * c0, -0.5 * a1 * b0},
+                             {0.5 * b1 * c0, 0.5 * a1 * c0, -0.5 * a1 * b1},
+                             {-0.5 * b1 * c0, 0.5 * a0 * c0, -0.5 * a0 * b1},
+                             {0.0, 0.0, 0.5}};
+    const double h2 = 2.0 / (1.0 - e2);
+    const double *v = verts + e * 15;
+    double deriv[9];
+    for (int c = 0; c < 3; ++c) {
+        double de[3] = {0, 0, 0};
+        for (int nv = 0; nv < 5; ++nv)
+            for (int a = 0; a < 3; ++a) de[a] += dN[nv][a] * v[nv * 3 + c];
+        deriv[0 * 3 + c] = h2 * de[0];
+        deriv[1 * 3 + c] = h2 * de[1];
+        deriv[2 * 3 + c] = a1 * h2 * de[0] + b1 * h2 * de[1] + de[2];
+    }
+    double J, fac[9];
+    factors_from_deriv(3, deriv, &J, fac);
+    jac[g] = J;
+    for (int d = 0; d < 9; ++d) df[(long)d * n + g] = fac[d];
+}
+
+hipError_t geom_pyr(const double *d_verts, long nElmt, const int *nq, const double *const *z,
+                    int deformed, double *d_jac, double *d_df, hipStream_t s)
+{
+    Pts pz[3] = {};
+    for (int d = 0; d < 3; ++d)
+        for (int i = 0; i < nq[d] && i < 16; ++i) pz[d].v[i] = z[d][i];
+    const long n = nElmt * (deformed ? (long)nq[0] * nq[1] * nq[2] : 1);
+    const int bs = 256;
+    geom_pyr_kernel<<<(unsigned)((n + bs - 1) / bs), bs, 0, s>>>(
+        d_verts, nElmt, nq[0], nq[1], nq[2], pz[0], pz[1], pz[2], deformed, d_jac, d_df);
+    return hipGetLastError();
+}
+
 // Straight-sided prisms, vertices in StdPrismExp order: 0..3 the xi2 = -1 quad (xi0, xi1
 // = (-1,-1), (1,-1), (1,1), (-1,1)), 4 and 5 the collapsed edge at xi2 = 1 (xi1 = -1, 1).
 // The map is linear in each of (xi0, xi2) and xi1 but not affine overall, so deformed

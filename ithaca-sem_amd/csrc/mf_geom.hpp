@@ -30,6 +30,11 @@ hipError_t geom_tri(const double *d_verts, long nElmt, int Q, int deformed, doub
 hipError_t geom_tet(const double *d_verts, long nElmt, int Q, int deformed, double *d_jac,
                     double *d_df, hipStream_t s);
 
+// Straight-sided pyramids (vertices nElmt x 5 x 3), nq[d] points z[d] along eta0 / eta1 /
+// eta2 (point (i, j, k) at index (k*nq1 + j)*nq0 + i).
+hipError_t geom_pyr(const double *d_verts, long nElmt, const int *nq, const double *const *z,
+                    int deformed, double *d_jac, double *d_df, hipStream_t s);
+
 // Straight-sided prisms (vertices nElmt x 6 x 3), nq[d] points z[d] along eta0 / xi1 / eta2
 // (point (i, j, k) at index (k*nq1 + j)*nq0 + i).
 hipError_t geom_prism(const double *d_verts, long nElmt, const int *nq, const double *const *z,

@@ -7,6 +7,7 @@
 #include "imf_common.hpp"
 #include "mf_aniso.hpp"
 #include "mf_prism.hpp"
+#include "mf_pyr.hpp"
 #include "mf_tet.hpp"
 #include "mf_tri.hpp"
 
@@ -39,8 +40,9 @@ struct imf_op_s {
     double *d_atab = nullptr;
     int bufLen = 0;
     // collapsed-coordinate shapes (FP64): triangles (mf_tri.hip; nqd[0] / nqd[1] points along
-    // eta0 / eta1), prisms (mf_prism.hip; nqd[0..2] along eta0 / xi1 / eta2) and tetrahedra
-    // (mf_tet.hip; nqd[0..2] along eta0 / eta1 / eta2), nmd[0] modes per direction, the table image in d_atab; geometry read unpacked from d_jac / d_df
+    // eta0 / eta1), prisms (mf_prism.hip; nqd[0..2] along eta0 / xi1 / eta2), tetrahedra
+    // (mf_tet.hip) and pyramids (mf_pyr.hip; nqd[0..2] along eta0 / eta1 / eta2), nmd[0]
+    // modes per direction, the table image in d_atab; geometry read unpacked from d_jac / d_df
     bool collapsed = false;
 };
 

@@ -63,6 +63,8 @@ def lib():
         L.imf_geom_prism_device.argtypes = [vp, i, i, i, i, i, vp, vp, vp]
         L.imf_geom_tet_device.argtypes = [vp, i, i, i, i, i, vp, vp, vp]
         L.imf_basis_modified_c.argtypes = [i, i, vp, vp, vp, vp, vp]
+        L.imf_basis_modified_pyr_c.argtypes = [i, i, vp, vp, vp, vp, vp]
+        L.imf_geom_pyr_device.argtypes = [vp, i, i, i, i, i, vp, vp, vp]
         _LIB = L
     return _LIB
 
@@ -116,12 +118,25 @@ def basis_modified_c(nm, nq):
     return dict(z=z, w=w, D=D, bdata=b, dbdata=db, nm=nm, nq=nq)
 
 
+def basis_modified_pyr_c(nm, nq):
+    """ModifiedPyr_C basis on nq Gauss-Radau (alpha = 2, beta = 0) points (Basis.cpp:
+    569-668), computed by the library: the collapsed eta2 direction of a pyramid.  w are the
+    Gauss-Radau weights (without the collapse factor 1/4); bdata has nm(nm+1)(2nm+1)/6
+    modes."""
+    M = nm * (nm + 1) * (2 * nm + 1) // 6
+    z, w = np.zeros(nq), np.zeros(nq)
+    D, b, db = np.zeros(nq * nq), np.zeros(M * nq), np.zeros(M * nq)
+    _check(lib().imf_basis_modified_pyr_c(nm, nq, _np_ptr(z), _np_ptr(w), _np_ptr(D),
+                                          _np_ptr(b), _np_ptr(db)))
+    return dict(z=z, w=w, D=D, bdata=b, dbdata=db, nm=nm, nq=nq)
+
+
 def geom_device(shape, verts, nq, z, deformed=True, stream=None):
     """Geometric factors of straight-sided elements on the GPU (GeomFactors.cpp:399-478).
 
     verts: cuda float64 tensor (nElmt, 8, 3) for hexes, (nElmt, 4, 2) for quads, (nElmt, 3, 2)
-    for triangles, (nElmt, 6, 3) for prisms or (nElmt, 4, 3) for tetrahedra (collapsed shapes:
-    nq the per-direction counts).
+    for triangles, (nElmt, 6, 3) for prisms, (nElmt, 4, 3) for tetrahedra or (nElmt, 5, 3) for
+    pyramids (collapsed shapes: nq the per-direction counts).
     Returns (jac, df) cuda tensors in the GetJac / GetDerivFactors layouts."""
     verts = verts.contiguous()
     E = verts.shape[0]
@@ -131,6 +146,16 @@ def geom_device(shape, verts, nq, z, deformed=True, stream=None):
         jac = torch.empty(npts, dtype=torch.float64, device=verts.device)
         df = torch.empty(4 * npts, dtype=torch.float64, device=verts.device)
         _check(lib().imf_geom_tri_device(ctypes.c_void_p(verts.data_ptr()), E, nq0, nq1,
+                                         int(deformed), ctypes.c_void_p(jac.data_ptr()),
+                                         ctypes.c_void_p(df.data_ptr()), _stream_ptr(stream)))
+        return jac, df
+    if shape == "pyr":  # (nElmt, 5, 3) vertices; nq = (nq0, nq1, nq2); z unused
+        nq0, nq1, nq2 = ([int(x) for x in nq] if isinstance(nq, (tuple, list))
+                         else (nq, nq, nq - 1))
+        npts = E * (nq0 * nq1 * nq2 if deformed else 1)
+        jac = torch.empty(npts, dtype=torch.float64, device=verts.device)
+        df = torch.empty(9 * npts, dtype=torch.float64, device=verts.device)
+        _check(lib().imf_geom_pyr_device(ctypes.c_void_p(verts.data_ptr()), E, nq0, nq1, nq2,
                                          int(deformed), ctypes.c_void_p(jac.data_ptr()),
                                          ctypes.c_void_p(df.data_ptr()), _stream_ptr(stream)))
         return jac, df
@@ -192,10 +217,12 @@ class Operator:
         modes, nq = nq0 (with nq1 = nq0, nq2 = nq0 - 1) or (nq0, nq1, nq2); basis the
         Modified_A, Modified_A and Modified_B tables of the three directions.  Tetrahedra
         ("*_Tet_*"): nm modes, nq = nq0 (with nq1 = nq2 = nq0 - 1) or (nq0, nq1, nq2); basis
-        the Modified_A, Modified_B and Modified_C tables."""
+        the Modified_A, Modified_B and Modified_C tables.  Pyramids ("*_Pyr_*"): nm modes,
+        nq = nq0 (with nq1 = nq0, nq2 = nq0 - 1) or (nq0, nq1, nq2); basis the Modified_A,
+        Modified_A and ModifiedPyr_C tables."""
         self.key = key
         shape = key.split("_")[1]
-        self.dim = 3 if shape in ("Hex", "Prism", "Tet") else 2
+        self.dim = 3 if shape in ("Hex", "Prism", "Tet", "Pyr") else 2
         self.nm, self.nq, self.nElmt, self.precision = nm, nq, nElmt, precision
         per_dir = isinstance(nm, (tuple, list))
         if shape == "Tri":
@@ -203,6 +230,14 @@ class Operator:
             nms, nqs = [nm, nm], [nq0, nq1]
             if basis is None:
                 basis = [basis_modified_a(nm, nq0), basis_modified_b(nm, nq1)]
+            bases = list(basis)
+        elif shape == "Pyr":
+            nqs = ([int(x) for x in nq] if isinstance(nq, (tuple, list))
+                   else [nq, nq, nq - 1])
+            nms = [nm] * 3
+            if basis is None:
+                basis = [basis_modified_a(nm, nqs[0]), basis_modified_a(nm, nqs[1]),
+                         basis_modified_pyr_c(nm, nqs[2])]
             bases = list(basis)
         elif shape == "Tet":
             nqs = ([int(x) for x in nq] if isinstance(nq, (tuple, list))

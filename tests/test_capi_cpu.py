@@ -73,6 +73,19 @@ def test_modified_c_helper_matches_oracle():
             assert np.allclose(a, b[k], atol=1e-14 * scale, rtol=1e-13), (nm, nq, k)
 
 
+def test_modified_pyr_c_helper_matches_oracle():
+    lib = ctypes.CDLL(LIB)
+    for nm, nq in [(2, 1), (2, 2), (6, 6), (4, 7), (8, 15)]:
+        M = nm * (nm + 1) * (2 * nm + 1) // 6
+        arrs = [np.zeros(nq), np.zeros(nq), np.zeros(nq * nq), np.zeros(M * nq), np.zeros(M * nq)]
+        st = lib.imf_basis_modified_pyr_c(nm, nq, *[a.ctypes.data_as(ctypes.c_void_p) for a in arrs])
+        assert st == 0
+        b = O.basis_pyr(nm, nq + 1, nq)
+        for a, k in zip(arrs, ["z2", "w2_raw", "D2", "bdata2", "dbdata2"]):
+            scale = max(1.0, np.abs(b["D2"]).max())
+            assert np.allclose(a, b[k], atol=1e-14 * scale, rtol=1e-13), (nm, nq, k)
+
+
 def test_create_rejects_bad_requests_without_gpu():
     lib = ctypes.CDLL(LIB)
     lib.imf_last_error.restype = ctypes.c_char_p

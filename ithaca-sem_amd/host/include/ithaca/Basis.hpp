@@ -1,6 +1,8 @@
-// LibUtilities::Basis for the Modified_A basis on Gauss-Lobatto-Legendre points
-// (LibUtilities/Foundations/Basis.cpp:217, :392-421; Points/GaussPoints.cpp), with the
-// accessors the MatrixFree Helper reads (MatrixFreeOps/Operator.hpp:223-273).
+// LibUtilities::Basis for the modified bases of the MatrixFree shapes
+// (LibUtilities/Foundations/Basis.cpp:217-668; Points/GaussPoints.cpp): Modified_A on
+// Gauss-Lobatto-Legendre points, Modified_B on Gauss-Radau alpha = 1 points, Modified_C and
+// ModifiedPyr_C on Gauss-Radau alpha = 2 points, with the accessors the MatrixFree Helper
+// reads (MatrixFreeOps/Operator.hpp:223-273).
 #pragma once
 
 #include <memory>
@@ -11,8 +13,13 @@
 namespace Nektar {
 namespace LibUtilities {
 
-enum BasisType { eNoBasisType, eModified_A };
-enum PointsType { eNoPointsType, eGaussLobattoLegendre };
+enum BasisType { eNoBasisType, eModified_A, eModified_B, eModified_C, eModifiedPyr_C };
+enum PointsType {
+    eNoPointsType,
+    eGaussLobattoLegendre,
+    eGaussRadauMAlpha1Beta0,
+    eGaussRadauMAlpha2Beta0
+};
 
 class PointsKey {
 public:
@@ -46,7 +53,9 @@ public:
     int GetNumPoints() const { return m_key.GetPointsKey().GetNumPoints(); }
     PointsType GetPointsType() const { return m_key.GetPointsKey().GetPointsType(); }
     const BasisKey &GetBasisKey() const { return m_key; }
-    // bdata[p*nq + i] = phi_p(z_i); dbdata[p*nq + i] = phi_p'(z_i)
+    // bdata[mode*nq + i] = phi_mode(z_i); dbdata[mode*nq + i] = phi_mode'(z_i) (Modified_A:
+    // nm modes; Modified_B: nm(nm+1)/2; Modified_C: nm(nm+1)(nm+2)/6; ModifiedPyr_C:
+    // nm(nm+1)(2nm+1)/6)
     const std::vector<NekDouble> &GetBdata() const { return m_bdata; }
     const std::vector<NekDouble> &GetDbdata() const { return m_dbdata; }
     const std::vector<NekDouble> &GetZ() const { return m_z; }
@@ -64,6 +73,19 @@ using BasisSharedPtr = std::shared_ptr<Basis>;
 inline BasisSharedPtr MakeModifiedA(int nm, int nq)
 {
     return std::make_shared<Basis>(BasisKey(eModified_A, nm, PointsKey(nq, eGaussLobattoLegendre)));
+}
+inline BasisSharedPtr MakeModifiedB(int nm, int nq)
+{
+    return std::make_shared<Basis>(BasisKey(eModified_B, nm, PointsKey(nq, eGaussRadauMAlpha1Beta0)));
+}
+inline BasisSharedPtr MakeModifiedC(int nm, int nq)
+{
+    return std::make_shared<Basis>(BasisKey(eModified_C, nm, PointsKey(nq, eGaussRadauMAlpha2Beta0)));
+}
+inline BasisSharedPtr MakeModifiedPyrC(int nm, int nq)
+{
+    return std::make_shared<Basis>(
+        BasisKey(eModifiedPyr_C, nm, PointsKey(nq, eGaussRadauMAlpha2Beta0)));
 }
 
 }  // namespace LibUtilities

@@ -25,17 +25,29 @@ namespace LibUtilities {
 
 Basis::Basis(const BasisKey &key) : m_key(key)
 {
-    ASSERTL0(key.GetBasisType() == eModified_A &&
-                 key.GetPointsKey().GetPointsType() == eGaussLobattoLegendre,
-             "Basis: only Modified_A on Gauss-Lobatto-Legendre points is provided");
+    const BasisType b = key.GetBasisType();
+    const PointsType pt = key.GetPointsKey().GetPointsType();
+    ASSERTL0((b == eModified_A && pt == eGaussLobattoLegendre) ||
+                 (b == eModified_B && pt == eGaussRadauMAlpha1Beta0) ||
+                 ((b == eModified_C || b == eModifiedPyr_C) && pt == eGaussRadauMAlpha2Beta0),
+             "Basis: provided are Modified_A on Gauss-Lobatto-Legendre points, Modified_B on "
+             "Gauss-Radau alpha = 1 points and Modified_C / ModifiedPyr_C on Gauss-Radau "
+             "alpha = 2 points");
     const int nm = key.GetNumModes(), nq = key.GetPointsKey().GetNumPoints();
+    const int rows = b == eModified_A ? nm
+                   : b == eModified_B ? nm * (nm + 1) / 2
+                   : b == eModified_C ? nm * (nm + 1) * (nm + 2) / 6
+                                      : nm * (nm + 1) * (2 * nm + 1) / 6;
     m_z.resize(nq);
     m_w.resize(nq);
     m_D.resize(nq * nq);
-    m_bdata.resize(nm * nq);
-    m_dbdata.resize(nm * nq);
-    check(imf_basis_modified_a(nm, nq, m_z.data(), m_w.data(), m_D.data(), m_bdata.data(),
-                               m_dbdata.data()));
+    m_bdata.resize((size_t)rows * nq);
+    m_dbdata.resize((size_t)rows * nq);
+    auto fn = b == eModified_A ? imf_basis_modified_a
+            : b == eModified_B ? imf_basis_modified_b
+            : b == eModified_C ? imf_basis_modified_c
+                               : imf_basis_modified_pyr_c;
+    check(fn(nm, nq, m_z.data(), m_w.data(), m_D.data(), m_bdata.data(), m_dbdata.data()));
 }
 
 }  // namespace LibUtilities
@@ -45,10 +57,34 @@ Basis::Basis(const BasisKey &key) : m_key(key)
 // ---------------------------------------------------------------------------
 namespace MatrixFree {
 
+namespace {
+
+// The basis types each shape's kernels read (StdTriExp, StdPrismExp, StdTetExp, StdPyrExp
+// and the tensor shapes): imf_create takes the tables by pointer and cannot check their
+// lengths, so a mismatch is refused here.
+void check_bases(const std::string &key, const std::vector<LibUtilities::BasisSharedPtr> &basis)
+{
+    using namespace LibUtilities;
+    const std::string shape = key.substr(key.find('_') + 1, key.rfind('_') - key.find('_') - 1);
+    std::vector<BasisType> want;
+    if (shape == "Quad") want = {eModified_A, eModified_A};
+    else if (shape == "Hex") want = {eModified_A, eModified_A, eModified_A};
+    else if (shape == "Tri") want = {eModified_A, eModified_B};
+    else if (shape == "Prism") want = {eModified_A, eModified_A, eModified_B};
+    else if (shape == "Tet") want = {eModified_A, eModified_B, eModified_C};
+    else if (shape == "Pyr") want = {eModified_A, eModified_A, eModifiedPyr_C};
+    bool ok = want.size() == basis.size();
+    for (size_t d = 0; ok && d < want.size(); ++d) ok = basis[d]->GetBasisType() == want[d];
+    ASSERTL0(ok, key + ": the bases do not match the shape's expansion");
+}
+
+}  // namespace
+
 Operator::Operator(const std::string &key, std::vector<LibUtilities::BasisSharedPtr> basis,
                    int nElmt)
     : m_basis(basis), m_nElmt(nElmt)
 {
+    check_bases(key, basis);
     const int dim = (int)basis.size();
     std::vector<int> nm(dim), nq(dim);
     std::vector<const double *> b(dim), db(dim), D(dim), w(dim);
@@ -173,12 +209,13 @@ OperatorSharedPtr make(const std::string &key, std::vector<LibUtilities::BasisSh
     return std::make_shared<T>(key, b, n);
 }
 
-// Keys registered by the reference for quads and hexes: MatrixFreeOps/BwdTrans.cpp:7-33,
-// IProductQuad.cpp, IProductHex.cpp, Helmholtz.cpp:7-23, PhysDeriv.cpp:14-36,
-// IProductWRTDerivBaseQuad.cpp, IProductWRTDerivBaseHex.cpp.
+// Keys registered by the reference: MatrixFreeOps/BwdTrans.cpp:7-35, IProduct{Quad,Hex,
+// Tri,Prism,Tet,Pyr}.cpp, Helmholtz.cpp:7-42, PhysDeriv.cpp:14-49,
+// IProductWRTDerivBase{Quad,Hex}.cpp.  IProductWRTDerivBase of the collapsed shapes is not
+// provided (imf_create rejects those keys).
 const bool registered = [] {
     auto &f = GetOperatorFactory();
-    for (const char *shape : {"Quad", "Hex"}) {
+    for (const char *shape : {"Quad", "Hex", "Tri", "Prism", "Tet", "Pyr"}) {
         const std::string s(shape);
         const std::string kb = "BwdTrans_" + s + "_Regular";
         f.RegisterCreatorFunction(kb, [kb](std::vector<LibUtilities::BasisSharedPtr> b, int n) {
@@ -212,11 +249,22 @@ const bool registered = [] {
 // ---------------------------------------------------------------------------
 namespace Collections {
 
+// StdExpansion::GetNcoeffs of the shape (StdTriData / StdTetData / StdPyrData /
+// StdPrismData::getNumberOfCoefficients for the collapsed shapes).
 int CollectionExp::GetNcoeffs() const
 {
-    int n = 1;
-    for (auto &b : basis) n *= b->GetNumModes();
-    return n;
+    const int m = basis[0]->GetNumModes();
+    switch (shape) {
+    case LibUtilities::eTriangle: return m * (m + 1) / 2;
+    case LibUtilities::ePrism: return m * m * (m + 1) / 2;
+    case LibUtilities::eTetrahedron: return m * (m + 1) * (m + 2) / 6;
+    case LibUtilities::ePyramid: return m * (m + 1) * (2 * m + 1) / 6;
+    default: {
+        int n = 1;
+        for (auto &b : basis) n *= b->GetNumModes();
+        return n;
+    }
+    }
 }
 
 int CollectionExp::GetTotPoints() const
@@ -416,7 +464,11 @@ OperatorSharedPtr create(const CollectionExp &e, CoalescedGeomDataSharedPtr g)
 
 const bool registered = [] {
     auto &f = GetOperatorFactory();
-    for (auto shape : {LibUtilities::eQuadrilateral, LibUtilities::eHexahedron}) {
+    for (auto shape : {LibUtilities::eQuadrilateral, LibUtilities::eHexahedron,
+                       LibUtilities::eTriangle, LibUtilities::ePrism, LibUtilities::eTetrahedron,
+                       LibUtilities::ePyramid}) {
+        const bool collapsed = shape != LibUtilities::eQuadrilateral &&
+                               shape != LibUtilities::eHexahedron;
         f.RegisterCreatorFunction(OperatorKey(shape, eBwdTrans, eMatrixFreeHip, false),
                                   create<BwdTrans_MatrixFreeHip>, "BwdTrans_MatrixFreeHip");
         f.RegisterCreatorFunction(OperatorKey(shape, eIProductWRTBase, eMatrixFreeHip, false),
@@ -426,6 +478,7 @@ const bool registered = [] {
                                   create<Helmholtz_MatrixFreeHip>, "Helmholtz_MatrixFreeHip");
         f.RegisterCreatorFunction(OperatorKey(shape, ePhysDeriv, eMatrixFreeHip, false),
                                   create<PhysDeriv_MatrixFreeHip>, "PhysDeriv_MatrixFreeHip");
+        if (collapsed) continue;
         f.RegisterCreatorFunction(OperatorKey(shape, eIProductWRTDerivBase, eMatrixFreeHip, false),
                                   create<IProductWRTDerivBase_MatrixFreeHip>,
                                   "IProductWRTDerivBase_MatrixFreeHip");

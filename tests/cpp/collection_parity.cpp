@@ -51,6 +51,129 @@ static std::vector<double> hex_mesh(int n, double jit, std::mt19937_64 &g)
     return v;
 }
 
+// The collapsed shapes through the Collections API: BwdTrans and Helmholtz of deformed
+// triangles, prisms, tetrahedra and pyramids against the oracle.
+static int collapsed_cases(std::mt19937_64 &g)
+{
+    using namespace LibUtilities;
+    std::uniform_real_distribution<double> U(-1, 1), J(-0.1, 0.1);
+    const int nm = 5, E = 37;
+    const double lam = 1.5;
+    int fails = 0;
+    auto scaled = [](const BasisSharedPtr &b, double f) {
+        std::vector<double> w(b->GetW());
+        for (auto &x : w) x *= f;
+        return w;
+    };
+    auto run = [&](const char *name, ShapeType shape, std::vector<BasisSharedPtr> bases,
+                   const double *ref_verts, int nv, int M, auto geom_fn, auto bwd_fn, auto helm_fn) {
+        const int dim = (int)bases.size();
+        int Q = 1;
+        for (auto &b : bases) Q *= b->GetNumPoints();
+        std::vector<double> verts(E * nv * dim);
+        for (int e = 0; e < E; ++e)
+            for (int i = 0; i < nv * dim; ++i) verts[e * nv * dim + i] = ref_verts[i] + J(g);
+        auto jac = std::make_shared<std::vector<double>>(E * Q);
+        auto df = std::make_shared<std::vector<double>>(dim * dim * E * Q);
+        geom_fn(verts.data(), jac->data(), df->data());
+        Collections::CollectionExp exp{shape, bases, E};
+        if (exp.GetNcoeffs() != M) {
+            printf("%s GetNcoeffs %d != %d (FAIL)\n", name, exp.GetNcoeffs(), M);
+            return 1;
+        }
+        auto geom = std::make_shared<Collections::CoalescedGeomData>(true, jac, df);
+        Collections::Collection coll(exp, geom,
+                                     Collections::SetFixedImpType(Collections::eMatrixFreeHip));
+        Array<OneD, NekDouble> u(E * M), phys(E * Q), out(E * M);
+        for (int i = 0; i < E * M; ++i) u[i] = U(g);
+        std::vector<double> rb(E * Q), rh(E * M);
+        coll.Initialise(Collections::eBwdTrans);
+        coll.ApplyOperator(Collections::eBwdTrans, u, phys);
+        bwd_fn(u.data(), rb.data());
+        double r = rel(phys.data(), rb.data(), E * Q);
+        printf("BwdTrans_MatrixFreeHip %s rel=%.3e\n", name, r);
+        int f = !(r < 1e-12);
+        StdRegions::ConstFactorMap factors;
+        factors[StdRegions::eFactorLambda] = lam;
+        coll.Initialise(Collections::eHelmholtz);
+        coll.ApplyOperator(Collections::eHelmholtz, u, out, factors);
+        helm_fn(jac->data(), df->data(), u.data(), rh.data());
+        r = rel(out.data(), rh.data(), E * M);
+        printf("Helmholtz_MatrixFreeHip %s deformed rel=%.3e\n", name, r);
+        return f + !(r < 1e-12);
+    };
+    {   // triangles: Modified_A x Modified_B
+        auto a = MakeModifiedA(nm, nm + 1), b = MakeModifiedB(nm, nm);
+        const double v[6] = {0, 0, 1, 0, 0.3, 1};
+        auto w1 = scaled(b, 0.5);
+        fails += run("tri", eTriangle, {a, b}, v, 3, nm * (nm + 1) / 2,
+            [&](const double *vv, double *j, double *d) { or_geom_tri(vv, E, nm + 1, nm, 1, j, d); },
+            [&](const double *in, double *o) {
+                or_bwdtrans_tri(nm, nm + 1, nm, a->GetBdata().data(), b->GetBdata().data(), E, in, o); },
+            [&](const double *j, const double *d, const double *in, double *o) {
+                or_helmholtz_tri(nm, nm + 1, nm, a->GetBdata().data(), a->GetDbdata().data(),
+                                 b->GetBdata().data(), b->GetDbdata().data(), a->GetD().data(),
+                                 b->GetD().data(), a->GetZ().data(), b->GetZ().data(),
+                                 a->GetW().data(), w1.data(), E, 1, j, d, lam, in, o); });
+    }
+    {   // prisms: Modified_A x Modified_A x Modified_B
+        auto a = MakeModifiedA(nm, nm + 1), b = MakeModifiedB(nm, nm);
+        const double v[18] = {0, 0, 0, 1, 0, 0, 1, 1, 0, 0, 1, 0, 0.5, 0, 1, 0.5, 1, 1};
+        auto w2 = scaled(b, 0.5);
+        fails += run("prism", ePrism, {a, a, b}, v, 6, nm * nm * (nm + 1) / 2,
+            [&](const double *vv, double *j, double *d) {
+                or_geom_prism(vv, E, nm + 1, nm + 1, nm, a->GetZ().data(), a->GetZ().data(),
+                              b->GetZ().data(), 1, j, d); },
+            [&](const double *in, double *o) {
+                or_bwdtrans_prism(nm, nm + 1, nm + 1, nm, a->GetBdata().data(), a->GetBdata().data(),
+                                  b->GetBdata().data(), E, in, o); },
+            [&](const double *j, const double *d, const double *in, double *o) {
+                or_helmholtz_prism(nm, nm + 1, nm + 1, nm, a->GetBdata().data(), a->GetDbdata().data(),
+                                   a->GetBdata().data(), a->GetDbdata().data(), b->GetBdata().data(),
+                                   b->GetDbdata().data(), a->GetD().data(), a->GetD().data(),
+                                   b->GetD().data(), a->GetZ().data(), b->GetZ().data(),
+                                   a->GetW().data(), a->GetW().data(), w2.data(), E, 1, j, d, lam,
+                                   in, o); });
+    }
+    {   // tetrahedra: Modified_A x Modified_B x Modified_C
+        auto a = MakeModifiedA(nm, nm + 1), b = MakeModifiedB(nm, nm), c = MakeModifiedC(nm, nm);
+        const double v[12] = {0, 0, 0, 1, 0, 0, 0, 1, 0, 0, 0, 1};
+        auto w1 = scaled(b, 0.5), w2 = scaled(c, 0.25);
+        fails += run("tet", eTetrahedron, {a, b, c}, v, 4, nm * (nm + 1) * (nm + 2) / 6,
+            [&](const double *vv, double *j, double *d) { or_geom_tet(vv, E, nm + 1, nm, nm, 1, j, d); },
+            [&](const double *in, double *o) {
+                or_bwdtrans_tet(nm, nm + 1, nm, nm, a->GetBdata().data(), b->GetBdata().data(),
+                                c->GetBdata().data(), E, in, o); },
+            [&](const double *j, const double *d, const double *in, double *o) {
+                or_helmholtz_tet(nm, nm + 1, nm, nm, a->GetBdata().data(), a->GetDbdata().data(),
+                                 b->GetBdata().data(), b->GetDbdata().data(), c->GetBdata().data(),
+                                 c->GetDbdata().data(), a->GetD().data(), b->GetD().data(),
+                                 c->GetD().data(), a->GetZ().data(), b->GetZ().data(),
+                                 c->GetZ().data(), a->GetW().data(), w1.data(), w2.data(), E, 1, j,
+                                 d, lam, in, o); });
+    }
+    {   // pyramids: Modified_A x Modified_A x ModifiedPyr_C
+        auto a = MakeModifiedA(nm, nm + 1), c = MakeModifiedPyrC(nm, nm);
+        const double v[15] = {0, 0, 0, 1, 0, 0, 1, 1, 0, 0, 1, 0, 0.5, 0.5, 0.866};
+        auto w2 = scaled(c, 0.25);
+        fails += run("pyr", ePyramid, {a, a, c}, v, 5, nm * (nm + 1) * (2 * nm + 1) / 6,
+            [&](const double *vv, double *j, double *d) {
+                or_geom_pyr(vv, E, nm + 1, nm + 1, nm, a->GetZ().data(), a->GetZ().data(),
+                            c->GetZ().data(), 1, j, d); },
+            [&](const double *in, double *o) {
+                or_bwdtrans_pyr(nm, nm + 1, nm + 1, nm, a->GetBdata().data(), a->GetBdata().data(),
+                                c->GetBdata().data(), E, in, o); },
+            [&](const double *j, const double *d, const double *in, double *o) {
+                or_helmholtz_pyr(nm, nm + 1, nm + 1, nm, a->GetBdata().data(), a->GetDbdata().data(),
+                                 a->GetBdata().data(), a->GetDbdata().data(), c->GetBdata().data(),
+                                 c->GetDbdata().data(), a->GetD().data(), a->GetD().data(),
+                                 c->GetD().data(), a->GetZ().data(), a->GetZ().data(),
+                                 c->GetZ().data(), a->GetW().data(), a->GetW().data(), w2.data(), E,
+                                 1, j, d, lam, in, o); });
+    }
+    return fails;
+}
+
 int main()
 {
     int fails = 0;
@@ -178,7 +301,7 @@ int main()
         }
     };
     expect_throw("unknown key", [&] {
-        Collections::CollectionExp et{LibUtilities::eTetrahedron, {B, B, B}, 4};
+        Collections::CollectionExp et{LibUtilities::eSegment, {B}, 4};
         Collections::Collection ct(et, geom, Collections::SetFixedImpType(Collections::eMatrixFreeHip));
         ct.Initialise(Collections::eHelmholtz);
     });
@@ -194,6 +317,12 @@ int main()
             "Helmholtz_Hex_Deformed", {LibUtilities::MakeModifiedA(6, 14), LibUtilities::MakeModifiedA(6, 14),
                                        LibUtilities::MakeModifiedA(6, 14)}, 2);
     });
+    expect_throw("bases that do not match the shape", [&] {
+        Collections::CollectionExp et{LibUtilities::eTetrahedron, {B, B, B}, 4};
+        Collections::Collection ct(et, geom, Collections::SetFixedImpType(Collections::eMatrixFreeHip));
+        ct.Initialise(Collections::eHelmholtz);
+    });
+    fails += collapsed_cases(g);
     printf("%s\n", fails ? "FAIL" : "PASS");
     return fails ? 1 : 0;
 }

@@ -179,7 +179,7 @@ def test_pipelined_host_apply_is_bitwise_the_device_apply(key, shape, nm, nq, de
 def test_error_paths():
     b = O.basis(6, 7)
     with pytest.raises(H.MFError, match="No such operator key"):
-        H.Operator("Helmholtz_Tet_Deformed", 6, 7, 4)
+        H.Operator("Helmholtz_Cube_Deformed", 6, 7, 4)
     with pytest.raises(H.MFError, match="No such operator key"):
         H.Operator("BwdTrans_Hex_Deformed", 6, 7, 4)
     with pytest.raises(H.MFError, match="modes / points combo"):

@@ -231,6 +231,22 @@ int imf_assembly_map_aniso(int nElmt, const int *nm3, const long long *vertex_id
 int imf_assembly_create_aniso(int nElmt, const int *nm3, const long long *vertex_ids,
                               int dirichlet, imf_asm *assembly);
 long imf_assembly_nglobal(imf_asm assembly);
+
+/* Boundary conditions that differ by region (ContField with a mix of Dirichlet, Neumann and
+ * Robin regions, e.g. Demos/MultiRegions/Tests/Helmholtz3D_CG_Hex_AllBCs.tst):
+ * imf_assembly_set_dirichlet_mask replaces the Dirichlet set by mask[g] != 0 over the
+ * nglobal modes (the closure of the Dirichlet faces; NULL: none); those modes are excluded
+ * from the solve as the create calls' dirichlet = 1 excludes every boundary mode.
+ * imf_assembly_set_robin gives the Robin trace terms alpha * (trace mass) as a sparse
+ * matrix R over the global modes (COO, nnz entries, repeated entries summed; nnz = 0
+ * removes it); the solve's operator becomes mask0(Assemble(H(GlobalToLocal(x))) + R x), as
+ * GlobalLinSysIterativeFull::v_DoMatrixMultiply adds AddRobinTraceContribution per element
+ * before the assembly (GlobalLinSysIterativeFull.cpp:232-252), and diag(R) joins the Jacobi
+ * preconditioner.  Neumann and Robin right-hand-side integrals are the caller's (host-side
+ * set-up, as the Dirichlet values).  Both calls are synchronous and copy their arrays. */
+int imf_assembly_set_dirichlet_mask(imf_asm assembly, const unsigned char *mask);
+int imf_assembly_set_robin(imf_asm assembly, long nnz, const int *rows, const int *cols,
+                           const double *vals);
 int imf_assembly_destroy(imf_asm assembly);
 
 /* AssemblyMapCG::GlobalToLocal (MultiRegions/AssemblyMap/AssemblyMapCG.cpp:2853-2876):

@@ -551,13 +551,16 @@ __global__ void assemble_dot3_kernel(long ng, const int *__restrict__ off,
                                      const signed char *__restrict__ sg,
                                      const double *__restrict__ loc,
                                      const unsigned char *__restrict__ mask,
+                                     const double *__restrict__ extra,
                                      const double *__restrict__ r, const double *__restrict__ w,
                                      double *__restrict__ s_out, double *__restrict__ part)
 {
     __shared__ double sh[3][RB / 64];
     double s0 = 0, s1 = 0, s2 = 0;
     for (long g = (long)blockIdx.x * RB + threadIdx.x; g < ng; g += (long)gridDim.x * RB) {
-        const double sv = assemble_one(g, off, inv, sg, loc, mask);
+        double sv = assemble_one(g, off, inv, sg, loc, mask);
+        // the Robin trace term of the global mode, when there is one (zero on Dirichlet modes)
+        if (extra && !(mask && mask[g])) sv += extra[g];
         s_out[g] = sv;
         const double rg = r[g], wg = w[g];
         s0 += rg * wg;
@@ -604,6 +607,28 @@ __global__ void dot3_final(int nb, const double *__restrict__ part, double *__re
 
 inline unsigned nblk(long n) { return (unsigned)((n + RB - 1) / RB); }
 
+// y = R x for the Robin matrix R in CSR form over the global modes
+__global__ void csr_spmv(long ng, const int *__restrict__ off, const int *__restrict__ col,
+                         const double *__restrict__ val, const double *__restrict__ x,
+                         double *__restrict__ y)
+{
+    const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= ng) return;
+    double s = 0.0;
+    for (int k = off[g]; k < off[g + 1]; ++k) s += val[k] * x[col[k]];
+    y[g] = s;
+}
+
+// d += diag(R)
+__global__ void csr_diag_add(long ng, const int *__restrict__ off, const int *__restrict__ col,
+                             const double *__restrict__ val, double *__restrict__ d)
+{
+    const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= ng) return;
+    for (int k = off[g]; k < off[g + 1]; ++k)
+        if (col[k] == g) d[g] += val[k];
+}
+
 // blocks of the fused assembly + dot pass (16 per CU on a 256-CU device)
 constexpr long kAsmParts = 4096;
 
@@ -619,6 +644,10 @@ struct imf_asm_s {
     signed char *d_sg = nullptr;
     int *d_off = nullptr, *d_inv = nullptr;  // inverse map (CSR), 32-bit
     unsigned char *d_mask = nullptr;  // global Dirichlet mask (NULL without Dirichlet)
+    // Robin trace terms: a sparse matrix over the global modes (CSR), and R w of the solve
+    long r_nnz = 0;
+    int *d_roff = nullptr, *d_rcol = nullptr;
+    double *d_rval = nullptr, *d_rw = nullptr;
     // solver workspace (allocated on the first solve)
     double *d_ws = nullptr, *d_loc = nullptr, *d_part = nullptr, *d_dots = nullptr;
     int nparts = 0;
@@ -660,9 +689,11 @@ int apply_global_dots(imf_asm a, int nops, const imf_op *ops, const double *w, c
             e0 += ops[g]->nElmt;
         }
     }
+    if (a->r_nnz)
+        csr_spmv<<<nblk(a->ng), RB, 0, s>>>(a->ng, a->d_roff, a->d_rcol, a->d_rval, w, a->d_rw);
     const int nb = (int)std::min<long>(kAsmParts, nblk(a->ng));
-    assemble_dot3_kernel<<<nb, RB, 0, s>>>(a->ng, a->d_off, a->d_inv, a->d_sg, l1, a->d_mask, r,
-                                           w, out, a->d_part);
+    assemble_dot3_kernel<<<nb, RB, 0, s>>>(a->ng, a->d_off, a->d_inv, a->d_sg, l1, a->d_mask,
+                                           a->r_nnz ? a->d_rw : nullptr, r, w, out, a->d_part);
     dot3_final<<<1, RB, 0, s>>>(nb, a->d_part, a->d_dots);
     ACHECK(hipGetLastError(), "assembly kernels");
     return IMF_OK;
@@ -838,9 +869,66 @@ int imf_assembly_destroy(imf_asm a)
     if (!a) return IMF_OK;
     for (void *p : {(void *)a->d_l2g, (void *)a->d_sg, (void *)a->d_off, (void *)a->d_inv,
                     (void *)a->d_mask, (void *)a->d_ws, (void *)a->d_loc, (void *)a->d_part,
-                    (void *)a->d_dots})
+                    (void *)a->d_dots, (void *)a->d_roff, (void *)a->d_rcol, (void *)a->d_rval,
+                    (void *)a->d_rw})
         if (p) (void)hipFree(p);
     delete a;
+    return IMF_OK;
+}
+
+int imf_assembly_set_dirichlet_mask(imf_asm a, const unsigned char *mask)
+{
+    if (!a) return afail(IMF_ERR_ARG, "imf_assembly_set_dirichlet_mask: null assembly");
+    a->diag_key.clear();  // the cached preconditioner excludes the Dirichlet modes
+    if (!mask) {
+        if (a->d_mask) (void)hipFree(a->d_mask);
+        a->d_mask = nullptr;
+        a->dirichlet = 0;
+        return IMF_OK;
+    }
+    if (!a->d_mask) ACHECK(hipMalloc(&a->d_mask, a->ng), "hipMalloc(Dirichlet mask)");
+    std::vector<unsigned char> m(mask, mask + a->ng);
+    for (auto &v : m) v = v ? 1 : 0;
+    ACHECK(hipMemcpy(a->d_mask, m.data(), a->ng, hipMemcpyHostToDevice), "hipMemcpy(Dirichlet mask)");
+    a->dirichlet = 1;
+    return IMF_OK;
+}
+
+int imf_assembly_set_robin(imf_asm a, long nnz, const int *rows, const int *cols,
+                           const double *vals)
+{
+    if (!a || nnz < 0 || (nnz > 0 && (!rows || !cols || !vals)))
+        return afail(IMF_ERR_ARG, "imf_assembly_set_robin: bad argument");
+    for (long k = 0; k < nnz; ++k)
+        if (rows[k] < 0 || rows[k] >= a->ng || cols[k] < 0 || cols[k] >= a->ng)
+            return afail(IMF_ERR_ARG, "imf_assembly_set_robin: index outside the global modes");
+    for (void *p : {(void *)a->d_roff, (void *)a->d_rcol, (void *)a->d_rval, (void *)a->d_rw})
+        if (p) (void)hipFree(p);
+    a->d_roff = a->d_rcol = nullptr;
+    a->d_rval = a->d_rw = nullptr;
+    a->r_nnz = 0;
+    a->diag_key.clear();  // the cached preconditioner includes diag(R)
+    if (nnz == 0) return IMF_OK;
+    // CSR by row, entries of a row in the order given (repeated entries are summed by the
+    // product)
+    std::vector<int> off(a->ng + 1, 0), col(nnz);
+    std::vector<double> val(nnz);
+    for (long k = 0; k < nnz; ++k) ++off[rows[k] + 1];
+    for (long g = 0; g < a->ng; ++g) off[g + 1] += off[g];
+    std::vector<int> fill(off.begin(), off.end() - 1);
+    for (long k = 0; k < nnz; ++k) {
+        const int at = fill[rows[k]]++;
+        col[at] = cols[k];
+        val[at] = vals[k];
+    }
+    ACHECK(hipMalloc(&a->d_roff, (a->ng + 1) * sizeof(int)), "hipMalloc(Robin)");
+    ACHECK(hipMalloc(&a->d_rcol, nnz * sizeof(int)), "hipMalloc(Robin)");
+    ACHECK(hipMalloc(&a->d_rval, nnz * sizeof(double)), "hipMalloc(Robin)");
+    ACHECK(hipMalloc(&a->d_rw, a->ng * sizeof(double)), "hipMalloc(Robin)");
+    ACHECK(hipMemcpy(a->d_roff, off.data(), (a->ng + 1) * sizeof(int), hipMemcpyHostToDevice), "hipMemcpy(Robin)");
+    ACHECK(hipMemcpy(a->d_rcol, col.data(), nnz * sizeof(int), hipMemcpyHostToDevice), "hipMemcpy(Robin)");
+    ACHECK(hipMemcpy(a->d_rval, val.data(), nnz * sizeof(double), hipMemcpyHostToDevice), "hipMemcpy(Robin)");
+    a->r_nnz = nnz;
     return IMF_OK;
 }
 
@@ -934,6 +1022,8 @@ int imf_helmsolve_cg_groups_device(imf_asm a, int nops, const imf_op *ops, const
             e0 += op->nElmt;
         }
         assemble_kernel<<<nblk(ng), RB, 0, s>>>(ng, a->d_off, a->d_inv, nullptr, a->d_loc, nullptr, dinv);
+        if (a->r_nnz)
+            csr_diag_add<<<nblk(ng), RB, 0, s>>>(ng, a->d_roff, a->d_rcol, a->d_rval, dinv);
         inv_masked<<<nblk(ng), RB, 0, s>>>(ng, a->d_mask, dinv);
         ACHECK(hipGetLastError(), "diagonal kernels");
         a->diag_key = key;

@@ -39,6 +39,8 @@ def _lib():
         L.imf_helmsolve_cg_device.argtypes = [vp, vp, vp, vp, d, i, ctypes.POINTER(i),
                                               ctypes.POINTER(d), vp]
         L.imf_assembly_last_error.restype = ctypes.c_char_p
+        L.imf_assembly_set_dirichlet_mask.argtypes = [vp, vp]
+        L.imf_assembly_set_robin.argtypes = [vp, l_, vp, vp, vp]
         _SET = True
     return L
 
@@ -246,6 +248,115 @@ def dirichlet_lift(coords, vertex_ids, nm, fn, nq=None):
     return g
 
 
+# Local faces of a hexahedron in the reference's order (SpatialDomains/HexGeom.cpp): the
+# normal direction d (0: xi0, 1: xi1, 2: xi2) and the side (0: xi_d = -1, 1: xi_d = +1).
+HEX_FACES = [(2, 0), (1, 0), (0, 1), (1, 1), (0, 0), (2, 1)]
+
+
+def _hex_map_derivs(coords_e, z):
+    """dx/dxi_a (a = 0, 1, 2) of the trilinear map at the GLL grid: (3, nq, nq, nq, 3)."""
+    lo, hi = 0.5 * (1.0 - z), 0.5 * (1.0 + z)
+    half = np.full_like(z, 0.5)
+    corners = [(0, 0, 0), (1, 0, 0), (1, 1, 0), (0, 1, 0),
+               (0, 0, 1), (1, 0, 1), (1, 1, 1), (0, 1, 1)]
+    out = np.zeros((3, len(z), len(z), len(z), 3))
+    for v, c in enumerate(corners):
+        for a in range(3):
+            f = [hi if c[d] else lo for d in range(3)]
+            f[a] = half if c[a] else -half
+            out[a] += np.einsum("k,j,i->kji", f[2], f[1], f[0])[..., None] * coords_e[v]
+    return out
+
+
+def hex_face_quadrature(coords_e, nm, face, nq=None):
+    """The quadrature of one face of a straight-sided hexahedron (vertices coords_e (8, 3)
+    in local order) on the element's own GLL points, as the reference integrates over the
+    face's trace expansion (its points are the element's points in the two tangential
+    directions): returns (local mode indices of the face's nm*nm modes, points (nq*nq, 3),
+    weights times the surface Jacobian (nq*nq), basis values (nq*nq, nm*nm)).  The other
+    element modes vanish on the face."""
+    nq = nm + 1 if nq is None else nq
+    z, w, B = _tables(nm, nq)
+    d, side = HEX_FACES[face]
+    t0, t1 = [a for a in range(3) if a != d]
+    P = _points(np.asarray(coords_e, dtype=np.float64)[None], z)[0]     # [k][j][i]
+    dX = _hex_map_derivs(np.asarray(coords_e, dtype=np.float64), z)
+    sl = [slice(None)] * 3
+    sl[2 - d] = 0 if side == 0 else nq - 1
+    sl = tuple(sl)
+    # face arrays indexed [j1][j0] with j0 along t0, j1 along t1 (t0 < t1: t0 is the
+    # faster array axis of the [k][j][i] grid)
+    pts = P[sl].reshape(-1, 3)
+    J = np.linalg.norm(np.cross(dX[t0][sl], dX[t1][sl]), axis=-1).reshape(-1)
+    W = np.outer(w, w).reshape(-1) * J
+    phi = np.einsum("jb,ia->jiba", B, B).reshape(nq * nq, nm * nm)  # [(j1, j0), (b, a)]
+    loc = np.arange(nm ** 3).reshape(nm, nm, nm)                      # [r][q][p]
+    ml = [slice(None)] * 3
+    ml[2 - d] = side
+    modes = loc[tuple(ml)].reshape(-1)                                  # [b][a]
+    return modes, pts, W, phi
+
+
+def boundary_terms(coords, vertex_ids, nm, neumann=(), robin=(), nq=None):
+    """The boundary integrals of a straight-sided hex mesh's Neumann and Robin conditions
+    (DisContField::v_... / ContField HelmSolve: the boundary expansions' IProductWRTBase of
+    the condition's values, and for Robin du/dn + alpha u = g the trace mass matrix times
+    alpha that GlobalLinSysIterativeFull::v_DoMatrixMultiply adds through
+    AddRobinTraceContribution, GlobalLinSysIterativeFull.cpp:232-252).  neumann: list of
+    (faces, g) with faces [(element, local face), ...] and g(points (n, 3)) -> du/dn;
+    robin: list of (faces, alpha, g).  Returns (rhs (nglobal,), rows, cols, vals): the
+    right-hand-side contribution and the Robin matrix in global coordinates (COO, with the
+    signs of the map)."""
+    coords = np.asarray(coords, dtype=np.float64)
+    l2g, sign, _, ng = assembly_map(vertex_ids, nm, "hex")
+    M = nm ** 3
+    rhs = np.zeros(ng)
+    rows, cols, vals = [], [], []
+    for faces, alpha, g in [(f, None, g) for f, g in neumann] + list(robin):
+        for e, lf in faces:
+            modes, pts, W, phi = hex_face_quadrature(coords[e], nm, lf, nq)
+            li = e * M + modes
+            gi, s = l2g[li], sign[li].astype(np.float64)
+            np.add.at(rhs, gi, s * (phi.T @ (W * g(pts))))
+            if alpha is not None:
+                Mf = alpha * (phi.T @ (W[:, None] * phi)) * np.outer(s, s)
+                rows.append(np.repeat(gi, len(gi)))
+                cols.append(np.tile(gi, len(gi)))
+                vals.append(Mf.ravel())
+    if rows:
+        return rhs, np.concatenate(rows), np.concatenate(cols), np.concatenate(vals)
+    return rhs, np.zeros(0, np.int64), np.zeros(0, np.int64), np.zeros(0)
+
+
+def dirichlet_lift_faces(coords, vertex_ids, nm, fn, faces, nq=None):
+    """dirichlet_lift restricted to the listed faces [(element, local face), ...] of a
+    straight-sided hex mesh, for conditions that differ by boundary region: returns
+    (g, mask), g the global vector of the projected Dirichlet values (zero elsewhere) and
+    mask the global modes on those faces' closure (vertices and edges included, which the
+    reference also fixes by the Dirichlet condition)."""
+    coords = np.asarray(coords, dtype=np.float64)
+    nq = nm + 1 if nq is None else nq
+    z, w, B = _tables(nm, nq)
+    l2g, sign, _, ng = assembly_map(vertex_ids, nm, "hex")
+    g = np.zeros(ng)
+    mask = np.zeros(ng, dtype=bool)
+    P = _points(coords, z)
+    M = nm ** 3
+    loc = np.arange(M).reshape(nm, nm, nm)
+    for e, lf in faces:
+        d, side = HEX_FACES[lf]
+        U = fn(P[e].reshape(-1, 3)).reshape(nq, nq, nq)
+        sl = [slice(None)] * 3
+        sl[2 - d] = 0 if side == 0 else nq - 1
+        ml = [slice(None)] * 3
+        ml[2 - d] = side
+        li = e * M + loc[tuple(ml)].ravel()
+        c = _constrained_2d(U[tuple(sl)], B, w)
+        g[l2g[li]] = sign[li] * c.ravel()
+        mask[l2g[li]] = True
+    return g, mask
+
+
 def _dirichlet_lift_aniso(coords, vertex_ids, nm3, fn, nq3=None):
     """dirichlet_lift for hexahedra with nm3[d] modes on nq3[d] GLL points (nm3[d] + 1 by
     default) along direction d, the same for every element: each boundary face is
@@ -311,6 +422,31 @@ class Assembly:
             self.nlocal = int(np.sum(nmv.astype(np.int64) ** dim))
         self._h = h
         self.nglobal = _lib().imf_assembly_nglobal(h)
+        self._robin = None
+
+    def set_dirichlet_mask(self, mask):
+        """Replace the Dirichlet modes by mask (nglobal booleans; None: no Dirichlet modes),
+        for conditions that differ by boundary region (see dirichlet_lift_faces)."""
+        if mask is None:
+            _check(_lib().imf_assembly_set_dirichlet_mask(self._h, None))
+            return
+        m = np.ascontiguousarray(np.asarray(mask).astype(np.uint8))
+        if m.shape != (self.nglobal,):
+            raise MFError(f"set_dirichlet_mask: expected {self.nglobal} entries, got {m.shape}")
+        _check(_lib().imf_assembly_set_dirichlet_mask(self._h, m.ctypes.data_as(ctypes.c_void_p)))
+
+    def set_robin(self, rows, cols, vals):
+        """The Robin trace matrix over the global modes (COO, e.g. from boundary_terms),
+        added to the solve's operator."""
+        r = np.ascontiguousarray(rows, dtype=np.int32)
+        c = np.ascontiguousarray(cols, dtype=np.int32)
+        v = np.ascontiguousarray(vals, dtype=np.float64)
+        if not (r.shape == c.shape == v.shape):
+            raise MFError("set_robin: rows, cols and vals differ in length")
+        _check(_lib().imf_assembly_set_robin(self._h, len(v), r.ctypes.data_as(ctypes.c_void_p),
+                                             c.ctypes.data_as(ctypes.c_void_p),
+                                             v.ctypes.data_as(ctypes.c_void_p)))
+        self._robin = (r.astype(np.int64), c.astype(np.int64), v) if len(v) else None
 
     def __del__(self):
         h = getattr(self, "_h", None)
@@ -349,10 +485,15 @@ class Assembly:
         order).  Dirichlet modes are zero, or, with `lift` (a global vector holding the
         Dirichlet values, e.g. from dirichlet_lift), equal to lift: the solve is then for
         x - lift with rhs - A lift, as the reference's ContField::HelmSolve treats
-        inhomogeneous Dirichlet conditions."""
+        inhomogeneous Dirichlet conditions (A includes the Robin matrix of set_robin)."""
         if lift is not None:
             lift = torch.as_tensor(lift, dtype=torch.float64, device=rhs.device)
             rhs = rhs - self.assemble(self.apply_local(op, self.global_to_local(lift)))
+            if self._robin is not None:  # the operator's Robin part acting on the lift
+                r, c, v = self._robin
+                lh = lift.cpu().numpy()
+                rl = np.bincount(r, weights=v * lh[c], minlength=self.nglobal)
+                rhs = rhs - torch.from_numpy(rl).to(rhs.device)
         x = torch.empty_like(rhs) if x is None else x
         it, res = ctypes.c_int(), ctypes.c_double()
         ops = list(op) if isinstance(op, (list, tuple)) else [op]

@@ -124,6 +124,8 @@ class Mesh:
             self.element_vertex_ids = np.array(
                 [self._hex_vertices(tuple(int(x) for x in h["f"])) for h in elements],
                 dtype=np.int64).reshape(-1, 8)
+            self.element_face_ids = np.array([[int(x) for x in h["f"]] for h in elements],
+                                             dtype=np.int64).reshape(-1, 6)
         else:
             self.elements = np.array([int(q["id"]) for q in elements], dtype=np.int64)
             self.element_vertex_ids = np.array(
@@ -200,6 +202,27 @@ class Mesh:
         bottom = chain(first_two(0, 1), [1, 2])
         top = chain(first_two(8, 9), [9, 10])
         return bottom + top
+
+    def boundary_faces(self, composite):
+        """(element row, local face) of every face of a face composite ("F[...]" text or
+        its id), local faces in the reference's hex order: 0 the xi2 = -1 face (vertices
+        0-3), 1 xi1 = -1, 2 xi0 = +1, 3 xi1 = +1, 4 xi0 = -1, 5 xi2 = +1
+        (SpatialDomains/HexGeom.cpp; the faces of the H records in that order)."""
+        if self.dim != 3:
+            raise NektarIOError("boundary_faces: hexahedral meshes only")
+        text = self.composites[int(composite)] if not isinstance(composite, str) else composite
+        where = {}
+        for row, fids in enumerate(self.element_face_ids):
+            for lf, f in enumerate(fids):
+                where.setdefault(int(f), []).append((row, lf))
+        out = []
+        for f in _composite_ids(text):
+            owners = where.get(int(f), [])
+            if len(owners) != 1:
+                raise NektarIOError(f"face {int(f)} belongs to {len(owners)} elements, "
+                                    "not to one (not a boundary face)")
+            out.append(owners[0])
+        return out
 
     # -- helpers -------------------------------------------------------------------
     def element_coords(self, element_ids=None):

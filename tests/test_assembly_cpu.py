@@ -331,3 +331,101 @@ def test_helmholtz3d_hex_heterogeneous_golden():
     print(f"L2 {np.sqrt(l2):.6e}  Linf {linf:.6e}")
     assert abs(float(f"{np.sqrt(l2):.6g}") - 5.23385e-05) <= 1e-12
     assert abs(float(f"{linf:.6g}") - 0.00014837) <= 1e-12
+
+
+# ---------------------------------------------------------------- mixed boundary conditions
+
+def _allbcs_problem():
+    """Demos/MultiRegions/Tests/Helmholtz3D_CG_Hex_AllBCs.tst: the session
+    Helmholtz3D_Hex_AllBCs_P6.xml (27 hexes, NUMMODES 4, lambda = 1, u = sin(pi x/2)
+    sin(pi y/2) sin(pi z/2)) with Dirichlet regions C[1], C[5], Neumann C[2], C[6] and Robin
+    C[3] (du/dn + 2u = g) and C[4] (du/dn + 3u = g); the condition values as the session
+    states them."""
+    import os
+    from ithaca_hip import nektar_io
+    gold = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "nektar")
+    msh = nektar_io.read_mesh(os.path.join(gold, "Helmholtz3D_Hex_AllBCs_P6.xml"))
+    pi = np.pi
+    S = lambda P, k: np.sin(pi / 2 * P[:, k])
+    C = lambda P, k: np.cos(pi / 2 * P[:, k])
+    uex = lambda P: S(P, 0) * S(P, 1) * S(P, 2)
+    F = {c: msh.boundary_faces(c) for c in range(1, 7)}
+    bc = dict(dirichlet=F[1] + F[5],
+              neumann=[(F[2], lambda P: -S(P, 0) * C(P, 1) * S(P, 2) * pi / 2),
+                       (F[6], lambda P: S(P, 0) * S(P, 1) * C(P, 2) * pi / 2)],
+              robin=[(F[3], 2.0, lambda P: C(P, 0) * S(P, 1) * S(P, 2) * pi / 2 + 2 * uex(P)),
+                     (F[4], 3.0, lambda P: S(P, 0) * C(P, 1) * S(P, 2) * pi / 2 + 3 * uex(P))])
+    return msh, uex, bc
+
+
+def _printed(v):
+    """The value as the reference's tester reads it from the program's output (the default
+    six significant digits of the stream)."""
+    return float(f"{v:.6g}")
+
+
+def test_helmholtz3d_hex_allbcs_golden():
+    """Helmholtz3D_CG_Hex_AllBCs.tst: L2 error 0.000416575, Linf 0.000871589 (tolerance
+    1e-12 on the printed values).  Here: the Dirichlet projection on the Dirichlet faces
+    only (dirichlet_lift_faces), the Neumann and Robin face integrals and the Robin trace
+    matrix (boundary_terms), dense element matrices and a sparse direct solve."""
+    msh, uex, bc = _allbcs_problem()
+    X, ids = msh.element_coords(), msh.vertex_ids()
+    nm = msh.expansions[0]["nummodes"]
+    nq, M, lam = nm + 1, nm ** 3, 1.0
+    g, dmask = assembly.dirichlet_lift_faces(X, ids, nm, uex, bc["dirichlet"])
+    brhs, rr, rc, rv = assembly.boundary_terms(X, ids, nm, bc["neumann"], bc["robin"])
+    l2g, sign, _, ng = assembly.assembly_map_hex(ids, nm)
+    mats = [DR.element_mats("hex", nm, nq, X[e]) for e in range(len(ids))]
+    rows, cols, vals, rhs = [rr], [rc], [rv], brhs.copy()
+    for e, m in enumerate(mats):
+        gi, s = l2g[e * M:(e + 1) * M], sign[e * M:(e + 1) * M].astype(float)
+        rows.append(np.repeat(gi, M))
+        cols.append(np.tile(gi, M))
+        vals.append((DR.helmholtz_matrix(m, lam) * np.outer(s, s)).ravel())
+        np.add.at(rhs, gi, s * (m["B"].T @ (m["W"] * m["J"] * (lam + 3 * np.pi ** 2 / 4) * uex(m["X"]))))
+    A = sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))),
+                      shape=(ng, ng))
+    x = g.copy()
+    free = ~dmask
+    x[free] = spla.spsolve(A[free][:, free].tocsc(), rhs[free] - A[free][:, dmask] @ g[dmask])
+    l2 = linf = 0.0
+    for e, m in enumerate(mats):
+        d = m["B"] @ (sign[e * M:(e + 1) * M] * x[l2g[e * M:(e + 1) * M]]) - uex(m["X"])
+        l2 += np.sum(m["W"] * m["J"] * d * d)
+        linf = max(linf, np.abs(d).max())
+    print(f"L2 {np.sqrt(l2):.9e}  Linf {linf:.9e}")
+    assert abs(_printed(np.sqrt(l2)) - 0.000416575) <= 1e-12
+    assert abs(_printed(linf) - 0.000871589) <= 1e-12
+
+
+def test_boundary_terms_integrate_the_faces():
+    """With g = 1 the Neumann integrals against the constant field's coefficients give the
+    areas of the faces (computed from the vertices as two triangles each), the Robin matrix
+    is symmetric and 1^T R 1 = alpha * area."""
+    msh, _, bc = _allbcs_problem()
+    X, ids = msh.element_coords(), msh.vertex_ids()
+    nm = 4
+    one = lambda P: np.ones(len(P))
+    g1, _ = assembly.dirichlet_lift_faces(X, ids, nm, one, [(e, f) for e in range(len(ids))
+                                                            for f in range(6)])
+    # the constant field: every vertex mode 1, every other mode 0
+    l2g, sign, _, ng = assembly.assembly_map_hex(ids, nm)
+    c1 = np.zeros(ng)
+    loc = np.arange(nm ** 3).reshape(nm, nm, nm)
+    vert = loc[:2, :2, :2].ravel()
+    for e in range(len(ids)):
+        c1[l2g[e * nm ** 3 + vert]] = 1.0
+    assert np.allclose(g1[np.abs(g1) > 0], c1[np.abs(g1) > 0])
+    corners = {0: (0, 1, 2, 3), 1: (0, 1, 5, 4), 2: (1, 2, 6, 5), 3: (3, 2, 6, 7),
+               4: (0, 3, 7, 4), 5: (4, 5, 6, 7)}
+    for faces in (bc["dirichlet"], bc["robin"][0][0]):
+        area = 0.0
+        for e, f in faces:
+            a, b, c, d = (X[e][v] for v in corners[f])
+            area += 0.5 * (np.linalg.norm(np.cross(b - a, c - a)) + np.linalg.norm(np.cross(c - a, d - a)))
+        rhs, rr, rc, rv = assembly.boundary_terms(X, ids, nm, [(faces, one)], [(faces, 2.5, one)])
+        assert abs(rhs @ c1 - 2 * area) < 1e-12 * area   # the Neumann and the Robin g = 1
+        R = sp.csr_matrix((rv, (rr, rc)), shape=(ng, ng))
+        assert abs(c1 @ (R @ c1) - 2.5 * area) < 1e-12 * area
+        assert abs(R - R.T).max() < 1e-15

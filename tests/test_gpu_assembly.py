@@ -267,6 +267,49 @@ def test_helmholtz3d_hex_golden_on_the_gpu():
     assert abs(linf - 5.66374e-07) <= 1e-12
 
 
+def test_helmholtz3d_hex_allbcs_golden_on_the_gpu():
+    """Demos/MultiRegions/Tests/Helmholtz3D_CG_Hex_AllBCs.tst (L2 0.000416575, Linf
+    0.000871589, tolerance 1e-12 on the printed values) through the GPU path: Dirichlet on
+    two regions (the mask of the Dirichlet faces' closure and their lift), Neumann on two
+    and Robin on two (the face integrals on the right-hand side and the Robin trace matrix
+    inside the GPU conjugate gradients)."""
+    from test_assembly_cpu import _allbcs_problem, _printed
+    msh, uex, bc = _allbcs_problem()
+    X, ids = msh.element_coords(), msh.vertex_ids()
+    nm = msh.expansions[0]["nummodes"]
+    nq, E, lam = nm + 1, len(ids), 1.0
+    print(f"[case] allbcs E={E} nm={nm}", flush=True)
+    b = H.basis_modified_a(nm, nq)
+    jac, df = H.geom_device("hex", _dev(X), nq, b["z"], True)
+    Hop = H.Operator("Helmholtz_Hex_Deformed", nm, nq, E, basis=b)
+    Hop.set_jac(jac)
+    Hop.set_df(df)
+    Hop.set_lambda(lam)
+    ipr = H.Operator("IProduct_Hex_Deformed", nm, nq, E, basis=b)
+    ipr.set_jac(jac)
+    bwd = H.Operator("BwdTrans_Hex_Regular", nm, nq, E, basis=b)
+    P = mesh.hex_quad_points(_dev(X), b["z"]).reshape(-1, 3).cpu().numpy()
+    A = assembly.Assembly(ids, nm, dirichlet=True)
+    g, dmask = assembly.dirichlet_lift_faces(X, ids, nm, uex, bc["dirichlet"])
+    brhs, rr, rc, rv = assembly.boundary_terms(X, ids, nm, bc["neumann"], bc["robin"])
+    A.set_dirichlet_mask(dmask)
+    A.set_robin(rr, rc, rv)
+    rhs = A.assemble(ipr.apply(_dev((lam + 3 * np.pi ** 2 / 4) * uex(P)))) + _dev(brhs)
+    x, it, res = A.helmsolve(Hop, rhs, tol=1e-14, max_iter=5000, lift=g)
+    d = bwd.apply(A.global_to_local(x)).cpu().numpy() - uex(P)
+    wq = np.einsum("k,j,i->kji", b["w"], b["w"], b["w"]).ravel()
+    wj = jac.cpu().numpy() * np.tile(wq, E)
+    l2, linf = np.sqrt(np.sum(wj * d * d)), np.abs(d).max()
+    print(f"iterations {it}, residual {res:.1e}, L2 {l2:.9e}, Linf {linf:.9e}")
+    assert abs(_printed(l2) - 0.000416575) <= 1e-12
+    assert abs(_printed(linf) - 0.000871589) <= 1e-12
+    # removing the Robin matrix and the mask again restores the all-Dirichlet assembly
+    A.set_robin([], [], [])
+    A.set_dirichlet_mask(None)
+    with pytest.raises(H.MFError):
+        A.set_robin([0], [A.nglobal], [1.0])
+
+
 # ---------------------------------------------------------------- variable order
 
 def _runs(nmv):

@@ -43,6 +43,10 @@ def parse():
     p.add_argument("--headline-only", action="store_true",
                    help="time the Helmholtz apply only (no further sections; for counter runs)")
     p.add_argument("--pod-snapshots", type=int, default=512)
+    p.add_argument("--rom-only", action="store_true",
+                   help="the batched reduced solves only, one JSON object (for counter runs)")
+    p.add_argument("--rom-host-gen", action="store_true",
+                   help="generate the batched systems on the host (numpy) instead of the GPU")
     return p.parse_args()
 
 
@@ -161,7 +165,12 @@ def rom_bench(args):
 
     res = {}
     for kind in ("normalised", "literal"):
-        bt = rom.affine_batch_device(args.rom_instances, seed=3, kind=kind)
+        if args.rom_host_gen:  # no device QR in the process (e.g. under counter collection)
+            bt = rom.affine_batch(args.rom_instances, seed=3, kind=kind)
+            bt["A"] = torch.from_numpy(bt["A"]).cuda()
+        else:
+            bt = rom.affine_batch_device(args.rom_instances, seed=3, kind=kind)
+        print(f"[bench] batched solves: {kind} batch generated", file=sys.stderr, flush=True)
         B = rom.Batch(bt["N"], bt["Q"])
         A = bt["A"]
         th = torch.from_numpy(bt["theta"]).cuda()
@@ -185,6 +194,7 @@ def rom_bench(args):
             torch.cuda.synchronize()
             xs[name] = x.cpu().numpy()
             times[name] = ms
+            print(f"[bench] batched solves: {kind} {name} {ms:.3f} ms", file=sys.stderr, flush=True)
             st = status.cpu().numpy()
             r[name] = {"solves_per_s": n / (ms * 1e-3), "ms_per_batch": ms,
                        "frac_of_hbm_floor": r["hbm_floor_ms"] / ms,
@@ -919,6 +929,9 @@ def main():
     args = parse()
     if args.op == "copy":
         return copy_calibration(args)
+    if args.rom_only:
+        print(json.dumps({"batched_reduced_solves": rom_bench(args)}))
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "one GPU only: --gpus must be 1 (the path's sharding "
                                    "over several GPUs is not implemented)"}))

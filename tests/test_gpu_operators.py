@@ -387,3 +387,32 @@ def test_element_counts(shape, nm, nq):
             op = _op(f"Helmholtz_{S}_Deformed", nm, nq, E, b, jac, df, prec=prec)
             op.set_lambda(1.5)
             assert rel(op.apply(_dev(u, dt)).double().cpu().numpy(), ref_h) < tol
+
+
+@pytest.mark.parametrize("nq", [5, 8])
+def test_reference_collection_lambda_zero_cases(nq):
+    """library/UnitTests/Collections/TestHexCollection.cpp:3649-3745 and :3747-3845
+    (TestHexHelmholtz_MatrixFree_UniformP and ..._Deformed_OverInt): ten copies of the hex
+    with vertex 6 moved to (2, 3, 4), Modified_A with 4 modes on 5 (and, over-integrated, 8)
+    GLL points, lambda = 0, every coefficient 1; the MatrixFree result must match the
+    elemental Helmholtz matrix to BOOST_CHECK_CLOSE's 1e-8 percent after values below 1e-14
+    are set to zero.  Here the elemental matrix is the dense one of tests/dense_ref.py."""
+    import dense_ref as DR
+    print(f"[case] reference lambda = 0 hex, nq={nq}", flush=True)
+    v = np.array([[-1, -1, -1], [1, -1, -1], [1, 1, -1], [-1, 1, -1],
+                  [-1, -1, 1], [1, -1, 1], [2, 3, 4], [-1, 1, 1]], dtype=float)
+    nm, E = 4, 10
+    verts = np.repeat(v[None], E, 0)
+    b = H.basis_modified_a(nm, nq)
+    jac, df = H.geom_device("hex", torch.from_numpy(verts).cuda(), nq, b["z"], True)
+    op = H.Operator("Helmholtz_Hex_Deformed", nm, nq, E, basis=b)
+    op.set_jac(jac)
+    op.set_df(df)
+    op.set_lambda(0.0)
+    got = op.apply(torch.ones(E * nm ** 3, dtype=torch.float64, device="cuda")).cpu().numpy()
+    m = DR.element_mats("hex", nm, nq, v)
+    ref = np.tile(DR.helmholtz_matrix(m, 0.0) @ np.ones(nm ** 3), E)
+    got[np.abs(got) < 1e-14] = 0.0
+    ref[np.abs(ref) < 1e-14] = 0.0
+    close = np.abs(got - ref) <= 1e-10 * np.minimum(np.abs(got), np.abs(ref))  # the strong form
+    assert np.all(close), np.abs(got - ref).max()

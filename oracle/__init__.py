@@ -455,3 +455,38 @@ def helmholtz_pyr(b, E, deformed, jac, df, lam, u):
                            _p(b["w1"]), _p(b["w2"]), E, int(deformed), _p(jac), _p(df),
                            ctypes.c_double(lam), _p(u), _p(out))
     return out
+
+
+# ---- IProductWRTDerivBase of the collapsed shapes ----
+
+def iprodderiv_tri(b, E, deformed, jac, df, f0, f1):
+    f0, f1 = (np.ascontiguousarray(x, dtype=np.float64) for x in (f0, f1))
+    out = np.zeros(E * b["M"])
+    lib().or_iprodderiv_tri(b["nm"], b["nq0"], b["nq1"], _p(b["bdata0"]), _p(b["dbdata0"]),
+                            _p(b["bdata1"]), _p(b["dbdata1"]), _p(b["z0"]), _p(b["z1"]),
+                            _p(b["w0"]), _p(b["w1"]), E, int(deformed), _p(jac), _p(df), _p(f0),
+                            _p(f1), _p(out))
+    return out
+
+
+def _iprodderiv3(fn, b, E, deformed, jac, df, f, with_z1):
+    f = [np.ascontiguousarray(x, dtype=np.float64) for x in f]
+    out = np.zeros(E * b["M"])
+    zs = [_p(b["z0"])] + ([_p(b["z1"])] if with_z1 else []) + [_p(b["z2"])]
+    fn(b["nm"], b["nq0"], b["nq1"], b["nq2"], _p(b["bdata0"]), _p(b["dbdata0"]),
+       _p(b["bdata1"]), _p(b["dbdata1"]), _p(b["bdata2"]), _p(b["dbdata2"]), *zs,
+       _p(b["w0"]), _p(b["w1"]), _p(b["w2"]), E, int(deformed), _p(jac), _p(df),
+       *[_p(x) for x in f], _p(out))
+    return out
+
+
+def iprodderiv_prism(b, E, deformed, jac, df, f):
+    return _iprodderiv3(lib().or_iprodderiv_prism, b, E, deformed, jac, df, f, False)
+
+
+def iprodderiv_tet(b, E, deformed, jac, df, f):
+    return _iprodderiv3(lib().or_iprodderiv_tet, b, E, deformed, jac, df, f, True)
+
+
+def iprodderiv_pyr(b, E, deformed, jac, df, f):
+    return _iprodderiv3(lib().or_iprodderiv_pyr, b, E, deformed, jac, df, f, True)

@@ -181,6 +181,31 @@ void or_helmholtz_pyr(int nm, int nq0, int nq1, int nq2, const double *b0, const
                       const double *w2, int E, int deformed, const double *jac,
                       const double *df, double lambda, const double *in, double *out);
 
+/* IProductWRTDerivBase of the collapsed shapes: out = sum_c (d phi / d x_c, in_c) */
+void or_iprodderiv_tri(int nm, int nq0, int nq1, const double *b0, const double *db0,
+                       const double *b1, const double *db1, const double *z0, const double *z1,
+                       const double *w0, const double *w1, int E, int deformed,
+                       const double *jac, const double *df, const double *in0,
+                       const double *in1, double *out);
+void or_iprodderiv_prism(int nm, int nq0, int nq1, int nq2, const double *b0,
+                         const double *db0, const double *b1, const double *db1,
+                         const double *b2, const double *db2, const double *z0,
+                         const double *z2, const double *w0, const double *w1, const double *w2,
+                         int E, int deformed, const double *jac, const double *df,
+                         const double *in0, const double *in1, const double *in2, double *out);
+void or_iprodderiv_tet(int nm, int nq0, int nq1, int nq2, const double *b0, const double *db0,
+                       const double *b1, const double *db1, const double *b2, const double *db2,
+                       const double *z0, const double *z1, const double *z2, const double *w0,
+                       const double *w1, const double *w2, int E, int deformed,
+                       const double *jac, const double *df, const double *in0,
+                       const double *in1, const double *in2, double *out);
+void or_iprodderiv_pyr(int nm, int nq0, int nq1, int nq2, const double *b0, const double *db0,
+                       const double *b1, const double *db1, const double *b2, const double *db2,
+                       const double *z0, const double *z1, const double *z2, const double *w0,
+                       const double *w1, const double *w2, int E, int deformed,
+                       const double *jac, const double *df, const double *in0,
+                       const double *in1, const double *in2, double *out);
+
 #ifdef __cplusplus
 }
 #endif

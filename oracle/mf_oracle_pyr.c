@@ -356,3 +356,51 @@ void or_helmholtz_pyr(int nm, int nq0, int nq1, int nq2, const double *b0, const
     free(dv1);
     free(dv2);
 }
+
+/* IProductWRTDerivBasePyrImpl (MatrixFreeOps/IProductWRTDerivBase.h:2056-): t_a =
+ * sum_c df[3c+a] in_c, tmp0 = 2/(1-eta2) (t0 + (1+eta0)/2 t2),
+ * tmp1 = 2/(1-eta2) (t1 + (1+eta1)/2 t2), tmp2 = t2 */
+void or_iprodderiv_pyr(int nm, int nq0, int nq1, int nq2, const double *b0, const double *db0,
+                       const double *b1, const double *db1, const double *b2, const double *db2,
+                       const double *z0, const double *z1, const double *z2, const double *w0,
+                       const double *w1, const double *w2, int E, int deformed,
+                       const double *jac, const double *df, const double *in0,
+                       const double *in1, const double *in2, double *out)
+{
+    const int M = nm * (nm + 1) * (2 * nm + 1) / 6, Q = nq0 * nq1 * nq2;
+    const size_t np = deformed ? (size_t)E * Q : (size_t)E;
+    double *tmp = (double *)malloc(sizeof(double) * 3 * Q);
+    double *ws = (double *)malloc(sizeof(double) * (nq1 * nq2 + nq2));
+    const double *in[3] = {in0, in1, in2};
+    for (int e = 0; e < E; ++e) {
+        const double *jp = jac + (deformed ? (size_t)e * Q : (size_t)e);
+        double f[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        if (!deformed)
+            for (int a = 0; a < 9; ++a) f[a] = df[a * np + e];
+        for (int k = 0, c = 0; k < nq2; ++k) {
+            const double f0 = 2.0 / (1.0 - z2[k]);
+            for (int j = 0; j < nq1; ++j) {
+                const double hf2 = 0.5 * (1.0 + z1[j]);
+                for (int i = 0; i < nq0; ++i, ++c) {
+                    if (deformed)
+                        for (int a = 0; a < 9; ++a) f[a] = df[a * np + (size_t)e * Q + c];
+                    double x[3], t[3];
+                    for (int d = 0; d < 3; ++d) x[d] = in[d][(size_t)e * Q + c];
+                    for (int a = 0; a < 3; ++a) t[a] = f[a] * x[0] + f[3 + a] * x[1] + f[6 + a] * x[2];
+                    const double hf1 = 0.5 * (1.0 + z0[i]);
+                    t[0] = t[0] * f0;
+                    t[0] = fma(hf1 * t[2], f0, t[0]);
+                    t[1] = t[1] * f0;
+                    t[1] = fma(hf2 * t[2], f0, t[1]);
+                    for (int a = 0; a < 3; ++a) tmp[a * Q + c] = t[a];
+                }
+            }
+        }
+        double *o = out + (size_t)e * M;
+        iprod_pyr_elmt(nm, nq0, nq1, nq2, db0, b1, b2, w0, w1, w2, deformed, jp, tmp, ws, ws + nq1 * nq2, o, 0, 0, 1.0);
+        iprod_pyr_elmt(nm, nq0, nq1, nq2, b0, db1, b2, w0, w1, w2, deformed, jp, tmp + Q, ws, ws + nq1 * nq2, o, 0, 1, 1.0);
+        iprod_pyr_elmt(nm, nq0, nq1, nq2, b0, b1, db2, w0, w1, w2, deformed, jp, tmp + 2 * Q, ws, ws + nq1 * nq2, o, 0, 1, 1.0);
+    }
+    free(tmp);
+    free(ws);
+}

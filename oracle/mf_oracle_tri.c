@@ -348,3 +348,45 @@ void or_helmholtz_tri(int nm, int nq0, int nq1, const double *b0, const double *
         free(bwd);
     }
 }
+
+/* IProductWRTDerivBaseTriImpl (MatrixFreeOps/IProductWRTDerivBase.h:671-, 2D coordinates):
+ * t_a = sum_c df[2c+a] in_c per point, then the collapsed chain rule
+ * tmp0 = 2/(1-eta1) (t0 + (1+eta0)/2 t1), tmp1 = t1, and the inner products with
+ * (dB0, B1) and (B0, dB1) */
+void or_iprodderiv_tri(int nm, int nq0, int nq1, const double *b0, const double *db0,
+                       const double *b1, const double *db1, const double *z0, const double *z1,
+                       const double *w0, const double *w1, int E, int deformed,
+                       const double *jac, const double *df, const double *in0,
+                       const double *in1, double *out)
+{
+    const int M = nm * (nm + 1) / 2, Q = nq0 * nq1;
+    const size_t np = deformed ? (size_t)E * Q : (size_t)E;
+    double *tmp0 = (double *)malloc(sizeof(double) * 2 * Q), *tmp1 = tmp0 + Q;
+    double *wsp = (double *)malloc(sizeof(double) * (nm > nq1 ? nm : nq1));
+    for (int e = 0; e < E; ++e) {
+        const double *jp = jac + (deformed ? (size_t)e * Q : (size_t)e);
+        double f[4] = {0, 0, 0, 0};
+        if (!deformed)
+            for (int a = 0; a < 4; ++a) f[a] = df[a * np + e];
+        for (int j = 0, c = 0; j < nq1; ++j) {
+            const double f0 = 2.0 / (1.0 - z1[j]);
+            for (int i = 0; i < nq0; ++i, ++c) {
+                if (deformed)
+                    for (int a = 0; a < 4; ++a) f[a] = df[a * np + (size_t)e * Q + c];
+                const double x0 = in0[(size_t)e * Q + c], x1 = in1[(size_t)e * Q + c];
+                double t0 = f[0] * x0 + f[2] * x1;
+                const double t1 = f[1] * x0 + f[3] * x1;
+                const double hf1 = 0.5 * (1.0 + z0[i]);
+                t0 = t0 * f0;
+                t0 = fma(hf1 * t1, f0, t0);
+                tmp0[c] = t0;
+                tmp1[c] = t1;
+            }
+        }
+        double *o = out + (size_t)e * M;
+        iprod_tri_elmt(nm, nq0, nq1, db0, b1, w0, w1, deformed, jp, tmp0, wsp, o, 0, 0, 1.0);
+        iprod_tri_elmt(nm, nq0, nq1, b0, db1, w0, w1, deformed, jp, tmp1, wsp, o, 0, 1, 1.0);
+    }
+    free(tmp0);
+    free(wsp);
+}

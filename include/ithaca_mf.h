@@ -107,6 +107,12 @@ enum imf_precision { IMF_FP64 = 0, IMF_FP32 = 1 };
  * nm - 1 <= nq[2] <= 16, within one workgroup's LDS).  Tables: Modified_A for directions
  * 0 and 1, ModifiedPyr_C (imf_basis_modified_pyr_c) for direction 2 with its raw
  * Gauss-Radau weights as w[2] (the factor 1/4 is applied here).  Layouts as for prisms.
+ * "IProductWRTDerivBase_{Tri,Prism,Tet,Pyr}_{Regular,Deformed}" (IProductWRTDerivBase
+ * {Tri,Prism,Tet,Pyr}.cpp; IProductWRTDerivBase.h:671-, :1630-, :2056-, :2484-): the
+ * same sizes, tables (dbdata required) and layouts as the shape's other operators; in =
+ * dim consecutive arrays of nElmt*Q, out = nElmt*M, out = sum_a (D_a B)^T W J t_a with
+ * t_a = sum_c df[c*dim+a] in_c pushed through the transpose of the collapsed chain rule;
+ * needs SetJac and SetDF; no transpose.
  * Also accepted, with no MatrixFreeOps counterpart: "Mass_Hex_Deformed" and
  * "Mass_Hex_Regular", the elemental mass action out = B^T diag(w J) B in of
  * StdRegions/StdExpansion.cpp:647-661 (MassMatrixOp_MatFree: BwdTrans, then

@@ -307,14 +307,14 @@ int create_tri(const std::string &k, int kind, int deformed, const int *nm, cons
 {
     if (precision != IMF_FP64 ||
         (kind != imf::kBwdTrans && kind != imf::kIProduct && kind != imf::kHelmholtz &&
-         kind != imf::kPhysDeriv))
+         kind != imf::kPhysDeriv && kind != imf::kIProdDeriv))
         return fail(IMF_ERR_UNSUPPORTED, k + ": triangles are supported for FP64 BwdTrans, "
-                                             "IProduct, Helmholtz and PhysDeriv");
+                                             "IProduct, Helmholtz, PhysDeriv and IProductWRTDerivBase");
     const int m = nm[0], q0 = nq[0], q1 = nq[1];
     if (m < 2 || m > 8 || q0 < 2 || q1 < 1 || q0 > 16 || q1 > 16 ||
         (kind != imf::kPhysDeriv && (nm[1] != m || q0 < m || q1 < m - 1)))
         return fail(IMF_ERR_UNSUPPORTED, k + ": # of modes / points combo not implemented.");
-    const bool helm = kind == imf::kHelmholtz;
+    const bool helm = kind == imf::kHelmholtz || kind == imf::kIProdDeriv;
     if (!D[0] || !D[1] || !w[0] || !w[1] ||
         (kind != imf::kPhysDeriv && (!bdata[0] || !bdata[1])) ||
         (helm && (!dbdata || !dbdata[0] || !dbdata[1])))
@@ -379,9 +379,9 @@ int create_prism(const std::string &k, int kind, int deformed, const int *nm, co
 {
     if (precision != IMF_FP64 ||
         (kind != imf::kBwdTrans && kind != imf::kIProduct && kind != imf::kHelmholtz &&
-         kind != imf::kPhysDeriv))
+         kind != imf::kPhysDeriv && kind != imf::kIProdDeriv))
         return fail(IMF_ERR_UNSUPPORTED, k + ": prisms are supported for FP64 BwdTrans, "
-                                             "IProduct, Helmholtz and PhysDeriv");
+                                             "IProduct, Helmholtz, PhysDeriv and IProductWRTDerivBase");
     const int m = nm[0], q0 = nq[0], q1 = nq[1], q2 = nq[2];
     if (m < 2 || m > 8 || q0 < 2 || q1 < 2 || q2 < 1 || q0 > 16 || q1 > 16 || q2 > 16 ||
         (kind != imf::kPhysDeriv &&
@@ -395,7 +395,7 @@ int create_prism(const std::string &k, int kind, int deformed, const int *nm, co
     if (sizeof(double) * (size_t)imf::prism_lds_len(m, q0, q1, q2) > (size_t)lds_max)
         return fail(IMF_ERR_UNSUPPORTED, k + ": an element of this size does not fit the LDS "
                                              "of one workgroup");
-    const bool helm = kind == imf::kHelmholtz, tabs = kind != imf::kPhysDeriv;
+    const bool helm = kind == imf::kHelmholtz || kind == imf::kIProdDeriv, tabs = kind != imf::kPhysDeriv;
     if (!D[0] || !D[1] || !D[2] || !w[0] || !w[1] || !w[2] ||
         (tabs && (!bdata[0] || !bdata[1] || !bdata[2])) ||
         (helm && (!dbdata || !dbdata[0] || !dbdata[1] || !dbdata[2])))
@@ -464,9 +464,9 @@ int create_tet(const std::string &k, int kind, int deformed, const int *nm, cons
 {
     if (precision != IMF_FP64 ||
         (kind != imf::kBwdTrans && kind != imf::kIProduct && kind != imf::kHelmholtz &&
-         kind != imf::kPhysDeriv))
+         kind != imf::kPhysDeriv && kind != imf::kIProdDeriv))
         return fail(IMF_ERR_UNSUPPORTED, k + ": tetrahedra are supported for FP64 BwdTrans, "
-                                             "IProduct, Helmholtz and PhysDeriv");
+                                             "IProduct, Helmholtz, PhysDeriv and IProductWRTDerivBase");
     const int m = nm[0], q0 = nq[0], q1 = nq[1], q2 = nq[2];
     if (m < 2 || m > 8 || q0 < 2 || q1 < 1 || q2 < 1 || q0 > 16 || q1 > 16 || q2 > 16 ||
         (kind != imf::kPhysDeriv &&
@@ -480,7 +480,7 @@ int create_tet(const std::string &k, int kind, int deformed, const int *nm, cons
     if (sizeof(double) * (size_t)imf::tet_lds_len(m, q0, q1, q2) > (size_t)lds_max)
         return fail(IMF_ERR_UNSUPPORTED, k + ": an element of this size does not fit the LDS "
                                              "of one workgroup");
-    const bool helm = kind == imf::kHelmholtz, tabs = kind != imf::kPhysDeriv;
+    const bool helm = kind == imf::kHelmholtz || kind == imf::kIProdDeriv, tabs = kind != imf::kPhysDeriv;
     if (!D[0] || !D[1] || !D[2] || !w[0] || !w[1] || !w[2] ||
         (tabs && (!bdata[0] || !bdata[1] || !bdata[2])) ||
         (helm && (!dbdata || !dbdata[0] || !dbdata[1] || !dbdata[2])))
@@ -545,9 +545,9 @@ int create_pyr(const std::string &k, int kind, int deformed, const int *nm, cons
 {
     if (precision != IMF_FP64 ||
         (kind != imf::kBwdTrans && kind != imf::kIProduct && kind != imf::kHelmholtz &&
-         kind != imf::kPhysDeriv))
+         kind != imf::kPhysDeriv && kind != imf::kIProdDeriv))
         return fail(IMF_ERR_UNSUPPORTED, k + ": pyramids are supported for FP64 BwdTrans, "
-                                             "IProduct, Helmholtz and PhysDeriv");
+                                             "IProduct, Helmholtz, PhysDeriv and IProductWRTDerivBase");
     const int m = nm[0], q0 = nq[0], q1 = nq[1], q2 = nq[2];
     if (m < 2 || m > 8 || q0 < 2 || q1 < 2 || q2 < 1 || q0 > 16 || q1 > 16 || q2 > 16 ||
         (kind != imf::kPhysDeriv &&
@@ -561,7 +561,7 @@ int create_pyr(const std::string &k, int kind, int deformed, const int *nm, cons
     if (sizeof(double) * (size_t)imf::pyr_lds_len(m, q0, q1, q2) > (size_t)lds_max)
         return fail(IMF_ERR_UNSUPPORTED, k + ": an element of this size does not fit the LDS "
                                              "of one workgroup");
-    const bool helm = kind == imf::kHelmholtz, tabs = kind != imf::kPhysDeriv;
+    const bool helm = kind == imf::kHelmholtz || kind == imf::kIProdDeriv, tabs = kind != imf::kPhysDeriv;
     if (!D[0] || !D[1] || !D[2] || !w[0] || !w[1] || !w[2] ||
         (tabs && (!bdata[0] || !bdata[1] || !bdata[2])) ||
         (helm && (!dbdata || !dbdata[0] || !dbdata[1] || !dbdata[2])))
@@ -696,8 +696,6 @@ int imf_create(const char *key, int dim, const int *nm, const int *nq,
     if (shape == imf::kTri || shape == imf::kPrism || shape == imf::kTet || shape == imf::kPyr) {
         if (dim != (shape == imf::kTri ? 2 : 3))
             return fail(IMF_ERR_ARG, "imf_create: dim does not match the shape of " + k);
-        if (kind == imf::kIProdDeriv)
-            return fail(IMF_ERR_UNSUPPORTED, "No such operator key: " + k);
         if (precision != IMF_FP64 && precision != IMF_FP32)
             return fail(IMF_ERR_ARG, "imf_create: unknown precision");
         if (nElmt <= 0) return fail(IMF_ERR_ARG, "imf_create: nElmt must be positive");
@@ -866,7 +864,8 @@ int imf_apply_transpose_device(imf_op op, const void *d_in, void *d_out, void *s
         return run(op, op->fwd, d_in, d_out, stream);
     if (op->collapsed) {
         if (op->kind == imf::kHelmholtz) return run(op, op->fwd, d_in, d_out, stream);
-        if (op->kind == imf::kPhysDeriv) return fail(IMF_ERR_UNSUPPORTED, "transpose not available");
+        if (op->kind == imf::kPhysDeriv || op->kind == imf::kIProdDeriv)
+            return fail(IMF_ERR_UNSUPPORTED, "transpose not available");
         return run(op, tri_transpose_marker(), d_in, d_out, stream);
     }
     if (!op->tr) return fail(IMF_ERR_UNSUPPORTED, "transpose not available");

@@ -8,7 +8,7 @@
 namespace imf {
 
 struct PrismArgs {
-    int kind;       // kBwdTrans, kIProduct, kHelmholtz or kPhysDeriv
+    int kind;       // kBwdTrans, kIProduct, kHelmholtz, kPhysDeriv or kIProdDeriv
     int transpose;  // Euclidean transpose (BwdTrans, IProduct)
     int geo;        // kUnit, kRegular (per element) or kDeformed (per point)
     int nm, nq0, nq1, nq2, nElmt;

@@ -194,7 +194,7 @@ __global__ __launch_bounds__(256) void pyr_kernel(PyrArgs a)
         const double *jac = a.jac ? a.jac + (deformed ? e * Q : e) : nullptr;
         if (coef_in)
             for (int t = tid; t < M; t += nt) cin[t] = a.in[e * M + t];
-        else
+        else if (a.kind != kIProdDeriv)
             for (int t = tid; t < Q; t += nt) u[t] = a.in[e * Q + t];
         __syncthreads();
 
@@ -213,6 +213,39 @@ __global__ __launch_bounds__(256) void pyr_kernel(PyrArgs a)
                 __syncthreads();
             }
             pyr_iprod(d, T.B0, T.B1, T.B2, u, sb, sa, cout, false, false, 1.0, tid, nt);
+            for (int t = tid; t < M; t += nt) a.out[e * M + t] = cout[t];
+        } else if (a.kind == kIProdDeriv) {
+            // IProductWRTDerivBase (IProductWRTDerivBase.h:2484-): per point t_a = sum_c df[3c+a] in_c,
+            // the transpose of PhysDeriv's collapsed chain rule, the weights w J, then the
+            // inner products with (dB0, B1, B2), (B0, dB1, B2) and (B0, B1, dB2)
+            const long nQ = (long)a.nElmt * Q;
+            for (int t = tid; t < Q; t += nt) {
+                const long g = deformed ? e * Q + t : e;
+                const double x0 = a.in[e * Q + t], x1 = a.in[nQ + e * Q + t],
+                             x2 = a.in[2 * nQ + e * Q + t];
+                double v[3];
+                for (int c = 0; c < 3; ++c) {
+                    double o = x0 * a.df[c * np + g];
+                    o = fma(x1, a.df[(3 + c) * np + g], o);
+                    v[c] = fma(x2, a.df[(6 + c) * np + g], o);
+                }
+                double v0 = v[0], v1 = v[1];
+                const double v2 = v[2];
+                // adjoint of (h2 D0, h2 D1, h0 h2 D0 + h1 h2 D1 + D2):
+                // (h2 (t0 + h0 t2), h2 (t1 + h1 t2), t2)
+                const int i = t % d.nq0, kj = t / d.nq0, j = kj % d.nq1, k = kj / d.nq1;
+                const double h2 = 2.0 / (1.0 - T.z2[k]);
+                v0 = h2 * fma(0.5 * (1.0 + T.z0[i]), v2, v0);
+                v1 = h2 * fma(0.5 * (1.0 + T.z1[j]), v2, v1);
+                const double wj = point_weight(T, d, t) * (deformed ? jac[t] : jac[0]);
+                d0[t] = wj * v0;
+                d1[t] = wj * v1;
+                d2[t] = wj * v2;
+            }
+            __syncthreads();
+            pyr_iprod(d, T.dB0, T.B1, T.B2, d0, sb, sa, cout, false, false, 1.0, tid, nt);
+            pyr_iprod(d, T.B0, T.dB1, T.B2, d1, sb, sa, cout, false, true, 1.0, tid, nt);
+            pyr_iprod(d, T.B0, T.B1, T.dB2, d2, sb, sa, cout, false, true, 1.0, tid, nt);
             for (int t = tid; t < M; t += nt) a.out[e * M + t] = cout[t];
         } else if (a.kind == kPhysDeriv) {
             pyr_deriv(T, d, u, d0, d1, d2, tid, nt);

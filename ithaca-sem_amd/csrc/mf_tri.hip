@@ -153,7 +153,7 @@ __global__ __launch_bounds__(TB) void tri_kernel(TriArgs a)
                          a.kind == kHelmholtz;
     if (coef_in) {
         for (int t = lane; t < M; t += 64) cin[t] = valid ? a.in[e * M + t] : 0.0;
-    } else {
+    } else if (a.kind != kIProdDeriv) {
         for (int t = lane; t < Q; t += 64) u[t] = valid ? a.in[e * Q + t] : 0.0;
     }
     __syncthreads();
@@ -173,6 +173,28 @@ __global__ __launch_bounds__(TB) void tri_kernel(TriArgs a)
     } else if (a.kind == kBwdTrans || a.kind == kIProduct) {
         const bool unit = a.kind == kBwdTrans;  // BwdTrans^T: unit weights, no Jacobian
         tri_iprod(T, T.B0, T.B1, nm, nq0, nq1, unit, deformed, jac, u, ps, cout, false, false,
+                  1.0, lane);
+        if (valid)
+            for (int t = lane; t < M; t += 64) a.out[e * M + t] = cout[t];
+    } else if (a.kind == kIProdDeriv) {
+        // IProductWRTDerivBase: per point t_a = sum_c df[2c+a] in_c, the transpose of the
+        // collapsed chain rule of PhysDeriv (tmp0 = h1 (t0 + h0 t1), tmp1 = t1), then the
+        // inner products with (dB0, B1) and (B0, dB1)
+        const long np = deformed ? (long)a.nElmt * Q : a.nElmt, nQ = (long)a.nElmt * Q;
+        for (int t = lane; t < Q; t += 64) {
+            const int j = t / nq0, i = t - j * nq0;
+            const long k = deformed ? ej * Q + t : ej;
+            const double f0 = a.df[k], f1 = a.df[np + k], f2 = a.df[2 * np + k],
+                         f3 = a.df[3 * np + k];
+            const double x0 = valid ? a.in[e * Q + t] : 0.0, x1 = valid ? a.in[nQ + e * Q + t] : 0.0;
+            const double t0 = fma(f2, x1, f0 * x0), t1 = fma(f3, x1, f1 * x0);
+            d0[t] = (2.0 / (1.0 - T.z1[j])) * fma(0.5 * (1.0 + T.z0[i]), t1, t0);
+            d1[t] = t1;
+        }
+        __syncthreads();
+        tri_iprod(T, T.dB0, T.B1, nm, nq0, nq1, false, deformed, jac, d0, ps, cout, false, false,
+                  1.0, lane);
+        tri_iprod(T, T.B0, T.dB1, nm, nq0, nq1, false, deformed, jac, d1, ps, cout, false, true,
                   1.0, lane);
         if (valid)
             for (int t = lane; t < M; t += 64) a.out[e * M + t] = cout[t];

@@ -211,8 +211,7 @@ OperatorSharedPtr make(const std::string &key, std::vector<LibUtilities::BasisSh
 
 // Keys registered by the reference: MatrixFreeOps/BwdTrans.cpp:7-35, IProduct{Quad,Hex,
 // Tri,Prism,Tet,Pyr}.cpp, Helmholtz.cpp:7-42, PhysDeriv.cpp:14-49,
-// IProductWRTDerivBase{Quad,Hex}.cpp.  IProductWRTDerivBase of the collapsed shapes is not
-// provided (imf_create rejects those keys).
+// IProductWRTDerivBase{Quad,Hex,Tri,Prism,Tet,Pyr}.cpp.
 const bool registered = [] {
     auto &f = GetOperatorFactory();
     for (const char *shape : {"Quad", "Hex", "Tri", "Prism", "Tet", "Pyr"}) {
@@ -467,8 +466,6 @@ const bool registered = [] {
     for (auto shape : {LibUtilities::eQuadrilateral, LibUtilities::eHexahedron,
                        LibUtilities::eTriangle, LibUtilities::ePrism, LibUtilities::eTetrahedron,
                        LibUtilities::ePyramid}) {
-        const bool collapsed = shape != LibUtilities::eQuadrilateral &&
-                               shape != LibUtilities::eHexahedron;
         f.RegisterCreatorFunction(OperatorKey(shape, eBwdTrans, eMatrixFreeHip, false),
                                   create<BwdTrans_MatrixFreeHip>, "BwdTrans_MatrixFreeHip");
         f.RegisterCreatorFunction(OperatorKey(shape, eIProductWRTBase, eMatrixFreeHip, false),
@@ -478,7 +475,6 @@ const bool registered = [] {
                                   create<Helmholtz_MatrixFreeHip>, "Helmholtz_MatrixFreeHip");
         f.RegisterCreatorFunction(OperatorKey(shape, ePhysDeriv, eMatrixFreeHip, false),
                                   create<PhysDeriv_MatrixFreeHip>, "PhysDeriv_MatrixFreeHip");
-        if (collapsed) continue;
         f.RegisterCreatorFunction(OperatorKey(shape, eIProductWRTDerivBase, eMatrixFreeHip, false),
                                   create<IProductWRTDerivBase_MatrixFreeHip>,
                                   "IProductWRTDerivBase_MatrixFreeHip");

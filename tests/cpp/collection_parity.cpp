@@ -66,7 +66,8 @@ static int collapsed_cases(std::mt19937_64 &g)
         return w;
     };
     auto run = [&](const char *name, ShapeType shape, std::vector<BasisSharedPtr> bases,
-                   const double *ref_verts, int nv, int M, auto geom_fn, auto bwd_fn, auto helm_fn) {
+                   const double *ref_verts, int nv, int M, auto geom_fn, auto bwd_fn, auto helm_fn,
+                   auto ipdb_fn) {
         const int dim = (int)bases.size();
         int Q = 1;
         for (auto &b : bases) Q *= b->GetNumPoints();
@@ -100,6 +101,20 @@ static int collapsed_cases(std::mt19937_64 &g)
         helm_fn(jac->data(), df->data(), u.data(), rh.data());
         r = rel(out.data(), rh.data(), E * M);
         printf("Helmholtz_MatrixFreeHip %s deformed rel=%.3e\n", name, r);
+        f += !(r < 1e-12);
+        // IProductWRTDerivBase through the multi-array ApplyOperator forms
+        Array<OneD, NekDouble> d0(E * Q), d1(E * Q), d2(E * Q);
+        for (int i = 0; i < E * Q; ++i) {
+            d0[i] = U(g);
+            d1[i] = U(g);
+            d2[i] = U(g);
+        }
+        coll.Initialise(Collections::eIProductWRTDerivBase);
+        if (dim == 2) coll.ApplyOperator(Collections::eIProductWRTDerivBase, d0, d1, out);
+        else coll.ApplyOperator(Collections::eIProductWRTDerivBase, d0, d1, d2, out);
+        ipdb_fn(jac->data(), df->data(), d0.data(), d1.data(), d2.data(), rh.data());
+        r = rel(out.data(), rh.data(), E * M);
+        printf("IProductWRTDerivBase_MatrixFreeHip %s deformed rel=%.3e\n", name, r);
         return f + !(r < 1e-12);
     };
     {   // triangles: Modified_A x Modified_B
@@ -114,7 +129,13 @@ static int collapsed_cases(std::mt19937_64 &g)
                 or_helmholtz_tri(nm, nm + 1, nm, a->GetBdata().data(), a->GetDbdata().data(),
                                  b->GetBdata().data(), b->GetDbdata().data(), a->GetD().data(),
                                  b->GetD().data(), a->GetZ().data(), b->GetZ().data(),
-                                 a->GetW().data(), w1.data(), E, 1, j, d, lam, in, o); });
+                                 a->GetW().data(), w1.data(), E, 1, j, d, lam, in, o); },
+            [&](const double *j, const double *d, const double *i0, const double *i1, const double *,
+                double *o) {
+                or_iprodderiv_tri(nm, nm + 1, nm, a->GetBdata().data(), a->GetDbdata().data(),
+                                  b->GetBdata().data(), b->GetDbdata().data(), a->GetZ().data(),
+                                  b->GetZ().data(), a->GetW().data(), w1.data(), E, 1, j, d, i0,
+                                  i1, o); });
     }
     {   // prisms: Modified_A x Modified_A x Modified_B
         auto a = MakeModifiedA(nm, nm + 1), b = MakeModifiedB(nm, nm);
@@ -133,7 +154,15 @@ static int collapsed_cases(std::mt19937_64 &g)
                                    b->GetDbdata().data(), a->GetD().data(), a->GetD().data(),
                                    b->GetD().data(), a->GetZ().data(), b->GetZ().data(),
                                    a->GetW().data(), a->GetW().data(), w2.data(), E, 1, j, d, lam,
-                                   in, o); });
+                                   in, o); },
+            [&](const double *j, const double *d, const double *i0, const double *i1,
+                const double *i2, double *o) {
+                or_iprodderiv_prism(nm, nm + 1, nm + 1, nm, a->GetBdata().data(),
+                                    a->GetDbdata().data(), a->GetBdata().data(),
+                                    a->GetDbdata().data(), b->GetBdata().data(),
+                                    b->GetDbdata().data(), a->GetZ().data(), b->GetZ().data(),
+                                    a->GetW().data(), a->GetW().data(), w2.data(), E, 1, j, d, i0,
+                                    i1, i2, o); });
     }
     {   // tetrahedra: Modified_A x Modified_B x Modified_C
         auto a = MakeModifiedA(nm, nm + 1), b = MakeModifiedB(nm, nm), c = MakeModifiedC(nm, nm);
@@ -150,7 +179,14 @@ static int collapsed_cases(std::mt19937_64 &g)
                                  c->GetDbdata().data(), a->GetD().data(), b->GetD().data(),
                                  c->GetD().data(), a->GetZ().data(), b->GetZ().data(),
                                  c->GetZ().data(), a->GetW().data(), w1.data(), w2.data(), E, 1, j,
-                                 d, lam, in, o); });
+                                 d, lam, in, o); },
+            [&](const double *j, const double *d, const double *i0, const double *i1,
+                const double *i2, double *o) {
+                or_iprodderiv_tet(nm, nm + 1, nm, nm, a->GetBdata().data(), a->GetDbdata().data(),
+                                  b->GetBdata().data(), b->GetDbdata().data(), c->GetBdata().data(),
+                                  c->GetDbdata().data(), a->GetZ().data(), b->GetZ().data(),
+                                  c->GetZ().data(), a->GetW().data(), w1.data(), w2.data(), E, 1, j,
+                                  d, i0, i1, i2, o); });
     }
     {   // pyramids: Modified_A x Modified_A x ModifiedPyr_C
         auto a = MakeModifiedA(nm, nm + 1), c = MakeModifiedPyrC(nm, nm);
@@ -169,7 +205,14 @@ static int collapsed_cases(std::mt19937_64 &g)
                                  c->GetDbdata().data(), a->GetD().data(), a->GetD().data(),
                                  c->GetD().data(), a->GetZ().data(), a->GetZ().data(),
                                  c->GetZ().data(), a->GetW().data(), a->GetW().data(), w2.data(), E,
-                                 1, j, d, lam, in, o); });
+                                 1, j, d, lam, in, o); },
+            [&](const double *j, const double *d, const double *i0, const double *i1,
+                const double *i2, double *o) {
+                or_iprodderiv_pyr(nm, nm + 1, nm + 1, nm, a->GetBdata().data(), a->GetDbdata().data(),
+                                  a->GetBdata().data(), a->GetDbdata().data(), c->GetBdata().data(),
+                                  c->GetDbdata().data(), a->GetZ().data(), a->GetZ().data(),
+                                  c->GetZ().data(), a->GetW().data(), a->GetW().data(), w2.data(), E,
+                                  1, j, d, i0, i1, i2, o); });
     }
     return fails;
 }

@@ -83,6 +83,16 @@ def test_pyr_operators_match_the_oracle(nm, nq, deformed):
     for c in range(3):
         assert _rel(g[c * E * Q:(c + 1) * E * Q], ref[c]) < 1e-12
 
+    ipd = H.Operator(f"IProductWRTDerivBase_Pyr_{geo}", nm, nq, E, basis=tabs)
+    ipd.set_jac(gj)
+    ipd.set_df(gd)
+    fs = [rng.uniform(-1, 1, E * Q) for _ in range(3)]
+    dfs = torch.from_numpy(np.concatenate(fs)).cuda()
+    got = ipd.apply(dfs).cpu().numpy()
+    assert _rel(got, O.iprodderiv_pyr(b, E, deformed, jac, df, fs)) < 1e-12
+    with pytest.raises(H.MFError):
+        ipd.apply_transpose(du)
+
     # Euclidean transposes: <A x, y> = <x, A^T y>
     for op, nin, nout in [(bw, M, Q), (ip, Q, M), (hm, M, M)]:
         x = torch.from_numpy(rng.uniform(-1, 1, E * nin)).cuda()
@@ -159,7 +169,8 @@ def test_pyr_rejects_what_it_does_not_cover():
     with pytest.raises(H.MFError):
         H.Operator("Helmholtz_Pyr_Regular", 4, 5, 4, precision=H.FP32, basis=tabs)
     with pytest.raises(H.MFError):
-        H.Operator("IProductWRTDerivBase_Pyr_Regular", 4, 5, 4, basis=tabs)
+        H.Operator("IProductWRTDerivBase_Pyr_Regular", 4, 5, 4, precision=H.FP32,
+                   basis=tabs)
     with pytest.raises(H.MFError):
         H.Operator("Helmholtz_Pyr_Regular", 9, 10, 4)
     with pytest.raises(H.MFError):  # one element's arrays exceed a workgroup's LDS

@@ -80,6 +80,16 @@ def test_tri_operators_match_the_oracle(nm, nq0, deformed):
     r0, r1 = O.physderiv_tri(b, E, deformed, df, f)
     assert _rel(g[:E * Q], r0) < 1e-12 and _rel(g[E * Q:], r1) < 1e-12
 
+    ipd = H.Operator(f"IProductWRTDerivBase_Tri_{geo}", nm, nq0, E, basis=tabs)
+    ipd.set_jac(jac)
+    ipd.set_df(df)
+    fs = [rng.uniform(-1, 1, E * Q) for _ in range(2)]
+    dfs = torch.from_numpy(np.concatenate(fs)).cuda()
+    got = ipd.apply(dfs).cpu().numpy()
+    assert _rel(got, O.iprodderiv_tri(b, E, deformed, jac, df, fs[0], fs[1])) < 1e-12
+    with pytest.raises(H.MFError):
+        ipd.apply_transpose(du)
+
     # Euclidean transposes: <A x, y> = <x, A^T y>
     for op, nin, nout in [(bw, M, Q), (ip, Q, M), (hm, M, M)]:
         x = torch.from_numpy(rng.uniform(-1, 1, E * nin)).cuda()
@@ -134,7 +144,8 @@ def test_tri_rejects_what_it_does_not_cover():
     with pytest.raises(H.MFError):
         H.Operator("Helmholtz_Tri_Regular", 6, 7, 4, precision=H.FP32, basis=tabs)
     with pytest.raises(H.MFError):
-        H.Operator("IProductWRTDerivBase_Tri_Regular", 6, 7, 4, basis=tabs)
+        H.Operator("IProductWRTDerivBase_Tri_Regular", 6, 7, 4, precision=H.FP32,
+                   basis=tabs)
     with pytest.raises(H.MFError):
         H.Operator("Helmholtz_Tri_Regular", 9, 10, 4)
     op = H.Operator("PhysDeriv_Tri_Regular", 6, 7, 4, basis=tabs)

@@ -8,17 +8,22 @@
 // semi-definite G, rotating the columns of W = G V until they are mutually orthogonal
 // gives lambda_j = ||w_j|| and eigenvectors v_j.  The np columns are split into nb blocks
 // of 16; a sweep is nb-1 rounds of the round-robin ("circle") pairing of blocks.  In a
-// round, a workgroup takes a block pair: the 32 x 32 Gram matrix P = W_pair^T W_pair is
-// formed with FP64 MFMA (the four waves sum disjoint row ranges), diagonalised in LDS by a
-// parallel two-sided Jacobi (16 disjoint rotations per step, one thread per 2 x 2 block of
-// P, rotations accumulated in R), and the pair's columns of W and V become W R and V R
-// (MFMA again).  The rotation of a column pair is the one-sided Jacobi rotation of the
-// same two columns, so one inner sweep is exactly the column-by-column method; the columns
-// themselves are touched twice per round instead of once per inner step.  All rounds and
-// sweeps run in one persistent launch; workgroups meet at a grid barrier between rounds
-// (an atomic arrival counter and a generation word, with a bounded wait), and stop
-// together after the first sweep without a rotation.  Eigenvalues are sorted on the
-// device (bitonic sort in one workgroup) and the eigenvectors gathered in that order.
+// round, S workgroups take a block pair: each forms the 32 x 32 Gram matrix
+// P = W_pair^T W_pair with FP64 MFMA (the waves sum disjoint row ranges), diagonalises it
+// in LDS by a parallel two-sided Jacobi (16 disjoint rotations per step, one thread per
+// 2 x 2 block of P, rotations accumulated in R), and the pair's columns of W and V become
+// W R and V R (MFMA again) on its own share of the rows.  The S workgroups of a pair
+// compute the same P and R bit for bit; a per-pair arrival count keeps any share from
+// writing W before every share has read it.  The rotation of a column pair is the
+// one-sided Jacobi rotation of the same two columns, so one inner sweep is exactly the
+// column-by-column method; the columns themselves are touched twice per round instead of
+// once per inner step.  All rounds and sweeps run in one persistent launch.  Rounds are
+// not separated by a grid barrier: each column block counts the shares done with it, and
+// a pair starts once its two blocks' previous round is done; workgroups meet at a grid
+// barrier (an atomic arrival counter and a generation word, with a bounded wait) after
+// each sweep, and stop together after the first sweep without a rotation.  Eigenvalues
+// are sorted on the device (bitonic sort in one workgroup) and the eigenvectors gathered
+// in that order.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -74,10 +79,32 @@ __device__ __forceinline__ void grid_sync(unsigned *bar, unsigned nblk, unsigned
     __syncthreads();
 }
 
+// Thread 0 waits (bounded like grid_sync) until both counters reach `want`, then acquires
+// at device scope, so that the writes released before those counts are visible to the
+// workgroup.
+__device__ __forceinline__ void wait_counts(const unsigned *c0, const unsigned *c1, unsigned want,
+                                            int *err)
+{
+    if (threadIdx.x == 0) {
+        long spins = 0;
+        while (__hip_atomic_load(c0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < want ||
+               __hip_atomic_load(c1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < want) {
+            __builtin_amdgcn_s_sleep(1);
+            if (++spins > (1L << 23)) {
+                atomicOr(err, 1);
+                break;
+            }
+        }
+        __threadfence();
+    }
+    __syncthreads();
+}
+
 typedef double d4 __attribute__((ext_vector_type(4)));
 
 constexpr int JB = 16;       // columns per block
 constexpr int JC = 2 * JB;   // columns of a block pair
+static_assert(JC == 32, "the circle-schedule position arithmetic assumes 32 columns per pair");
 constexpr int JP = JC + 1;   // padded LDS row
 constexpr int NW = 8;        // waves per workgroup
 constexpr int JT = 64 * NW;  // threads per workgroup
@@ -97,6 +124,9 @@ struct EigArgs {
     unsigned *bar;   // [2] grid barrier (zeroed)
     int *sweeps;     // sweeps performed
     int *err;        // barrier expiry
+    int split;       // workgroups per block pair (each updates a share of the rows)
+    unsigned *cnt;   // [nb / 2] arrivals per block pair (zeroed)
+    unsigned *ver;   // [nb] per column block: shares done with it, over all rounds (zeroed)
 };
 
 // 1/x and 1/sqrt(x) from the hardware approximations (about 2^-23 relative) refined by
@@ -125,29 +155,26 @@ __device__ __forceinline__ double rsq_nr(double x)
 // zeta = (aqq - app) / (2 apq), computed to about 1e-14: nearly parallel columns (the
 // columns of a rank-deficient Gram matrix) need the angle to about 1/kappa of the pair, so
 // an FP32 angle would stall them; a rotation by a slightly different angle is still a
-// rotation.  c = (1 + t^2)^(-1/2) is refined to full precision, so c^2 + s^2 = 1 to
-// rounding and V stays orthogonal.
+// rotation.  c = (1 + t^2)^(-1/2) is refined to full precision (two Newton steps from the
+// hardware estimate), so c^2 + s^2 = 1 to rounding and V stays orthogonal.  No branches:
+// the special cases (no rotation, |zeta| > 1e60 where t ~ 1/(2 zeta), a NaN zeta giving
+// t = 0) are selected at the end.
 __device__ __forceinline__ bool rot_params(double app, double aqq, double apq, double tol2,
                                            double floor_, double &c, double &s)
 {
-    if (apq == 0.0 || apq * apq <= tol2 * (app * aqq) || fabs(apq) <= floor_) {
-        c = 1.0;
-        s = 0.0;
-        return false;
-    }
-    const double zeta = (aqq - app) * rcp_nr<1>(2.0 * apq);
+    const bool rot = apq != 0.0 && apq * apq > tol2 * (app * aqq) && fabs(apq) > floor_;
+    const double zeta = (aqq - app) * rcp_nr<1>(rot ? 2.0 * apq : 1.0);
     const double az = fabs(zeta);
-    double t;
-    if (!(az <= 1e60)) {
-        t = az == az ? 0.5 / zeta : 0.0;  // t ~ 1/(2 zeta); no rotation for a NaN
-    } else {
-        const double h = fma(zeta, zeta, 1.0);
-        t = rcp_nr<1>(az + h * rsq_nr<1>(h));
-        if (zeta < 0.0) t = -t;
-    }
-    c = rsq_nr<3>(fma(t, t, 1.0));
-    s = c * t;
-    return true;
+    const bool big = !(az <= 1e60);
+    const double zc = big ? 1.0 : az;  // finite on the ordinary path
+    const double h = fma(zc, zc, 1.0);
+    const double tn = rcp_nr<1>(zc + h * rsq_nr<1>(h));
+    const double tb = az == az ? 0.5 * rcp_nr<1>(zeta) : 0.0;
+    const double t = big ? tb : (zeta < 0.0 ? -tn : tn);
+    const double cc = rsq_nr<2>(fma(t, t, 1.0));
+    c = rot ? cc : 1.0;
+    s = rot ? cc * t : 0.0;
+    return rot;
 }
 
 // One persistent launch for all sweeps; JT threads, one block pair at a time.
@@ -156,10 +183,11 @@ __global__ __launch_bounds__(JT) void gram_jacobi_kernel(EigArgs a)
     __shared__ double Pw[NW / 2][JC][JP];  // partial Gram matrices (waves w and w + NW/2)
     __shared__ double Pb[2][JC][JP];       // the pair's Gram matrix, current and next
     __shared__ double R[JC][JP];           // accumulated rotation
+    __shared__ double Cs[JB][2];           // the rotations (c, s) of one inner step
     __shared__ double s_red[NW];
     const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
     const int i16 = lane & 15, g4 = lane >> 4;
-    const int np = a.np, nb = a.nb, nrb = np / 16;
+    const int np = a.np, nb = a.nb, nrb = np / 16, S = a.split, nitems = nb / 2 * S;
 
     // ||G||_F, summed in the same order by every workgroup
     double fro = 0.0;
@@ -176,8 +204,17 @@ __global__ __launch_bounds__(JT) void gram_jacobi_kernel(EigArgs a)
     int sw = 0;
     for (; sw < a.max_sweeps; ++sw) {
         for (int r = 0; r < nb - 1; ++r) {
-            for (int k = blockIdx.x; k < nb / 2; k += gridDim.x) {
+            // work item (k, sp): block pair k of the round, row share sp.  The S workgroups of
+            // a pair form the same P and R (same data, same order of operations); each
+            // updates the rows [rb0, rb1) (16-row blocks) of the pair's columns.
+            for (int item = blockIdx.x; item < nitems; item += gridDim.x) {
+                const int k = item / S, sp = item - k * S;
+                const int rb0 = sp * nrb / S, rb1 = (sp + 1) * nrb / S;
                 const int bi = player(k, r, nb), bj = player(nb - 1 - k, r, nb);
+                // the two blocks' previous round is complete: every share of the pairs that
+                // held them has written its rows (no grid barrier between rounds)
+                wait_counts(&a.ver[bi], &a.ver[bj], (unsigned)S * (unsigned)(sw * (nb - 1) + r),
+                            a.err);
                 auto gcol = [&](int c) { return c < JB ? bi * JB + c : bj * JB + (c - JB); };
                 // 1. P = W_pair^T W_pair.  Lane (i16, g4) holds rows 4 g4 .. 4 g4 + 3 of a
                 // 16-row block for column i16 of each block; MFMA step t pairs row 4 g4 + t
@@ -252,8 +289,8 @@ __global__ __launch_bounds__(JT) void gram_jacobi_kernel(EigArgs a)
                 auto load_group = [&](int g) {
 #pragma unroll
                     for (int u = 0; u < GU; ++u) {
-                        // rows past the end: the last block (in bounds), not used
-                        const int rb = min(wv + NW * (g * GU + u), nrb - 1);
+                        // rows past the share: the last block (in bounds), not used
+                        const int rb = min(rb0 + wv + NW * (g * GU + u), nrb - 1);
 #pragma unroll
                         for (int kk = 0; kk < 8; ++kk) {
                             aw[u][kk] = a.W[ka(kk) + rb * 16];
@@ -263,13 +300,17 @@ __global__ __launch_bounds__(JT) void gram_jacobi_kernel(EigArgs a)
                 };
                 load_group(0);
                 __syncthreads();
+                // every row of W this workgroup reads from the other shares is consumed
+                // (P is formed): count the arrival; the shares write W only after all S
+                // arrivals of this round
+                if (S > 1 && tid == 0) atomicAdd(&a.cnt[k], 1u);
 
                 // 2. Pairs already orthogonal skip the rest.  Otherwise two-sided Jacobi on
-                // P: in step st the 16 pairs of the circle schedule rotate at once; thread
-                // (k1, k2) of the first half forms the 2 x 2 block of J^T P J on rows
-                // {p1, q1} and columns {p2, q2} (P double-buffered), the same thread of the
-                // second half updates that block of R <- R J.  The blocks partition the
-                // matrices.
+                // P: in step st the 16 pairs of the circle schedule rotate at once; threads
+                // 0..15 form the 16 rotations into Cs, then thread (k1, k2) of the first
+                // half forms the 2 x 2 block of J^T P J on rows {p1, q1} and columns
+                // {p2, q2} (P double-buffered), the same thread of the second half updates
+                // that block of R <- R J.  The blocks partition the matrices.
                 int need = 0;
                 for (int idx = tid; idx < JC * JC; idx += JT) {
                     const int ii = idx / JC, jj = idx - ii * JC;
@@ -284,23 +325,32 @@ __global__ __launch_bounds__(JT) void gram_jacobi_kernel(EigArgs a)
                 if (__syncthreads_or(need)) {
                     const bool rhalf = tid >= JT / 2;
                     const int hb = rhalf ? tid - JT / 2 : tid;
-                    const int k1 = hb >> 4, k2 = hb & 15;
+                    const int k1 = hb >> 4, k2 = hb & 15, ka = tid & (JB - 1);
                     int cur = 0;
 #pragma unroll 1
                     for (int it = 0; it < a.max_inner; ++it) {
                         int any = 0;
-                        int pl[4] = {player(k1, 0, JC), player(JC - 1 - k1, 0, JC), player(k2, 0, JC),
-                                     player(JC - 1 - k2, 0, JC)};
+                        int pl[6] = {player(k1, 0, JC), player(JC - 1 - k1, 0, JC), player(k2, 0, JC),
+                                     player(JC - 1 - k2, 0, JC), player(ka, 0, JC),
+                                     player(JC - 1 - ka, 0, JC)};
 #pragma unroll 1
                         for (int st = 0; st < JC - 1; ++st) {
                             const int p1 = pl[0], q1 = pl[1], p2 = pl[2], q2 = pl[3];
                             double(*P)[JP] = Pb[cur];
-                            double c2, s2;
-                            rot_params(P[p2][p2], P[q2][q2], P[p2][q2], tol2, floor_, c2, s2);
+                            // each rotation once: thread k < 16 forms pair k's (c, s)
+                            if (tid < JB) {
+                                const int pa = pl[4], qa = pl[5];
+                                double c, sn;
+                                if (rot_params(P[pa][pa], P[qa][qa], P[pa][qa], tol2, floor_, c, sn))
+                                    any = 1;
+                                Cs[tid][0] = c;
+                                Cs[tid][1] = sn;
+                            }
+                            __syncthreads();
+                            const double c2 = Cs[k2][0], s2 = Cs[k2][1];
                             if (!rhalf) {
                                 double(*Pn)[JP] = Pb[cur ^ 1];
-                                double c1, s1;
-                                const bool r1 = rot_params(P[p1][p1], P[q1][q1], P[p1][q1], tol2, floor_, c1, s1);
+                                const double c1 = Cs[k1][0], s1 = Cs[k1][1];
                                 const double mpp = P[p1][p2], mpq = P[p1][q2], mqp = P[q1][p2], mqq = P[q1][q2];
                                 const double xpp = c2 * mpp - s2 * mpq, xpq = s2 * mpp + c2 * mpq;
                                 const double xqp = c2 * mqp - s2 * mqq, xqq = s2 * mqp + c2 * mqq;
@@ -308,7 +358,6 @@ __global__ __launch_bounds__(JT) void gram_jacobi_kernel(EigArgs a)
                                 Pn[p1][q2] = c1 * xpq - s1 * xqq;
                                 Pn[q1][p2] = s1 * xpp + c1 * xqp;
                                 Pn[q1][q2] = s1 * xpq + c1 * xqq;
-                                if (k1 == k2 && r1) any = 1;
                             } else {
                                 const double rpp = R[p1][p2], rpq = R[p1][q2], rqp = R[q1][p2], rqq = R[q1][q2];
                                 R[p1][p2] = c2 * rpp - s2 * rpq;
@@ -317,9 +366,15 @@ __global__ __launch_bounds__(JT) void gram_jacobi_kernel(EigArgs a)
                                 R[q1][q2] = s2 * rqp + c2 * rqq;
                             }
                             // the circle schedule: every position but the first advances
-                            // by one, 31 wrapping to 1
+                            // by one, 31 wrapping to 1 (integer arithmetic: a conditional
+                            // form is compiled into divergent branches)
 #pragma unroll
-                            for (int u = 0; u < 4; ++u) pl[u] = pl[u] == 0 ? 0 : (pl[u] == JC - 1 ? 1 : pl[u] + 1);
+                            for (int u = 0; u < 6; ++u) {
+                                int v = pl[u];
+                                v += (v + JC - 1) >> 5;  // +1 unless v = 0
+                                v -= (v >> 5) * (JC - 1);  // JC -> 1
+                                pl[u] = v;
+                            }
                             cur ^= 1;
                             __syncthreads();
                         }
@@ -331,14 +386,28 @@ __global__ __launch_bounds__(JT) void gram_jacobi_kernel(EigArgs a)
                 // 3. The pair's columns of W and V <- (W, V) R: A = the rows loaded above,
                 // B = R; each wave writes only its own rows.
                 if (pair_rot) {
+                    if (S > 1) {  // the other shares have read W for their P
+                        if (tid == 0) {
+                            const unsigned target = (unsigned)S * (unsigned)(sw * (nb - 1) + r + 1);
+                            long spins = 0;
+                            while (__hip_atomic_load(&a.cnt[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+                                __builtin_amdgcn_s_sleep(1);
+                                if (++spins > (1L << 23)) {
+                                    atomicOr(a.err, 1);
+                                    break;
+                                }
+                            }
+                        }
+                        __syncthreads();
+                    }
                     const long kc0 = (long)gcol(i16) * np + g4, kc1 = (long)gcol(JB + i16) * np + g4;
-                    for (int g = 0; wv + NW * g * GU < nrb; ++g) {
+                    for (int g = 0; rb0 + wv + NW * g * GU < rb1; ++g) {
                         if (g > 0) load_group(g);
 #pragma unroll
                         for (int u = 0; u < GU; ++u) {
-                            const int rb = wv + NW * (g * GU + u);
+                            const int rb = rb0 + wv + NW * (g * GU + u);
                             const int r0 = rb * 16;
-                            if (rb < nrb) {
+                            if (rb < rb1) {
                             d4 ow0 = {0, 0, 0, 0}, ow1 = {0, 0, 0, 0}, ov0 = {0, 0, 0, 0}, ov1 = {0, 0, 0, 0};
 #pragma unroll
                             for (int kk = 0; kk < 8; ++kk) {
@@ -360,12 +429,18 @@ __global__ __launch_bounds__(JT) void gram_jacobi_kernel(EigArgs a)
                     }
                     if (tid == 0) atomicOr(&a.rot[sw], 1);
                 }
-                __syncthreads();  // LDS reused by the next block pair of this workgroup
+                // this share is done with blocks bi and bj: release its writes and count it
+                __syncthreads();
+                if (tid == 0) {
+                    __threadfence();
+                    atomicAdd(&a.ver[bi], 1u);
+                    atomicAdd(&a.ver[bj], 1u);
+                }
             }
-            grid_sync(a.bar, gridDim.x, gen, a.err);
         }
-        // every flag of this sweep was set before the last barrier: all workgroups read
-        // the same value and stop together
+        grid_sync(a.bar, gridDim.x, gen, a.err);
+        // every flag of this sweep was set before the barrier: all workgroups read the same
+        // value and stop together
         if (__hip_atomic_load(&a.rot[sw], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == 0) {
             ++sw;
             break;
@@ -474,10 +549,12 @@ constexpr EigPlan eig_plan(int n)
 }
 
 // Workspace layout (bytes, each part 256-byte aligned): W, V (np^2 doubles each), norms
-// (2 np: of W's and of V's columns), sorted values (n), perm (n ints), rot flags, barrier words, sweeps, err.
+// (2 np: of W's and of V's columns), sorted values (n), perm (n ints), then the ints: rot
+// flags, barrier words, sweeps, err, the per-pair arrival counters (np / JC) and the
+// per-block version counters (np / JB).
 struct EigWs {
     double *W, *V, *nrm, *ev;
-    int *perm, *ints;  // ints: rot[EIG_MAX_SWEEPS], bar[2], sweeps, err
+    int *perm, *ints;  // ints: rot[EIG_MAX_SWEEPS], bar[2], sweeps, err, cnt[np / JC], ver[np / JB]
 };
 
 constexpr size_t al256(size_t b) { return (b + 255) / 256 * 256; }
@@ -487,7 +564,7 @@ constexpr size_t eig_ws_bytes(int n)
     const EigPlan p = eig_plan(n);
     return 2 * al256(sizeof(double) * (size_t)p.np * p.np) + al256(sizeof(double) * 2 * p.np) +
            al256(sizeof(double) * n) + al256(sizeof(int) * n) +
-           al256(sizeof(int) * (EIG_MAX_SWEEPS + 4));
+           al256(sizeof(int) * (EIG_MAX_SWEEPS + 4 + p.np / JC + p.np / JB));
 }
 
 inline EigWs eig_ws(int n, void *base)
@@ -525,10 +602,20 @@ int efail(int c, const std::string &m)
         if (e_ != hipSuccess) return efail(3, std::string(what) + ": " + hipGetErrorString(e_)); \
     } while (0)
 
+// Row shares per block pair: the most (up to 4) for which every work item of a round has
+// a workgroup of its own (the shares of a pair wait for each other), one per CU.
+int eig_split(int nb, int ncu)
+{
+    int sp = 4;
+    while (sp > 1 && (nb / 2) * sp > ncu) sp >>= 1;
+    return sp;
+}
+
 hipError_t launch_gram_jacobi(const irom::EigArgs &a, int ncu, hipStream_t s)
 {
-    // every workgroup must be resident at once (grid barrier): at most one per CU here
-    const int grid = std::max(1, std::min(a.nb / 2, ncu));
+    // every workgroup must be resident at once (grid barrier): at most one per CU here;
+    // with more than one share per pair, one workgroup per work item
+    const int grid = std::max(1, std::min(a.nb / 2 * a.split, ncu));
     void *args[] = {(void *)&a};
     return hipLaunchCooperativeKernel((const void *)&irom::gram_jacobi_kernel, dim3(grid),
                                       dim3(irom::JT), args, 0, s);
@@ -546,14 +633,16 @@ int eig_run(int n, const double *d_G, double *h_evals, double *d_evecs, double t
     IROM_ECHECK(hipGetDevice(&dev), "hipGetDevice");
     IROM_ECHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev), "attr");
     int *rot = w.ints, *bar = w.ints + irom::EIG_MAX_SWEEPS, *sw_d = bar + 2, *err_d = sw_d + 1;
-    const long tot = std::max<long>((long)p.np * p.np, irom::EIG_MAX_SWEEPS + 4);
+    const int nints = irom::EIG_MAX_SWEEPS + 4 + p.np / irom::JC + p.np / irom::JB;
+    const long tot = std::max<long>((long)p.np * p.np, nints);
     irom::eig_init<<<(unsigned)((tot + 255) / 256), 256, 0, s>>>(n, p.np, d_G, w.W, w.V, w.ints,
-                                                                 irom::EIG_MAX_SWEEPS + 4);
+                                                                 nints);
     IROM_ECHECK(hipGetLastError(), "eig_init");
     // one inner sweep per round: more inner sweeps saved 1-2 outer sweeps of 13 but cost
     // more than they saved
     irom::EigArgs a{n, p.np, p.nb, tol, max_sweeps, 1, d_G, w.W, w.V, rot,
-                    (unsigned *)bar, sw_d, err_d};
+                    (unsigned *)bar, sw_d, err_d, eig_split(p.nb, ncu),
+                    (unsigned *)(err_d + 1), (unsigned *)(err_d + 1) + p.np / irom::JC};
     IROM_ECHECK(launch_gram_jacobi(a, ncu, s), "gram_jacobi launch");
     irom::column_norms<<<p.np, irom::ET, 0, s>>>(p.np, w.W, w.nrm);
     irom::column_norms<<<p.np, irom::ET, 0, s>>>(p.np, w.V, w.nrm + p.np);

@@ -95,7 +95,7 @@ __device__ __forceinline__ void wait_counts(const unsigned *c0, const unsigned *
                 break;
             }
         }
-        __threadfence();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     }
     __syncthreads();
 }
@@ -432,7 +432,7 @@ __global__ __launch_bounds__(JT) void gram_jacobi_kernel(EigArgs a)
                 // this share is done with blocks bi and bj: release its writes and count it
                 __syncthreads();
                 if (tid == 0) {
-                    __threadfence();
+                    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
                     atomicAdd(&a.ver[bi], 1u);
                     atomicAdd(&a.ver[bj], 1u);
                 }

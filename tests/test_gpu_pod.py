@@ -78,6 +78,35 @@ def test_eig_sym_parity(n):
     print(f"sweeps {sweeps}")
 
 
+def test_eig_sym_two_workgroups_per_block_pair():
+    """Order 2100: 66 block pairs, so each pair's rows are split over two workgroups instead
+    of four (at most one workgroup per CU); a POD-like spectrum (20 decaying modes over a
+    noise floor) against LAPACK eigvalsh.  eig_sym's default tolerance is n eps (pairs stop
+    rotating below n eps relative), so the eigenvalues are normwise accurate to a multiple of
+    n eps ||G||: the bars are 10 n eps lambda_max for the eigenvalues and 100 n eps
+    lambda_max for the residuals (at n = 512 the first is the 1e-12 of the test above);
+    the eigenvectors are products of rotations, orthonormal to 1e-12 whatever the order."""
+    n = 2100
+    print(f"[case] eig n={n}", flush=True)
+    rng = np.random.default_rng(21)
+    a = rng.standard_normal((n, 20)) * (2.0 ** -np.arange(20))[None, :]
+    B = np.concatenate([a, 1e-6 * rng.standard_normal((n, 2 * n))], axis=1)
+    G = B @ B.T
+    G = 0.5 * (G + G.T)
+    ev, V, sweeps = pod.eig_sym(_dev(G))
+    V = V.cpu().numpy()
+    ref = np.sort(np.linalg.eigvalsh(G))[::-1]
+    neps = n * np.finfo(np.float64).eps
+    err = np.abs(ev - ref).max() / ref[0]
+    res = np.linalg.norm(G @ V.T - V.T * ev, axis=0).max() / ref[0]
+    orth = np.abs(V @ V.T - np.eye(n)).max()
+    print(f"sweeps {sweeps}: eigenvalue error {err:.2e}, residual {res:.2e}, "
+          f"orthogonality {orth:.2e} (n eps = {neps:.2e})")
+    assert err < 10 * neps
+    assert res < 100 * neps
+    assert orth < 1e-12
+
+
 def test_pod_pipeline_small_mesh():
     """Mass-weighted POD of 24 snapshots on a 4^3 deformed hex mesh vs a dense FP64
     restatement (oracle mass operator + LAPACK eigh)."""

@@ -107,6 +107,15 @@ enum imf_precision { IMF_FP64 = 0, IMF_FP32 = 1 };
  * nm - 1 <= nq[2] <= 16, within one workgroup's LDS).  Tables: Modified_A for directions
  * 0 and 1, ModifiedPyr_C (imf_basis_modified_pyr_c) for direction 2 with its raw
  * Gauss-Radau weights as w[2] (the factor 1/4 is applied here).  Layouts as for prisms.
+ * Segments, "BwdTrans_Seg_Regular", "{IProduct,PhysDeriv,IProductWRTDerivBase}_Seg_{Regular,
+ * Deformed}" (FP64; BwdTrans.cpp, IProductSeg.cpp, PhysDeriv.cpp, IProductWRTDerivBaseSeg.cpp;
+ * BwdTrans.h:16-, IProduct.h:17-, PhysDeriv.h:17-, IProductWRTDerivBase.h:18-): one Modified_A
+ * direction of nm[0] modes on nq[0] GLL points (1 <= nm <= 16, nm <= nq <= 32; PhysDeriv any
+ * nq in 1..32) with tables bdata[0], dbdata[0] (IProductWRTDerivBase), D[0], w[0]; dim is the
+ * coordinate dimension (1 to 3) of the mesh the segments lie in: SetDF takes dim factors per
+ * element (_Regular) or per point (_Deformed), dim-major, PhysDeriv's out and
+ * IProductWRTDerivBase's in are dim arrays of nElmt*nq back to back.  No segment Helmholtz
+ * (the reference registers none).  Transposes: BwdTrans and IProduct.
  * "IProductWRTDerivBase_{Tri,Prism,Tet,Pyr}_{Regular,Deformed}" (IProductWRTDerivBase
  * {Tri,Prism,Tet,Pyr}.cpp; IProductWRTDerivBase.h:671-, :1630-, :2056-, :2484-): the
  * same sizes, tables (dbdata required) and layouts as the shape's other operators; in =
@@ -333,6 +342,11 @@ int imf_geom_quad_device(const double *d_verts, int nElmt, int nq, const double 
  * are constant: one value per element (the reference's eRegular factors), or nq0*nq1
  * copies when deformed != 0.  Layouts as above (4 factors, d = c*2 + a). */
 int imf_geom_tri_device(const double *d_verts, int nElmt, int nq0, int nq1, int deformed,
+                        double *d_jac, double *d_df, void *stream);
+/* Straight segments (SegGeom; GeomFactors.cpp:399-478 for one expansion direction):
+ * d_verts nElmt x 2 x ncoord (the two end points); J = |t| and df[c] = t_c / |t|^2 with
+ * t = (x1 - x0) / 2, per element (deformed = 0) or repeated per point (deformed = 1). */
+int imf_geom_seg_device(const double *d_verts, int nElmt, int ncoord, int nq, int deformed,
                         double *d_jac, double *d_df, void *stream);
 /* Straight-sided pyramids (PyrGeom, SpatialDomains/PyrGeom.cpp; GeomFactors.cpp):
  * d_verts nElmt x 5 x 3 in StdPyrExp vertex order (xi = (-1,-1,-1), (1,-1,-1), (1,1,-1),

@@ -22,7 +22,7 @@ enum Kind : int {
     kPhysDeriv = 4,   // quadrature values -> dim physical derivatives
     kIProdDeriv = 5   // IProductWRTDerivBase: dim quadrature fields -> coefficients
 };
-enum Shape : int { kQuad = 2, kHex = 3, kTri = 12, kPrism = 13, kTet = 14, kPyr = 15 };  // collapsed shapes from kTri
+enum Shape : int { kQuad = 2, kHex = 3, kTri = 12, kPrism = 13, kTet = 14, kPyr = 15, kSeg = 16 };  // collapsed shapes from kTri; kSeg: one-dimensional
 
 // Geometry modes.  kUnit: no geometry (the Euclidean transposes).  kDeformed:
 // per-point packed geometry.  kRegular: per-element packed geometry.

@@ -20,6 +20,7 @@
 #include "mf_geom.hpp"
 #include "mf_launch.hpp"
 #include "mf_op.hpp"
+#include "mf_seg.hpp"
 
 namespace {
 
@@ -39,7 +40,11 @@ int hip_fail(hipError_t e, const char *what)
 size_t elt_size(const imf_op op) { return op->prec == IMF_FP64 ? sizeof(double) : sizeof(float); }
 
 size_t jac_len(const imf_op op) { return (size_t)op->nElmt * (op->deformed ? op->Q : 1); }
-size_t df_len(const imf_op op) { return (size_t)op->dim * op->dim * jac_len(op); }
+// segments: one factor per coordinate (dim is their coordinate dimension)
+size_t df_len(const imf_op op)
+{
+    return (size_t)op->dim * (op->shape == imf::kSeg ? 1 : op->dim) * jac_len(op);
+}
 
 size_t geom_len(const imf_op op)
 {
@@ -146,6 +151,25 @@ int run(imf_op op, imf::LaunchFn fn, const void *in, void *out, void *stream, in
                                    : needs_df(op) ? "Jacobian and derivative factors not set"
                                                   : "Jacobian not set");
     if (!in || !out) return fail(IMF_ERR_ARG, "null input/output array");
+    if (op->collapsed && op->shape == imf::kSeg) {
+        if (nvec != 1 || gmap) return fail(IMF_ERR_UNSUPPORTED, "segments: one vector per apply");
+        imf::SegArgs t{};
+        t.kind = op->kind;
+        t.transpose = fn == tri_transpose_marker() ? 1 : 0;
+        t.geo = op->kind == imf::kBwdTrans ? imf::kUnit : (op->deformed ? imf::kDeformed : imf::kRegular);
+        t.nm = op->nm;
+        t.nq = op->nq;
+        t.nElmt = op->nElmt;
+        t.ncoord = op->dim;
+        t.tab = op->d_atab;
+        t.in = (const double *)in;
+        t.out = (double *)out;
+        t.jac = op->d_jac;
+        t.df = op->d_df;
+        hipError_t e = imf::launch_seg(t, (hipStream_t)stream);
+        if (e != hipSuccess) return hip_fail(e, "kernel launch");
+        return IMF_OK;
+    }
     if (op->collapsed && op->shape == imf::kPyr) {
         if (nvec != 1 || gmap) return fail(IMF_ERR_UNSUPPORTED, "pyramids: one vector per apply");
         imf::PyrArgs t{};
@@ -290,6 +314,54 @@ int create_aniso(const std::string &k, int kind, int shape, int deformed, const 
         (void)hipFree(op->d_atab);
         delete op;
         return hip_fail(e, "anisotropic basis tables");
+    }
+    *out = op;
+    return IMF_OK;
+}
+
+// Segments: one Modified_A direction (mf_seg.hip), FP64.  bdata/dbdata nm*nq, D nq^2, w nq
+// (GLL); ncoord (the dim argument) is the coordinate dimension of the mesh the segments lie
+// in: PhysDeriv gives, and IProductWRTDerivBase takes, ncoord arrays of nElmt*nq values.
+int create_seg(const std::string &k, int kind, int deformed, int ncoord, int m, int q,
+               const double *const *bdata, const double *const *dbdata,
+               const double *const *D, const double *const *w, int nElmt, int precision,
+               imf_op *out)
+{
+    if (precision != IMF_FP64 ||
+        (kind != imf::kBwdTrans && kind != imf::kIProduct && kind != imf::kPhysDeriv &&
+         kind != imf::kIProdDeriv))
+        return fail(IMF_ERR_UNSUPPORTED, k + ": segments are supported for FP64 BwdTrans, "
+                                             "IProduct, PhysDeriv and IProductWRTDerivBase");
+    if (m < 1 || m > 16 || q < 1 || q > 32 || (kind != imf::kPhysDeriv && q < m))
+        return fail(IMF_ERR_UNSUPPORTED, k + ": # of modes / points combo not implemented.");
+    const bool tabs = kind != imf::kPhysDeriv, deriv = kind == imf::kIProdDeriv;
+    if (!D[0] || !w[0] || (tabs && !bdata[0]) || (deriv && (!dbdata || !dbdata[0])))
+        return fail(IMF_ERR_ARG, k + ": missing basis tables");
+    std::vector<double> tab;
+    auto put = [&](const double *p, size_t n) {
+        if (p) tab.insert(tab.end(), p, p + n);
+        else tab.insert(tab.end(), n, 0.0);
+    };
+    put(tabs ? bdata[0] : nullptr, (size_t)m * q);
+    put(deriv ? dbdata[0] : nullptr, (size_t)m * q);
+    put(D[0], (size_t)q * q);
+    put(w[0], q);
+    imf_op op = new imf_op_s();
+    op->kind = kind; op->shape = imf::kSeg; op->deformed = deformed; op->prec = precision;
+    op->dim = ncoord; op->nm = m; op->nq = q; op->nElmt = nElmt;
+    op->collapsed = true;
+    op->nmd[0] = m;
+    op->nqd[0] = q;
+    op->M = (size_t)m;
+    op->Q = (size_t)q;
+    op->w.assign(w[0], w[0] + q);
+    hipError_t e = hipMalloc(&op->d_atab, tab.size() * sizeof(double));
+    if (e == hipSuccess)
+        e = hipMemcpy(op->d_atab, tab.data(), tab.size() * sizeof(double), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        (void)hipFree(op->d_atab);
+        delete op;
+        return hip_fail(e, "segment basis tables");
     }
     *out = op;
     return IMF_OK;
@@ -687,12 +759,23 @@ int imf_create(const char *key, int dim, const int *nm, const int *nq,
     else if (rest == "Tet_Deformed") { shape = imf::kTet; deformed = 1; }
     else if (rest == "Pyr_Regular") { shape = imf::kPyr; deformed = 0; }
     else if (rest == "Pyr_Deformed") { shape = imf::kPyr; deformed = 1; }
+    else if (rest == "Seg_Regular") { shape = imf::kSeg; deformed = 0; }
+    else if (rest == "Seg_Deformed") { shape = imf::kSeg; deformed = 1; }
     else return fail(IMF_ERR_UNSUPPORTED, "No such operator key: " + k);
     // The reference registers BwdTrans only as _Regular (MatrixFreeOps/BwdTrans.cpp:7-33).
     if (kind == imf::kBwdTrans && deformed)
         return fail(IMF_ERR_UNSUPPORTED, "No such operator key: " + k);
     if (kind == imf::kMass && shape != imf::kHex)
         return fail(IMF_ERR_UNSUPPORTED, "No such operator key: " + k);
+    if (shape == imf::kSeg) {
+        // the reference registers no segment Helmholtz (MatrixFreeOps/Helmholtz.cpp)
+        if (kind == imf::kHelmholtz) return fail(IMF_ERR_UNSUPPORTED, "No such operator key: " + k);
+        if (dim < 1 || dim > 3)
+            return fail(IMF_ERR_ARG, "imf_create: dim of " + k + " is the coordinate dimension, 1 to 3");
+        if (nElmt <= 0) return fail(IMF_ERR_ARG, "imf_create: nElmt must be positive");
+        return create_seg(k, kind, deformed, dim, nm[0], nq[0], bdata, dbdata, D, w, nElmt,
+                          precision, out);
+    }
     if (shape == imf::kTri || shape == imf::kPrism || shape == imf::kTet || shape == imf::kPyr) {
         if (dim != (shape == imf::kTri ? 2 : 3))
             return fail(IMF_ERR_ARG, "imf_create: dim does not match the shape of " + k);
@@ -1132,6 +1215,17 @@ int imf_geom_tri_device(const double *d_verts, int nElmt, int nq0, int nq1, int 
     hipError_t e = imf::geom_tri(d_verts, nElmt, nq0 * nq1, deformed, d_jac, d_df,
                                  (hipStream_t)stream);
     return e == hipSuccess ? IMF_OK : hip_fail(e, "geom_tri");
+}
+
+int imf_geom_seg_device(const double *d_verts, int nElmt, int ncoord, int nq, int deformed,
+                        double *d_jac, double *d_df, void *stream)
+{
+    if (!d_verts || !d_jac || !d_df || nElmt <= 0 || ncoord < 1 || ncoord > 3 || nq < 1 ||
+        nq > 64)
+        return fail(IMF_ERR_ARG, "imf_geom_seg_device: bad argument");
+    hipError_t e = imf::geom_seg(d_verts, nElmt, ncoord, nq, deformed, d_jac, d_df,
+                                 (hipStream_t)stream);
+    return e == hipSuccess ? IMF_OK : hip_fail(e, "geom_seg");
 }
 
 int imf_geom_pyr_device(const double *d_verts, int nElmt, int nq0, int nq1, int nq2,

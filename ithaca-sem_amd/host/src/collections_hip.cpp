@@ -67,7 +67,8 @@ void check_bases(const std::string &key, const std::vector<LibUtilities::BasisSh
     using namespace LibUtilities;
     const std::string shape = key.substr(key.find('_') + 1, key.rfind('_') - key.find('_') - 1);
     std::vector<BasisType> want;
-    if (shape == "Quad") want = {eModified_A, eModified_A};
+    if (shape == "Seg") want = {eModified_A};
+    else if (shape == "Quad") want = {eModified_A, eModified_A};
     else if (shape == "Hex") want = {eModified_A, eModified_A, eModified_A};
     else if (shape == "Tri") want = {eModified_A, eModified_B};
     else if (shape == "Prism") want = {eModified_A, eModified_A, eModified_B};
@@ -209,12 +210,15 @@ OperatorSharedPtr make(const std::string &key, std::vector<LibUtilities::BasisSh
     return std::make_shared<T>(key, b, n);
 }
 
-// Keys registered by the reference: MatrixFreeOps/BwdTrans.cpp:7-35, IProduct{Quad,Hex,
-// Tri,Prism,Tet,Pyr}.cpp, Helmholtz.cpp:7-42, PhysDeriv.cpp:14-49,
-// IProductWRTDerivBase{Quad,Hex,Tri,Prism,Tet,Pyr}.cpp.
+// Keys registered by the reference: MatrixFreeOps/BwdTrans.cpp:7-35, IProduct{Seg,Quad,Hex,
+// Tri,Prism,Tet,Pyr}.cpp, Helmholtz.cpp:7-42 (no segments), PhysDeriv.cpp:14-49,
+// IProductWRTDerivBase{Seg,Quad,Hex,Tri,Prism,Tet,Pyr}.cpp.  Segments made here lie in a
+// one-dimensional mesh (one derivative factor per point: the basis count is the dimension);
+// segments of 2D and 3D meshes are created through imf_create with dim = the coordinate
+// dimension.
 const bool registered = [] {
     auto &f = GetOperatorFactory();
-    for (const char *shape : {"Quad", "Hex", "Tri", "Prism", "Tet", "Pyr"}) {
+    for (const char *shape : {"Seg", "Quad", "Hex", "Tri", "Prism", "Tet", "Pyr"}) {
         const std::string s(shape);
         const std::string kb = "BwdTrans_" + s + "_Regular";
         f.RegisterCreatorFunction(kb, [kb](std::vector<LibUtilities::BasisSharedPtr> b, int n) {
@@ -232,6 +236,7 @@ const bool registered = [] {
             f.RegisterCreatorFunction(ki, [ki](std::vector<LibUtilities::BasisSharedPtr> b, int n) {
                 return make<IProduct>(ki, b, n);
             });
+            if (s == "Seg") continue;
             f.RegisterCreatorFunction(kh, [kh](std::vector<LibUtilities::BasisSharedPtr> b, int n) {
                 return make<Helmholtz>(kh, b, n);
             });
@@ -442,9 +447,9 @@ public:
         Array<OneD, Array<OneD, NekDouble>> in(m_dim);
         const size_t n = entry0.size();
         in[0] = Array<OneD, NekDouble>(n, entry0.data());
-        in[1] = entry1;
-        if (m_dim == 3) in[2] = entry2;
-        (*m_oper)(in, m_dim == 3 ? entry3 : entry2);
+        Array<OneD, NekDouble> *ents[3] = {&entry1, &entry2, &entry3};
+        for (int d = 1; d < m_dim; ++d) in[d] = *ents[d - 1];
+        (*m_oper)(in, *ents[m_dim - 1]);
     }
     void operator()(int, const Array<OneD, const NekDouble> &, Array<OneD, NekDouble> &,
                     Array<OneD, NekDouble> &) final
@@ -463,16 +468,17 @@ OperatorSharedPtr create(const CollectionExp &e, CoalescedGeomDataSharedPtr g)
 
 const bool registered = [] {
     auto &f = GetOperatorFactory();
-    for (auto shape : {LibUtilities::eQuadrilateral, LibUtilities::eHexahedron,
-                       LibUtilities::eTriangle, LibUtilities::ePrism, LibUtilities::eTetrahedron,
-                       LibUtilities::ePyramid}) {
+    for (auto shape : {LibUtilities::eSegment, LibUtilities::eQuadrilateral,
+                       LibUtilities::eHexahedron, LibUtilities::eTriangle, LibUtilities::ePrism,
+                       LibUtilities::eTetrahedron, LibUtilities::ePyramid}) {
         f.RegisterCreatorFunction(OperatorKey(shape, eBwdTrans, eMatrixFreeHip, false),
                                   create<BwdTrans_MatrixFreeHip>, "BwdTrans_MatrixFreeHip");
         f.RegisterCreatorFunction(OperatorKey(shape, eIProductWRTBase, eMatrixFreeHip, false),
                                   create<IProductWRTBase_MatrixFreeHip>,
                                   "IProductWRTBase_MatrixFreeHip");
-        f.RegisterCreatorFunction(OperatorKey(shape, eHelmholtz, eMatrixFreeHip, false),
-                                  create<Helmholtz_MatrixFreeHip>, "Helmholtz_MatrixFreeHip");
+        if (shape != LibUtilities::eSegment)  // no segment Helmholtz in the reference
+            f.RegisterCreatorFunction(OperatorKey(shape, eHelmholtz, eMatrixFreeHip, false),
+                                      create<Helmholtz_MatrixFreeHip>, "Helmholtz_MatrixFreeHip");
         f.RegisterCreatorFunction(OperatorKey(shape, ePhysDeriv, eMatrixFreeHip, false),
                                   create<PhysDeriv_MatrixFreeHip>, "PhysDeriv_MatrixFreeHip");
         f.RegisterCreatorFunction(OperatorKey(shape, eIProductWRTDerivBase, eMatrixFreeHip, false),

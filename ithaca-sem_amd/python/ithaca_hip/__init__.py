@@ -60,6 +60,7 @@ def lib():
         L.imf_geom_hex_aniso_device.argtypes = [vp, i, vp, vp, i, vp, vp, vp]
         L.imf_basis_modified_b.argtypes = [i, i, vp, vp, vp, vp, vp]
         L.imf_geom_tri_device.argtypes = [vp, i, i, i, i, vp, vp, vp]
+        L.imf_geom_seg_device.argtypes = [vp, i, i, i, i, vp, vp, vp]
         L.imf_geom_prism_device.argtypes = [vp, i, i, i, i, i, vp, vp, vp]
         L.imf_geom_tet_device.argtypes = [vp, i, i, i, i, i, vp, vp, vp]
         L.imf_basis_modified_c.argtypes = [i, i, vp, vp, vp, vp, vp]
@@ -134,12 +135,22 @@ def basis_modified_pyr_c(nm, nq):
 def geom_device(shape, verts, nq, z, deformed=True, stream=None):
     """Geometric factors of straight-sided elements on the GPU (GeomFactors.cpp:399-478).
 
-    verts: cuda float64 tensor (nElmt, 8, 3) for hexes, (nElmt, 4, 2) for quads, (nElmt, 3, 2)
+    verts: cuda float64 tensor (nElmt, 2, ncoord) for segments, (nElmt, 8, 3) for hexes,
+    (nElmt, 4, 2) for quads, (nElmt, 3, 2)
     for triangles, (nElmt, 6, 3) for prisms, (nElmt, 4, 3) for tetrahedra or (nElmt, 5, 3) for
     pyramids (collapsed shapes: nq the per-direction counts).
     Returns (jac, df) cuda tensors in the GetJac / GetDerivFactors layouts."""
     verts = verts.contiguous()
     E = verts.shape[0]
+    if shape == "seg":  # (nElmt, 2, ncoord) end points; nq points; z unused (linear map)
+        ncoord = verts.shape[2]
+        npts = E * (int(nq) if deformed else 1)
+        jac = torch.empty(npts, dtype=torch.float64, device=verts.device)
+        df = torch.empty(ncoord * npts, dtype=torch.float64, device=verts.device)
+        _check(lib().imf_geom_seg_device(ctypes.c_void_p(verts.data_ptr()), E, ncoord, int(nq),
+                                         int(deformed), ctypes.c_void_p(jac.data_ptr()),
+                                         ctypes.c_void_p(df.data_ptr()), _stream_ptr(stream)))
+        return jac, df
     if shape == "tri":  # (nElmt, 3, 2) vertices; nq = (nq0, nq1); z unused (linear map)
         nq0, nq1 = (int(nq[0]), int(nq[1])) if isinstance(nq, (tuple, list)) else (nq, nq - 1)
         npts = E * (nq0 * nq1 if deformed else 1)
@@ -209,7 +220,7 @@ class Operator:
     """One MatrixFree operator instance: ``GetOperatorFactory().CreateInstance(key, basis,
     nElmt)`` of the reference, implemented on the GPU."""
 
-    def __init__(self, key, nm, nq, nElmt, precision=FP64, basis=None):
+    def __init__(self, key, nm, nq, nElmt, precision=FP64, basis=None, coord_dim=1):
         """nm, nq: the modes and points of every direction, or (hexes) tuples of the three
         directions' counts with basis a list of the three directions' tables.  Triangles
         ("*_Tri_*"): nm modes, nq = nq0 (with nq1 = nq0 - 1) or (nq0, nq1); basis the
@@ -219,13 +230,22 @@ class Operator:
         ("*_Tet_*"): nm modes, nq = nq0 (with nq1 = nq2 = nq0 - 1) or (nq0, nq1, nq2); basis
         the Modified_A, Modified_B and Modified_C tables.  Pyramids ("*_Pyr_*"): nm modes,
         nq = nq0 (with nq1 = nq0, nq2 = nq0 - 1) or (nq0, nq1, nq2); basis the Modified_A,
-        Modified_A and ModifiedPyr_C tables."""
+        Modified_A and ModifiedPyr_C tables.  Segments ("*_Seg_*"): nm modes on nq GLL points,
+        basis one Modified_A table; coord_dim (1 to 3) is the coordinate dimension of the mesh
+        the segments lie in (the number of derivative factors, PhysDeriv outputs and
+        IProductWRTDerivBase inputs)."""
         self.key = key
         shape = key.split("_")[1]
         self.dim = 3 if shape in ("Hex", "Prism", "Tet", "Pyr") else 2
         self.nm, self.nq, self.nElmt, self.precision = nm, nq, nElmt, precision
         per_dir = isinstance(nm, (tuple, list))
-        if shape == "Tri":
+        if shape == "Seg":
+            # the C ABI reads direction 0's tables; dim carries the coordinate dimension
+            self.dim = int(coord_dim)
+            if basis is None:
+                basis = basis_modified_a(nm, nq)
+            nms, nqs, bases = [nm] * self.dim, [nq] * self.dim, [basis] * self.dim
+        elif shape == "Tri":
             nq0, nq1 = (int(nq[0]), int(nq[1])) if isinstance(nq, (tuple, list)) else (nq, nq - 1)
             nms, nqs = [nm, nm], [nq0, nq1]
             if basis is None:

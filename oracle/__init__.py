@@ -490,3 +490,46 @@ def iprodderiv_tet(b, E, deformed, jac, df, f):
 
 def iprodderiv_pyr(b, E, deformed, jac, df, f):
     return _iprodderiv3(lib().or_iprodderiv_pyr, b, E, deformed, jac, df, f, True)
+
+
+# ------------------------------------------------------------------ segments
+
+def geom_seg(verts, nq, deformed=False):
+    """verts: (E, 2, ncoord) end points -> (jac, df) in the GetJac / GetDerivFactors layouts."""
+    verts = np.ascontiguousarray(verts, dtype=np.float64)
+    E, ncoord = verts.shape[0], verts.shape[2]
+    n = E * nq if deformed else E
+    jac = np.zeros(n); df = np.zeros(ncoord * n)
+    lib().or_geom_seg(_p(verts), E, ncoord, nq, int(deformed), _p(jac), _p(df))
+    return jac, df
+
+
+def bwdtrans_seg(b, E, u):
+    u = np.ascontiguousarray(u, dtype=np.float64)
+    out = np.zeros(E * b["nq"])
+    lib().or_bwdtrans_seg(b["nm"], b["nq"], _p(b["bdata"]), E, _p(u), _p(out))
+    return out
+
+
+def iproduct_seg(b, E, deformed, jac, f):
+    f = np.ascontiguousarray(f, dtype=np.float64)
+    out = np.zeros(E * b["nm"])
+    lib().or_iproduct_seg(b["nm"], b["nq"], _p(b["bdata"]), _p(b["w"]), E, int(deformed),
+                          _p(jac), _p(f), _p(out))
+    return out
+
+
+def physderiv_seg(b, E, deformed, ncoord, df, f):
+    f = np.ascontiguousarray(f, dtype=np.float64)
+    out = np.zeros(ncoord * E * b["nq"])
+    lib().or_physderiv_seg(b["nq"], _p(b["D"]), E, int(deformed), ncoord, _p(df), _p(f), _p(out))
+    return out
+
+
+def iprodderiv_seg(b, E, deformed, ncoord, jac, df, f):
+    """f: ncoord arrays of E*nq point values back to back."""
+    f = np.ascontiguousarray(f, dtype=np.float64)
+    out = np.zeros(E * b["nm"])
+    lib().or_iprodderiv_seg(b["nm"], b["nq"], _p(b["dbdata"]), _p(b["w"]), E, int(deformed),
+                            ncoord, _p(jac), _p(df), _p(f), _p(out))
+    return out

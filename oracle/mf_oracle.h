@@ -206,6 +206,17 @@ void or_iprodderiv_pyr(int nm, int nq0, int nq1, int nq2, const double *b0, cons
                        const double *jac, const double *df, const double *in0,
                        const double *in1, const double *in2, double *out);
 
+void or_geom_seg(const double *verts, int E, int ncoord, int nq, int deformed, double *jac,
+                 double *df);
+void or_bwdtrans_seg(int nm, int nq, const double *b, int E, const double *in, double *out);
+void or_iproduct_seg(int nm, int nq, const double *b, const double *w, int E, int deformed,
+                     const double *jac, const double *in, double *out);
+void or_physderiv_seg(int nq, const double *D, int E, int deformed, int ncoord,
+                      const double *df, const double *in, double *out);
+void or_iprodderiv_seg(int nm, int nq, const double *db, const double *w, int E, int deformed,
+                       int ncoord, const double *jac, const double *df, const double *in,
+                       double *out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -43,6 +43,7 @@ size_t jac_len(const imf_op op) { return (size_t)op->nElmt * (op->deformed ? op-
 // segments: one factor per coordinate (dim is their coordinate dimension)
 size_t df_len(const imf_op op)
 {
+    if (op->ncoord == 3) return 6 * jac_len(op);  // a triangle in 3D space: 3 x 2 factors
     return (size_t)op->dim * (op->shape == imf::kSeg ? 1 : op->dim) * jac_len(op);
 }
 
@@ -243,6 +244,7 @@ int run(imf_op op, imf::LaunchFn fn, const void *in, void *out, void *stream, in
         t.nq0 = op->nqd[0];
         t.nq1 = op->nqd[1];
         t.nElmt = op->nElmt;
+        t.ncoord = op->ncoord == 3 ? 3 : 2;
         t.lambda = op->lambda;
         t.tab = op->d_atab;
         t.in = (const double *)in;
@@ -884,7 +886,7 @@ size_t imf_in_size(imf_op op)
     switch (op->kind) {
     case imf::kIProduct:
     case imf::kPhysDeriv: return (size_t)op->nElmt * op->Q;
-    case imf::kIProdDeriv: return (size_t)op->dim * op->nElmt * op->Q;
+    case imf::kIProdDeriv: return (size_t)(op->ncoord ? op->ncoord : op->dim) * op->nElmt * op->Q;
     default: return (size_t)op->nElmt * op->M;
     }
 }
@@ -925,6 +927,18 @@ int imf_set_df_device(imf_op op, const double *d_df, size_t n)
     if (int s = check_op(op)) return s;
     return set_array(op, &op->d_df, &op->n_df, d_df, n, df_len(op), "SetDF",
                      hipMemcpyDeviceToDevice);
+}
+
+int imf_set_coord_dim(imf_op op, int ncoord)
+{
+    if (int s = check_op(op)) return s;
+    if (op->shape != imf::kTri || op->kind != imf::kIProdDeriv)
+        return fail(IMF_ERR_UNSUPPORTED,
+                    "SetCoordDim: only triangles' IProductWRTDerivBase takes 3D coordinates");
+    if (ncoord != 2 && ncoord != 3) return fail(IMF_ERR_ARG, "SetCoordDim: ncoord must be 2 or 3");
+    if (op->d_df) return fail(IMF_ERR_ARG, "SetCoordDim: call before SetDF");
+    op->ncoord = ncoord == 3 ? 3 : 0;
+    return IMF_OK;
 }
 
 int imf_set_lambda(imf_op op, double lambda)

@@ -13,6 +13,7 @@
 
 struct imf_op_s {
     int kind = 0, shape = 0, deformed = 0, prec = 0, dim = 0, nm = 0, nq = 0;
+    int ncoord = 0;  // triangles' IProductWRTDerivBase in 3D space: 3 (0: the shape's dim)
     int nElmt = 0;
     size_t M = 0, Q = 0;
     std::vector<double> w;                 // direction-0 weights (host)

@@ -177,7 +177,8 @@ __global__ __launch_bounds__(TB) void tri_kernel(TriArgs a)
         if (valid)
             for (int t = lane; t < M; t += 64) a.out[e * M + t] = cout[t];
     } else if (a.kind == kIProdDeriv) {
-        // IProductWRTDerivBase: per point t_a = sum_c df[2c+a] in_c, the transpose of the
+        // IProductWRTDerivBase: per point t_a = sum_c df[2c+a] in_c over the ncoord fields (a
+        // triangle in 3D space has three, IProductWRTDerivBase.h:906-), the transpose of the
         // collapsed chain rule of PhysDeriv (tmp0 = h1 (t0 + h0 t1), tmp1 = t1), then the
         // inner products with (dB0, B1) and (B0, dB1)
         const long np = deformed ? (long)a.nElmt * Q : a.nElmt, nQ = (long)a.nElmt * Q;
@@ -187,7 +188,12 @@ __global__ __launch_bounds__(TB) void tri_kernel(TriArgs a)
             const double f0 = a.df[k], f1 = a.df[np + k], f2 = a.df[2 * np + k],
                          f3 = a.df[3 * np + k];
             const double x0 = valid ? a.in[e * Q + t] : 0.0, x1 = valid ? a.in[nQ + e * Q + t] : 0.0;
-            const double t0 = fma(f2, x1, f0 * x0), t1 = fma(f3, x1, f1 * x0);
+            double t0 = fma(f2, x1, f0 * x0), t1 = fma(f3, x1, f1 * x0);
+            if (a.ncoord == 3) {
+                const double x2 = valid ? a.in[2 * nQ + e * Q + t] : 0.0;
+                t0 = fma(a.df[4 * np + k], x2, t0);
+                t1 = fma(a.df[5 * np + k], x2, t1);
+            }
             d0[t] = (2.0 / (1.0 - T.z1[j])) * fma(0.5 * (1.0 + T.z0[i]), t1, t0);
             d1[t] = t1;
         }

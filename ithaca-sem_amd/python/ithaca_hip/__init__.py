@@ -61,6 +61,7 @@ def lib():
         L.imf_basis_modified_b.argtypes = [i, i, vp, vp, vp, vp, vp]
         L.imf_geom_tri_device.argtypes = [vp, i, i, i, i, vp, vp, vp]
         L.imf_geom_seg_device.argtypes = [vp, i, i, i, i, vp, vp, vp]
+        L.imf_set_coord_dim.argtypes = [vp, i]
         L.imf_geom_prism_device.argtypes = [vp, i, i, i, i, i, vp, vp, vp]
         L.imf_geom_tet_device.argtypes = [vp, i, i, i, i, i, vp, vp, vp]
         L.imf_basis_modified_c.argtypes = [i, i, vp, vp, vp, vp, vp]
@@ -233,7 +234,8 @@ class Operator:
         Modified_A and ModifiedPyr_C tables.  Segments ("*_Seg_*"): nm modes on nq GLL points,
         basis one Modified_A table; coord_dim (1 to 3) is the coordinate dimension of the mesh
         the segments lie in (the number of derivative factors, PhysDeriv outputs and
-        IProductWRTDerivBase inputs)."""
+        IProductWRTDerivBase input  IProductWRTDerivBase_Tri_* takes coord_dim = 3 for triangles in 3D
+        space (three fields, six derivative factors per point)."""
         self.key = key
         shape = key.split("_")[1]
         self.dim = 3 if shape in ("Hex", "Prism", "Tet", "Pyr") else 2
@@ -295,6 +297,8 @@ class Operator:
                                 arr(self._keep[1]), arr(self._keep[2]), arr(self._keep[3]),
                                 nElmt, precision, ctypes.byref(h)))
         self._h = h
+        if shape == "Tri" and int(coord_dim) == 3:  # IProductWRTDerivBase on a surface in 3D
+            _check(lib().imf_set_coord_dim(h, 3))
         self.dtype = torch.float64 if precision == FP64 else torch.float32
 
     def __del__(self):

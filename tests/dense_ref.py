@@ -176,14 +176,16 @@ def tri_element_mats(nm, nq0, nq1, verts):
     dV0 = np.stack([a * xi0 ** max(a - 1, 0) * xi1 ** c for a, c in pw], 1)
     dV1 = np.stack([c * xi0 ** a * xi1 ** max(c - 1, 0) for a, c in pw], 1)
     G = [dV0 @ coef, dV1 @ coef]
-    v = np.asarray(verts, dtype=float).reshape(3, 2)
+    v = np.asarray(verts, dtype=float)
+    nc = v.shape[1]                                                # 2, or 3 for a surface
     A = np.stack([0.5 * (v[1] - v[0]), 0.5 * (v[2] - v[0])], 1)   # A[c, a] = dx_c / dxi_a
-    Ainv = np.linalg.inv(A)
-    Gp = [Ainv[0, c] * G[0] + Ainv[1, c] * G[1] for c in range(2)]
+    Ainv = np.linalg.pinv(A)                                       # the inverse when nc = 2
+    Gp = [Ainv[0, c] * G[0] + Ainv[1, c] * G[1] for c in range(nc)]
     X = np.stack([-(xi0 + xi1) / 2 * v[0][c] + (1 + xi0) / 2 * v[1][c] + (1 + xi1) / 2 * v[2][c]
-                  for c in range(2)])
+                  for c in range(nc)])
     W = 0.5 * w0[I0] * w1[I1]
-    return dict(B=B, Gphys=Gp, W=W, J=abs(np.linalg.det(A)), X=X, z0=z0, z1=z1, w0=w0,
+    J = np.sqrt(np.linalg.det(A.T @ A))                            # |det A| when nc = 2
+    return dict(B=B, Gphys=Gp, W=W, J=J, X=X, A=A, Ainv=Ainv, z0=z0, z1=z1, w0=w0,
                 w1_raw=w1, fit_residual=np.abs(V @ coef - B).max())
 
 

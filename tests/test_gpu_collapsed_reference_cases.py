@@ -200,3 +200,44 @@ def test_reference_matrixfree_cases(name, opname, shape, verts, nm, nqb, lam, E)
 def _coords(shape, nm, nq, v):
     """Physical coordinates of the element's points (the dense reference's map)."""
     return list(_mats(shape, max(nm, 2), nq, v)["X"])
+
+
+@pytest.mark.parametrize("deformed", [True, False])
+def test_reference_tri_iprodderiv_in_3d_space_case(deformed):
+    """TestTriCollection.cpp:2233 TestTriIProductWRTDerivBase_MatrixFree_UniformP_Deformed_
+    MultiElmt_ThreeD: the triangle (-1, -1, 0), (1, -1, 0), (-1, 2, 1) in 3D space, 4 modes,
+    5 x 4 points, fields sin x cos y, cos x sin y and cos x sin z, the three directions'
+    results summed.  The derivative factors of a surface are the pseudo-inverse of its 3 x 2
+    map (dxi_a/dx_c, six per point), J = sqrt(det(A^T A))."""
+    print(f"[case] tri in 3D space, IProductWRTDerivBase deformed={deformed}", flush=True)
+    v = np.array([(-1, -1, 0), (1, -1, 0), (-1, 2, 1)], dtype=np.float64)
+    nm, nq = 4, (5, 4)
+    m = R.tri_element_mats(nm, nq[0], nq[1], v)
+    x, y, z = m["X"]
+    f = [np.sin(x) * np.cos(y), np.cos(x) * np.sin(y), np.cos(x) * np.sin(z)]
+    ref = sum(m["Gphys"][c].T @ (m["W"] * m["J"] * f[c]) for c in range(3))
+    Q = nq[0] * nq[1]
+    key = f"IProductWRTDerivBase_Tri_{'Deformed' if deformed else 'Regular'}"
+    op = H.Operator(key, nm, nq, E, coord_dim=3)
+    assert op.in_size == 3 * E * Q
+    n = E * Q if deformed else E
+    op.set_jac(np.full(n, m["J"]))
+    # GetDerivFactors order: factor 2c + a is dxi_a / dx_c
+    op.set_df(np.concatenate([np.full(n, m["Ainv"][a, c]) for c in range(3) for a in range(2)]))
+    inp = torch.from_numpy(np.concatenate([np.tile(fc, E) for fc in f])).cuda()
+    got = op.apply(inp).cpu().numpy().reshape(E, -1)
+    for e in range(E):
+        assert np.linalg.norm(got[e] - ref) < 1e-10 * np.linalg.norm(ref), e
+
+
+def test_coord_dim_is_refused_where_the_reference_has_no_3d_form():
+    print("[case] coord_dim refusals", flush=True)
+    with pytest.raises(H.MFError):
+        H.Operator("PhysDeriv_Tri_Regular", 4, (5, 4), E, coord_dim=3)
+    op = H.Operator("IProductWRTDerivBase_Tri_Regular", 4, (5, 4), E)
+    op.set_jac(np.ones(E))
+    op.set_df(np.zeros(4 * E))
+    with pytest.raises(H.MFError):  # after SetDF
+        H._check(H.lib().imf_set_coord_dim(op._h, 3))
+    with pytest.raises(H.MFError):
+        H._check(H.lib().imf_set_coord_dim(op._h, 4))

@@ -158,10 +158,11 @@ int imf_set_df_device(imf_op op, const double *d_df, size_t n);
 /* Helmholtz::SetLambda (Operator.hpp:193-196). */
 int imf_set_lambda(imf_op op, double lambda);
 
-/* IProductWRTDerivBase of triangles in 3D space (a surface mesh): ncoord = 3 fields in, and
- * SetDF then takes 6 factors per point (regular: per element), t_a = sum_c df[2c+a] in_c,
- * as MatrixFreeOps/IProductWRTDerivBase.h:906- does for three inputs.  Call before SetDF;
- * ncoord = 2 restores the default.  Other operators and shapes: IMF_ERR_UNSUPPORTED. */
+/* IProductWRTDerivBase of triangles and quadrilaterals in 3D space (a surface mesh), FP64:
+ * ncoord = 3 fields in, and SetDF then takes 6 factors per point (regular: per element),
+ * t_a = sum_c df[2c+a] in_c, as MatrixFreeOps/IProductWRTDerivBase.h:545- (quads) and :906-
+ * (triangles) do for three inputs.  Call before SetDF; ncoord = 2 restores the default.
+ * Other operators and shapes: IMF_ERR_UNSUPPORTED. */
 int imf_set_coord_dim(imf_op op, int ncoord);
 
 /* operator()(in, out) (Operator.hpp:70-72, 99-101, 176-178) on host arrays: copies in,

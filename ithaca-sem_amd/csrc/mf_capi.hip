@@ -143,6 +143,22 @@ int check_op(const imf_op op)
 hipError_t tri_transpose_fn(const imf::LaunchArgs &) { return hipErrorInvalidValue; }
 imf::LaunchFn tri_transpose_marker() { return &tri_transpose_fn; }
 
+// Quadrilaterals in 3D space, IProductWRTDerivBase (IProductWRTDerivBase.h:545-, three
+// inputs): y_a = sum_c df[2c+a] in_c per point, after which the packed kernel applies
+// identity factors times w J.  One thread per point; df [6][nElmt*Q] deformed, [6][nElmt]
+// regular.
+__global__ void quad_coord3_contract(const double *__restrict__ in,
+                                     const double *__restrict__ df, long nQ, int Q, int deformed,
+                                     double *__restrict__ y)
+{
+    const long g = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= nQ) return;
+    const long np = deformed ? nQ : nQ / Q, k = deformed ? g : g / Q;
+    const double x0 = in[g], x1 = in[nQ + g], x2 = in[2 * nQ + g];
+    y[g] = fma(df[4 * np + k], x2, fma(df[2 * np + k], x1, df[k] * x0));
+    y[nQ + g] = fma(df[5 * np + k], x2, fma(df[3 * np + k], x1, df[np + k] * x0));
+}
+
 int run(imf_op op, imf::LaunchFn fn, const void *in, void *out, void *stream, int nvec = 1,
         long ldin = 0, long ldout = 0, const int *gmap = nullptr,
         const signed char *gsign = nullptr)
@@ -261,6 +277,15 @@ int run(imf_op op, imf::LaunchFn fn, const void *in, void *out, void *stream, in
                                          (hipStream_t)stream);
         if (e != hipSuccess) return hip_fail(e, "kernel launch");
         return IMF_OK;
+    }
+    if (op->ncoord == 3) {  // quads in 3D space: contract the three fields first
+        if (nvec != 1 || gmap) return fail(IMF_ERR_UNSUPPORTED, "3D coordinates: one vector per apply");
+        const long nQ = (long)op->nElmt * op->Q;
+        quad_coord3_contract<<<(unsigned)((nQ + 255) / 256), 256, 0, (hipStream_t)stream>>>(
+            (const double *)in, op->d_df3, nQ, (int)op->Q, op->deformed, op->d_tmp3);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return hip_fail(e, "kernel launch");
+        in = op->d_tmp3;
     }
     imf::LaunchArgs a{op->tab.data(), op->nElmt, op->lambda, op->d_geom, in, out,
                       (hipStream_t)stream, nvec, ldin, ldout, gmap, gsign};
@@ -860,6 +885,8 @@ int imf_destroy(imf_op op)
     if (!op) return IMF_OK;
     (void)hipFree(op->d_jac);
     (void)hipFree(op->d_df);
+    (void)hipFree(op->d_df3);
+    (void)hipFree(op->d_tmp3);
     (void)hipFree(op->d_geom);
     (void)hipFree(op->d_in);
     (void)hipFree(op->d_out);
@@ -908,9 +935,28 @@ int imf_set_jac(imf_op op, const double *jac, size_t n)
                      hipMemcpyHostToDevice);
 }
 
+// Quadrilaterals in 3D space: keep the six raw factors for the contraction and hand the
+// packing identity factors (1, 0, 0, 1), so that the packed geometry is w J per point.
+int set_df_quad3(imf_op op, const double *df, size_t n, hipMemcpyKind kind)
+{
+    if (int s = set_array(op, &op->d_df3, &op->n_df3, df, n, df_len(op), "SetDF", kind)) return s;
+    const size_t np = jac_len(op);
+    if (!op->d_tmp3) {
+        hipError_t e = hipMalloc(&op->d_tmp3, 2 * (size_t)op->nElmt * op->Q * sizeof(double));
+        if (e != hipSuccess) return hip_fail(e, "hipMalloc(contracted fields)");
+    }
+    std::vector<double> id(4 * np, 0.0);
+    std::fill(id.begin(), id.begin() + np, 1.0);
+    std::fill(id.begin() + 3 * np, id.end(), 1.0);
+    return set_array(op, &op->d_df, &op->n_df, id.data(), 4 * np, 4 * np, "SetDF",
+                     hipMemcpyHostToDevice);
+}
+
 int imf_set_df(imf_op op, const double *df, size_t n)
 {
     if (int s = check_op(op)) return s;
+    if (op->ncoord == 3 && op->shape == imf::kQuad)
+        return set_df_quad3(op, df, n, hipMemcpyHostToDevice);
     return set_array(op, &op->d_df, &op->n_df, df, n, df_len(op), "SetDF",
                      hipMemcpyHostToDevice);
 }
@@ -925,6 +971,8 @@ int imf_set_jac_device(imf_op op, const double *d_jac, size_t n)
 int imf_set_df_device(imf_op op, const double *d_df, size_t n)
 {
     if (int s = check_op(op)) return s;
+    if (op->ncoord == 3 && op->shape == imf::kQuad)
+        return set_df_quad3(op, d_df, n, hipMemcpyDeviceToDevice);
     return set_array(op, &op->d_df, &op->n_df, d_df, n, df_len(op), "SetDF",
                      hipMemcpyDeviceToDevice);
 }
@@ -932,9 +980,10 @@ int imf_set_df_device(imf_op op, const double *d_df, size_t n)
 int imf_set_coord_dim(imf_op op, int ncoord)
 {
     if (int s = check_op(op)) return s;
-    if (op->shape != imf::kTri || op->kind != imf::kIProdDeriv)
-        return fail(IMF_ERR_UNSUPPORTED,
-                    "SetCoordDim: only triangles' IProductWRTDerivBase takes 3D coordinates");
+    if ((op->shape != imf::kTri && op->shape != imf::kQuad) || op->kind != imf::kIProdDeriv ||
+        op->prec != IMF_FP64 || op->aniso)
+        return fail(IMF_ERR_UNSUPPORTED, "SetCoordDim: only the FP64 IProductWRTDerivBase of "
+                                         "triangles and quadrilaterals takes 3D coordinates");
     if (ncoord != 2 && ncoord != 3) return fail(IMF_ERR_ARG, "SetCoordDim: ncoord must be 2 or 3");
     if (op->d_df) return fail(IMF_ERR_ARG, "SetCoordDim: call before SetDF");
     op->ncoord = ncoord == 3 ? 3 : 0;

@@ -13,7 +13,11 @@
 
 struct imf_op_s {
     int kind = 0, shape = 0, deformed = 0, prec = 0, dim = 0, nm = 0, nq = 0;
-    int ncoord = 0;  // triangles' IProductWRTDerivBase in 3D space: 3 (0: the shape's dim)
+    int ncoord = 0;  // IProductWRTDerivBase of triangles / quads in 3D space: 3 (0: the shape's dim)
+    // quads with ncoord = 3: the 6 raw factors, and the two contracted fields that the packed
+    // kernel (given identity factors) reads
+    double *d_df3 = nullptr, *d_tmp3 = nullptr;
+    size_t n_df3 = 0;
     int nElmt = 0;
     size_t M = 0, Q = 0;
     std::vector<double> w;                 // direction-0 weights (host)

@@ -234,7 +234,7 @@ class Operator:
         Modified_A and ModifiedPyr_C tables.  Segments ("*_Seg_*"): nm modes on nq GLL points,
         basis one Modified_A table; coord_dim (1 to 3) is the coordinate dimension of the mesh
         the segments lie in (the number of derivative factors, PhysDeriv outputs and
-        IProductWRTDerivBase input  IProductWRTDerivBase_Tri_* takes coord_dim = 3 for triangles in 3D
+        IProductWRTDerivBase input  IProductWRTDerivBase_{Tri,Quad}_* take coord_dim = 3 for elements in 3D
         space (three fields, six derivative factors per point)."""
         self.key = key
         shape = key.split("_")[1]
@@ -297,7 +297,7 @@ class Operator:
                                 arr(self._keep[1]), arr(self._keep[2]), arr(self._keep[3]),
                                 nElmt, precision, ctypes.byref(h)))
         self._h = h
-        if shape == "Tri" and int(coord_dim) == 3:  # IProductWRTDerivBase on a surface in 3D
+        if shape in ("Tri", "Quad") and int(coord_dim) == 3:  # IPDB on a surface in 3D
             _check(lib().imf_set_coord_dim(h, 3))
         self.dtype = torch.float64 if precision == FP64 else torch.float32
 

@@ -51,7 +51,8 @@ def hex_map(verts, xi):
 
 def quad_map(verts, xi):
     corners = np.array([[0, 0], [1, 0], [1, 1], [0, 1]])
-    x = np.zeros(2); A = np.zeros((2, 2))
+    nc = np.shape(verts)[1]                # 2, or 3 for a quad in 3D space
+    x = np.zeros(nc); A = np.zeros((nc, 2))
     for v, c in enumerate(corners):
         f = np.where(c, 0.5 * (1 + xi), 0.5 * (1 - xi))
         d = np.where(c, 0.5, -0.5)

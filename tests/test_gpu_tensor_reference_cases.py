@@ -119,3 +119,43 @@ def test_reference_tensor_matrixfree_cases(name, opname, shape, verts, nm, nq, l
     got = op.apply(torch.from_numpy(np.tile(inp, E)).cuda()).cpu().numpy().reshape(E, -1)
     for e in range(E):
         assert np.linalg.norm(got[e] - ref) < 1e-10 * np.linalg.norm(ref), (name, e)
+
+
+def test_reference_quad_iprodderiv_in_3d_space_case():
+    """TestQuadCollection.cpp:2049 TestQuadIProductWRTDerivBase_MatrixFree_UniformP_Deformed_
+    ThreeD: the quadrilateral (-1, -1, 0), (3, -1, 0), (1, 1, 1), (-1, 1, 1) in 3D space (a
+    trapezoid: per-point factors), 5 modes, 6 GLL points, fields sin x cos y, cos x sin y and
+    cos x sin z, the three directions' results summed, 10 elements.  Per point the factors are
+    the pseudo-inverse of the 3 x 2 map (dxi_a/dx_c) and J = sqrt(det(A^T A))."""
+    print("[case] quad in 3D space, IProductWRTDerivBase", flush=True)
+    v = np.array([(-1, -1, 0), (3, -1, 0), (1, 1, 1), (-1, 1, 1)], dtype=np.float64)
+    nm, nq, E = 5, 6, 10
+    z, w = DR.gll(nq)
+    B1, dB1 = DR.modified_a(nm, z)
+    G = [np.kron(B1, dB1), np.kron(dB1, B1)]
+    W = np.kron(w, w)
+    Q = nq * nq
+    X, J, P = np.zeros((Q, 3)), np.zeros(Q), np.zeros((Q, 2, 3))
+    for j in range(nq):
+        for i in range(nq):
+            x, A = DR.quad_map(v, np.array([z[i], z[j]]))
+            X[j * nq + i], P[j * nq + i] = x, np.linalg.pinv(A)
+            J[j * nq + i] = np.sqrt(np.linalg.det(A.T @ A))
+    assert np.ptp(J) > 1e-3  # not a parallelogram: the deformed path
+    f = [np.sin(X[:, 0]) * np.cos(X[:, 1]), np.cos(X[:, 0]) * np.sin(X[:, 1]),
+         np.cos(X[:, 0]) * np.sin(X[:, 2])]
+    ref = sum((P[:, a, c][:, None] * G[a]).T @ (W * J * f[c]) for c in range(3) for a in range(2))
+    op = H.Operator("IProductWRTDerivBase_Quad_Deformed", nm, nq, E, coord_dim=3)
+    assert op.in_size == 3 * E * Q
+    op.set_jac(np.tile(J, E))
+    op.set_df(np.concatenate([np.tile(P[:, a, c], E) for c in range(3) for a in range(2)]))
+    inp = torch.from_numpy(np.concatenate([np.tile(fc, E) for fc in f])).cuda()
+    got = op.apply(inp).cpu().numpy().reshape(E, -1)
+    for e in range(E):
+        assert np.linalg.norm(got[e] - ref) < 1e-10 * np.linalg.norm(ref), e
+    # the same operator on two coordinates is unchanged by the setter's default
+    op2 = H.Operator("IProductWRTDerivBase_Quad_Deformed", nm, nq, E)
+    assert op2.in_size == 2 * E * Q
+    with pytest.raises(H.MFError):
+        H.Operator("IProductWRTDerivBase_Quad_Deformed", nm, nq, E, precision=H.FP32,
+                   coord_dim=3)

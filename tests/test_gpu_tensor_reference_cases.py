@@ -8,8 +8,8 @@ MatrixFree operator with the element's own routine to BOOST_CHECK_CLOSE 1e-8 (pe
 here the GPU operator is compared with independent dense matrices (tests/dense_ref.py;
 the oracle's collocation derivative for PhysDeriv) to 1e-10 relative.  The hexahedral
 Helmholtz cases (:3649, :3747) are in tests/test_gpu_operators.py; the quadrilateral
-embedded in three dimensions (:2049, a 2D element with 3D coordinates) is not covered:
-these quadrilateral operators take 2D coordinates."""
+embedded in three dimensions (:2049, a 2D element with 3D coordinates) is the last test
+below."""
 import numpy as np
 import pytest
 import torch
